@@ -1,0 +1,860 @@
+/*
+ * FP64 CPU restatement of the GFS hybrid sigma-pressure dynamical core step
+ * (TEST INFRASTRUCTURE ONLY -- see oracle.h). Each function names the
+ * reference lines it follows. Index comments use the reference's 1-based
+ * level numbering: L(X,k) is level k of a 3-D grid array.
+ *
+ * Single-precision literals: the reference compiled with default_real=2
+ * (kinds.f90:100) still writes constants as default-real literals
+ * (physcons.f90:17-56, semimp_data.f90:74-88, phy_run_hs.f90:42-49,
+ * pressure_data.f90:92-93). They are rounded to float here with SGL() so the
+ * restatement carries the same values as that build.
+ *
+ * Pinning: calc_pressdata against closed forms; getomega/getpresgrad/
+ * energy conversion/continuity against the independently built linear
+ * operators of init_semimpdata (linearisation test); init_semimpdata by
+ * (I + c^2 dt^2 n(n+1) ym) d = I and the tableau identities; advance by
+ * the IMEX order and a resting-state invariant; HS by closed-form columns.
+ * The exact vertical differencing of getvadv / getvadv_tracers has no closed
+ * form and no reference-run vectors here: parity unpinned beyond the
+ * constant-field and positivity properties.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define SGL(x) ((double)(float)(x))
+
+/* IMEX tableau, semimp_data.f90:74-88 (default-real literals) */
+#define A21 SGL(1. / 3.)
+#define A31 SGL(1. / 6.)
+#define A32 SGL(1. / 2.)
+#define AA21 SGL(1. / 6.)
+#define AA22 SGL(1. / 6.)
+#define AA31 SGL(1. / 3.)
+#define AA32 0.0
+#define AA33 SGL(1. / 3.)
+#define B1 SGL(1. / 2.)
+#define B2 SGL(-1. / 2.)
+#define B3 1.0
+#define BB1 SGL(3. / 8.)
+#define BB2 0.0
+#define BB3 SGL(3. / 8.)
+#define BB4 SGL(1. / 4.)
+
+struct orc_model {
+    orc_cfg c;
+    orc_sht *sht;
+    int nlon, nlat, nlev, ntrac, T, nlm;
+    long ng;
+    double *ak, *bk, *ck, *dbk, *si, *sl;
+    double *lats, *areawts, *lap, *sinlat, *coslat;
+    int *degree;
+    /* spectral state */
+    ocplx *vrt, *div, *tv, *trac, *lnps, *topo;
+    /* grid fields */
+    double *ug, *vg, *vrtg, *divg, *tvg, *tracg, *dlnpdtg, *etadot, *lnpsg, *psg, *dlnpsdt;
+    double *dphisdx, *dphisdy, *dlnpsdx, *dlnpsdy;
+    double *pk, *dpk, *alfa, *rlnp, *prs;
+    double *w[10]; /* work grids (nlev) */
+    double *cb, *db, *cg, *wk1;
+    /* semi-implicit and damping operators */
+    double *amhyb, *bmhyb, *svhyb, *tor_hyb, *d_hyb_m;
+    double *disspec, *diff_prof, *dmp_prof;
+    double pdryini, kappa;
+    ocplx *lin_d[3], *lin_t[3], *lin_p[3]; /* linear terms of the RK stages */
+    int have_grid;
+};
+
+#define L(X, k) ((X) + (long)((k) - 1) * m->ng)
+
+void orc_cfg_defaults(orc_cfg *c) {
+    memset(c, 0, sizeof(*c));
+    c->explicit_ = 0;
+    c->physics = 1;
+    c->massfix = 1;
+    c->ndiss = 0;
+    c->efold = 0.0;
+    c->hdif_fac = 1.0;
+    c->hdif_fac2 = 1.0;
+    c->fshk = 0.0;
+    c->taustratdamp = SGL(5. * 86400.);
+    /* physcons.f90:23-36 */
+    c->rerth = SGL(6.3712e+6);
+    c->grav = SGL(9.80665e+0);
+    c->omega = SGL(7.2921e-5);
+    c->rd = SGL(2.8705e+2);
+    c->rv = SGL(4.6150e+2);
+    c->cp = SGL(1.0046e+3);
+    c->cvap = SGL(1.8460e+3);
+}
+
+/* counter-based RNG: splitmix64 finaliser over a mixed (seed, stream, index) key */
+uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ULL ^ (stream + 0x632BE59BD9B4E019ULL) * 0xBF58476D1CE4E5B9ULL;
+    z += index * 0x94D049BB133111EBULL + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double orc_rng_uniform(uint64_t seed, uint64_t stream, uint64_t index) {
+    return (double)(orc_rng_u64(seed, stream, index) >> 11) * 0x1p-53;
+}
+
+/* ---- pressure_data.f90:58-100 calc_pressdata ---------------------- */
+static void calc_press_cols(int nlev, long ncol, const double *ak, const double *bk, double rk,
+                            const double *lnpsg, double *psg, double *pk, double *dpk,
+                            double *alfa, double *rlnp, double *prs) {
+#pragma omp parallel for
+    for (long i = 0; i < ncol; i++) {
+        double ps = exp(lnpsg[i]);
+        if (psg) psg[i] = ps;
+        for (int k = 1; k <= nlev + 1; k++) pk[(long)(k - 1) * ncol + i] = ak[k - 1] + bk[k - 1] * ps;
+        for (int k = 1; k <= nlev; k++) {
+            double p1 = pk[(long)(k - 1) * ncol + i], p2 = pk[(long)k * ncol + i];
+            double d = p2 - p1;
+            dpk[(long)(k - 1) * ncol + i] = d;
+            prs[(long)(nlev - k) * ncol + i] =
+                pow((pow(p2, rk) * p2 - pow(p1, rk) * p1) / ((rk + 1.0) * d), 1.0 / rk);
+        }
+        alfa[i] = SGL(log(2.));
+        rlnp[i] = SGL(99999.99);
+        for (int k = 2; k <= nlev; k++) {
+            double p1 = pk[(long)(k - 1) * ncol + i], p2 = pk[(long)k * ncol + i];
+            double r = log(p2 / p1);
+            rlnp[(long)(k - 1) * ncol + i] = r;
+            alfa[(long)(k - 1) * ncol + i] = 1.0 - (p1 / dpk[(long)(k - 1) * ncol + i]) * r;
+        }
+    }
+}
+
+void orc_calc_pressdata(int nlev, int ncol, const double *ak, const double *bk, double rk,
+                        const double *lnpsg, double *pk, double *dpk, double *alfa,
+                        double *rlnp, double *prs) {
+    calc_press_cols(nlev, ncol, ak, bk, rk, lnpsg, NULL, pk, dpk, alfa, rlnp, prs);
+}
+
+/* ---- semimp_data.f90:127-256 init_semimpdata ---------------------- */
+static int invert(int n, double *a /* col-major n*n, replaced by inverse */) {
+    double *b = calloc((size_t)n * n, sizeof(double));
+    for (int i = 0; i < n; i++) b[i + i * n] = 1.0;
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (fabs(a[r + c * n]) > fabs(a[piv + c * n])) piv = r;
+        if (a[piv + c * n] == 0.0) { free(b); return -1; }
+        if (piv != c)
+            for (int j = 0; j < n; j++) {
+                double t = a[c + j * n]; a[c + j * n] = a[piv + j * n]; a[piv + j * n] = t;
+                t = b[c + j * n]; b[c + j * n] = b[piv + j * n]; b[piv + j * n] = t;
+            }
+        double d = 1.0 / a[c + c * n];
+        for (int j = 0; j < n; j++) { a[c + j * n] *= d; b[c + j * n] *= d; }
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            double f = a[r + c * n];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; j++) { a[r + j * n] -= f * a[c + j * n]; b[r + j * n] -= f * b[c + j * n]; }
+        }
+    }
+    memcpy(a, b, sizeof(double) * n * n);
+    free(b);
+    return 0;
+}
+
+static void init_semimp(orc_model *m) {
+    int nl = m->nlev;
+    double rd = m->c.rd, kappa = m->kappa, rerth = m->c.rerth, dt = m->c.dt;
+    const double tref = 300.0, pref = 800.e2; /* semimp_data.f90:122-123 */
+    double *pkref = malloc(sizeof(double) * (nl + 1)), *dpkref = malloc(sizeof(double) * nl);
+    double *alfaref = malloc(sizeof(double) * nl);
+    double *yecm = calloc((size_t)nl * nl, sizeof(double)), *tecm = calloc((size_t)nl * nl, sizeof(double));
+    double *ym = calloc((size_t)nl * nl, sizeof(double));
+#define CM(A, i, j) A[((i) - 1) + (long)((j) - 1) * nl]
+    for (int k = 1; k <= nl + 1; k++) pkref[k - 1] = m->ak[k - 1] + m->bk[k - 1] * pref;
+    for (int k = 1; k <= nl; k++) dpkref[k - 1] = pkref[k] - pkref[k - 1];
+    alfaref[0] = SGL(log(2.));
+    for (int k = 2; k <= nl; k++)
+        alfaref[k - 1] = 1.0 - (pkref[k - 1] / dpkref[k - 1]) * log(pkref[k] / pkref[k - 1]);
+    for (int ir = 1; ir <= nl; ir++) {
+        CM(yecm, ir, ir) = alfaref[ir - 1] * rd;
+        for (int ic = ir + 1; ic <= nl; ic++) CM(yecm, ir, ic) = rd * log(pkref[ic] / pkref[ic - 1]);
+    }
+    for (int ir = 1; ir <= nl; ir++) {
+        CM(tecm, ir, ir) = kappa * tref * alfaref[ir - 1];
+        for (int ic = 1; ic <= ir - 1; ic++)
+            CM(tecm, ir, ic) = (kappa * tref * dpkref[ic - 1] / dpkref[ir - 1]) * log(pkref[ir] / pkref[ir - 1]);
+    }
+    for (int j = 1; j <= nl; j++) {
+        m->svhyb[j - 1] = dpkref[nl - j] / pref;
+        for (int k = 1; k <= nl; k++) {
+            CM(m->amhyb, k, j) = CM(yecm, nl + 1 - k, nl + 1 - j) / (rerth * rerth);
+            CM(m->bmhyb, k, j) = CM(tecm, nl + 1 - k, nl + 1 - j);
+        }
+    }
+    for (int k = 0; k < nl; k++) m->tor_hyb[k] = rd * tref / (rerth * rerth);
+    for (int i = 1; i <= nl; i++)
+        for (int j = 1; j <= nl; j++) {
+            double s = m->tor_hyb[i - 1] * m->svhyb[j - 1];
+            for (int k = 1; k <= nl; k++) s += CM(m->amhyb, i, k) * CM(m->bmhyb, k, j);
+            CM(ym, i, j) = s;
+        }
+    const double consts[3] = {AA22, AA33, BB4};
+    for (int s = 0; s < 3; s++)
+        for (int nn = 1; nn <= m->T + 1; nn++) {
+            int n = nn - 1;
+            double rnn1 = (double)n * (n + 1);
+            double *d = m->d_hyb_m + ((long)s * (m->T + 1) + (nn - 1)) * nl * nl;
+            double cdt = consts[s] * dt;
+            for (int i = 1; i <= nl; i++)
+                for (int j = 1; j <= nl; j++) CM(d, i, j) = (i == j ? 1.0 : 0.0) + cdt * cdt * rnn1 * CM(ym, i, j);
+            if (invert(nl, d) != 0) fprintf(stderr, "oracle: singular semi-implicit matrix\n");
+        }
+#undef CM
+    free(pkref); free(dpkref); free(alfaref); free(yecm); free(tecm); free(ym);
+}
+
+/* ---- dyn_init.f90:450-494 setdampspec ------------------------------ */
+static void setdampspec(orc_model *m) {
+    int ndiss = m->c.ndiss == 0 ? 8 : m->c.ndiss;
+    double efold = m->c.efold, fshk = m->c.fshk;
+    double rerth = m->c.rerth, hf2 = m->c.hdif_fac2, hf = m->c.hdif_fac;
+    int T = m->T;
+    if (efold <= 0) {
+        if (T > 170)
+            efold = 3600. / (hf2 * SGL(pow((float)T / 170.f, 4)) * SGL(1.1));
+        else if (T == 126 || T == 170)
+            efold = 1. / (hf2 * SGL(12.E15) / pow(rerth, 4) * SGL(80. * 81.) * SGL(80. * 81.));
+        else
+            efold = 1. / (hf2 * SGL(3.E15) / pow(rerth, 4) * SGL(80. * 81.) * SGL(80. * 81.));
+        efold = 2. * efold;
+    }
+    if (fshk <= 0) {
+        fshk = 1.0 * hf;
+        if (T > 170) fshk = SGL(2.2) * hf;
+        if (T == 126) fshk = SGL(1.5) * hf;
+    }
+    double slrd0 = SGL(0.002), dmp1 = 1. / m->c.taustratdamp;
+    for (int k = 0; k < m->nlev; k++) {
+        m->dmp_prof[k] = 0.0;
+        m->diff_prof[k] = pow(m->sl[k], log(1. / fshk));
+        if (m->sl[k] < slrd0) m->dmp_prof[k] = dmp1 * log(slrd0 / m->sl[k]);
+    }
+    double minlap = -(double)T * (T + 1);
+    for (int k = 0; k < m->nlm; k++) {
+        double r = m->lap[k] / minlap, p = 1.0;
+        for (int i = 0; i < ndiss / 2; i++) p *= r;
+        m->disspec[k] = -(1. / efold) * p;
+    }
+}
+
+orc_model *orc_model_create(const orc_cfg *c) {
+    orc_model *m = calloc(1, sizeof(orc_model));
+    m->c = *c;
+    m->nlon = c->nlon; m->nlat = c->nlat; m->nlev = c->nlev; m->ntrac = c->ntrac; m->T = c->ntrunc;
+    m->sht = orc_sht_create(c->nlon, c->nlat, c->ntrunc);
+    m->nlm = orc_sht_nlm(m->sht);
+    m->ng = (long)c->nlon * c->nlat;
+    m->kappa = c->rd / c->cp;
+    int nl = c->nlev, nlm = m->nlm;
+    long ng = m->ng;
+    m->ak = malloc(sizeof(double) * (nl + 1)); m->bk = malloc(sizeof(double) * (nl + 1));
+    memcpy(m->ak, c->ak, sizeof(double) * (nl + 1));
+    memcpy(m->bk, c->bk, sizeof(double) * (nl + 1));
+    m->ck = malloc(sizeof(double) * nl); m->dbk = malloc(sizeof(double) * nl);
+    m->si = malloc(sizeof(double) * (nl + 1)); m->sl = malloc(sizeof(double) * nl);
+    for (int k = 1; k <= nl; k++) {
+        m->dbk[k - 1] = m->bk[k] - m->bk[k - 1];
+        m->ck[k - 1] = m->ak[k] * m->bk[k - 1] - m->ak[k - 1] * m->bk[k];
+    }
+    /* pressure_data.f90:63-70 */
+    for (int k = 1; k <= nl + 1; k++) m->si[nl + 1 - k] = m->ak[k - 1] / SGL(101300.0) + m->bk[k - 1];
+    for (int k = 0; k < nl; k++) m->sl[k] = 0.5 * (m->si[k] + m->si[k + 1]);
+    m->lats = malloc(sizeof(double) * c->nlat);
+    m->areawts = malloc(sizeof(double) * ng);
+    m->lap = malloc(sizeof(double) * nlm);
+    m->degree = malloc(sizeof(int) * nlm);
+    orc_sht_griddata(m->sht, m->lats, NULL, m->areawts, m->degree, NULL, m->lap, NULL);
+    m->sinlat = malloc(sizeof(double) * ng);
+    m->coslat = malloc(sizeof(double) * ng);
+    for (int j = 0; j < c->nlat; j++)
+        for (int i = 0; i < c->nlon; i++) {
+            m->sinlat[(long)j * c->nlon + i] = sin(m->lats[j]);
+            m->coslat[(long)j * c->nlon + i] = cos(m->lats[j]);
+        }
+    size_t s3 = sizeof(ocplx) * nlm * nl;
+    m->vrt = calloc(1, s3); m->div = calloc(1, s3); m->tv = calloc(1, s3);
+    m->trac = calloc(1, s3 * (c->ntrac > 0 ? c->ntrac : 1));
+    m->lnps = calloc(nlm, sizeof(ocplx)); m->topo = calloc(nlm, sizeof(ocplx));
+    size_t g3 = sizeof(double) * ng * nl, g2 = sizeof(double) * ng;
+    m->ug = malloc(g3); m->vg = malloc(g3); m->vrtg = malloc(g3); m->divg = malloc(g3);
+    m->tvg = malloc(g3); m->tracg = malloc(g3 * (c->ntrac > 0 ? c->ntrac : 1));
+    m->dlnpdtg = malloc(g3); m->etadot = malloc(g3 + g2); m->prs = malloc(g3);
+    m->pk = malloc(g3 + g2); m->dpk = malloc(g3); m->alfa = malloc(g3); m->rlnp = malloc(g3);
+    for (int i = 0; i < 10; i++) m->w[i] = malloc(g3);
+    m->cb = malloc(g3); m->db = malloc(g3); m->cg = malloc(g3); m->wk1 = malloc(g3 + g2);
+    m->lnpsg = malloc(g2); m->psg = malloc(g2); m->dlnpsdt = malloc(g2);
+    m->dphisdx = calloc(ng, sizeof(double)); m->dphisdy = calloc(ng, sizeof(double));
+    m->dlnpsdx = malloc(g2); m->dlnpsdy = malloc(g2);
+    m->amhyb = malloc(sizeof(double) * nl * nl); m->bmhyb = malloc(sizeof(double) * nl * nl);
+    m->svhyb = malloc(sizeof(double) * nl); m->tor_hyb = malloc(sizeof(double) * nl);
+    m->d_hyb_m = malloc(sizeof(double) * nl * nl * (m->T + 1) * 3);
+    m->disspec = malloc(sizeof(double) * nlm);
+    m->diff_prof = malloc(sizeof(double) * nl); m->dmp_prof = malloc(sizeof(double) * nl);
+    for (int s = 0; s < 3; s++) {
+        m->lin_d[s] = malloc(s3); m->lin_t[s] = malloc(s3); m->lin_p[s] = malloc(sizeof(ocplx) * nlm);
+    }
+    init_semimp(m);
+    setdampspec(m);
+    return m;
+}
+
+void orc_model_destroy(orc_model *m) {
+    if (!m) return;
+    orc_sht_destroy(m->sht);
+    void *ptrs[] = {m->ak, m->bk, m->ck, m->dbk, m->si, m->sl, m->lats, m->areawts, m->lap,
+                    m->degree, m->sinlat, m->coslat, m->vrt, m->div, m->tv, m->trac, m->lnps,
+                    m->topo, m->ug, m->vg, m->vrtg, m->divg, m->tvg, m->tracg, m->dlnpdtg,
+                    m->etadot, m->prs, m->pk, m->dpk, m->alfa, m->rlnp, m->cb, m->db, m->cg,
+                    m->wk1, m->lnpsg, m->psg, m->dlnpsdt, m->dphisdx, m->dphisdy, m->dlnpsdx,
+                    m->dlnpsdy, m->amhyb, m->bmhyb, m->svhyb, m->tor_hyb, m->d_hyb_m,
+                    m->disspec, m->diff_prof, m->dmp_prof};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    for (int i = 0; i < 10; i++) free(m->w[i]);
+    for (int s = 0; s < 3; s++) { free(m->lin_d[s]); free(m->lin_t[s]); free(m->lin_p[s]); }
+    free(m);
+}
+
+const orc_sht *orc_model_sht(const orc_model *m) { return m->sht; }
+
+void orc_set_state(orc_model *m, const ocplx *vrt, const ocplx *div, const ocplx *tv,
+                   const ocplx *trac, const ocplx *lnps) {
+    size_t s3 = sizeof(ocplx) * m->nlm * m->nlev;
+    if (vrt) memcpy(m->vrt, vrt, s3);
+    if (div) memcpy(m->div, div, s3);
+    if (tv) memcpy(m->tv, tv, s3);
+    if (trac && m->ntrac > 0) memcpy(m->trac, trac, s3 * m->ntrac);
+    if (lnps) memcpy(m->lnps, lnps, sizeof(ocplx) * m->nlm);
+}
+
+void orc_get_state(const orc_model *m, ocplx *vrt, ocplx *div, ocplx *tv, ocplx *trac,
+                   ocplx *lnps) {
+    size_t s3 = sizeof(ocplx) * m->nlm * m->nlev;
+    if (vrt) memcpy(vrt, m->vrt, s3);
+    if (div) memcpy(div, m->div, s3);
+    if (tv) memcpy(tv, m->tv, s3);
+    if (trac && m->ntrac > 0) memcpy(trac, m->trac, s3 * m->ntrac);
+    if (lnps) memcpy(lnps, m->lnps, sizeof(ocplx) * m->nlm);
+}
+
+void orc_set_pdryini(orc_model *m, double p) { m->pdryini = p; }
+double orc_get_pdryini(const orc_model *m) { return m->pdryini; }
+
+static double area_sum(const orc_model *m, const double *x) {
+    double s = 0.0;
+    for (long i = 0; i < m->ng; i++) s += m->areawts[i] * x[i];
+    return s;
+}
+
+/* ---- dyn_run.f90:307-373 getomega ---------------------------------- */
+static void getomega(orc_model *m) {
+    int nl = m->nlev;
+    long ng = m->ng;
+    double *cg = m->cg, *cb = m->cb, *db = m->db, *wb = m->w[8], *wc = m->w[9];
+    double *dpk = m->dpk, *dbk = m->dbk, *bk = m->bk, *ck = m->ck, *psg = m->psg;
+#pragma omp parallel for
+    for (long i = 0; i < ng; i++) {
+        for (int k = 1; k <= nl; k++)
+            L(cg, k)[i] = L(m->ug, nl + 1 - k)[i] * m->dlnpsdx[i] + L(m->vg, nl + 1 - k)[i] * m->dlnpsdy[i];
+        L(db, 1)[i] = L(m->divg, nl)[i] * L(dpk, 1)[i];
+        L(cb, 1)[i] = L(cg, 1)[i] * dbk[0];
+        for (int k = 1; k <= nl - 1; k++) {
+            L(db, k + 1)[i] = L(db, k)[i] + L(m->divg, nl - k)[i] * L(dpk, k + 1)[i];
+            L(cb, k + 1)[i] = L(cb, k)[i] + L(cg, k + 1)[i] * dbk[k];
+        }
+        m->dlnpsdt[i] = -L(db, nl)[i] / psg[i] - L(cb, nl)[i];
+        L(m->etadot, 1)[i] = 0.0;
+        L(m->etadot, nl + 1)[i] = 0.0;
+        for (int k = 1; k <= nl - 1; k++)
+            L(m->etadot, k + 1)[i] = -psg[i] * (bk[k] * m->dlnpsdt[i] + L(cb, k)[i]) - L(db, k)[i];
+        L(wb, 1)[i] = L(m->alfa, 1)[i] * (L(m->divg, nl)[i] * L(dpk, 1)[i] + psg[i] * L(cb, 1)[i] * dbk[0]);
+        for (int k = 2; k <= nl; k++)
+            L(wb, k)[i] = L(m->rlnp, k)[i] * (L(db, k - 1)[i] + psg[i] * L(cb, k - 1)[i]) +
+                          L(m->alfa, k)[i] * (L(m->divg, nl + 1 - k)[i] * L(dpk, k)[i] +
+                                              psg[i] * L(cg, k)[i] * dbk[k - 1]);
+        L(wc, 1)[i] = psg[i] * L(cg, 1)[i] * dbk[0];
+        for (int k = 2; k <= nl; k++)
+            L(wc, k)[i] = psg[i] * L(cg, k)[i] * (dbk[k - 1] + ck[k - 1] * L(m->rlnp, k)[i] / L(dpk, k)[i]);
+        for (int k = 1; k <= nl; k++)
+            L(m->dlnpdtg, nl + 1 - k)[i] = (L(wc, k)[i] - L(wb, k)[i]) / L(dpk, k)[i];
+    }
+}
+
+/* ---- dyn_run.f90:375-451 getpresgrad ------------------------------- */
+static void getpresgrad(orc_model *m, const double *dtvdx, const double *dtvdy, double *prsgx,
+                        double *prsgy) {
+    int nl = m->nlev;
+    long ng = m->ng;
+    double rd = m->c.rd;
+    double *cofa = m->w[8], *cofb = m->w[9], *px3u = m->cb, *px3v = m->db;
+    double *dpk = m->dpk, *dbk = m->dbk, *bk = m->bk, *pk = m->pk, *psg = m->psg;
+    const double *tvg = m->tvg;
+#pragma omp parallel for
+    for (long i = 0; i < ng; i++) {
+        L(cofb, 1)[i] = -(1. / L(dpk, 1)[i]) * (L(m->alfa, 1)[i] * dbk[0]);
+        for (int k = 2; k <= nl; k++)
+            L(cofb, k)[i] = -(1. / L(dpk, k)[i]) * (bk[k - 1] * L(m->rlnp, k)[i] + L(m->alfa, k)[i] * dbk[k - 1]);
+        for (int k = 1; k <= nl; k++) {
+            L(prsgx, nl - k + 1)[i] = L(cofb, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdx[i];
+            L(prsgy, nl - k + 1)[i] = L(cofb, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdy[i];
+        }
+        for (int k = 1; k <= nl; k++)
+            L(cofa, k)[i] = -(1. / L(dpk, k)[i]) *
+                            (bk[k] * L(pk, k)[i] / L(pk, k + 1)[i] - bk[k - 1] +
+                             L(m->rlnp, k)[i] * (bk[k - 1] - L(pk, k)[i] * dbk[k - 1] / L(dpk, k)[i]));
+        L(cofb, nl)[i] = 0.0;
+        L(cofb, nl - 1)[i] = -rd * (bk[nl] / L(pk, nl + 1)[i] - bk[nl - 1] / L(pk, nl)[i]) * L(tvg, 1)[i];
+        for (int k = 2; k <= nl - 1; k++)
+            L(cofb, nl - k)[i] = L(cofb, nl + 1 - k)[i] -
+                                 rd * (bk[nl + 1 - k] / L(pk, nl + 2 - k)[i] - bk[nl - k] / L(pk, nl + 1 - k)[i]) *
+                                     L(tvg, k)[i];
+        L(px3u, nl)[i] = 0.0;
+        L(px3v, nl)[i] = 0.0;
+        L(px3u, nl - 1)[i] = -rd * L(m->rlnp, nl)[i] * L(dtvdx, 1)[i];
+        L(px3v, nl - 1)[i] = -rd * L(m->rlnp, nl)[i] * L(dtvdy, 1)[i];
+        for (int k = 2; k <= nl - 1; k++) {
+            L(px3u, nl - k)[i] = L(px3u, nl + 1 - k)[i] - rd * L(m->rlnp, nl + 1 - k)[i] * L(dtvdx, k)[i];
+            L(px3v, nl - k)[i] = L(px3v, nl + 1 - k)[i] - rd * L(m->rlnp, nl + 1 - k)[i] * L(dtvdy, k)[i];
+        }
+        for (int k = 1; k <= nl; k++) {
+            L(prsgx, nl - k + 1)[i] = L(prsgx, nl - k + 1)[i] - m->dphisdx[i] +
+                                      L(cofb, k)[i] * psg[i] * m->dlnpsdx[i] + L(px3u, k)[i] -
+                                      rd * L(m->alfa, k)[i] * L(dtvdx, nl + 1 - k)[i] -
+                                      L(cofa, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdx[i];
+            L(prsgy, nl - k + 1)[i] = L(prsgy, nl - k + 1)[i] - m->dphisdy[i] +
+                                      L(cofb, k)[i] * psg[i] * m->dlnpsdy[i] + L(px3v, k)[i] -
+                                      rd * L(m->alfa, k)[i] * L(dtvdy, nl + 1 - k)[i] -
+                                      L(cofa, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdy[i];
+        }
+    }
+}
+
+/* ---- dyn_run.f90:453-476 getvadv ----------------------------------- */
+static void getvadv(orc_model *m, const double *datag, double *vadv) {
+    int nl = m->nlev;
+    long ng = m->ng;
+    const double *et = m->etadot, *dpk = m->dpk;
+#pragma omp parallel for
+    for (long i = 0; i < ng; i++) {
+        L(vadv, nl)[i] = (0.5 / L(dpk, 1)[i]) * L(et, 2)[i] * (L(datag, nl - 1)[i] - L(datag, nl)[i]);
+        L(vadv, 1)[i] = (0.5 / L(dpk, nl)[i]) * L(et, nl)[i] * (L(datag, 1)[i] - L(datag, 2)[i]);
+        for (int k = 2; k <= nl - 1; k++)
+            L(vadv, nl + 1 - k)[i] =
+                (0.5 / L(dpk, k)[i]) * (L(et, k + 1)[i] * (L(datag, nl - k)[i] - L(datag, nl + 1 - k)[i]) +
+                                        L(et, k)[i] * (L(datag, nl + 1 - k)[i] - L(datag, nl + 2 - k)[i]));
+    }
+}
+
+/* ---- dyn_run.f90:478-603 getvadv_tracers (Thuburn 1993 limiter) ----- */
+static void getvadv_tracers(orc_model *m, const double *datag, double *vadv) {
+    int nl = m->nlev;
+    long ng = m->ng;
+    const double *et = m->etadot, *dpk = m->dpk;
+    const double epstiny = 2.2250738585072014e-308; /* tiny(real(8)) */
+#pragma omp parallel
+    {
+        double *half = malloc(sizeof(double) * (nl + 1)), *dd = malloc(sizeof(double) * (nl + 1));
+#pragma omp for
+        for (long i = 0; i < ng; i++) {
+#define D(k) L(datag, k)[i]
+            for (int k = 1; k <= nl - 1; k++) {
+                half[k] = 0.5 * (D(nl - k) + D(nl + 1 - k));
+                dd[k] = D(nl - k) - D(nl + 1 - k);
+            }
+            half[0] = D(nl);
+            half[nl] = D(1);
+            if (D(nl) >= 0) dd[0] = D(nl) - fmax(0., 2. * D(nl) - D(nl - 1));
+            else dd[0] = D(nl) - fmin(0., 2. * D(nl) - D(nl - 1));
+            if (D(1) >= 0) dd[nl] = fmax(0., 2. * D(1) - D(2)) - D(1);
+            else dd[nl] = fmin(0., 2. * D(1) - D(2)) - D(1);
+            for (int k = 1; k <= nl - 1; k++)
+                if (fabs(dd[k]) < epstiny) dd[k] = copysign(epstiny, dd[k]);
+            for (int k = 1; k <= nl - 1; k++) {
+                double phi;
+                if (L(et, k + 1)[i] > 0.) {
+                    phi = dd[k - 1] / dd[k];
+                    half[k] = D(nl + 1 - k) + (phi + fabs(phi)) / (1. + fabs(phi)) * (half[k] - D(nl + 1 - k));
+                } else {
+                    phi = dd[k + 1] / dd[k];
+                    half[k] = D(nl - k) + (phi + fabs(phi)) / (1. + fabs(phi)) * (half[k] - D(nl - k));
+                }
+            }
+            for (int k = 1; k <= nl; k++)
+                L(vadv, nl + 1 - k)[i] = (1. / L(dpk, k)[i]) *
+                                         ((half[k] * L(et, k + 1)[i] - half[k - 1] * L(et, k)[i]) +
+                                          (D(nl + 1 - k) * (L(et, k)[i] - L(et, k + 1)[i])));
+#undef D
+        }
+        free(half); free(dd);
+    }
+}
+
+/* ---- dyn_run.f90:33-305 getdyntend --------------------------------- */
+void orc_getdyntend(orc_model *m, int rkstage, int inverse_only, ocplx *dvrt, ocplx *ddiv,
+                    ocplx *dtv, ocplx *dtrac, ocplx *dlnps) {
+    const orc_sht *h = m->sht;
+    int nl = m->nlev, nlm = m->nlm, ntrac = m->ntrac;
+    long ng = m->ng;
+    double rerth = m->c.rerth;
+    double *prsgx = m->w[0], *prsgy = m->w[1], *vadvu = m->w[2], *vadvv = m->w[3];
+    double *vadvt = m->w[4], *vadvq = m->w[5], *dtvdx = m->w[6], *dtvdy = m->w[7];
+
+    /* dyn_run.f90:101-117 */
+    orc_getuv(h, m->vrt, m->div, m->ug, m->vg, nl, rerth);
+    orc_spectogrd(h, m->vrt, m->vrtg, nl);
+    orc_spectogrd(h, m->div, m->divg, nl);
+    orc_spectogrd(h, m->tv, m->tvg, nl);
+    orc_getgrad(h, m->tv, dtvdx, dtvdy, nl, rerth);
+    if (ntrac > 0) orc_spectogrd(h, m->trac, m->tracg, nl * ntrac);
+    /* 126-131 */
+    orc_spectogrd(h, m->lnps, m->lnpsg, 1);
+    calc_press_cols(nl, ng, m->ak, m->bk, m->kappa, m->lnpsg, m->psg, m->pk, m->dpk, m->alfa,
+                    m->rlnp, m->prs);
+    /* 135-153: dry-mass diagnostic, pdryini set on first use */
+    if (rkstage == 1) {
+        double pmoist = 0.0;
+        if (ntrac > 0)
+            for (int k = 1; k <= nl; k++)
+                for (int nt = 0; nt < ntrac; nt++) {
+                    const double *q = m->tracg + ((long)nt * nl + (k - 1)) * ng;
+                    const double *dp = L(m->dpk, nl - k + 1);
+                    for (long i = 0; i < ng; i++) pmoist += m->areawts[i] * dp[i] * q[i];
+                }
+        double pdry = area_sum(m, m->psg) - pmoist;
+        if (m->pdryini == 0.0) m->pdryini = pdry;
+    }
+    /* 158-166 */
+    orc_getgrad(h, m->lnps, m->dlnpsdx, m->dlnpsdy, 1, rerth);
+    getomega(m);
+    m->have_grid = 1;
+    if (inverse_only) return;
+    /* 179-188 */
+    orc_grdtospec(h, m->dlnpsdt, dlnps, 1);
+    getpresgrad(m, dtvdx, dtvdy, prsgx, prsgy);
+    getvadv(m, m->ug, vadvu);
+    getvadv(m, m->vg, vadvv);
+    getvadv(m, m->tvg, vadvt);
+    /* 191-214: planetary vorticity and energy conversion */
+    double kappa = m->kappa, cvcp = m->c.cvap / m->c.cp - 1.0;
+    double twoom = 2. * m->c.omega;
+#pragma omp parallel for
+    for (long i = 0; i < ng * nl; i++) {
+        double t = kappa * m->dlnpdtg[i] * m->tvg[i];
+        if (ntrac > 0) t = t / (1. + cvcp * m->tracg[i]);
+        vadvq[i] = t;
+    }
+    /* 220-272 */
+#pragma omp parallel for
+    for (long idx = 0; idx < ng * nl; idx++) {
+        long i = idx % ng;
+        double f = twoom * m->sinlat[i];
+        vadvv[idx] -= prsgy[idx];
+        vadvu[idx] -= prsgx[idx];
+        double u = m->ug[idx], v = m->vg[idx], z = m->vrtg[idx];
+        m->w[8][idx] = -u * dtvdx[idx] - v * dtvdy[idx] - vadvt[idx] + vadvq[idx];
+        prsgx[idx] = u * (z + f) + vadvv[idx];
+        prsgy[idx] = v * (z + f) - vadvu[idx];
+        m->w[9][idx] = 0.5 * (u * u + v * v);
+    }
+    orc_grdtospec(h, m->w[8], dtv, nl);
+    orc_getvrtdivspec(h, prsgx, prsgy, ddiv, dvrt, nl, rerth); /* swapped outputs */
+    ocplx *ke = malloc(sizeof(ocplx) * nlm * nl);
+    orc_grdtospec(h, m->w[9], ke, nl);
+    for (long idx = 0; idx < (long)nlm * nl; idx++) {
+        int k = idx % nlm;
+        dvrt[idx] = -dvrt[idx];
+        ddiv[idx] = ddiv[idx] - (m->lap[k] / (rerth * rerth)) * ke[idx];
+    }
+    free(ke);
+    /* 279-291: tracers */
+    for (int nt = 0; nt < ntrac; nt++) {
+        double *q = m->tracg + (long)nt * nl * ng;
+        getvadv_tracers(m, q, vadvq);
+        orc_getgrad(h, m->trac + (long)nt * nl * nlm, dtvdx, dtvdy, nl, rerth);
+#pragma omp parallel for
+        for (long idx = 0; idx < ng * nl; idx++)
+            prsgx[idx] = -m->ug[idx] * dtvdx[idx] - m->vg[idx] * dtvdy[idx] - vadvq[idx];
+        orc_grdtospec(h, prsgx, dtrac + (long)nt * nl * nlm, nl);
+    }
+}
+
+/* ---- phy_run_hs.f90:22-84 ------------------------------------------ */
+void orc_hs_phytend(orc_model *m, ocplx *dvrt, ocplx *ddiv, ocplx *dtv) {
+    int nl = m->nlev;
+    long ng = m->ng;
+    const double sigbot = SGL(0.7), delthz = 10., tempstrat = 200.;
+    const double kdrag = SGL(1.f / (1.f * 86400.f)), krada = SGL(1.f / (40.f * 86400.f));
+    const double kradb = SGL(1.f / (4.f * 86400.f)), p0 = 1.e5, deltmp = 60.;
+    const double expo = SGL(2.f / 7.f);
+    double *ft = m->w[0], *fv = m->w[1], *fd = m->w[2];
+#pragma omp parallel for
+    for (long idx = 0; idx < ng * nl; idx++) {
+        long i = idx % ng;
+        double prs = m->prs[idx], sl = m->sinlat[i], cl = m->coslat[i];
+        double blprof = prs / m->psg[i];
+        double teq = pow(prs / p0, expo) * (315. - deltmp * sl * sl - delthz * log(prs / p0) * cl * cl);
+        blprof = (blprof - sigbot) / (1. - sigbot);
+        if (blprof < 0) blprof = 0;
+        if (teq < tempstrat) teq = tempstrat;
+        ft[idx] = (krada + (kradb - krada) * blprof * cl * cl * cl * cl) * (teq - m->tvg[idx]);
+        fv[idx] = -(blprof * kdrag) * m->vrtg[idx];
+        fd[idx] = -(blprof * kdrag) * m->divg[idx];
+    }
+    orc_grdtospec(m->sht, ft, dtv, nl);
+    orc_grdtospec(m->sht, fv, dvrt, nl);
+    orc_grdtospec(m->sht, fd, ddiv, nl);
+}
+
+/* ---- run.f90:194-565 advance ---------------------------------------- */
+static void semi_implicit(orc_model *m, int stage, ocplx **dd, ocplx **dT, ocplx **dp,
+                          const ocplx *div0, const ocplx *tv0, const ocplx *lnps0,
+                          const double *ca, const double *caa, double cimp);
+
+void orc_advance(orc_model *m, int nsteps) {
+    int nl = m->nlev, nlm = m->nlm, ntrac = m->ntrac;
+    double dt = m->c.dt;
+    size_t s3 = sizeof(ocplx) * nlm * nl, st = s3 * (ntrac > 0 ? ntrac : 1), s1 = sizeof(ocplx) * nlm;
+    ocplx *vrt0 = malloc(s3), *div0 = malloc(s3), *tv0 = malloc(s3), *tr0 = malloc(st), *lp0 = malloc(s1);
+    ocplx *dv[3], *dd[3], *dT[3], *dq[3], *dp[3];
+    for (int s = 0; s < 3; s++) {
+        dv[s] = malloc(s3); dd[s] = malloc(s3); dT[s] = malloc(s3); dq[s] = malloc(st); dp[s] = malloc(s1);
+    }
+    /* explicit / implicit coefficient rows per stage (run.f90:313-489) */
+    const double ca[3][3] = {{A21, 0, 0}, {A31, A32, 0}, {B1, B2, B3}};
+    const double caa[3][3] = {{AA21, 0, 0}, {AA31, AA32, 0}, {BB1, BB2, BB3}};
+    const double cimp[3] = {AA22, AA33, BB4};
+    for (int step = 0; step < nsteps; step++) {
+        memcpy(vrt0, m->vrt, s3); memcpy(div0, m->div, s3); memcpy(tv0, m->tv, s3);
+        if (ntrac > 0) memcpy(tr0, m->trac, st);
+        memcpy(lp0, m->lnps, s1);
+        for (int s = 0; s < 3; s++) {
+            orc_getdyntend(m, s + 1, 0, dv[s], dd[s], dT[s], dq[s], dp[s]);
+            /* vorticity and tracers: always explicit */
+            for (long i = 0; i < (long)nlm * nl; i++) {
+                ocplx acc = 0.0;
+                for (int r = 0; r <= s; r++) acc += ca[s][r] * dv[r][i];
+                m->vrt[i] = vrt0[i] + dt * acc;
+            }
+            for (long i = 0; i < (long)nlm * nl * ntrac; i++) {
+                ocplx acc = 0.0;
+                for (int r = 0; r <= s; r++) acc += ca[s][r] * dq[r][i];
+                m->trac[i] = tr0[i] + dt * acc;
+            }
+            if (m->c.explicit_) {
+                for (long i = 0; i < (long)nlm * nl; i++) {
+                    ocplx a1 = 0.0, a2 = 0.0;
+                    for (int r = 0; r <= s; r++) { a1 += ca[s][r] * dd[r][i]; a2 += ca[s][r] * dT[r][i]; }
+                    m->div[i] = div0[i] + dt * a1;
+                    m->tv[i] = tv0[i] + dt * a2;
+                }
+                for (int k = 0; k < nlm; k++) {
+                    ocplx a = 0.0;
+                    for (int r = 0; r <= s; r++) a += ca[s][r] * dp[r][k];
+                    m->lnps[k] = lp0[k] + dt * a;
+                }
+            } else {
+                semi_implicit(m, s, dd, dT, dp, div0, tv0, lp0, ca[s], caa[s], cimp[s]);
+            }
+        }
+        /* run.f90:493-502 implicit damping */
+#pragma omp parallel for
+        for (int k = 0; k < nl; k++)
+            for (int n = 0; n < nlm; n++) {
+                long i = (long)k * nlm + n;
+                double fvd = 1. - (m->disspec[n] * m->diff_prof[k] - m->dmp_prof[k]) * dt;
+                double ft = 1. - m->disspec[n] * m->diff_prof[k] * dt;
+                m->vrt[i] /= fvd;
+                m->div[i] /= fvd;
+                m->tv[i] /= ft;
+                for (int nt = 0; nt < ntrac; nt++) m->trac[(long)nt * nl * nlm + i] /= ft;
+            }
+        /* run.f90:509-538 time-split physics */
+        if (m->c.physics != 0) {
+            orc_getdyntend(m, 3, 1, NULL, NULL, NULL, NULL, NULL);
+            if (m->c.physics == 1) {
+                orc_hs_phytend(m, dv[1], dd[1], dT[1]);
+                for (long i = 0; i < (long)nlm * nl; i++) {
+                    m->vrt[i] += dt * dv[1][i];
+                    m->div[i] += dt * dd[1][i];
+                    m->tv[i] += dt * dT[1][i];
+                }
+                for (int k = 0; k < nlm; k++) dp[1][k] = 0.0;
+            }
+            if (m->c.massfix) {
+                /* dyn_run.f90:605-634 with pwat = 0 (stub/HS physics data) */
+                double pmean = area_sum(m, m->psg);
+                double pcorr = (m->pdryini + m->c.grav * 0.0) / pmean;
+                double *g = m->w[3];
+                for (long i = 0; i < m->ng; i++) g[i] = log(m->psg[i] * pcorr);
+                ocplx *ws = malloc(s1);
+                orc_grdtospec(m->sht, g, ws, 1);
+                for (int k = 0; k < nlm; k++) dp[1][k] = (ws[k] - m->lnps[k]) / dt;
+                free(ws);
+            }
+            for (int k = 0; k < nlm; k++) m->lnps[k] += dt * dp[1][k];
+        }
+    }
+    free(vrt0); free(div0); free(tv0); free(tr0); free(lp0);
+    for (int s = 0; s < 3; s++) {
+        free(dv[s]); free(dd[s]); free(dT[s]); free(dq[s]); free(dp[s]);
+    }
+}
+
+/* One semi-implicit stage of run.f90:326-350 / 386-416 / 450-488 for every
+ * spectral coefficient: remove the linear terms from the stage tendency,
+ * form the explicit update with the (a, aa) row of the tableau, solve for
+ * the new divergence with d_hyb_m(:,:,n+1,stage) and back-substitute. The
+ * linear terms of each stage are kept in the model for the later stages. */
+static void semi_implicit(orc_model *m, int stage, ocplx **dd, ocplx **dT, ocplx **dp,
+                          const ocplx *div0, const ocplx *tv0, const ocplx *lnps0,
+                          const double *ca, const double *caa, double cimp) {
+    int nl = m->nlev, nlm = m->nlm;
+    double dt = m->c.dt;
+    ocplx **g_lin_d = m->lin_d, **g_lin_t = m->lin_t, **g_lin_p = m->lin_p;
+    const double *A = m->amhyb, *Bm = m->bmhyb, *sv = m->svhyb, *tor = m->tor_hyb;
+#pragma omp parallel
+    {
+        ocplx *tmp = malloc(sizeof(ocplx) * nl), *rhs = malloc(sizeof(ocplx) * nl);
+        ocplx *tvc = malloc(sizeof(ocplx) * nl), *dvc = malloc(sizeof(ocplx) * nl);
+#pragma omp for schedule(static)
+        for (int n = 0; n < nlm; n++) {
+            double lapn = m->lap[n];
+            /* linear terms of the stage's input state */
+            const ocplx *tvin = (stage == 0) ? tv0 : m->tv;
+            const ocplx *dvin = (stage == 0) ? div0 : m->div;
+            ocplx lpin = (stage == 0) ? lnps0[n] : m->lnps[n];
+            for (int k = 0; k < nl; k++) { tvc[k] = tvin[(long)k * nlm + n]; dvc[k] = dvin[(long)k * nlm + n]; }
+            ocplx dpl = 0.0;
+            for (int i = 0; i < nl; i++) {
+                ocplx a = 0.0, b = 0.0;
+                for (int j = 0; j < nl; j++) { a += A[i + j * nl] * tvc[j]; b += Bm[i + j * nl] * dvc[j]; }
+                ocplx ddl = -lapn * (a + tor[i] * lpin);
+                ocplx dtl = -b;
+                g_lin_d[stage][(long)i * nlm + n] = ddl;
+                g_lin_t[stage][(long)i * nlm + n] = dtl;
+                dd[stage][(long)i * nlm + n] -= ddl;
+                dT[stage][(long)i * nlm + n] -= dtl;
+                dpl -= sv[i] * dvc[i];
+            }
+            g_lin_p[stage][n] = dpl;
+            dp[stage][n] -= dpl;
+            /* explicit part */
+            for (int i = 0; i < nl; i++) {
+                long id = (long)i * nlm + n;
+                ocplx a1 = 0.0, a2 = 0.0;
+                for (int r = 0; r <= stage; r++) {
+                    a1 += ca[r] * dd[r][id] + caa[r] * g_lin_d[r][id];
+                    a2 += ca[r] * dT[r][id] + caa[r] * g_lin_t[r][id];
+                }
+                dvc[i] = div0[id] + dt * a1;
+                tvc[i] = tv0[id] + dt * a2;
+            }
+            ocplx a3 = 0.0;
+            for (int r = 0; r <= stage; r++) a3 += ca[r] * dp[r][n] + caa[r] * g_lin_p[r][n];
+            ocplx lpn = lnps0[n] + dt * a3;
+            if (stage < 2 || fabs(cimp) > 1.e-5) {
+                /* implicit solve */
+                for (int i = 0; i < nl; i++) {
+                    ocplx a = 0.0;
+                    for (int j = 0; j < nl; j++) a += A[i + j * nl] * tvc[j];
+                    rhs[i] = dvc[i] - cimp * lapn * dt * (a + tor[i] * lpn);
+                }
+                const double *D = m->d_hyb_m + ((long)stage * (m->T + 1) + m->degree[n]) * nl * nl;
+                for (int i = 0; i < nl; i++) {
+                    ocplx a = 0.0;
+                    for (int j = 0; j < nl; j++) a += D[i + j * nl] * rhs[j];
+                    tmp[i] = a;
+                }
+                for (int i = 0; i < nl; i++) {
+                    ocplx b = 0.0;
+                    for (int j = 0; j < nl; j++) b += Bm[i + j * nl] * tmp[j];
+                    tvc[i] -= cimp * dt * b;
+                    lpn -= cimp * dt * sv[i] * tmp[i];
+                }
+                for (int i = 0; i < nl; i++) dvc[i] = tmp[i];
+            }
+            for (int i = 0; i < nl; i++) {
+                m->div[(long)i * nlm + n] = dvc[i];
+                m->tv[(long)i * nlm + n] = tvc[i];
+            }
+            m->lnps[n] = lpn;
+        }
+        free(tmp); free(rhs); free(tvc); free(dvc);
+    }
+}
+
+/* ---- accessors ------------------------------------------------------ */
+void orc_get_grid(const orc_model *m, int which, double *out) {
+    long ng = m->ng, n3 = ng * m->nlev;
+    const double *src = NULL;
+    long n = n3;
+    switch (which) {
+    case 0: src = m->ug; break;
+    case 1: src = m->vg; break;
+    case 2: src = m->vrtg; break;
+    case 3: src = m->divg; break;
+    case 4: src = m->tvg; break;
+    case 5: src = m->dlnpdtg; break;
+    case 6: src = m->etadot; n = n3 + ng; break;
+    case 7: src = m->psg; n = ng; break;
+    case 8: src = m->dlnpsdt; n = ng; break;
+    case 9: src = m->prs; break;
+    case 10: src = m->pk; n = n3 + ng; break;
+    case 11: src = m->dpk; break;
+    case 12: src = m->alfa; break;
+    case 13: src = m->rlnp; break;
+    case 14: src = m->dlnpsdx; n = ng; break;
+    case 15: src = m->dlnpsdy; n = ng; break;
+    case 16: src = m->tracg; n = n3 * (m->ntrac > 0 ? m->ntrac : 0); break;
+    default: return;
+    }
+    if (n > 0) memcpy(out, src, sizeof(double) * n);
+}
+
+void orc_get_semimp(const orc_model *m, double *amhyb, double *bmhyb, double *svhyb,
+                    double *tor_hyb, double *d_hyb_m) {
+    int nl = m->nlev;
+    if (amhyb) memcpy(amhyb, m->amhyb, sizeof(double) * nl * nl);
+    if (bmhyb) memcpy(bmhyb, m->bmhyb, sizeof(double) * nl * nl);
+    if (svhyb) memcpy(svhyb, m->svhyb, sizeof(double) * nl);
+    if (tor_hyb) memcpy(tor_hyb, m->tor_hyb, sizeof(double) * nl);
+    if (d_hyb_m) memcpy(d_hyb_m, m->d_hyb_m, sizeof(double) * nl * nl * (m->T + 1) * 3);
+}
+
+void orc_get_damp(const orc_model *m, double *disspec, double *diff_prof, double *dmp_prof) {
+    if (disspec) memcpy(disspec, m->disspec, sizeof(double) * m->nlm);
+    if (diff_prof) memcpy(diff_prof, m->diff_prof, sizeof(double) * m->nlev);
+    if (dmp_prof) memcpy(dmp_prof, m->dmp_prof, sizeof(double) * m->nlev);
+}
+
+void orc_step_diag(const orc_model *m, double *out4) {
+    long ng = m->ng, n3 = ng * m->nlev;
+    double spdmax = 0.0, pmin = 1e300, pmax = -1e300, pst = 0.0;
+    for (long i = 0; i < n3; i++) {
+        double s = sqrt(m->ug[i] * m->ug[i] + m->vg[i] * m->vg[i]);
+        if (s > spdmax) spdmax = s;
+    }
+    for (long i = 0; i < ng; i++) {
+        double p = m->psg[i];
+        if (p < pmin) pmin = p;
+        if (p > pmax) pmax = p;
+        double t = 36. * p * m->dlnpsdt[i];
+        pst += t * t * m->areawts[i];
+    }
+    out4[0] = spdmax; out4[1] = pmin; out4[2] = pmax; out4[3] = sqrt(pst);
+}

@@ -1,0 +1,136 @@
+/*
+ * FP64 CPU oracle for the GFS spectral transform-and-tendency step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP product
+ * path is compared against; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it. Nothing in gfs-dycore_amd/ links
+ * or calls it, and the product fails loudly when its HIP library is missing.
+ *
+ * It restates, in plain C with OpenMP, the arithmetic of:
+ *   src/shtns.f90            (transform wrapper over SHTns: scalings, role swaps)
+ *   SHTns itself             (external C library, version unpinned, absent from
+ *                             the image; restated from its published definition:
+ *                             orthonormal harmonics, no Condon-Shortley phase,
+ *                             Gauss grid, spheroidal/toroidal vector transforms)
+ *   src/pressure_data.f90    (calc_pressdata)
+ *   src/dyn_run.f90          (getdyntend, getomega, getpresgrad, getvadv,
+ *                             getvadv_tracers, dry_mass_fixer)
+ *   src/semimp_data.f90      (IMEX tableau, init_semimpdata; LAPACK dgetrf/dgetri
+ *                             replaced by Gauss-Jordan inversion with pivoting)
+ *   src/run.f90              (advance)
+ *   src/dyn_init.f90         (setdampspec)
+ *   src/phy_run_hs.f90       (Held-Suarez getphytend)
+ *
+ * Pinning status of each function is listed in DESIGN.md and in the header
+ * comment of each source file.
+ *
+ * Layouts follow the reference (Fortran column-major):
+ *   spectral  (ndimspec, nlev)  -> spec[lev*ndimspec + k], complex double,
+ *             k m-major: k(m,n) = m*(2T+3-m)/2 + (n-m)
+ *   grid      (nlon, nlat, nlev) -> grid[(lev*nlat + j)*nlon + i], lat north->south
+ *   state/grid levels bottom->top; pk/dpk/alfa/rlnp/etadot top->bottom.
+ */
+#ifndef GFS_ORACLE_H
+#define GFS_ORACLE_H
+#include <complex.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double _Complex ocplx;
+
+/* ---- spherical harmonic transforms (shtns.f90 + SHTns) ---- */
+typedef struct orc_sht orc_sht;
+
+orc_sht *orc_sht_create(int nlon, int nlat, int ntrunc);
+void orc_sht_destroy(orc_sht *h);
+int orc_sht_nlm(const orc_sht *h);
+/* module data of shtns.f90: lats (nlat, radians, N->S), gauwts (nlat, sum 2),
+ * areawts (nlon*nlat, sum 1), degree/order (nlm), lap/invlap (nlm). */
+void orc_sht_griddata(const orc_sht *h, double *lats, double *gauwts, double *areawts,
+                      int *degree, int *order, double *lap, double *invlap);
+/* associated Legendre table for order m at latitude j (0..nlat-1), degrees m..T+1 */
+void orc_sht_legendre(const orc_sht *h, int m, int j, double *out);
+
+void orc_spectogrd(const orc_sht *h, const ocplx *spec, double *grid, int nfield);
+void orc_grdtospec(const orc_sht *h, const double *grid, ocplx *spec, int nfield);
+void orc_getuv(const orc_sht *h, const ocplx *vrt, const ocplx *div, double *u, double *v,
+               int nfield, double rsphere);
+void orc_getvrtdivspec(const orc_sht *h, const double *u, const double *v, ocplx *vrt,
+                       ocplx *div, int nfield, double rsphere);
+void orc_getgrad(const orc_sht *h, const ocplx *spec, double *gx, double *gy, int nfield,
+                 double rsphere);
+/* raw real<->Fourier transforms used by the transforms (exposed for tests) */
+void orc_fft_analysis(int nlon, int mmax, const double *row, ocplx *coef);
+void orc_fft_synthesis(int nlon, int mmax, const ocplx *coef, double *row);
+/* Gauss-Legendre nodes (cos colatitude, N->S) and weights (sum 2) */
+void orc_gauss(int nlat, double *mu, double *w);
+
+/* ---- dynamical core ---- */
+typedef struct {
+    int nlon, nlat, ntrunc, nlev, ntrac;
+    double dt;
+    const double *ak, *bk; /* nlev+1, top -> bottom (as pressure_data.f90) */
+    int explicit_;         /* params.f90 explicit */
+    int physics;           /* 0 adiabatic (stub), 1 Held-Suarez */
+    int massfix;
+    int ndiss;             /* 0 -> 8 */
+    double efold;          /* <=0 -> GFS default by truncation */
+    double hdif_fac, hdif_fac2, fshk, taustratdamp;
+    /* physical constants (physcons.f90); orc_cfg_defaults fills them */
+    double rerth, grav, omega, rd, rv, cp, cvap;
+} orc_cfg;
+
+typedef struct orc_model orc_model;
+
+void orc_cfg_defaults(orc_cfg *c);
+orc_model *orc_model_create(const orc_cfg *c);
+void orc_model_destroy(orc_model *m);
+const orc_sht *orc_model_sht(const orc_model *m);
+
+/* state: vrt,div,tv (ndimspec,nlev), trac (ndimspec,nlev,ntrac), lnps (ndimspec) */
+void orc_set_state(orc_model *m, const ocplx *vrt, const ocplx *div, const ocplx *tv,
+                   const ocplx *trac, const ocplx *lnps);
+void orc_get_state(const orc_model *m, ocplx *vrt, ocplx *div, ocplx *tv, ocplx *trac,
+                   ocplx *lnps);
+void orc_set_pdryini(orc_model *m, double pdryini);
+double orc_get_pdryini(const orc_model *m);
+
+/* dyn_run.f90:33 getdyntend; outputs may be NULL when inverse_only */
+void orc_getdyntend(orc_model *m, int rkstage, int inverse_only, ocplx *dvrt, ocplx *ddiv,
+                    ocplx *dtv, ocplx *dtrac, ocplx *dlnps);
+/* run.f90:194 advance, nsteps times */
+void orc_advance(orc_model *m, int nsteps);
+/* phy_run_hs.f90:22 getphytend on the model's current grid fields */
+void orc_hs_phytend(orc_model *m, ocplx *dvrt, ocplx *ddiv, ocplx *dtv);
+
+/* grid diagnostics held by the model after getdyntend (grid_data.f90 names):
+ * which: 0 ug,1 vg,2 vrtg,3 divg,4 virtempg,5 dlnpdtg,6 etadot(nlev+1),
+ * 7 psg(2d),8 dlnpsdt(2d),9 prs,10 pk(nlev+1),11 dpk,12 alfa,13 rlnp,
+ * 14 dlnpsdx(2d), 15 dlnpsdy(2d), 16 tracerg (nlev*ntrac) */
+void orc_get_grid(const orc_model *m, int which, double *out);
+/* one-off operators: amhyb,bmhyb (nlev*nlev col-major), svhyb,tor_hyb (nlev),
+ * d_hyb_m (nlev,nlev,ntrunc+1,3) col-major */
+void orc_get_semimp(const orc_model *m, double *amhyb, double *bmhyb, double *svhyb,
+                    double *tor_hyb, double *d_hyb_m);
+void orc_get_damp(const orc_model *m, double *disspec, double *diff_prof, double *dmp_prof);
+/* per-step diagnostics of run.f90:159-162: spdmax, min ps, max ps (Pa), pstend */
+void orc_step_diag(const orc_model *m, double *out4);
+
+/* pressure_data.f90:58 calc_pressdata on a standalone set of columns
+ * lnpsg (ncol) -> pk (nlev+1,ncol), dpk, alfa, rlnp (nlev,ncol) top->bottom,
+ * prs (nlev,ncol) bottom->top; arrays column-fastest: x[k*ncol+i] */
+void orc_calc_pressdata(int nlev, int ncol, const double *ak, const double *bk, double rk,
+                        const double *lnpsg, double *pk, double *dpk, double *alfa,
+                        double *rlnp, double *prs);
+
+/* counter-based RNG shared with the product: splitmix64 of (seed, stream, index) */
+uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index);
+double orc_rng_uniform(uint64_t seed, uint64_t stream, uint64_t index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
