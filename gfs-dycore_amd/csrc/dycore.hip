@@ -1,0 +1,1370 @@
+// Device-resident GFS hybrid sigma-pressure dynamical core step.
+//
+// State lives in HBM in a coefficient-major layout spec[k][field] (complex),
+// field = member * nfs + {vrt[nlev], div[nlev], tv[nlev], trac[ntrac*nlev], lnps},
+// so one launch covers every level, variable and ensemble member:
+//   getdyntend (dyn_run.f90:33-305): one batched synthesis of all grid fields,
+//     one column kernel (pressure_data.f90:58 calc_pressdata, dyn_run.f90:307
+//     getomega, :375 getpresgrad, :453 getvadv, :190-272 tendency assembly), one
+//     batched analysis scattered straight into tendency fields.
+//   advance (run.f90:194-565): IMEX RK3 stages with the semi-implicit solve per
+//     spectral coefficient (one wavefront per coefficient, lane = level), implicit
+//     damping, then the time-split Held-Suarez physics (phy_run_hs.f90:22) and the
+//     dry-mass fixer (dyn_run.f90:605) with an FP64 global sum.
+#include <cmath>
+#include <cstring>
+
+#include "engine.h"
+
+namespace gfsdc {
+
+#define SGLF(x) ((double)(float)(x))
+
+// IMEX tableau, semimp_data.f90:74-88 (default-real literals)
+static const double TAB_A[3][3] = {{SGLF(1. / 3.), 0, 0}, {SGLF(1. / 6.), SGLF(1. / 2.), 0},
+                                   {SGLF(1. / 2.), SGLF(-1. / 2.), 1.0}};
+static const double TAB_AA[3][3] = {{SGLF(1. / 6.), 0, 0}, {SGLF(1. / 3.), 0.0, 0},
+                                    {SGLF(3. / 8.), 0.0, SGLF(3. / 8.)}};
+static const double TAB_IMP[3] = {SGLF(1. / 6.), SGLF(1. / 3.), SGLF(1. / 4.)};
+
+constexpr int MAXLEV = 64;  // levels map onto the 64 lanes of a wavefront in the solve
+
+struct Coeffs3 {
+    double a[3], aa[3];
+};
+
+struct DycoreBuffers {
+    int nl = 0, ntr = 0, nm = 0;
+    int nfs = 0;    // state fields per member
+    int nsyn = 0;   // synthesised grid fields per member
+    int nan = 0;    // analysed grid fields per member
+    int nhs = 0;    // Held-Suarez forcing fields per member (3 nlev + 1)
+    long ng = 0;
+    void *spec = nullptr, *spec0 = nullptr, *ten[3] = {nullptr, nullptr, nullptr};
+    void *lin[3] = {nullptr, nullptr, nullptr};  // linear terms: div[nl], tv[nl], lnps per member
+    void *tphys = nullptr;                       // physics tendencies
+    void *gsyn = nullptr, *gan = nullptr;        // grid fields (T)
+    void *etadot = nullptr, *dlnpdt = nullptr;   // per member (nl+1)*ng, nl*ng
+    void *psg = nullptr, *dlnpsdt = nullptr;     // per member ng
+    void *dphisdx = nullptr, *dphisdy = nullptr; // per member ng
+    void *topo = nullptr;                        // spectral orography per member (complex)
+    void *Work = nullptr;                        // Work<T>*
+    // operators (T)
+    void *at = nullptr, *bt = nullptr, *dt3 = nullptr;  // transposed amhyb, bmhyb, d_hyb_m
+    void *sv = nullptr, *tor = nullptr;
+    void *disspec = nullptr, *diffp = nullptr, *dmpp = nullptr;
+    void *ak = nullptr, *bk = nullptr, *ck = nullptr, *dbk = nullptr;  // T
+    void *sinlat = nullptr, *coslat = nullptr, *areaw = nullptr;      // T, nlat; areaw FP64 nlat
+    int *kbydeg = nullptr;                       // coefficients sorted by degree
+    int *degree = nullptr;                       // degree of coefficient k
+    double *pdryini = nullptr, *pmean = nullptr; // FP64 per member
+    double *diag = nullptr;                      // FP64 4 per member
+    SynthDesc *sdesc = nullptr;                  // nm*nsyn
+    AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr;
+    unsigned char *rflag_syn = nullptr, *rflag_an = nullptr, *rflag_hs = nullptr;
+    // host copies of the FP64 operators
+    std::vector<double> amhyb, bmhyb, svhyb, tor_hyb, d_hyb_m, disspec_h, diff_h, dmp_h, sl;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<std::string> evname;
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+
+// Column kernel: calc_pressdata + getomega (+ getpresgrad, getvadv and the
+// tendency assembly unless inverse_only). One thread per grid column and member.
+// Level indices: K counts interfaces/layers top -> bottom (1-based, as pk, dpk,
+// alfa, rlnp, etadot), L counts model levels bottom -> top (as u, v, tv).
+template <typename T>
+__global__ __launch_bounds__(256) void k_columns(
+    const T *__restrict__ gsyn, T *__restrict__ gan, T *__restrict__ etadot, T *__restrict__ dlnpdt,
+    T *__restrict__ psg_out, T *__restrict__ dlnpsdt_out, const T *__restrict__ dphisdx,
+    const T *__restrict__ dphisdy, const T *__restrict__ ak_, const T *__restrict__ bk_,
+    const T *__restrict__ ck_, const T *__restrict__ dbk_, const T *__restrict__ sinlat, int nl, int ntr,
+    int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only) {
+    __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1], ck[MAXLEV], dbk[MAXLEV];
+    for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
+        ak[i] = ak_[i];
+        bk[i] = bk_[i];
+        if (i < nl) {
+            ck[i] = ck_[i];
+            dbk[i] = dbk_[i];
+        }
+    }
+    __syncthreads();
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = blockIdx.y;
+    if (col >= ng) return;
+    const T *G = gsyn + (long)mem * nsyn * ng + col;
+    const T *U = G, *V = G + (long)nl * ng, *VRT = G + 2L * nl * ng, *DIV = G + 3L * nl * ng;
+    const T *TV = G + 4L * nl * ng, *TX = G + 5L * nl * ng, *TY = G + 6L * nl * ng, *Q = G + 7L * nl * ng;
+    const T *LNPS = G + (7L + ntr) * nl * ng;
+    const T lnps = LNPS[0], dpx = LNPS[ng], dpy = LNPS[2 * ng];
+    const T ps = exp(lnps);
+    T *ET = etadot + (long)mem * (nl + 1) * ng + col;
+    T *WD = dlnpdt + (long)mem * nl * ng + col;
+    const T alfa1 = T(SGLF(0.69314718055994530942)), rlnp1 = T(SGLF(99999.99));
+#define LV(P, L) (P)[(long)((L) - 1) * ng]
+    auto pk = [&](int K) { return ak[K - 1] + bk[K - 1] * ps; };
+    // pass 1: column totals of the continuity integrals (getomega db, cb)
+    T db = 0, cb = 0;
+    for (int K = 1; K <= nl; K++) {
+        int L = nl + 1 - K;
+        T cg = LV(U, L) * dpx + LV(V, L) * dpy;
+        db += LV(DIV, L) * (pk(K + 1) - pk(K));
+        cb += cg * dbk[K - 1];
+    }
+    const T dlnpsdt = -db / ps - cb;
+    // pass 2: etadot at interfaces and omega/p (dlnpdtg) on layers
+    db = 0;
+    cb = 0;
+    ET[0] = T(0);
+    ET[(long)nl * ng] = T(0);
+    for (int K = 1; K <= nl; K++) {
+        int L = nl + 1 - K;
+        T p1 = pk(K), p2 = pk(K + 1), dpk = p2 - p1;
+        T rlnp = (K == 1) ? rlnp1 : log(p2 / p1);
+        T alfa = (K == 1) ? alfa1 : T(1) - (p1 / dpk) * rlnp;
+        T cg = LV(U, L) * dpx + LV(V, L) * dpy;
+        T dv = LV(DIV, L);
+        T workb, workc;
+        if (K == 1) {
+            workb = alfa * (dv * dpk + ps * (cg * dbk[0]) * dbk[0]);
+            workc = ps * cg * dbk[0];
+        } else {
+            workb = rlnp * (db + ps * cb) + alfa * (dv * dpk + ps * cg * dbk[K - 1]);
+            workc = ps * cg * (dbk[K - 1] + ck[K - 1] * rlnp / dpk);
+        }
+        LV(WD, L) = (workc - workb) / dpk;
+        db += dv * dpk;
+        cb += cg * dbk[K - 1];
+        if (K <= nl - 1) ET[(long)K * ng] = -ps * (bk[K] * dlnpsdt + cb) - db;
+    }
+    psg_out[(long)mem * ng + col] = ps;
+    dlnpsdt_out[(long)mem * ng + col] = dlnpsdt;
+    if (inverse_only) return;
+    T *A = gan + (long)mem * nan * ng + col;
+    T *TT = A, *FX = A + (long)nl * ng, *FY = A + 2L * nl * ng, *KE = A + 3L * nl * ng;
+    A[4L * nl * ng] = dlnpsdt;
+    const T f = twoom * sinlat[col / nlon];
+    const T dphx = dphisdx[(long)mem * ng + col], dphy = dphisdy[(long)mem * ng + col];
+    // pass 3: bottom -> top (K = nl..1): getpresgrad accumulations, getvadv, assembly
+    T cofb_acc = 0, px3u = 0, px3v = 0;
+    for (int K = nl; K >= 1; K--) {
+        int L = nl + 1 - K;
+        T p1 = pk(K), p2 = pk(K + 1), dpk = p2 - p1;
+        T rlnp = (K == 1) ? rlnp1 : log(p2 / p1);
+        T alfa = (K == 1) ? alfa1 : T(1) - (p1 / dpk) * rlnp;
+        T cofb0 = (K == 1) ? -(T(1) / dpk) * (alfa * dbk[0]) : -(T(1) / dpk) * (bk[K - 1] * rlnp + alfa * dbk[K - 1]);
+        T cofa = -(T(1) / dpk) * (bk[K] * p1 / p2 - bk[K - 1] + rlnp * (bk[K - 1] - p1 * dbk[K - 1] / dpk));
+        T tv = LV(TV, L), tx = LV(TX, L), ty = LV(TY, L);
+        T prsgx = cofb0 * rd * tv * ps * dpx - dphx + cofb_acc * ps * dpx + px3u - rd * alfa * tx -
+                  cofa * rd * tv * ps * dpx;
+        T prsgy = cofb0 * rd * tv * ps * dpy - dphy + cofb_acc * ps * dpy + px3v - rd * alfa * ty -
+                  cofa * rd * tv * ps * dpy;
+        // getvadv for u, v, tv at level L
+        T u = LV(U, L), v = LV(V, L);
+        T vu, vv, vt;
+        T h = T(0.5) / dpk;
+        if (L == nl) {
+            T e2 = ET[ng];
+            vu = h * e2 * (LV(U, nl - 1) - u);
+            vv = h * e2 * (LV(V, nl - 1) - v);
+            vt = h * e2 * (LV(TV, nl - 1) - tv);
+        } else if (L == 1) {
+            T en = ET[(long)(nl - 1) * ng];
+            vu = h * en * (u - LV(U, 2));
+            vv = h * en * (v - LV(V, 2));
+            vt = h * en * (tv - LV(TV, 2));
+        } else {
+            T eb = ET[(long)K * ng], ea = ET[(long)(K - 1) * ng];
+            vu = h * (eb * (LV(U, L - 1) - u) + ea * (u - LV(U, L + 1)));
+            vv = h * (eb * (LV(V, L - 1) - v) + ea * (v - LV(V, L + 1)));
+            vt = h * (eb * (LV(TV, L - 1) - tv) + ea * (tv - LV(TV, L + 1)));
+        }
+        T vq = kappa * LV(WD, L) * tv;
+        if (ntr > 0) vq = vq / (T(1) + cvcp * LV(Q, L));
+        vv -= prsgy;
+        vu -= prsgx;
+        T z = LV(VRT, L) + f;
+        LV(TT, L) = -u * tx - v * ty - vt + vq;
+        LV(FX, L) = u * z + vv;
+        LV(FY, L) = v * z - vu;
+        LV(KE, L) = T(0.5) * (u * u + v * v);
+        // accumulations for the layer above (getpresgrad cofb and px3 recursions)
+        cofb_acc -= rd * (bk[K] / p2 - bk[K - 1] / p1) * tv;
+        px3u -= rd * rlnp * tx;
+        px3v -= rd * rlnp * ty;
+    }
+#undef LV
+}
+
+// Held-Suarez forcing on the post-dynamics grid fields (phy_run_hs.f90:42-77)
+// and the dry-mass-fixer field log(ps * pcorr) (dyn_run.f90:626-631).
+template <typename T>
+__global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, const T *__restrict__ psg,
+                                                    T *__restrict__ gan, const T *__restrict__ ak_,
+                                                    const T *__restrict__ bk_, const T *__restrict__ sinlat,
+                                                    const double *__restrict__ pdryini,
+                                                    const double *__restrict__ pmean, int nl, int ntr, int nsyn,
+                                                    int nhs, long ng, int nlon, T kappa, int massfix) {
+    __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1];
+    for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
+        ak[i] = ak_[i];
+        bk[i] = bk_[i];
+    }
+    __syncthreads();
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = blockIdx.y;
+    if (col >= ng) return;
+    const T *G = gsyn + (long)mem * nsyn * ng + col;
+    const T *VRT = G + 2L * nl * ng, *DIV = G + 3L * nl * ng, *TV = G + 4L * nl * ng;
+    T *A = gan + (long)mem * nhs * ng + col;
+    const T ps = psg[(long)mem * ng + col];
+    const T sl = sinlat[col / nlon];
+    const T cl2 = T(1) - sl * sl;
+    const T sigbot = T(SGLF(0.7)), delthz = T(10), tempstrat = T(200);
+    const T kdrag = T(SGLF(1.f / 86400.f)), krada = T(SGLF(1.f / (40.f * 86400.f)));
+    const T kradb = T(SGLF(1.f / (4.f * 86400.f))), p0 = T(1.e5), deltmp = T(60);
+    const T expo = T(SGLF(2.f / 7.f));
+    for (int L = 1; L <= nl; L++) {
+        int K = nl + 1 - L;
+        T p1 = ak[K - 1] + bk[K - 1] * ps, p2 = ak[K] + bk[K] * ps, dpk = p2 - p1;
+        // Phillips layer pressure, pressure_data.f90:87-88
+        T prs = pow((pow(p2, kappa) * p2 - pow(p1, kappa) * p1) / ((kappa + T(1)) * dpk), T(1) / kappa);
+        T blprof = prs / ps;
+        T teq = pow(prs / p0, expo) * (T(315) - deltmp * sl * sl - delthz * log(prs / p0) * cl2);
+        blprof = (blprof - sigbot) / (T(1) - sigbot);
+        if (blprof < T(0)) blprof = T(0);
+        if (teq < tempstrat) teq = tempstrat;
+        long o = (long)(L - 1) * ng;
+        A[o] = (krada + (kradb - krada) * blprof * cl2 * cl2) * (teq - TV[o]);
+        A[(long)nl * ng + o] = -(blprof * kdrag) * VRT[o];
+        A[2L * nl * ng + o] = -(blprof * kdrag) * DIV[o];
+    }
+    if (massfix) {
+        T pcorr = T(pdryini[mem] / pmean[mem]);
+        A[3L * nl * ng] = log(ps * pcorr);
+    }
+}
+
+// area-weighted global mean of ps per member (FP64); optionally sets pdryini
+// where it is still 0 (dyn_run.f90:146-149).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_ps_mean(const T *__restrict__ psg, const double *__restrict__ areaw,
+                                                  double *__restrict__ out, double *__restrict__ pdryini,
+                                                  long ng, int nlon) {
+    const int mem = blockIdx.x;
+    const T *p = psg + (long)mem * ng;
+    double s = 0.0;
+    for (long i = threadIdx.x; i < ng; i += blockDim.x) s += areaw[i / nlon] * (double)p[i];
+    __shared__ double red[1024];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (out) out[mem] = red[0];
+        if (pdryini && pdryini[mem] == 0.0) pdryini[mem] = red[0];
+    }
+}
+
+// Per-step diagnostics of run.f90:159-162 (spdmax, min/max ps, pstend) per member.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_diag(const T *__restrict__ gsyn, const T *__restrict__ psg,
+                                               const T *__restrict__ dlnpsdt, const double *__restrict__ areaw,
+                                               double *__restrict__ out, int nl, int nsyn, long ng, int nlon) {
+    const int mem = blockIdx.x;
+    const T *U = gsyn + (long)mem * nsyn * ng, *V = U + (long)nl * ng;
+    const T *P = psg + (long)mem * ng, *D = dlnpsdt + (long)mem * ng;
+    double smax = 0, pmin = 1e300, pmax = -1e300, pst = 0;
+    for (long i = threadIdx.x; i < (long)nl * ng; i += blockDim.x) {
+        double s = sqrt((double)U[i] * U[i] + (double)V[i] * V[i]);
+        smax = s > smax ? s : smax;
+    }
+    for (long i = threadIdx.x; i < ng; i += blockDim.x) {
+        double p = P[i];
+        pmin = p < pmin ? p : pmin;
+        pmax = p > pmax ? p : pmax;
+        double t = 36.0 * p * (double)D[i];
+        pst += t * t * areaw[i / nlon];
+    }
+    __shared__ double r[4][1024];
+    r[0][threadIdx.x] = smax;
+    r[1][threadIdx.x] = pmin;
+    r[2][threadIdx.x] = pmax;
+    r[3][threadIdx.x] = pst;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            int a = threadIdx.x, b = threadIdx.x + w;
+            r[0][a] = r[0][a] > r[0][b] ? r[0][a] : r[0][b];
+            r[1][a] = r[1][a] < r[1][b] ? r[1][a] : r[1][b];
+            r[2][a] = r[2][a] > r[2][b] ? r[2][a] : r[2][b];
+            r[3][a] += r[3][b];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[4 * mem + 0] = r[0][0];
+        out[4 * mem + 1] = r[1][0];
+        out[4 * mem + 2] = r[2][0];
+        out[4 * mem + 3] = sqrt(r[3][0]);
+    }
+}
+
+// matvec over levels inside one wavefront: lane i returns sum_j M^T[j][i] x_j,
+// x_j taken from lane j through LDS (x holds re/im of this lane's level)
+template <typename T>
+__device__ inline cplx<T> wave_matvec(const T *__restrict__ Mt, cplx<T> x, cplx<T> *sh, int nl, int lane) {
+    sh[lane] = x;
+    __syncthreads();
+    cplx<T> acc{T(0), T(0)};
+    if (lane < nl) {
+        for (int j = 0; j < nl; j++) {
+            T a = Mt[j * nl + lane];
+            cplx<T> xj = sh[j];
+            acc.x += a * xj.x;
+            acc.y += a * xj.y;
+        }
+    }
+    __syncthreads();
+    return acc;
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v, T *sh, int lane) {
+    sh[lane] = v;
+    __syncthreads();
+    T s = T(0);
+    for (int j = 0; j < 64; j++) s += sh[j];
+    __syncthreads();
+    return s;
+}
+
+// One IMEX RK stage for every spectral coefficient (run.f90:313-350, 374-417,
+// 440-489) and, after the last stage, the implicit damping (run.f90:493-502).
+// One wavefront per (coefficient, member), lane = model level (bottom -> top).
+// Workgroup = 4 wavefronts with the same total wavenumber (shared d_hyb_m in cache).
+template <typename T>
+__global__ __launch_bounds__(256) void k_si_stage(
+    cplx<T> *__restrict__ spec, const cplx<T> *__restrict__ spec0, cplx<T> *__restrict__ ten0,
+    cplx<T> *__restrict__ ten1, cplx<T> *__restrict__ ten2, cplx<T> *__restrict__ lin0,
+    cplx<T> *__restrict__ lin1, cplx<T> *__restrict__ lin2, const T *__restrict__ At, const T *__restrict__ Bt,
+    const T *__restrict__ Dt, const T *__restrict__ sv, const T *__restrict__ tor, const T *__restrict__ disspec,
+    const T *__restrict__ diffp, const T *__restrict__ dmpp, const int *__restrict__ kbydeg,
+    const int *__restrict__ degree, int stage, Coeffs3 cf, T cimp, int do_implicit, int explicit_, int damp,
+    int nl, int ntr, int nfs, int nm, int nlm, int ntrunc, T dt) {
+    __shared__ cplx<T> shc[4][64];
+    __shared__ T shr[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long item = (long)blockIdx.x * 4 + wave;  // (member, coefficient in degree order)
+    const bool active = item < (long)nlm * nm;
+    const int mem = active ? (int)(item / nlm) : 0;
+    const int k = active ? kbydeg[item % nlm] : 0;
+    const int n = degree[k];
+    const long NFS = (long)nfs * nm;
+    const long base = (long)k * NFS + (long)mem * nfs;
+    const long nlin = 2L * nl + 1;
+    const long lbase = (long)k * nlin * nm + (long)mem * nlin;
+    const T lap = -T(n) * T(n + 1);
+    const bool lv = active && lane < nl;
+    cplx<T> *tn[3] = {ten0, ten1, ten2};
+    cplx<T> *ln[3] = {lin0, lin1, lin2};
+    const cplx<T> z{T(0), T(0)};
+    // vorticity and tracers: always explicit
+    if (lv) {
+        cplx<T> acc = z;
+        for (int r = 0; r <= stage; r++) {
+            cplx<T> t = tn[r][base + lane];
+            acc.x += T(cf.a[r]) * t.x;
+            acc.y += T(cf.a[r]) * t.y;
+        }
+        cplx<T> o = spec0[base + lane];
+        spec[base + lane] = cplx<T>{o.x + dt * acc.x, o.y + dt * acc.y};
+        for (int q = 0; q < ntr; q++) {
+            long off = base + 3L * nl + (long)q * nl + lane;
+            cplx<T> a2 = z;
+            for (int r = 0; r <= stage; r++) {
+                cplx<T> t = tn[r][off];
+                a2.x += T(cf.a[r]) * t.x;
+                a2.y += T(cf.a[r]) * t.y;
+            }
+            cplx<T> o2 = spec0[off];
+            spec[off] = cplx<T>{o2.x + dt * a2.x, o2.y + dt * a2.y};
+        }
+    }
+    const long idv = base + nl + lane, itv = base + 2L * nl + lane, ilp = base + 3L * nl + (long)ntr * nl;
+    if (explicit_) {
+        if (lv) {
+            cplx<T> a1 = z, a2 = z;
+            for (int r = 0; r <= stage; r++) {
+                cplx<T> d = tn[r][idv], t = tn[r][itv];
+                a1.x += T(cf.a[r]) * d.x; a1.y += T(cf.a[r]) * d.y;
+                a2.x += T(cf.a[r]) * t.x; a2.y += T(cf.a[r]) * t.y;
+            }
+            cplx<T> d0 = spec0[idv], t0 = spec0[itv];
+            spec[idv] = cplx<T>{d0.x + dt * a1.x, d0.y + dt * a1.y};
+            spec[itv] = cplx<T>{t0.x + dt * a2.x, t0.y + dt * a2.y};
+        }
+        if (active && lane == 0) {
+            cplx<T> a = z;
+            for (int r = 0; r <= stage; r++) {
+                cplx<T> p = tn[r][ilp];
+                a.x += T(cf.a[r]) * p.x; a.y += T(cf.a[r]) * p.y;
+            }
+            cplx<T> p0 = spec0[ilp];
+            spec[ilp] = cplx<T>{p0.x + dt * a.x, p0.y + dt * a.y};
+        }
+    } else {
+        // linear terms of this stage's input state (run.f90:329-335)
+        const cplx<T> *sin_ = (stage == 0) ? spec0 : spec;
+        cplx<T> tvin = lv ? sin_[itv] : z, dvin = lv ? sin_[idv] : z;
+        cplx<T> lpin = active ? sin_[ilp] : z;
+        cplx<T> Atv = wave_matvec(At, tvin, shc[wave], nl, lane);
+        cplx<T> Bdv = wave_matvec(Bt, dvin, shc[wave], nl, lane);
+        T svl = lv ? sv[lane] : T(0), torl = lv ? tor[lane] : T(0);
+        T sdr = wave_sum(svl * dvin.x, shr[wave], lane), sdi = wave_sum(svl * dvin.y, shr[wave], lane);
+        cplx<T> ddl{-lap * (Atv.x + torl * lpin.x), -lap * (Atv.y + torl * lpin.y)};
+        cplx<T> dtl{-Bdv.x, -Bdv.y};
+        cplx<T> dpl{-sdr, -sdi};
+        cplx<T> *L = ln[stage];
+        if (lv) {
+            L[lbase + lane] = ddl;
+            L[lbase + nl + lane] = dtl;
+            tn[stage][idv] = cplx<T>{tn[stage][idv].x - ddl.x, tn[stage][idv].y - ddl.y};
+            tn[stage][itv] = cplx<T>{tn[stage][itv].x - dtl.x, tn[stage][itv].y - dtl.y};
+        }
+        if (active && lane == 0) {
+            L[lbase + 2L * nl] = dpl;
+            tn[stage][ilp] = cplx<T>{tn[stage][ilp].x - dpl.x, tn[stage][ilp].y - dpl.y};
+        }
+        __syncthreads();
+        // explicit part (run.f90:337-342 and the later-stage forms)
+        cplx<T> dvc = z, tvc = z, lpc = z;
+        if (lv) {
+            cplx<T> a1 = z, a2 = z;
+            for (int r = 0; r <= stage; r++) {
+                cplx<T> d = tn[r][idv], t = tn[r][itv];
+                cplx<T> ld = ln[r][lbase + lane], lt = ln[r][lbase + nl + lane];
+                a1.x += T(cf.a[r]) * d.x + T(cf.aa[r]) * ld.x; a1.y += T(cf.a[r]) * d.y + T(cf.aa[r]) * ld.y;
+                a2.x += T(cf.a[r]) * t.x + T(cf.aa[r]) * lt.x; a2.y += T(cf.a[r]) * t.y + T(cf.aa[r]) * lt.y;
+            }
+            cplx<T> d0 = spec0[idv], t0 = spec0[itv];
+            dvc = cplx<T>{d0.x + dt * a1.x, d0.y + dt * a1.y};
+            tvc = cplx<T>{t0.x + dt * a2.x, t0.y + dt * a2.y};
+        }
+        if (active) {
+            cplx<T> a = z;
+            for (int r = 0; r <= stage; r++) {
+                cplx<T> p = tn[r][ilp], lp = ln[r][lbase + 2L * nl];
+                a.x += T(cf.a[r]) * p.x + T(cf.aa[r]) * lp.x; a.y += T(cf.a[r]) * p.y + T(cf.aa[r]) * lp.y;
+            }
+            cplx<T> p0 = spec0[ilp];
+            lpc = cplx<T>{p0.x + dt * a.x, p0.y + dt * a.y};
+        }
+        if (do_implicit) {
+            // rhs, solve and back-substitution (run.f90:344-349)
+            cplx<T> At2 = wave_matvec(At, tvc, shc[wave], nl, lane);
+            cplx<T> rhs{dvc.x - cimp * lap * dt * (At2.x + torl * lpc.x),
+                        dvc.y - cimp * lap * dt * (At2.y + torl * lpc.y)};
+            const T *D = Dt + ((long)stage * (ntrunc + 1) + n) * nl * nl;
+            cplx<T> dn = wave_matvec(D, rhs, shc[wave], nl, lane);
+            cplx<T> Bd = wave_matvec(Bt, dn, shc[wave], nl, lane);
+            T s2r = wave_sum(svl * dn.x, shr[wave], lane), s2i = wave_sum(svl * dn.y, shr[wave], lane);
+            tvc.x -= cimp * dt * Bd.x;
+            tvc.y -= cimp * dt * Bd.y;
+            lpc.x -= cimp * dt * s2r;
+            lpc.y -= cimp * dt * s2i;
+            dvc = dn;
+        }
+        if (lv) {
+            spec[idv] = dvc;
+            spec[itv] = tvc;
+        }
+        if (active && lane == 0) spec[ilp] = lpc;
+    }
+    if (damp && lv) {
+        // run.f90:493-502
+        T ds = disspec[k];
+        T fv = T(1) - (ds * diffp[lane] - dmpp[lane]) * dt;
+        T ft = T(1) - ds * diffp[lane] * dt;
+        cplx<T> a = spec[base + lane], d = spec[idv], t = spec[itv];
+        spec[base + lane] = cplx<T>{a.x / fv, a.y / fv};
+        spec[idv] = cplx<T>{d.x / fv, d.y / fv};
+        spec[itv] = cplx<T>{t.x / ft, t.y / ft};
+        for (int q = 0; q < ntr; q++) {
+            long off = base + 3L * nl + (long)q * nl + lane;
+            cplx<T> c = spec[off];
+            spec[off] = cplx<T>{c.x / ft, c.y / ft};
+        }
+    }
+}
+
+// time-split physics update (run.f90:526-538)
+template <typename T>
+__global__ void k_phys_update(cplx<T> *__restrict__ spec, const cplx<T> *__restrict__ tp, int nl, int ntr, int nfs,
+                              int nm, int nlm, T dt, int massfix) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nfst = (long)nfs * nm;
+    if (i >= nfst * nlm) return;
+    int f = (int)(i % nfst) % nfs;
+    cplx<T> s = spec[i], t = tp[i];
+    if (f < 3 * nl + ntr * nl) {
+        spec[i] = cplx<T>{s.x + dt * t.x, s.y + dt * t.y};
+    } else if (massfix) {
+        // dlnpsspecdt = (workspec - lnpsspec)/dt; lnpsspec += dt*dlnpsspecdt
+        cplx<T> d{(t.x - s.x) / dt, (t.y - s.y) / dt};
+        spec[i] = cplx<T>{s.x + dt * d.x, s.y + dt * d.y};
+    }
+}
+
+template <typename T> __global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// reference layout (ndimspec, nfield-run) <-> state layout spec[k][member*nfs + off + f]
+template <typename T>
+__global__ void k_pack(cplx<T> *__restrict__ spec, const cplx<T> *__restrict__ ref, int nfr, int off, int nfs,
+                       int nm, int nlm, int to_state) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long tot = (long)nfr * nm * nlm;
+    if (i >= tot) return;
+    int k = (int)(i % nlm);
+    long fr = i / nlm;  // member * nfr + f
+    int mem = (int)(fr / nfr), f = (int)(fr % nfr);
+    long si = (long)k * nfs * nm + (long)mem * nfs + off + f;
+    if (to_state)
+        spec[si] = ref[i];
+    else
+        const_cast<cplx<T> *>(ref)[i] = spec[si];
+}
+
+// grid fields of a member-major field set -> caller layout
+template <typename T>
+__global__ void k_grid_gather(const T *__restrict__ src, long mstride, T *__restrict__ dst, long per_mem, int nm) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per_mem * nm) return;
+    long mem = i / per_mem, r = i % per_mem;
+    dst[i] = src[mem * mstride + r];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+static int invert_fp64(int n, std::vector<double> &a) {
+    std::vector<double> b(n * n, 0.0);
+    for (int i = 0; i < n; i++) b[i + i * n] = 1.0;
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs(a[r + c * n]) > std::fabs(a[piv + c * n])) piv = r;
+        if (a[piv + c * n] == 0.0) return -1;
+        if (piv != c)
+            for (int j = 0; j < n; j++) {
+                std::swap(a[c + j * n], a[piv + j * n]);
+                std::swap(b[c + j * n], b[piv + j * n]);
+            }
+        double d = 1.0 / a[c + c * n];
+        for (int j = 0; j < n; j++) {
+            a[c + j * n] *= d;
+            b[c + j * n] *= d;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            double f = a[r + c * n];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; j++) {
+                a[r + j * n] -= f * a[c + j * n];
+                b[r + j * n] -= f * b[c + j * n];
+            }
+        }
+    }
+    a = b;
+    return 0;
+}
+
+template <typename T> static T *dalloc(size_t n) {
+    T *p = nullptr;
+    if (hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+    return p;
+}
+
+template <typename T> static int dupload(void **d, const std::vector<T> &h, hipStream_t s) {
+    *d = dalloc<T>(h.size());
+    if (!*d) {
+        set_error("device allocation failed");
+        return GFSDC_ENOMEM;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+template <typename T> static std::vector<T> castv(const std::vector<double> &v) {
+    return std::vector<T>(v.begin(), v.end());
+}
+
+uint64_t rng_u64(uint64_t seed, uint64_t stream, uint64_t index) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ULL ^ (stream + 0x632BE59BD9B4E019ULL) * 0xBF58476D1CE4E5B9ULL;
+    z += index * 0x94D049BB133111EBULL + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+template <typename T> int Engine<T>::dyn_init() {
+    const int nl = cfg.nlev, ntr = cfg.ntrac, nm = cfg.nmem, T_ = plan.ntrunc, nlm = plan.nlm;
+    if (nl < 2 || nl > MAXLEV) {
+        set_error("nlev must be in [2, 64]");
+        return GFSDC_EINVAL;
+    }
+    if (ntr != 0) {
+        set_error("tracers (ntrac > 0) are not supported by this build of the dycore");
+        return GFSDC_EINVAL;
+    }
+    if (cfg.physics != GFSDC_PHYS_NONE && cfg.physics != GFSDC_PHYS_HS) {
+        set_error("physics must be GFSDC_PHYS_NONE or GFSDC_PHYS_HS");
+        return GFSDC_EINVAL;
+    }
+    DycoreBuffers *B = new DycoreBuffers();
+    dyn = B;
+    B->nl = nl;
+    B->ntr = ntr;
+    B->nm = nm;
+    B->nfs = 3 * nl + ntr * nl + 1;
+    B->nsyn = 7 * nl + ntr * nl + 3;
+    B->nan = 4 * nl + 1 + ntr * nl;
+    B->nhs = 3 * nl + 1;
+    B->ng = (long)plan.nlat * plan.nlon;
+    const long ng = B->ng, NFS = (long)B->nfs * nm;
+    // ---- one-off operators in FP64 on the host (semimp_data.f90:127-256, dyn_init.f90:450-494)
+    const double rd = cfg.rd, kappa = cfg.rd / cfg.cp, rerth = cfg.rerth, dt = cfg.dt;
+    std::vector<double> dbk(nl), ck(nl), si(nl + 1);
+    B->sl.resize(nl);
+    for (int k = 1; k <= nl; k++) {
+        dbk[k - 1] = bk[k] - bk[k - 1];
+        ck[k - 1] = ak[k] * bk[k - 1] - ak[k - 1] * bk[k];
+    }
+    for (int k = 1; k <= nl + 1; k++) si[nl + 1 - k] = ak[k - 1] / SGLF(101300.0) + bk[k - 1];
+    for (int k = 0; k < nl; k++) B->sl[k] = 0.5 * (si[k] + si[k + 1]);
+    {
+        const double tref = 300.0, pref = 800.e2;
+        std::vector<double> pkref(nl + 1), dpkref(nl), alfaref(nl), yecm(nl * nl, 0.0), tecm(nl * nl, 0.0);
+        for (int k = 0; k <= nl; k++) pkref[k] = ak[k] + bk[k] * pref;
+        for (int k = 0; k < nl; k++) dpkref[k] = pkref[k + 1] - pkref[k];
+        alfaref[0] = SGLF(0.69314718055994530942);
+        for (int k = 2; k <= nl; k++)
+            alfaref[k - 1] = 1.0 - (pkref[k - 1] / dpkref[k - 1]) * std::log(pkref[k] / pkref[k - 1]);
+#define CM(A, i, j) A[((i) - 1) + (size_t)((j) - 1) * nl]
+        for (int ir = 1; ir <= nl; ir++) {
+            CM(yecm, ir, ir) = alfaref[ir - 1] * rd;
+            for (int ic = ir + 1; ic <= nl; ic++) CM(yecm, ir, ic) = rd * std::log(pkref[ic] / pkref[ic - 1]);
+            CM(tecm, ir, ir) = kappa * tref * alfaref[ir - 1];
+            for (int ic = 1; ic <= ir - 1; ic++)
+                CM(tecm, ir, ic) = (kappa * tref * dpkref[ic - 1] / dpkref[ir - 1]) * std::log(pkref[ir] / pkref[ir - 1]);
+        }
+        B->amhyb.assign(nl * nl, 0.0);
+        B->bmhyb.assign(nl * nl, 0.0);
+        B->svhyb.assign(nl, 0.0);
+        B->tor_hyb.assign(nl, rd * tref / (rerth * rerth));
+        for (int j = 1; j <= nl; j++) {
+            B->svhyb[j - 1] = dpkref[nl - j] / pref;
+            for (int k = 1; k <= nl; k++) {
+                CM(B->amhyb, k, j) = CM(yecm, nl + 1 - k, nl + 1 - j) / (rerth * rerth);
+                CM(B->bmhyb, k, j) = CM(tecm, nl + 1 - k, nl + 1 - j);
+            }
+        }
+        std::vector<double> ym(nl * nl);
+        for (int i = 1; i <= nl; i++)
+            for (int j = 1; j <= nl; j++) {
+                double s = B->tor_hyb[i - 1] * B->svhyb[j - 1];
+                for (int k = 1; k <= nl; k++) s += CM(B->amhyb, i, k) * CM(B->bmhyb, k, j);
+                CM(ym, i, j) = s;
+            }
+        B->d_hyb_m.assign((size_t)nl * nl * (T_ + 1) * 3, 0.0);
+        for (int s = 0; s < 3; s++)
+            for (int n = 0; n <= T_; n++) {
+                std::vector<double> m(nl * nl);
+                double c = TAB_IMP[s] * dt, rnn1 = (double)n * (n + 1);
+                for (int i = 1; i <= nl; i++)
+                    for (int j = 1; j <= nl; j++) CM(m, i, j) = (i == j ? 1.0 : 0.0) + c * c * rnn1 * CM(ym, i, j);
+                if (invert_fp64(nl, m)) {
+                    set_error("singular semi-implicit matrix");
+                    return GFSDC_EINVAL;
+                }
+                std::copy(m.begin(), m.end(), B->d_hyb_m.begin() + ((size_t)s * (T_ + 1) + n) * nl * nl);
+            }
+#undef CM
+    }
+    {  // setdampspec
+        int ndiss = cfg.ndiss == 0 ? 8 : cfg.ndiss;
+        double efold = cfg.efold, fshk = cfg.fshk, hf2 = cfg.hdif_fac2, hf = cfg.hdif_fac;
+        if (efold <= 0) {
+            if (T_ > 170)
+                efold = 3600. / (hf2 * SGLF(std::pow((float)T_ / 170.f, 4)) * SGLF(1.1));
+            else if (T_ == 126 || T_ == 170)
+                efold = 1. / (hf2 * SGLF(12.E15) / std::pow(rerth, 4) * SGLF(80. * 81.) * SGLF(80. * 81.));
+            else
+                efold = 1. / (hf2 * SGLF(3.E15) / std::pow(rerth, 4) * SGLF(80. * 81.) * SGLF(80. * 81.));
+            efold = 2. * efold;
+        }
+        if (fshk <= 0) {
+            fshk = 1.0 * hf;
+            if (T_ > 170) fshk = SGLF(2.2) * hf;
+            if (T_ == 126) fshk = SGLF(1.5) * hf;
+        }
+        double slrd0 = SGLF(0.002), dmp1 = 1. / cfg.taustratdamp;
+        B->diff_h.resize(nl);
+        B->dmp_h.resize(nl);
+        for (int k = 0; k < nl; k++) {
+            B->dmp_h[k] = 0.0;
+            B->diff_h[k] = std::pow(B->sl[k], std::log(1. / fshk));
+            if (B->sl[k] < slrd0) B->dmp_h[k] = dmp1 * std::log(slrd0 / B->sl[k]);
+        }
+        B->disspec_h.resize(nlm);
+        double minlap = -(double)T_ * (T_ + 1);
+        int k = 0;
+        for (int m = 0; m <= T_; m++)
+            for (int n = m; n <= T_; n++, k++) {
+                double r = (-(double)n * (n + 1)) / minlap, p = 1.0;
+                for (int i = 0; i < ndiss / 2; i++) p *= r;
+                B->disspec_h[k] = -(1. / efold) * p;
+            }
+    }
+    // ---- device operators (transposed for the per-lane matvec)
+    auto transp = [&](const std::vector<double> &cm, size_t off) {
+        std::vector<T> t(nl * nl);
+        for (int i = 0; i < nl; i++)
+            for (int j = 0; j < nl; j++) t[j * nl + i] = (T)cm[off + i + (size_t)j * nl];
+        return t;
+    };
+    int rc;
+    if ((rc = dupload<T>(&B->at, transp(B->amhyb, 0), stream))) return rc;
+    if ((rc = dupload<T>(&B->bt, transp(B->bmhyb, 0), stream))) return rc;
+    {
+        std::vector<T> d3((size_t)nl * nl * (T_ + 1) * 3);
+        for (size_t b = 0; b < (size_t)(T_ + 1) * 3; b++) {
+            std::vector<T> t = transp(B->d_hyb_m, b * nl * nl);
+            std::copy(t.begin(), t.end(), d3.begin() + b * nl * nl);
+        }
+        if ((rc = dupload<T>(&B->dt3, d3, stream))) return rc;
+    }
+    if ((rc = dupload<T>(&B->sv, castv<T>(B->svhyb), stream))) return rc;
+    if ((rc = dupload<T>(&B->tor, castv<T>(B->tor_hyb), stream))) return rc;
+    if ((rc = dupload<T>(&B->disspec, castv<T>(B->disspec_h), stream))) return rc;
+    if ((rc = dupload<T>(&B->diffp, castv<T>(B->diff_h), stream))) return rc;
+    if ((rc = dupload<T>(&B->dmpp, castv<T>(B->dmp_h), stream))) return rc;
+    if ((rc = dupload<T>(&B->ak, castv<T>(ak), stream))) return rc;
+    if ((rc = dupload<T>(&B->bk, castv<T>(bk), stream))) return rc;
+    if ((rc = dupload<T>(&B->ck, castv<T>(ck), stream))) return rc;
+    if ((rc = dupload<T>(&B->dbk, castv<T>(dbk), stream))) return rc;
+    {
+        std::vector<double> sl(plan.nlat), area(plan.nlat);
+        double tot = 0;
+        for (int j = 0; j < plan.nlat; j++) tot += plan.w[j] * plan.nlon;
+        for (int j = 0; j < plan.nlat; j++) {
+            sl[j] = plan.mu[j];  // sin(lat) = cos(colatitude)
+            area[j] = plan.w[j] / tot;
+        }
+        if ((rc = dupload<T>(&B->sinlat, castv<T>(sl), stream))) return rc;
+        if ((rc = dupload<double>(&B->areaw, area, stream))) return rc;
+    }
+    {
+        std::vector<int> deg(nlm), byd;
+        int k = 0;
+        for (int m = 0; m <= T_; m++)
+            for (int n = m; n <= T_; n++) deg[k++] = n;
+        for (int n = 0; n <= T_; n++)
+            for (int m = 0; m <= n; m++) byd.push_back(plan.koff_h[m] + n - m);
+        void *p;
+        if ((rc = dupload<int>(&p, deg, stream))) return rc;
+        B->degree = (int *)p;
+        if ((rc = dupload<int>(&p, byd, stream))) return rc;
+        B->kbydeg = (int *)p;
+    }
+    // ---- state and work buffers
+    size_t nspec = (size_t)nlm * NFS, nlin = (size_t)nlm * nm * (2 * nl + 1);
+    B->spec = dalloc<cplx<T>>(nspec);
+    B->spec0 = dalloc<cplx<T>>(nspec);
+    for (int s = 0; s < 3; s++) {
+        B->ten[s] = dalloc<cplx<T>>(nspec);
+        B->lin[s] = dalloc<cplx<T>>(nlin);
+    }
+    B->tphys = dalloc<cplx<T>>(nspec);
+    B->topo = dalloc<cplx<T>>((size_t)nlm * nm);
+    B->gsyn = dalloc<T>((size_t)nm * B->nsyn * ng);
+    B->gan = dalloc<T>((size_t)nm * B->nan * ng);
+    B->etadot = dalloc<T>((size_t)nm * (nl + 1) * ng);
+    B->dlnpdt = dalloc<T>((size_t)nm * nl * ng);
+    B->psg = dalloc<T>((size_t)nm * ng);
+    B->dlnpsdt = dalloc<T>((size_t)nm * ng);
+    B->dphisdx = dalloc<T>((size_t)nm * ng);
+    B->dphisdy = dalloc<T>((size_t)nm * ng);
+    B->pdryini = dalloc<double>(nm);
+    B->pmean = dalloc<double>(nm);
+    B->diag = dalloc<double>(4 * nm);
+    void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
+                    B->tphys, B->topo, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
+                    B->dphisdy, B->pdryini, B->pmean, B->diag};
+    for (void *p : ptrs)
+        if (!p) {
+            set_error("device allocation failed for the dycore state");
+            return GFSDC_ENOMEM;
+        }
+    GFSDC_CHECK(hipMemsetAsync(B->spec, 0, sizeof(cplx<T>) * nspec, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->topo, 0, sizeof(cplx<T>) * nlm * nm, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->dphisdx, 0, sizeof(T) * nm * ng, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->dphisdy, 0, sizeof(T) * nm * ng, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->pdryini, 0, sizeof(double) * nm, stream));
+    Work<T> *W = new Work<T>();
+    B->Work = W;
+    if ((rc = work_alloc(*W, plan, nm * std::max(B->nsyn, B->nan)))) return rc;
+    // ---- descriptors
+    std::vector<SynthDesc> sd((size_t)nm * B->nsyn);
+    std::vector<unsigned char> rs((size_t)nm * B->nsyn, 0), ra((size_t)nm * B->nan, 0), rh((size_t)nm * B->nan, 0);
+    std::vector<AnalDesc> ad((size_t)nm * B->nfs), ah((size_t)nm * B->nfs);
+    const double a = rerth, ra_ = 1.0 / rerth;
+    for (int mem = 0; mem < nm; mem++) {
+        SynthDesc *d = &sd[(size_t)mem * B->nsyn];
+        unsigned char *r = &rs[(size_t)mem * B->nsyn];
+        int s0 = mem * B->nfs;  // source field base of the member
+        int vrt = s0, div = s0 + nl, tv = s0 + 2 * nl, tr = s0 + 3 * nl, lp = s0 + 3 * nl + ntr * nl;
+        for (int k = 0; k < nl; k++) {
+            d[k] = SynthDesc{SK_UHAT, vrt + k, div + k, a, a, 1, 1};           // ug (getuv)
+            d[nl + k] = SynthDesc{SK_VHAT, vrt + k, div + k, a, a, 1, 1};      // vg
+            d[2 * nl + k] = SynthDesc{SK_COPY, vrt + k, -1, 1.0, 0.0, 0, 0};   // vrtg
+            d[3 * nl + k] = SynthDesc{SK_COPY, div + k, -1, 1.0, 0.0, 0, 0};   // divg
+            d[4 * nl + k] = SynthDesc{SK_COPY, tv + k, -1, 1.0, 0.0, 0, 0};    // virtempg
+            d[5 * nl + k] = SynthDesc{SK_UHAT, -1, tv + k, 0.0, ra_, 0, 0};    // dvirtempdx (getgrad)
+            d[6 * nl + k] = SynthDesc{SK_VHAT, -1, tv + k, 0.0, ra_, 0, 0};    // dvirtempdy
+            r[k] = r[nl + k] = r[5 * nl + k] = r[6 * nl + k] = 1;
+        }
+        for (int k = 0; k < ntr * nl; k++) d[7 * nl + k] = SynthDesc{SK_COPY, tr + k, -1, 1.0, 0.0, 0, 0};
+        int o = 7 * nl + ntr * nl;
+        d[o] = SynthDesc{SK_COPY, lp, -1, 1.0, 0.0, 0, 0};      // lnpsg
+        d[o + 1] = SynthDesc{SK_UHAT, -1, lp, 0.0, ra_, 0, 0};  // dlnpsdx
+        d[o + 2] = SynthDesc{SK_VHAT, -1, lp, 0.0, ra_, 0, 0};  // dlnpsdy
+        r[o + 1] = r[o + 2] = 1;
+        // analysis: grid fields of the member (Tten, Fx, Fy, KE, dlnpsdt, dq) -> tendencies
+        int g0 = mem * B->nan, h0 = mem * B->nhs;
+        AnalDesc *e = &ad[(size_t)mem * B->nfs];
+        AnalDesc *hh = &ah[(size_t)mem * B->nfs];
+        for (int k = 0; k < nl; k++) {
+            // dyn_run.f90:250-252,268-271: getvrtdivspec(Fx,Fy,ddiv,dvrt); dvrt = -dvrt; ddiv -= lap/a^2 KE
+            e[k] = AnalDesc{AK_DIV, g0 + nl + k, g0 + 2 * nl + k, -1.0, 0, 0, -1, 0.0};
+            e[nl + k] = AnalDesc{AK_VRT, g0 + nl + k, g0 + 2 * nl + k, 1.0, 0, 0, g0 + 3 * nl + k,
+                                 1.0 / (rerth * rerth)};
+            e[2 * nl + k] = AnalDesc{AK_COPY, g0 + k, -1, 1.0, 0, 0, -1, 0.0};
+            ra[g0 + nl + k] = ra[g0 + 2 * nl + k] = 1;
+            // Held-Suarez: (Tforce, vrt force, div force, log(ps*pcorr))
+            hh[k] = AnalDesc{AK_COPY, h0 + nl + k, -1, 1.0, 0, 0, -1, 0.0};
+            hh[nl + k] = AnalDesc{AK_COPY, h0 + 2 * nl + k, -1, 1.0, 0, 0, -1, 0.0};
+            hh[2 * nl + k] = AnalDesc{AK_COPY, h0 + k, -1, 1.0, 0, 0, -1, 0.0};
+        }
+        for (int k = 0; k < ntr * nl; k++)
+            e[3 * nl + k] = AnalDesc{AK_COPY, g0 + 4 * nl + 1 + k, -1, 1.0, 0, 0, -1, 0.0};
+        e[3 * nl + ntr * nl] = AnalDesc{AK_COPY, g0 + 4 * nl, -1, 1.0, 0, 0, -1, 0.0};
+        hh[3 * nl + ntr * nl] = AnalDesc{AK_COPY, h0 + 3 * nl, -1, 1.0, 0, 0, -1, 0.0};
+    }
+    void *p;
+    if ((rc = dupload<SynthDesc>(&p, sd, stream))) return rc;
+    B->sdesc = (SynthDesc *)p;
+    if ((rc = dupload<AnalDesc>(&p, ad, stream))) return rc;
+    B->adesc_dyn = (AnalDesc *)p;
+    if ((rc = dupload<AnalDesc>(&p, ah, stream))) return rc;
+    B->adesc_hs = (AnalDesc *)p;
+    if ((rc = dupload<unsigned char>(&p, rs, stream))) return rc;
+    B->rflag_syn = (unsigned char *)p;
+    if ((rc = dupload<unsigned char>(&p, ra, stream))) return rc;
+    B->rflag_an = (unsigned char *)p;
+    if ((rc = dupload<unsigned char>(&p, rh, stream))) return rc;
+    B->rflag_hs = (unsigned char *)p;
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> void Engine<T>::dyn_destroy() {
+    DycoreBuffers *B = dyn;
+    if (!B) return;
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    for (auto e : B->ev) (void)hipEventDestroy(e);
+    void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
+                    B->tphys, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
+                    B->dphisdy, B->topo, B->at, B->bt, B->dt3, B->sv, B->tor, B->disspec, B->diffp, B->dmpp,
+                    B->ak, B->bk, B->ck, B->dbk, B->sinlat, B->coslat, B->areaw, B->kbydeg, B->degree,
+                    B->pdryini, B->pmean, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->rflag_syn,
+                    B->rflag_an, B->rflag_hs};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    if (B->Work) {
+        work_free(*(Work<T> *)B->Work);
+        delete (Work<T> *)B->Work;
+    }
+    delete B;
+    dyn = nullptr;
+}
+
+// ---- step pieces ------------------------------------------------------------
+
+template <typename T> struct Step {
+    Engine<T> &E;
+    DycoreBuffers &B;
+    Work<T> &W;
+    hipStream_t s;
+    void mark(const char *name) {
+        if (!B.profiling) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        B.ev.push_back(e);
+        B.evname.push_back(name);
+    }
+    void synth_all() {
+        const Plan<T> &p = E.plan;
+        int nf = B.nm * B.nsyn;
+        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc, nf, W.sx, W.ldc, 1.0, s);
+        mark("synth_gather");
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+        mark("leg_synth");
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_syn, s);
+        mark("fft_synth");
+    }
+    void columns(int inverse_only) {
+        const gfsdc_cfg &c = E.cfg;
+        dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+        hipLaunchKernelGGL(k_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (T *)B.gan, (T *)B.etadot,
+                           (T *)B.dlnpdt, (T *)B.psg, (T *)B.dlnpsdt, (const T *)B.dphisdx, (const T *)B.dphisdy,
+                           (const T *)B.ak, (const T *)B.bk, (const T *)B.ck, (const T *)B.dbk,
+                           (const T *)B.sinlat, B.nl, B.ntr, B.nsyn, B.nan, B.ng, E.plan.nlon, (T)c.rd,
+                           (T)(c.rd / c.cp), (T)(c.cvap / c.cp - 1.0), (T)(2.0 * c.omega), inverse_only);
+        mark(inverse_only ? "columns_inverse" : "columns");
+    }
+    void ps_mean(bool set_pdry, bool mean) {
+        hipLaunchKernelGGL(k_ps_mean<T>, dim3(B.nm), dim3(1024), 0, s, (const T *)B.psg, (const double *)B.areaw,
+                           mean ? B.pmean : (double *)nullptr, set_pdry ? B.pdryini : (double *)nullptr, B.ng,
+                           E.plan.nlon);
+        mark("ps_mean");
+    }
+    void analyse(const AnalDesc *desc, const unsigned char *rflag, int ngf, cplx<T> *dst) {
+        const Plan<T> &p = E.plan;
+        launch_fft_anal(p, (const T *)B.gan, B.ng, ngf, rflag, W.fa, W.ldc, s);
+        mark("fft_anal");
+        launch_leg_anal(p, W.fa, W.ldc, W.sx, W.ldc, 2 * ngf, s);
+        mark("leg_anal");
+        launch_anal_scatter(p, W.sx, W.ldc, desc, B.nm * B.nfs, dst, 1, (long)B.nfs * B.nm, E.cfg.rerth, s);
+        mark("anal_scatter");
+    }
+    void getdyntend(int stage) {
+        synth_all();
+        columns(0);
+        if (stage == 0) ps_mean(true, false);  // dyn_run.f90:135-153 on this stage's psg
+        analyse(B.adesc_dyn, B.rflag_an, B.nm * B.nan, (cplx<T> *)B.ten[stage]);
+    }
+    void si_stage(int stage, bool damp) {
+        const gfsdc_cfg &c = E.cfg;
+        Coeffs3 cf;
+        for (int r = 0; r < 3; r++) {
+            cf.a[r] = TAB_A[stage][r];
+            cf.aa[r] = TAB_AA[stage][r];
+        }
+        int do_imp = (stage < 2 || std::fabs(TAB_IMP[stage]) > 1.e-5) ? 1 : 0;
+        long items = (long)E.plan.nlm * B.nm;
+        hipLaunchKernelGGL(k_si_stage<T>, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, (cplx<T> *)B.spec,
+                           (const cplx<T> *)B.spec0, (cplx<T> *)B.ten[0], (cplx<T> *)B.ten[1],
+                           (cplx<T> *)B.ten[2], (cplx<T> *)B.lin[0], (cplx<T> *)B.lin[1], (cplx<T> *)B.lin[2],
+                           (const T *)B.at, (const T *)B.bt, (const T *)B.dt3, (const T *)B.sv, (const T *)B.tor,
+                           (const T *)B.disspec, (const T *)B.diffp, (const T *)B.dmpp, B.kbydeg, B.degree, stage,
+                           cf, (T)TAB_IMP[stage], do_imp, c.explicit_, damp ? 1 : 0, B.nl, B.ntr, B.nfs, B.nm,
+                           E.plan.nlm, E.plan.ntrunc, (T)c.dt);
+        mark("si_stage");
+    }
+    void physics() {
+        const gfsdc_cfg &c = E.cfg;
+        synth_all();
+        columns(1);
+        if (c.physics == GFSDC_PHYS_HS) {
+            ps_mean(false, c.massfix != 0);
+            dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+            hipLaunchKernelGGL(k_hs_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg, (T *)B.gan,
+                               (const T *)B.ak, (const T *)B.bk, (const T *)B.sinlat, B.pdryini, B.pmean, B.nl,
+                               B.ntr, B.nsyn, B.nhs, B.ng, E.plan.nlon, (T)(c.rd / c.cp), c.massfix);
+            mark("hs_columns");
+            // HS fields per member: T, vrt, div forcing and the mass-fixer field
+            analyse(B.adesc_hs, B.rflag_hs, B.nm * B.nhs, (cplx<T> *)B.tphys);
+            long n = (long)E.plan.nlm * B.nfs * B.nm;
+            hipLaunchKernelGGL(k_phys_update<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               (cplx<T> *)B.spec, (const cplx<T> *)B.tphys, B.nl, B.ntr, B.nfs, B.nm, E.plan.nlm,
+                               (T)c.dt, c.massfix);
+            mark("phys_update");
+        }
+    }
+    void step() {
+        long n = 2L * E.plan.nlm * B.nfs * B.nm;
+        hipLaunchKernelGGL(k_copy<T>, dim3(2048), dim3(256), 0, s, (T *)B.spec0, (const T *)B.spec, n);
+        mark("save_state");
+        for (int st = 0; st < 3; st++) {
+            getdyntend(st);
+            si_stage(st, st == 2);
+        }
+        if (E.cfg.physics != GFSDC_PHYS_NONE) physics();
+    }
+};
+
+template <typename T>
+int Engine<T>::set_state(const void *vrt, const void *div, const void *tv, const void *trac, const void *lnps,
+                         int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore (nlev == 0)");
+        return GFSDC_ESTATE;
+    }
+    const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
+    struct Part {
+        const void *p;
+        int nfr, off;
+    } parts[] = {{vrt, nl, 0}, {div, nl, nl}, {tv, nl, 2 * nl}, {trac, B->ntr * nl, 3 * nl},
+                 {lnps, 1, 3 * nl + B->ntr * nl}};
+    for (auto &pt : parts) {
+        if (!pt.p || pt.nfr == 0) continue;
+        const T *in;
+        size_t n = (size_t)pt.nfr * nm * nlm * 2;
+        int rc = input(pt.p, n, dtype, loc, 0, &in);
+        if (rc) return rc;
+        long tot = (long)pt.nfr * nm * nlm;
+        hipLaunchKernelGGL(k_pack<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
+                           (cplx<T> *)B->spec, (const cplx<T> *)in, pt.nfr, pt.off, B->nfs, nm, nlm, 1);
+        GFSDC_CHECK(hipGetLastError());
+    }
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T>
+int Engine<T>::get_state(void *vrt, void *div, void *tv, void *trac, void *lnps, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore (nlev == 0)");
+        return GFSDC_ESTATE;
+    }
+    const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
+    struct Part {
+        void *p;
+        int nfr, off;
+    } parts[] = {{vrt, nl, 0}, {div, nl, nl}, {tv, nl, 2 * nl}, {trac, B->ntr * nl, 3 * nl},
+                 {lnps, 1, 3 * nl + B->ntr * nl}};
+    for (auto &pt : parts) {
+        if (!pt.p || pt.nfr == 0) continue;
+        size_t n = (size_t)pt.nfr * nm * nlm;
+        int rc = ensure_stage(3, n * sizeof(cplx<T>));
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pack<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, (cplx<T> *)B->spec,
+                           (const cplx<T> *)stage[3], pt.nfr, pt.off, B->nfs, nm, nlm, 0);
+        GFSDC_CHECK(hipGetLastError());
+        if ((rc = output((const T *)stage[3], pt.p, n * 2, dtype, loc, 1))) return rc;
+    }
+    return 0;
+}
+
+template <typename T> int Engine<T>::set_topo(const void *topo, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    const int nlm = plan.nlm, nm = B->nm;
+    const T *in;
+    int rc = input(topo, (size_t)nlm * nm * 2, dtype, loc, 0, &in);
+    if (rc) return rc;
+    // dyn_init.f90:84 getgrad(grav*topospec) -> dphisdx, dphisdy
+    Work<T> &W = *(Work<T> *)B->Work;
+    std::vector<SynthDesc> sd(2 * nm);
+    double g = cfg.grav / cfg.rerth;
+    for (int mem = 0; mem < nm; mem++) {
+        sd[mem] = SynthDesc{SK_UHAT, -1, mem, 0.0, g, 0, 0};
+        sd[nm + mem] = SynthDesc{SK_VHAT, -1, mem, 0.0, g, 0, 0};
+    }
+    if ((rc = upload_descs(sd, {}, std::vector<unsigned char>(2 * nm, 1)))) return rc;
+    GFSDC_CHECK(hipMemcpyAsync(B->topo, in, sizeof(cplx<T>) * nlm * nm, hipMemcpyDeviceToDevice, stream));
+    launch_synth_gather(plan, (const cplx<T> *)B->topo, nlm, 1, d_sdesc, 2 * nm, W.sx, W.ldc, 1.0, stream);
+    launch_leg_synth(plan, W.sx, W.ldc, W.fs, W.ldc, 4 * nm, stream);
+    // dphisdx then dphisdy are contiguous (nm fields each)
+    launch_fft_synth(plan, W.fs, W.ldc, (T *)B->dphisdx, B->ng, nm, d_rflag, stream);
+    GFSDC_CHECK(hipGetLastError());
+    // second half of the fields (dphisdy): shift the Fourier columns by nm fields
+    launch_fft_synth(plan, W.fs + 2 * nm, W.ldc, (T *)B->dphisdy, B->ng, nm, d_rflag + nm, stream);
+    GFSDC_CHECK(hipGetLastError());
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T>
+int Engine<T>::getdyntend(int rkstage, int inverse_only, void *dvrt, void *ddiv, void *dtv, void *dtrac,
+                          void *dlnps, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    if (rkstage < 1 || rkstage > 3) {
+        set_error("rkstage must be 1, 2 or 3");
+        return GFSDC_EINVAL;
+    }
+    Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    if (inverse_only) {
+        st.synth_all();
+        st.columns(1);
+        if (rkstage == 1) st.ps_mean(true, false);
+        GFSDC_CHECK(hipGetLastError());
+        GFSDC_CHECK(hipStreamSynchronize(stream));
+        return 0;
+    }
+    st.synth_all();
+    st.columns(0);
+    if (rkstage == 1) st.ps_mean(true, false);  // dyn_run.f90:135-153
+    st.analyse(B->adesc_dyn, B->rflag_an, B->nm * B->nan, (cplx<T> *)B->ten[0]);
+    GFSDC_CHECK(hipGetLastError());
+    const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
+    struct Part {
+        void *p;
+        int nfr, off;
+    } parts[] = {{dvrt, nl, 0}, {ddiv, nl, nl}, {dtv, nl, 2 * nl}, {dtrac, B->ntr * nl, 3 * nl},
+                 {dlnps, 1, 3 * nl + B->ntr * nl}};
+    for (auto &pt : parts) {
+        if (!pt.p || pt.nfr == 0) continue;
+        size_t n = (size_t)pt.nfr * nm * nlm;
+        int rc = ensure_stage(3, n * sizeof(cplx<T>));
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pack<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           (cplx<T> *)B->ten[0], (const cplx<T> *)stage[3], pt.nfr, pt.off, B->nfs, nm, nlm, 0);
+        GFSDC_CHECK(hipGetLastError());
+        if ((rc = output((const T *)stage[3], pt.p, n * 2, dtype, loc, 1))) return rc;
+    }
+    return 0;
+}
+
+template <typename T> int Engine<T>::advance(int nsteps, int use_graph) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    if (!use_graph) {
+        for (int i = 0; i < nsteps; i++) st.step();
+        GFSDC_CHECK(hipGetLastError());
+        return 0;
+    }
+    if (!B->gexec) {
+        GFSDC_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        st.step();
+        GFSDC_CHECK(hipStreamEndCapture(stream, &B->graph));
+        GFSDC_CHECK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < nsteps; i++) GFSDC_CHECK(hipGraphLaunch(B->gexec, stream));
+    return 0;
+}
+
+template <typename T> int Engine<T>::hs_phytend(void *dvrt, void *ddiv, void *dtv, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    dim3 g((unsigned)((B->ng + 255) / 256), B->nm);
+    hipLaunchKernelGGL(k_hs_columns<T>, g, dim3(256), 0, stream, (const T *)B->gsyn, (const T *)B->psg,
+                       (T *)B->gan, (const T *)B->ak, (const T *)B->bk, (const T *)B->sinlat, B->pdryini, B->pmean,
+                       B->nl, B->ntr, B->nsyn, B->nhs, B->ng, plan.nlon, (T)(cfg.rd / cfg.cp), 0);
+    st.analyse(B->adesc_hs, B->rflag_hs, B->nm * B->nhs, (cplx<T> *)B->tphys);
+    GFSDC_CHECK(hipGetLastError());
+    const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
+    struct Part {
+        void *p;
+        int off;
+    } parts[] = {{dvrt, 0}, {ddiv, nl}, {dtv, 2 * nl}};
+    for (auto &pt : parts) {
+        if (!pt.p) continue;
+        size_t n = (size_t)nl * nm * nlm;
+        int rc = ensure_stage(3, n * sizeof(cplx<T>));
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pack<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, (cplx<T> *)B->tphys,
+                           (const cplx<T> *)stage[3], nl, pt.off, B->nfs, nm, nlm, 0);
+        if ((rc = output((const T *)stage[3], pt.p, n * 2, dtype, loc, 1))) return rc;
+    }
+    return 0;
+}
+
+template <typename T> int Engine<T>::get_grid(int which, void *out, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    const int nl = B->nl, nm = B->nm;
+    const long ng = B->ng;
+    const T *src = nullptr;
+    long per = 0, mstride = 0;
+    switch (which) {
+    case GF_UG: src = (const T *)B->gsyn; per = nl * ng; mstride = (long)B->nsyn * ng; break;
+    case GF_VG: src = (const T *)B->gsyn + nl * ng; per = nl * ng; mstride = (long)B->nsyn * ng; break;
+    case GF_VRT: src = (const T *)B->gsyn + 2 * nl * ng; per = nl * ng; mstride = (long)B->nsyn * ng; break;
+    case GF_DIV: src = (const T *)B->gsyn + 3 * nl * ng; per = nl * ng; mstride = (long)B->nsyn * ng; break;
+    case GF_TV: src = (const T *)B->gsyn + 4 * nl * ng; per = nl * ng; mstride = (long)B->nsyn * ng; break;
+    case GF_DLNPDT: src = (const T *)B->dlnpdt; per = nl * ng; mstride = per; break;
+    case GF_ETADOT: src = (const T *)B->etadot; per = (nl + 1) * ng; mstride = per; break;
+    case GF_PSG: src = (const T *)B->psg; per = ng; mstride = ng; break;
+    case GF_DLNPSDT: src = (const T *)B->dlnpsdt; per = ng; mstride = ng; break;
+    case GF_DLNPSDX: src = (const T *)B->gsyn + (7L + B->ntr) * nl * ng + ng; per = ng; mstride = (long)B->nsyn * ng; break;
+    case GF_DLNPSDY: src = (const T *)B->gsyn + (7L + B->ntr) * nl * ng + 2 * ng; per = ng; mstride = (long)B->nsyn * ng; break;
+    default:
+        set_error("unknown grid field");
+        return GFSDC_EINVAL;
+    }
+    int rc = ensure_stage(3, sizeof(T) * per * nm);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_grid_gather<T>, dim3((unsigned)((per * nm + 255) / 256)), dim3(256), 0, stream, src,
+                       mstride, (T *)stage[3], per, nm);
+    GFSDC_CHECK(hipGetLastError());
+    return output((const T *)stage[3], out, (size_t)per * nm, dtype, loc, 1);
+}
+
+template <typename T>
+int Engine<T>::get_semimp(double *a, double *b, double *sv, double *tor, double *d) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    if (a) std::copy(B->amhyb.begin(), B->amhyb.end(), a);
+    if (b) std::copy(B->bmhyb.begin(), B->bmhyb.end(), b);
+    if (sv) std::copy(B->svhyb.begin(), B->svhyb.end(), sv);
+    if (tor) std::copy(B->tor_hyb.begin(), B->tor_hyb.end(), tor);
+    if (d) std::copy(B->d_hyb_m.begin(), B->d_hyb_m.end(), d);
+    return 0;
+}
+
+template <typename T> int Engine<T>::get_damp(double *ds, double *df, double *dm) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    if (ds) std::copy(B->disspec_h.begin(), B->disspec_h.end(), ds);
+    if (df) std::copy(B->diff_h.begin(), B->diff_h.end(), df);
+    if (dm) std::copy(B->dmp_h.begin(), B->dmp_h.end(), dm);
+    return 0;
+}
+
+template <typename T> int Engine<T>::step_diag(double *out) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    hipLaunchKernelGGL(k_diag<T>, dim3(B->nm), dim3(1024), 0, stream, (const T *)B->gsyn, (const T *)B->psg,
+                       (const T *)B->dlnpsdt, (const double *)B->areaw, B->diag, B->nl, B->nsyn, B->ng, plan.nlon);
+    GFSDC_CHECK(hipGetLastError());
+    GFSDC_CHECK(hipMemcpyAsync(out, B->diag, sizeof(double) * 4 * B->nm, hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> int Engine<T>::set_pdryini(const double *p) {
+    if (!dyn) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(dyn->pdryini, p, sizeof(double) * dyn->nm, hipMemcpyHostToDevice, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> int Engine<T>::get_pdryini(double *p) {
+    if (!dyn) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(p, dyn->pdryini, sizeof(double) * dyn->nm, hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// Resting isothermal state plus random vorticity (counter-based RNG, stream =
+// level, index = spectral coefficient; member i uses seed + i).
+template <typename T>
+int Engine<T>::init_rest_state(double t0, double ps0, double vort_rms, int nmin, int nmax, unsigned long long seed) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    const int nl = B->nl, nm = B->nm, nlm = plan.nlm, T_ = plan.ntrunc;
+    const double sq4pi = std::sqrt(4.0 * M_PI);
+    std::vector<double> vrt((size_t)nm * nl * nlm * 2, 0.0), tv((size_t)nm * nl * nlm * 2, 0.0),
+        lp((size_t)nm * nlm * 2, 0.0), zero((size_t)nm * nl * nlm * 2, 0.0);
+    for (int mem = 0; mem < nm; mem++) {
+        for (int l = 0; l < nl; l++) {
+            double *v = &vrt[((size_t)mem * nl + l) * nlm * 2];
+            double ss = 0.0;
+            int k = 0;
+            for (int m = 0; m <= T_; m++)
+                for (int n = m; n <= T_; n++, k++) {
+                    if (n < nmin || n > nmax) continue;
+                    double u = (double)(rng_u64(seed + mem, (uint64_t)l, (uint64_t)k) >> 11) * 0x1p-53;
+                    double th = 2.0 * M_PI * u;
+                    v[2 * k] = std::cos(th);
+                    v[2 * k + 1] = (m == 0) ? 0.0 : std::sin(th);
+                    ss += (m == 0 ? 1.0 : 2.0) * (v[2 * k] * v[2 * k] + v[2 * k + 1] * v[2 * k + 1]);
+                }
+            double rms = std::sqrt(ss / (4.0 * M_PI));
+            double sc = rms > 0 ? vort_rms / rms : 0.0;
+            for (int i = 0; i < nlm * 2; i++) v[i] *= sc;
+            tv[((size_t)mem * nl + l) * nlm * 2] = t0 * sq4pi;
+        }
+        lp[(size_t)mem * nlm * 2] = std::log(ps0) * sq4pi;
+    }
+    int rc = set_state(vrt.data(), zero.data(), tv.data(), nullptr, lp.data(), GFSDC_DT_F64, GFSDC_HOST);
+    if (rc) return rc;
+    std::vector<double> z(nm, 0.0);
+    return set_pdryini(z.data());
+}
+
+template <typename T> int Engine<T>::profile_step(char names[][48], double *ms, int nmax) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    for (auto e : B->ev) (void)hipEventDestroy(e);
+    B->ev.clear();
+    B->evname.clear();
+    B->profiling = true;
+    Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    st.mark("start");
+    st.step();
+    B->profiling = false;
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    int n = 0;
+    for (size_t i = 1; i < B->ev.size() && n < nmax; i++, n++) {
+        float t = 0;
+        GFSDC_CHECK(hipEventElapsedTime(&t, B->ev[i - 1], B->ev[i]));
+        std::strncpy(names[n], B->evname[i].c_str(), 47);
+        names[n][47] = 0;
+        ms[n] = t;
+    }
+    return n;
+}
+
+template struct Engine<float>;
+template struct Engine<double>;
+
+}  // namespace gfsdc

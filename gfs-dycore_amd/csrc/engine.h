@@ -1,0 +1,80 @@
+// Engine<T>: the device-resident transform and dycore machinery behind one
+// gfsdc context, at compute precision T (double = FP64 path, float = FP32 path).
+#pragma once
+#include <memory>
+
+#include "gfsdc_internal.h"
+
+namespace gfsdc {
+
+// Grid-space fields held by the dycore after getdyntend (grid_data.f90 names).
+enum GridField : int {
+    GF_UG = 0, GF_VG, GF_VRT, GF_DIV, GF_TV, GF_DLNPDT, GF_ETADOT, GF_PSG, GF_DLNPSDT, GF_PRS,
+    GF_DLNPSDX = 14, GF_DLNPSDY = 15
+};
+
+struct DycoreBuffers;  // defined in dycore.hip
+
+template <typename T> struct Engine {
+    gfsdc_cfg cfg{};
+    std::vector<double> ak, bk;
+    hipStream_t stream = nullptr;
+    Plan<T> plan;
+    Work<T> work;                 // per-field API workspace
+    // staging for caller buffers
+    void *stage[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t stage_sz[4] = {0, 0, 0, 0};
+    SynthDesc *d_sdesc = nullptr;
+    AnalDesc *d_adesc = nullptr;
+    unsigned char *d_rflag = nullptr;
+    int desc_cap = 0;
+    std::vector<SynthDesc> h_sdesc;
+    std::vector<AnalDesc> h_adesc;
+    std::vector<unsigned char> h_rflag;
+    DycoreBuffers *dyn = nullptr;
+
+    int init(const gfsdc_cfg &c);
+    void destroy();
+    int ensure_stage(int i, size_t bytes);
+    int upload_descs(const std::vector<SynthDesc> &sd, const std::vector<AnalDesc> &ad,
+                     const std::vector<unsigned char> &rf);
+    // caller buffer (n real values of dtype at loc) -> device T buffer
+    int input(const void *p, size_t n, int dtype, int loc, int slot, const T **out);
+    // device T buffer -> caller buffer
+    int output(const T *d, void *p, size_t n, int dtype, int loc, int slot);
+
+    int spectogrd(const void *spec, void *grid, int nf, int dtype, int loc);
+    int grdtospec(const void *grid, void *spec, int nf, int dtype, int loc);
+    int sphtor_to_spat(const void *s, const void *t, void *u, void *v, int nf, double sscale, int sinv,
+                       double tscale, int tinv, int dtype, int loc);
+    int spat_to_sphtor(const void *u, const void *v, void *s, void *t, int nf, double rsphere, int inv,
+                       int dtype, int loc);
+
+    // dycore (dycore.hip)
+    int dyn_init();
+    void dyn_destroy();
+    int set_state(const void *vrt, const void *div, const void *tv, const void *trac, const void *lnps,
+                  int dtype, int loc);
+    int get_state(void *vrt, void *div, void *tv, void *trac, void *lnps, int dtype, int loc);
+    int set_topo(const void *topo, int dtype, int loc);
+    int getdyntend(int rkstage, int inverse_only, void *dvrt, void *ddiv, void *dtv, void *dtrac,
+                   void *dlnps, int dtype, int loc);
+    int advance(int nsteps, int use_graph);
+    int hs_phytend(void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
+    int get_grid(int which, void *out, int dtype, int loc);
+    int get_semimp(double *amhyb, double *bmhyb, double *svhyb, double *tor_hyb, double *d_hyb_m);
+    int get_damp(double *disspec, double *diff_prof, double *dmp_prof);
+    int step_diag(double *out);
+    int set_pdryini(const double *p);
+    int get_pdryini(double *p);
+    int init_rest_state(double t0, double ps0, double vort_rms, int nmin, int nmax, unsigned long long seed);
+    int profile_step(char names[][48], double *ms, int nmax);
+};
+
+}  // namespace gfsdc
+
+struct gfsdc_ctx {
+    int precision = GFSDC_FP64;
+    gfsdc::Engine<float> *ef = nullptr;
+    gfsdc::Engine<double> *ed = nullptr;
+};

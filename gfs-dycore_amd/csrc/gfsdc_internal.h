@@ -1,0 +1,113 @@
+// Internal declarations of the gfsdc HIP library (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gfsdc.h"
+
+#define GFSDC_CHECK(expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            gfsdc::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
+            return GFSDC_EHIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+namespace gfsdc {
+
+void set_error(const std::string &msg);
+
+template <typename T> struct cplx { T x, y; };
+
+// Legendre GEMM tile shape (rows x columns of one workgroup, 4 waves).
+constexpr int GEMM_BM = 64;
+constexpr int GEMM_BN = 128;
+constexpr int GEMM_BK = 16;
+constexpr int COL_PAD = GEMM_BN;  // column padding of transform work buffers
+
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// Transform plan: Gauss grid, Legendre tables and FFT twiddles for one
+// (nlon, nlat, ntrunc) at precision T. Tables are generated on the device in FP64.
+template <typename T> struct Plan {
+    int nlon = 0, nlat = 0, nhalf = 0, nhp = 0, ntrunc = 0, nlm = 0, nlme = 0;
+    int nrad = 0, rad[16] = {0};
+    T *plm = nullptr;        // per m: [even i'][nhp] then [odd i'][nhp], n = m + 2i' (+1)
+    long *poff = nullptr;    // device, per m (ntrunc+2 entries)
+    int *keoff = nullptr;    // device, extended spectral offset of order m
+    int *koff = nullptr;     // device, spectral offset of order m
+    T *eps = nullptr;        // device, eps_n^m at extended index (n = m..T+1) plus n=T+2 row
+    int *eoff = nullptr;     // device, offset of order m in eps (T+3-m entries)
+    T *rcos = nullptr;       // device, 1/cos(lat), nlat
+    T *wfac = nullptr;       // device, 2 pi w_j / nlon, nlat
+    T *tw = nullptr;         // device, complex twiddles exp(-2 pi i t/nlon), 2*nlon
+    int *mtile_m = nullptr;  // device, analysis m-tile -> m
+    int *mtile_i = nullptr;  // device, analysis m-tile -> first row
+    int n_mtiles = 0;
+    int2 *kemn = nullptr;    // device, extended index -> (m, n)
+    int2 *kmn = nullptr;     // device, spectral index -> (m, n)
+    double *mu_d = nullptr, *w_d = nullptr;  // FP64 nodes / weights (device)
+    std::vector<double> mu, w;               // host copies
+    std::vector<int> keoff_h, koff_h;
+    std::vector<long> poff_h;
+};
+
+template <typename T> int plan_create(Plan<T> &p, int nlon, int nlat, int ntrunc, hipStream_t s);
+template <typename T> void plan_destroy(Plan<T> &p);
+
+// Descriptors of the synthesis gather (spectral state -> extended spectral
+// columns) and of the analysis scatter (extended columns -> spectral output).
+enum SynthKind : int { SK_ZERO = 0, SK_COPY = 1, SK_UHAT = 2, SK_VHAT = 3 };
+struct SynthDesc {
+    int kind;
+    int a, b;          // source fields (S and T of the vector pair; a for COPY)
+    double sa, sb;     // multipliers of S and T
+    int inv_a, inv_b;  // multiply S / T by invlap(n)
+};
+enum AnalKind : int { AK_COPY = 0, AK_VRT = 1, AK_DIV = 2 };
+struct AnalDesc {
+    int kind;
+    int a, b;      // analysed columns (field indices): COPY uses a; VRT/DIV use (U~=a, V~=b)
+    double scale;  // multiplier of the result
+    int inv;       // 1: also multiply by invlap(n) (SHTns S/T instead of vorticity/divergence)
+    int accum;     // 1: out += result, 0: out = result
+    int c;         // optional third column (-1: none) added as sc * n(n+1) * X_c
+    double sc;
+};
+
+// Kernel launchers (sht_kernels.hip), all asynchronous on stream s.
+// Spectral source element (field f, coefficient k) sits at src[f*sf + k*sk] (complex).
+template <typename T>
+void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc,
+                         int nfo, T *sx, int ldx, double rsphere, hipStream_t s);
+template <typename T>
+void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s);
+template <typename T>
+void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
+                      const unsigned char *rcos_flag, hipStream_t s);
+template <typename T>
+void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rcos_flag,
+                     T *fa, int ldf, hipStream_t s);
+template <typename T>
+void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s);
+template <typename T>
+void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc *desc, int nfo,
+                         cplx<T> *dst, long df, long dk, double rsphere, hipStream_t s);
+
+// Workspace for one batched transform pipeline.
+template <typename T> struct Work {
+    int ncap = 0;          // field capacity
+    int ldc = 0;           // padded column count (2*ncap rounded to COL_PAD)
+    T *sx = nullptr;       // nlme x ldc
+    T *fs = nullptr;       // (T+1) x nlat x ldc (synthesis Fourier)
+    T *fa = nullptr;       // (T+1) x 2 x nhp x ldc (analysis Fourier, zero padded)
+    T *grid = nullptr;     // ncap x nlat x nlon
+};
+template <typename T> int work_alloc(Work<T> &w, const Plan<T> &p, int nfield);
+template <typename T> void work_free(Work<T> &w);
+
+}  // namespace gfsdc

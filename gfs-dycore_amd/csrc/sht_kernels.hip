@@ -1,0 +1,808 @@
+// Spherical-harmonic transform kernels for gfx950 (CDNA4).
+//
+// Synthesis of a batch of fields (spectral -> grid) runs as
+//   gather      spectral state -> extended spectral columns (n <= T+1), applying the
+//               U = u cos(lat), V = v cos(lat) recurrences for vector fields
+//   Legendre    per zonal wavenumber m an MFMA GEMM over (latitude pairs) x (degrees)
+//               x (2 * fields), split by the parity of n - m so one GEMM per parity
+//               covers both hemispheres (P_n^m(-mu) = (-1)^(n-m) P_n^m(mu))
+//   FFT         per latitude pair a complex Stockham FFT in LDS packing the two
+//               hemispheres' real rows as z = x_north + i x_south
+// and analysis is the mirror image (FFT -> Legendre GEMM -> scatter).
+// The arithmetic follows the transforms src/shtns.f90 requests from SHTns:
+// orthonormal harmonics without Condon-Shortley phase on a Gauss grid.
+#include <cmath>
+
+#include "gfsdc_internal.h"
+
+namespace gfsdc {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MF;
+// Both take A[i][k] from lane i + 16k and B[k][j] from lane j + 16k; the row of
+// accumulator element r of a lane differs: 4*(lane/16) + r for the f32 form,
+// lane/16 + 4*r for the f64 form (measured on gfx950).
+template <> struct MF<float> {
+    typedef floatx4 acc;
+    __device__ static inline acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int r) { return (lane >> 4) * 4 + r; }
+};
+template <> struct MF<double> {
+    typedef doublex4 acc;
+    __device__ static inline acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+// ---------------------------------------------------------------------------
+// Table generation (FP64 arithmetic on the device)
+
+// Gauss-Legendre nodes mu (cos colatitude, north -> south) and weights (sum 2)
+// by Newton iteration on P_nlat, one thread per root pair.
+__global__ void k_gauss(double *mu, double *w, int nlat) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nlat + 1) / 2) return;
+    double x = cos(M_PI * (i + 0.75) / (nlat + 0.5));
+    double p0, p1, dp;
+    for (int it = 0; it < 100; it++) {
+        p0 = 1.0;
+        p1 = x;
+        for (int k = 2; k <= nlat; k++) {
+            double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = nlat * (x * p1 - p0) / (x * x - 1.0);
+        double dx = p1 / dp;
+        x -= dx;
+        if (fabs(dx) < 1e-16) break;
+    }
+    p0 = 1.0;
+    p1 = x;
+    for (int k = 2; k <= nlat; k++) {
+        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    dp = nlat * (x * p1 - p0) / (x * x - 1.0);
+    mu[i] = x;
+    mu[nlat - 1 - i] = -x;
+    w[i] = w[nlat - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
+}
+
+__device__ inline double eps_nm_d(int n, int m) {
+    if (n <= m) return 0.0;
+    return sqrt(((double)n * n - (double)m * m) / (4.0 * n * n - 1.0));
+}
+
+// Pbar_n^m(mu_j), n = m..T+1, FP64 three-term recurrence in extended exponent
+// (2^500 steps) so high orders near the poles neither underflow early nor
+// lose the values that grow back; stored in the parity-split table layout.
+template <typename T>
+__global__ void k_legendre_table(T *plm, const long *poff, const double *mu, int nhalf, int nhp,
+                                 int ntrunc) {
+    int m = blockIdx.y;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nhp) return;
+    int K = ntrunc + 2 - m, Ke = (K + 1) >> 1;
+    T *pe = plm + poff[m], *po = pe + (long)Ke * nhp;
+    if (j >= nhalf) {
+        for (int i = 0; i < K; i++) ((i & 1) ? po : pe)[(long)(i >> 1) * nhp + j] = T(0);
+        return;
+    }
+    const double BIG = 0x1p500, SMALL = 0x1p-500;
+    double x = mu[j];
+    double s = sqrt((1.0 - x) * (1.0 + x));
+    double p = sqrt(1.0 / (4.0 * M_PI));
+    int e = 0;
+    for (int k = 1; k <= m; k++) {
+        p *= sqrt((2.0 * k + 1.0) / (2.0 * k)) * s;
+        if (p != 0.0 && fabs(p) < SMALL) {
+            p *= BIG;
+            e--;
+        }
+    }
+    double pm2 = 0.0, pm1 = p;
+    for (int i = 0; i < K; i++) {
+        int n = m + i;
+        double val;
+        if (i == 0) {
+            val = pm1;
+        } else {
+            val = (x * pm1 - eps_nm_d(n - 1, m) * pm2) / eps_nm_d(n, m);
+            pm2 = pm1;
+            pm1 = val;
+            if (e < 0 && fabs(val) > BIG) {
+                pm1 *= SMALL;
+                pm2 *= SMALL;
+                val *= SMALL;
+                e++;
+            }
+        }
+        double out = (e == 0) ? val : (e == -1 ? val * SMALL : 0.0);
+        ((i & 1) ? po : pe)[(long)(i >> 1) * nhp + j] = (T)out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Legendre synthesis GEMM. One workgroup = one order m, 64 latitude rows, 128
+// columns; 4 waves in a 2x2 arrangement, each wave 32 x 64 via 2 x 4 MFMA
+// 16x16x4 blocks for the even and for the odd degrees.
+// fs[m][j][c] = E + O (north row j), fs[m][nlat-1-j][c] = E - O (south row).
+constexpr int LDS_PAD = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, const long *__restrict__ poff,
+                                                   const int *__restrict__ keoff, const T *__restrict__ sx,
+                                                   int ldx, T *__restrict__ fs, int ldf, int nhalf, int nhp,
+                                                   int nlat, int ntrunc, int ntj) {
+    typedef typename MF<T>::acc acc_t;
+    const int tc = blockIdx.x;
+    const int mt = blockIdx.y;
+    const int m = mt / ntj, tj = mt - m * ntj;
+    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
+    const T *Pe = plm + poff[m];
+    const T *Po = Pe + (long)Ke * nhp;
+    const T *S = sx + (long)keoff[m] * ldx;
+    const int j0 = tj * GEMM_BM, c0 = tc * GEMM_BN;
+    __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
+    __shared__ T Bs[2][GEMM_BK][GEMM_BN + LDS_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    acc_t E[2][4], O[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            E[a][b] = acc_t{0, 0, 0, 0};
+            O[a][b] = acc_t{0, 0, 0, 0};
+        }
+    for (int k0 = 0; k0 < Ke; k0 += GEMM_BK) {
+#pragma unroll
+        for (int t = tid; t < GEMM_BK * GEMM_BM; t += 256) {
+            int kk = t / GEMM_BM, jj = t - kk * GEMM_BM, i = k0 + kk;
+            As[0][kk][jj] = (i < Ke) ? Pe[(long)i * nhp + j0 + jj] : T(0);
+            As[1][kk][jj] = (i < Ko) ? Po[(long)i * nhp + j0 + jj] : T(0);
+        }
+#pragma unroll
+        for (int t = tid; t < GEMM_BK * GEMM_BN; t += 256) {
+            int kk = t / GEMM_BN, cc = t - kk * GEMM_BN, i = k0 + kk;
+            Bs[0][kk][cc] = (i < Ke) ? S[(long)(2 * i) * ldx + c0 + cc] : T(0);
+            Bs[1][kk][cc] = (i < Ko) ? S[(long)(2 * i + 1) * ldx + c0 + cc] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GEMM_BK; ks += 4) {
+            const int kr = ks + (lane >> 4), li = lane & 15;
+            T ae[2], ao[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) {
+                ae[bi] = As[0][kr][wm + bi * 16 + li];
+                ao[bi] = As[1][kr][wm + bi * 16 + li];
+            }
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) {
+                T be = Bs[0][kr][wn + bj * 16 + li];
+                T bo = Bs[1][kr][wn + bj * 16 + li];
+#pragma unroll
+                for (int bi = 0; bi < 2; bi++) {
+                    E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
+                    O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int j = j0 + wm + bi * 16 + MF<T>::row(lane, r);
+                int c = c0 + wn + bj * 16 + (lane & 15);
+                if (j < nhalf) {
+                    T e = E[bi][bj][r], o = O[bi][bj][r];
+                    fs[((long)m * nlat + j) * ldf + c] = e + o;
+                    fs[((long)m * nlat + (nlat - 1 - j)) * ldf + c] = e - o;
+                }
+            }
+}
+
+// Legendre analysis GEMM. One workgroup = one order m, 64 output degrees of
+// each parity, 128 columns; K runs over the (zero-padded) latitude pairs.
+// fa[m][0][j][c] holds w_j (F_north + F_south), fa[m][1][j][c] w_j (F_north - F_south).
+template <typename T>
+__global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, const long *__restrict__ poff,
+                                                  const int *__restrict__ keoff, const int *__restrict__ mtile_m,
+                                                  const int *__restrict__ mtile_i, const T *__restrict__ fa,
+                                                  int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc) {
+    typedef typename MF<T>::acc acc_t;
+    const int tc = blockIdx.x;
+    const int m = mtile_m[blockIdx.y], i0 = mtile_i[blockIdx.y];
+    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
+    const T *Pe = plm + poff[m];
+    const T *Po = Pe + (long)Ke * nhp;
+    const T *Fe = fa + (long)(2 * m) * nhp * ldf;
+    const T *Fo = Fe + (long)nhp * ldf;
+    const int c0 = tc * GEMM_BN;
+    __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
+    __shared__ T Bs[2][GEMM_BK][GEMM_BN + LDS_PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    acc_t E[2][4], O[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            E[a][b] = acc_t{0, 0, 0, 0};
+            O[a][b] = acc_t{0, 0, 0, 0};
+        }
+    for (int j0 = 0; j0 < nhp; j0 += GEMM_BK) {
+#pragma unroll
+        for (int t = tid; t < GEMM_BK * GEMM_BM; t += 256) {
+            int ii = t / GEMM_BK, kk = t - ii * GEMM_BK, i = i0 + ii;
+            As[0][kk][ii] = (i < Ke) ? Pe[(long)i * nhp + j0 + kk] : T(0);
+            As[1][kk][ii] = (i < Ko) ? Po[(long)i * nhp + j0 + kk] : T(0);
+        }
+#pragma unroll
+        for (int t = tid; t < GEMM_BK * GEMM_BN; t += 256) {
+            int kk = t / GEMM_BN, cc = t - kk * GEMM_BN;
+            Bs[0][kk][cc] = Fe[(long)(j0 + kk) * ldf + c0 + cc];
+            Bs[1][kk][cc] = Fo[(long)(j0 + kk) * ldf + c0 + cc];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GEMM_BK; ks += 4) {
+            const int kr = ks + (lane >> 4), li = lane & 15;
+            T ae[2], ao[2];
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) {
+                ae[bi] = As[0][kr][wm + bi * 16 + li];
+                ao[bi] = As[1][kr][wm + bi * 16 + li];
+            }
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) {
+                T be = Bs[0][kr][wn + bj * 16 + li];
+                T bo = Bs[1][kr][wn + bj * 16 + li];
+#pragma unroll
+                for (int bi = 0; bi < 2; bi++) {
+                    E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
+                    O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    T *X = sx + (long)keoff[m] * ldx;
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int i = i0 + wm + bi * 16 + MF<T>::row(lane, r);
+                int c = c0 + wn + bj * 16 + (lane & 15);
+                if (i < Ke) X[(long)(2 * i) * ldx + c] = E[bi][bj][r];
+                if (i < Ko) X[(long)(2 * i + 1) * ldx + c] = O[bi][bj][r];
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Stockham FFT in LDS: NFFT transforms of length N (radices 2, 3, 4), in place
+// with a barrier between the load and store half of every stage.
+template <typename T> __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return cplx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+template <typename T, int R> __device__ inline void small_dft(cplx<T> *v, int sign) {
+    if (R == 2) {
+        cplx<T> a = v[0], b = v[1];
+        v[0] = cplx<T>{a.x + b.x, a.y + b.y};
+        v[1] = cplx<T>{a.x - b.x, a.y - b.y};
+    } else if (R == 3) {
+        const T s3 = T(0.86602540378443864676) * T(sign);
+        cplx<T> a0 = v[0], a1 = v[1], a2 = v[2];
+        cplx<T> t1{a1.x + a2.x, a1.y + a2.y}, t2{a1.x - a2.x, a1.y - a2.y};
+        cplx<T> b{a0.x - T(0.5) * t1.x, a0.y - T(0.5) * t1.y};
+        v[0] = cplx<T>{a0.x + t1.x, a0.y + t1.y};
+        v[1] = cplx<T>{b.x - s3 * t2.y, b.y + s3 * t2.x};
+        v[2] = cplx<T>{b.x + s3 * t2.y, b.y - s3 * t2.x};
+    } else if (R == 4) {
+        cplx<T> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        cplx<T> s02{a0.x + a2.x, a0.y + a2.y}, d02{a0.x - a2.x, a0.y - a2.y};
+        cplx<T> s13{a1.x + a3.x, a1.y + a3.y}, d13{a1.x - a3.x, a1.y - a3.y};
+        // w * d13 with w = sign * i
+        cplx<T> wd{-T(sign) * d13.y, T(sign) * d13.x};
+        v[0] = cplx<T>{s02.x + s13.x, s02.y + s13.y};
+        v[2] = cplx<T>{s02.x - s13.x, s02.y - s13.y};
+        v[1] = cplx<T>{d02.x + wd.x, d02.y + wd.y};
+        v[3] = cplx<T>{d02.x - wd.x, d02.y - wd.y};
+    }
+}
+
+template <typename T, int R, int MAXB>
+__device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cplx<T> *__restrict__ tw,
+                                 int sign) {
+    const int nbf = N / R, total = nbf * nfft;
+    const int tstep = N / (Ns * R);
+    cplx<T> v[MAXB][R];
+#pragma unroll
+    for (int bb = 0; bb < MAXB; bb++) {
+        int b = threadIdx.x + bb * 256;
+        if (b < total) {
+            int q = b / nbf, j = b - q * nbf;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[bb][r] = buf[q * N + j + r * nbf];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bb = 0; bb < MAXB; bb++) {
+        int b = threadIdx.x + bb * 256;
+        if (b < total) {
+            int q = b / nbf, j = b - q * nbf;
+            int jm = j % Ns;
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                cplx<T> t = tw[jm * r * tstep];
+                if (sign > 0) t.y = -t.y;
+                v[bb][r] = cmul(v[bb][r], t);
+            }
+            small_dft<T, R>(v[bb], sign);
+            int idx = (j / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[q * N + idx + r * Ns] = v[bb][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int NFFT, int MAXN>
+__device__ inline void fft_run(cplx<T> *buf, int N, const int *rad, int nrad, const cplx<T> *tw, int sign) {
+    int Ns = 1;
+    for (int s = 0; s < nrad; s++) {
+        int R = rad[s];
+        if (R == 4)
+            fft_stage<T, 4, (NFFT * MAXN / 4 + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
+        else if (R == 3)
+            fft_stage<T, 3, (NFFT * MAXN / 3 + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
+        else
+            fft_stage<T, 2, (NFFT * MAXN / 2 + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
+        Ns *= R;
+    }
+}
+
+struct Radices {
+    int n, r[16];
+};
+
+template <typename T> struct FftCfg;
+template <> struct FftCfg<float> { static constexpr int NFFT = 8, MAXN = 768; };
+template <> struct FftCfg<double> { static constexpr int NFFT = 4, MAXN = 768; };
+
+// Fourier synthesis: one workgroup per latitude pair and NFFT fields.
+template <typename T, int NFFT, int MAXN>
+__global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int ldf, T *__restrict__ grid,
+                                                   long gstride, int nf, const unsigned char *__restrict__ rflag,
+                                                   const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
+                                                   int nlon, int nlat, int ntrunc, Radices rd) {
+    __shared__ cplx<T> buf[NFFT * MAXN];
+    const int jp = blockIdx.x, f0 = blockIdx.y * NFFT;
+    const int j = jp, j2 = nlat - 1 - jp, N = nlon, tid = threadIdx.x;
+    for (int t = tid; t < NFFT * N; t += 256) buf[t] = cplx<T>{T(0), T(0)};
+    __syncthreads();
+    for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
+        int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
+        if (f >= nf) continue;
+        const T *rn = fs + ((long)m * nlat + j) * ldf + 2 * f;
+        const T *rs = fs + ((long)m * nlat + j2) * ldf + 2 * f;
+        T xr = rn[0], xi = rn[1], yr = rs[0], yi = rs[1];
+        if (m == 0) {
+            xi = T(0);
+            yi = T(0);
+        }
+        buf[q * N + m] = cplx<T>{xr - yi, xi + yr};
+        if (m > 0) buf[q * N + N - m] = cplx<T>{xr + yi, -xi + yr};
+    }
+    __syncthreads();
+    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tw, +1);
+    const T rc = rcos[j];
+    for (int t = tid; t < NFFT * N; t += 256) {
+        int q = t / N, i = t - q * N, f = f0 + q;
+        if (f >= nf) continue;
+        T s = rflag[f] ? rc : T(1);
+        grid[f * gstride + (long)j * nlon + i] = buf[t].x * s;
+        grid[f * gstride + (long)j2 * nlon + i] = buf[t].y * s;
+    }
+}
+
+// Fourier analysis: rows of a latitude pair -> weighted symmetric / antisymmetric
+// Fourier coefficients, m = 0..T.
+template <typename T, int NFFT, int MAXN>
+__global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, long gstride, int nf,
+                                                  const unsigned char *__restrict__ rflag,
+                                                  const T *__restrict__ rcos, const T *__restrict__ wfac,
+                                                  const cplx<T> *__restrict__ tw, T *__restrict__ fa, int ldf,
+                                                  int nlon, int nlat, int nhp, int ntrunc, Radices rd) {
+    __shared__ cplx<T> buf[NFFT * MAXN];
+    const int jp = blockIdx.x, f0 = blockIdx.y * NFFT;
+    const int j = jp, j2 = nlat - 1 - jp, N = nlon, tid = threadIdx.x;
+    for (int t = tid; t < NFFT * N; t += 256) {
+        int q = t / N, i = t - q * N, f = f0 + q;
+        T x = T(0), y = T(0);
+        if (f < nf) {
+            x = grid[f * gstride + (long)j * nlon + i];
+            y = grid[f * gstride + (long)j2 * nlon + i];
+        }
+        buf[t] = cplx<T>{x, y};
+    }
+    __syncthreads();
+    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tw, -1);
+    const T wj = wfac[j], rc = rcos[j];
+    for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
+        int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
+        if (f >= nf) continue;
+        cplx<T> zk = buf[q * N + m], zn = buf[q * N + (m == 0 ? 0 : N - m)];
+        T xr = T(0.5) * (zk.x + zn.x), xi = T(0.5) * (zk.y - zn.y);
+        T yr = T(0.5) * (zk.y + zn.y), yi = T(-0.5) * (zk.x - zn.x);
+        T wf = rflag[f] ? wj * rc : wj;
+        T *pe = fa + ((long)(2 * m) * nhp + jp) * ldf + 2 * f;
+        T *po = fa + ((long)(2 * m + 1) * nhp + jp) * ldf + 2 * f;
+        pe[0] = wf * (xr + yr);
+        pe[1] = wf * (xi + yi);
+        po[0] = wf * (xr - yr);
+        po[1] = wf * (xi - yi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather into extended spectral columns and scatter out of them.
+template <typename T>
+__device__ inline cplx<T> src_coef(const cplx<T> *__restrict__ src, long sf, long sk, int f, const int *koff,
+                                   int m, int n, int ntrunc, double sc, int inv) {
+    if (f < 0 || n < m || n > ntrunc) return cplx<T>{T(0), T(0)};
+    cplx<T> v = src[f * sf + (long)(koff[m] + n - m) * sk];
+    T s = T(sc);
+    if (inv) s *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
+    return cplx<T>{v.x * s, v.y * s};
+}
+
+template <typename T>
+__global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk, const SynthDesc *__restrict__ desc,
+                               int nfo, T *__restrict__ sx, int ldx, const int2 *__restrict__ kemn,
+                               const int *__restrict__ koff, const T *__restrict__ eps, const int *__restrict__ eoff,
+                               int ntrunc) {
+    const int ke = blockIdx.y;
+    const int fo = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fo >= nfo) return;
+    const int2 mn = kemn[ke];
+    const int m = mn.x, n = mn.y;
+    const SynthDesc d = desc[fo];
+    cplx<T> r{T(0), T(0)};
+    if (d.kind == SK_COPY) {
+        r = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
+    } else if (d.kind == SK_UHAT || d.kind == SK_VHAT) {
+        const T *ep = eps + eoff[m];
+        T en = ep[n - m], en1 = ep[n + 1 - m];
+        T cm1 = T(n - 1) * en, cp1 = T(n + 2) * en1, fm = T(m);
+        if (d.kind == SK_UHAT) {
+            // U = (n-1) eps_n S_{n-1} - (n+2) eps_{n+1} S_{n+1} + i m T_n
+            cplx<T> sm = src_coef(src, sf, sk, d.a, koff, m, n - 1, ntrunc, d.sa, d.inv_a);
+            cplx<T> sp = src_coef(src, sf, sk, d.a, koff, m, n + 1, ntrunc, d.sa, d.inv_a);
+            cplx<T> t0 = src_coef(src, sf, sk, d.b, koff, m, n, ntrunc, d.sb, d.inv_b);
+            r.x = cm1 * sm.x - cp1 * sp.x - fm * t0.y;
+            r.y = cm1 * sm.y - cp1 * sp.y + fm * t0.x;
+        } else {
+            // V = i m S_n - (n-1) eps_n T_{n-1} + (n+2) eps_{n+1} T_{n+1}
+            cplx<T> s0 = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
+            cplx<T> tm = src_coef(src, sf, sk, d.b, koff, m, n - 1, ntrunc, d.sb, d.inv_b);
+            cplx<T> tp = src_coef(src, sf, sk, d.b, koff, m, n + 1, ntrunc, d.sb, d.inv_b);
+            r.x = -fm * s0.y - cm1 * tm.x + cp1 * tp.x;
+            r.y = fm * s0.x - cm1 * tm.y + cp1 * tp.y;
+        }
+    }
+    T *o = sx + (long)ke * ldx + 2 * fo;
+    o[0] = r.x;
+    o[1] = r.y;
+}
+
+template <typename T>
+__global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc *__restrict__ desc, int nfo,
+                               cplx<T> *__restrict__ dst, long df, long dk, const int2 *__restrict__ kmn,
+                               const int *__restrict__ keoff, const T *__restrict__ eps,
+                               const int *__restrict__ eoff, T rrs) {
+    const int k = blockIdx.y;
+    const int fo = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fo >= nfo) return;
+    const int2 mn = kmn[k];
+    const int m = mn.x, n = mn.y;
+    const AnalDesc d = desc[fo];
+    const T *X = sx + (long)(keoff[m] + n - m) * ldx;
+    cplx<T> r;
+    if (d.kind == AK_COPY) {
+        r = cplx<T>{X[2 * d.a], X[2 * d.a + 1]};
+    } else {
+        const T *ep = eps + eoff[m];
+        T en = ep[n - m], en1 = ep[n + 1 - m];
+        T cp = T(n) * en1, cm = T(n + 1) * en, fm = T(m);
+        int ca = (d.kind == AK_VRT) ? d.b : d.a;  // i m (V for vrt, U for div)
+        int cb = (d.kind == AK_VRT) ? d.a : d.b;  // recurrence partner (U for vrt, V for div)
+        T sgn = (d.kind == AK_VRT) ? T(-1) : T(1);
+        cplx<T> z0{X[2 * ca], X[2 * ca + 1]};
+        cplx<T> zp{X[ldx + 2 * cb], X[ldx + 2 * cb + 1]};
+        cplx<T> zm{T(0), T(0)};
+        if (n > m) zm = cplx<T>{X[-ldx + 2 * cb], X[-ldx + 2 * cb + 1]};
+        // vrt = (i m V_n - n eps_{n+1} U_{n+1} + (n+1) eps_n U_{n-1}) / a
+        // div = (i m U_n + n eps_{n+1} V_{n+1} - (n+1) eps_n V_{n-1}) / a
+        r.x = (-fm * z0.y + sgn * (cp * zp.x - cm * zm.x)) * rrs;
+        r.y = (fm * z0.x + sgn * (cp * zp.y - cm * zm.y)) * rrs;
+    }
+    if (d.c >= 0) {
+        T q = T(d.sc) * T(n) * T(n + 1);
+        r.x += q * X[2 * d.c];
+        r.y += q * X[2 * d.c + 1];
+    }
+    T s = T(d.scale);
+    if (d.inv) s *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
+    cplx<T> *o = dst + fo * df + (long)k * dk;
+    if (d.accum) {
+        o->x += s * r.x;
+        o->y += s * r.y;
+    } else {
+        *o = cplx<T>{s * r.x, s * r.y};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plan construction
+
+static Radices factor_nlon(int n) {
+    Radices r{};
+    r.n = 0;
+    while (n % 4 == 0 && n > 4) {
+        r.r[r.n++] = 4;
+        n /= 4;
+    }
+    while (n % 3 == 0) {
+        r.r[r.n++] = 3;
+        n /= 3;
+    }
+    while (n % 4 == 0) {
+        r.r[r.n++] = 4;
+        n /= 4;
+    }
+    while (n % 2 == 0) {
+        r.r[r.n++] = 2;
+        n /= 2;
+    }
+    if (n != 1) r.n = -1;
+    return r;
+}
+
+template <typename T> static int upload(T **d, const void *h, size_t bytes, hipStream_t s) {
+    GFSDC_CHECK(hipMalloc((void **)d, bytes > 0 ? bytes : 1));
+    if (bytes) GFSDC_CHECK(hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+template <typename T> int plan_create(Plan<T> &p, int nlon, int nlat, int ntrunc, hipStream_t s) {
+    if (nlon <= 0 || nlat <= 1 || ntrunc < 1 || nlat % 2 || 2 * ntrunc >= nlon || nlat < ntrunc + 1) {
+        set_error("plan: need even nlat >= ntrunc+1 and nlon > 2*ntrunc");
+        return GFSDC_EINVAL;
+    }
+    Radices rd = factor_nlon(nlon);
+    if (rd.n <= 0 || nlon > FftCfg<T>::MAXN) {
+        set_error("plan: nlon must be 2^a 3^b and <= 768");
+        return GFSDC_EINVAL;
+    }
+    p.nlon = nlon;
+    p.nlat = nlat;
+    p.nhalf = nlat / 2;
+    p.nhp = round_up(p.nhalf, GEMM_BM);
+    p.ntrunc = ntrunc;
+    p.nlm = (ntrunc + 1) * (ntrunc + 2) / 2;
+    p.nlme = p.nlm + ntrunc + 1;
+    p.nrad = rd.n;
+    for (int i = 0; i < rd.n; i++) p.rad[i] = rd.r[i];
+    const int T_ = ntrunc;
+    // offsets
+    p.poff_h.resize(T_ + 2);
+    p.keoff_h.resize(T_ + 2);
+    p.koff_h.resize(T_ + 2);
+    std::vector<int> eoff_h(T_ + 2);
+    long po = 0;
+    int ke = 0, k = 0, eo = 0;
+    for (int m = 0; m <= T_ + 1; m++) {
+        p.poff_h[m] = po;
+        p.keoff_h[m] = ke;
+        p.koff_h[m] = k;
+        eoff_h[m] = eo;
+        if (m <= T_) {
+            po += (long)(T_ + 2 - m) * p.nhp;
+            ke += T_ + 2 - m;
+            k += T_ + 1 - m;
+            eo += T_ + 3 - m;
+        }
+    }
+    std::vector<T> epsv(eo);
+    for (int m = 0; m <= T_; m++)
+        for (int n = m; n <= T_ + 2; n++) {
+            double e = (n <= m) ? 0.0 : std::sqrt(((double)n * n - (double)m * m) / (4.0 * n * n - 1.0));
+            epsv[eoff_h[m] + n - m] = (T)e;
+        }
+    std::vector<int2> kemn(p.nlme), kmn(p.nlm);
+    ke = 0;
+    k = 0;
+    for (int m = 0; m <= T_; m++)
+        for (int n = m; n <= T_ + 1; n++) {
+            kemn[ke++] = int2{m, n};
+            if (n <= T_) kmn[k++] = int2{m, n};
+        }
+    // analysis m-tiles: rows [i0, i0+BM) of the even block of order m
+    std::vector<int> mt_m, mt_i;
+    for (int m = 0; m <= T_; m++) {
+        int Ke = (T_ + 3 - m) / 2;
+        for (int i0 = 0; i0 < Ke; i0 += GEMM_BM) {
+            mt_m.push_back(m);
+            mt_i.push_back(i0);
+        }
+    }
+    p.n_mtiles = (int)mt_m.size();
+    int rc;
+    if ((rc = upload(&p.poff, p.poff_h.data(), sizeof(long) * (T_ + 2), s))) return rc;
+    if ((rc = upload(&p.keoff, p.keoff_h.data(), sizeof(int) * (T_ + 2), s))) return rc;
+    if ((rc = upload(&p.koff, p.koff_h.data(), sizeof(int) * (T_ + 2), s))) return rc;
+    if ((rc = upload(&p.eoff, eoff_h.data(), sizeof(int) * (T_ + 2), s))) return rc;
+    if ((rc = upload(&p.eps, epsv.data(), sizeof(T) * eo, s))) return rc;
+    if ((rc = upload(&p.mtile_m, mt_m.data(), sizeof(int) * mt_m.size(), s))) return rc;
+    if ((rc = upload(&p.mtile_i, mt_i.data(), sizeof(int) * mt_i.size(), s))) return rc;
+    if ((rc = upload(&p.kemn, kemn.data(), sizeof(int2) * kemn.size(), s))) return rc;
+    if ((rc = upload(&p.kmn, kmn.data(), sizeof(int2) * kmn.size(), s))) return rc;
+    // Gauss nodes and Legendre tables in FP64 on the device
+    GFSDC_CHECK(hipMalloc((void **)&p.mu_d, sizeof(double) * nlat));
+    GFSDC_CHECK(hipMalloc((void **)&p.w_d, sizeof(double) * nlat));
+    hipLaunchKernelGGL(k_gauss, dim3((nlat / 2 + 63) / 64), dim3(64), 0, s, p.mu_d, p.w_d, nlat);
+    GFSDC_CHECK(hipGetLastError());
+    GFSDC_CHECK(hipMalloc((void **)&p.plm, sizeof(T) * (size_t)po));
+    hipLaunchKernelGGL(k_legendre_table<T>, dim3(p.nhp / 64, T_ + 1), dim3(64), 0, s, p.plm, p.poff, p.mu_d,
+                       p.nhalf, p.nhp, T_);
+    GFSDC_CHECK(hipGetLastError());
+    p.mu.resize(nlat);
+    p.w.resize(nlat);
+    GFSDC_CHECK(hipMemcpyAsync(p.mu.data(), p.mu_d, sizeof(double) * nlat, hipMemcpyDeviceToHost, s));
+    GFSDC_CHECK(hipMemcpyAsync(p.w.data(), p.w_d, sizeof(double) * nlat, hipMemcpyDeviceToHost, s));
+    GFSDC_CHECK(hipStreamSynchronize(s));
+    std::vector<T> rcos(nlat), wfac(nlat), tw(2 * nlon);
+    for (int j = 0; j < nlat; j++) {
+        rcos[j] = (T)(1.0 / std::sqrt((1.0 - p.mu[j]) * (1.0 + p.mu[j])));
+        wfac[j] = (T)(2.0 * M_PI * p.w[j] / nlon);
+    }
+    for (int t = 0; t < nlon; t++) {
+        tw[2 * t] = (T)std::cos(2.0 * M_PI * t / nlon);
+        tw[2 * t + 1] = (T)(-std::sin(2.0 * M_PI * t / nlon));
+    }
+    if ((rc = upload(&p.rcos, rcos.data(), sizeof(T) * nlat, s))) return rc;
+    if ((rc = upload(&p.wfac, wfac.data(), sizeof(T) * nlat, s))) return rc;
+    if ((rc = upload(&p.tw, tw.data(), sizeof(T) * 2 * nlon, s))) return rc;
+    GFSDC_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+template <typename T> void plan_destroy(Plan<T> &p) {
+    void *ptrs[] = {p.plm, p.poff, p.keoff, p.koff, p.eps, p.eoff, p.rcos, p.wfac,
+                    p.tw, p.mtile_m, p.mtile_i, p.mu_d, p.w_d, p.kemn, p.kmn};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p = Plan<T>();
+}
+
+template <typename T> int work_alloc(Work<T> &w, const Plan<T> &p, int nfield) {
+    if (nfield <= w.ncap) return 0;
+    work_free(w);
+    w.ncap = nfield;
+    w.ldc = round_up(2 * nfield, COL_PAD);
+    size_t nsx = (size_t)p.nlme * w.ldc, nfs = (size_t)(p.ntrunc + 1) * p.nlat * w.ldc;
+    size_t nfa = (size_t)(p.ntrunc + 1) * 2 * p.nhp * w.ldc, ng = (size_t)nfield * p.nlat * p.nlon;
+    GFSDC_CHECK(hipMalloc((void **)&w.sx, sizeof(T) * nsx));
+    GFSDC_CHECK(hipMalloc((void **)&w.fs, sizeof(T) * nfs));
+    GFSDC_CHECK(hipMalloc((void **)&w.fa, sizeof(T) * nfa));
+    GFSDC_CHECK(hipMalloc((void **)&w.grid, sizeof(T) * ng));
+    GFSDC_CHECK(hipMemset(w.sx, 0, sizeof(T) * nsx));
+    GFSDC_CHECK(hipMemset(w.fs, 0, sizeof(T) * nfs));
+    GFSDC_CHECK(hipMemset(w.fa, 0, sizeof(T) * nfa));
+    return 0;
+}
+
+template <typename T> void work_free(Work<T> &w) {
+    if (w.sx) (void)hipFree(w.sx);
+    if (w.fs) (void)hipFree(w.fs);
+    if (w.fa) (void)hipFree(w.fa);
+    if (w.grid) (void)hipFree(w.grid);
+    w = Work<T>();
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+template <typename T>
+void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc, int nfo,
+                         T *sx, int ldx, double rsphere, hipStream_t s) {
+    (void)rsphere;
+    dim3 grid((nfo + 63) / 64, p.nlme);
+    hipLaunchKernelGGL(k_synth_gather<T>, grid, dim3(64), 0, s, src, sf, sk, desc, nfo, sx, ldx, p.kemn, p.koff,
+                       p.eps, p.eoff, p.ntrunc);
+}
+
+template <typename T>
+void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s) {
+    int ntj = p.nhp / GEMM_BM, ntc = (ncol + GEMM_BN - 1) / GEMM_BN;
+    hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc, (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff, p.keoff,
+                       sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj);
+}
+
+template <typename T>
+void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s) {
+    int ntc = (ncol + GEMM_BN - 1) / GEMM_BN;
+    hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc, p.n_mtiles), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
+                       p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc);
+}
+
+static Radices radices_of(int n, const int *r) {
+    Radices rd{};
+    rd.n = n;
+    for (int i = 0; i < n; i++) rd.r[i] = r[i];
+    return rd;
+}
+
+template <typename T>
+void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
+                      const unsigned char *rflag, hipStream_t s) {
+    constexpr int NF = FftCfg<T>::NFFT, MN = FftCfg<T>::MAXN;
+    dim3 g(p.nhalf, (nf + NF - 1) / NF);
+    hipLaunchKernelGGL((k_fft_synth<T, NF, MN>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, p.rcos,
+                       (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad));
+}
+
+template <typename T>
+void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
+                     int ldf, hipStream_t s) {
+    constexpr int NF = FftCfg<T>::NFFT, MN = FftCfg<T>::MAXN;
+    dim3 g(p.nhalf, (nf + NF - 1) / NF);
+    hipLaunchKernelGGL((k_fft_anal<T, NF, MN>), g, dim3(256), 0, s, grid, gstride, nf, rflag, p.rcos, p.wfac,
+                       (const cplx<T> *)p.tw, fa, ldf, p.nlon, p.nlat, p.nhp, p.ntrunc,
+                       radices_of(p.nrad, p.rad));
+}
+
+template <typename T>
+void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc *desc, int nfo, cplx<T> *dst,
+                         long df, long dk, double rsphere, hipStream_t s) {
+    dim3 grid((nfo + 63) / 64, p.nlm);
+    hipLaunchKernelGGL(k_anal_scatter<T>, grid, dim3(64), 0, s, sx, ldx, desc, nfo, dst, df, dk, p.kmn, p.keoff,
+                       p.eps, p.eoff, (T)(1.0 / rsphere));
+}
+
+#define INST(T)                                                                                                \
+    template int plan_create<T>(Plan<T> &, int, int, int, hipStream_t);                                        \
+    template void plan_destroy<T>(Plan<T> &);                                                                  \
+    template int work_alloc<T>(Work<T> &, const Plan<T> &, int);                                               \
+    template void work_free<T>(Work<T> &);                                                                     \
+    template void launch_synth_gather<T>(const Plan<T> &, const cplx<T> *, long, long, const SynthDesc *, int,  \
+                                         T *, int, double, hipStream_t);                                       \
+    template void launch_leg_synth<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);            \
+    template void launch_leg_anal<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);             \
+    template void launch_fft_synth<T>(const Plan<T> &, const T *, int, T *, long, int, const unsigned char *,  \
+                                      hipStream_t);                                                            \
+    template void launch_fft_anal<T>(const Plan<T> &, const T *, long, int, const unsigned char *, T *, int,   \
+                                     hipStream_t);                                                             \
+    template void launch_anal_scatter<T>(const Plan<T> &, const T *, int, const AnalDesc *, int, cplx<T> *,    \
+                                         long, long, double, hipStream_t);
+INST(float)
+INST(double)
+
+}  // namespace gfsdc

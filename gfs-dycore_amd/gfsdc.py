@@ -1,0 +1,309 @@
+"""Python host binding of libgfsdc (ctypes over the C ABI in include/gfsdc.h).
+
+Mirrors the public interface of the reference's transform module
+(src/shtns.f90: spectogrd, grdtospec, getuv, getvrtdivspec, getgrad and the
+module data lats/gauwts/areawts/degree/order/lap/invlap) and of the dynamics
+call sequence (src/dyn_run.f90 getdyntend, src/run.f90 advance) so tests read
+like the reference's own usage. Arrays are numpy, in the reference layout:
+spectral complex (nfield, ndimspec), grid real (nfield, nlat, nlon).
+
+The HIP library is required: importing this module raises if libgfsdc.so is
+missing, and every call raises GfsdcError on a non-zero status.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libgfsdc.so")
+
+FP64, FP32 = 0, 1
+HOST, DEVICE = 0, 1
+DT_F64, DT_F32 = 0, 1
+PHYS_NONE, PHYS_HS = 0, 1
+
+
+class GfsdcError(RuntimeError):
+    pass
+
+
+class Cfg(C.Structure):
+    _fields_ = [
+        ("nlon", C.c_int), ("nlat", C.c_int), ("ntrunc", C.c_int), ("nlev", C.c_int),
+        ("ntrac", C.c_int), ("nmem", C.c_int), ("precision", C.c_int), ("dt", C.c_double),
+        ("ak", C.POINTER(C.c_double)), ("bk", C.POINTER(C.c_double)),
+        ("explicit_", C.c_int), ("physics", C.c_int), ("massfix", C.c_int), ("ndiss", C.c_int),
+        ("efold", C.c_double), ("hdif_fac", C.c_double), ("hdif_fac2", C.c_double),
+        ("fshk", C.c_double), ("taustratdamp", C.c_double),
+        ("rerth", C.c_double), ("grav", C.c_double), ("omega", C.c_double), ("rd", C.c_double),
+        ("rv", C.c_double), ("cp", C.c_double), ("cvap", C.c_double), ("device", C.c_int),
+    ]
+
+
+SYMBOLS = [
+    "shtns_use_threads_", "shtns_set_size_", "shtns_precompute_", "shtns_calc_nlm_",
+    "shtns_cos_array_", "shtns_gauss_wts_", "shtns_spat_to_sh_", "shtns_sh_to_spat_",
+    "shtns_sphtor_to_spat_", "shtns_spat_to_sphtor_", "shtns_reset_",
+    "gfsdc_cfg_defaults", "gfsdc_create", "gfsdc_destroy", "gfsdc_last_error", "gfsdc_nlm",
+    "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_grdtospec", "gfsdc_getuv",
+    "gfsdc_getvrtdivspec", "gfsdc_getgrad", "gfsdc_set_state", "gfsdc_get_state",
+    "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
+    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_get_grid", "gfsdc_get_semimp",
+    "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_sync",
+    "gfsdc_stream", "gfsdc_profile_step",
+]
+
+_LIB = None
+
+
+def load():
+    """Load libgfsdc.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise GfsdcError(f"{LIBPATH} is missing: build it with `make -C {_HERE}` "
+                         "(or __graft_entry__.build())")
+    L = C.CDLL(LIBPATH)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    L.gfsdc_last_error.restype = C.c_char_p
+    L.gfsdc_cfg_defaults.argtypes = [C.POINTER(Cfg)]
+    L.gfsdc_create.argtypes = [C.POINTER(Cfg), C.POINTER(vp)]
+    L.gfsdc_destroy.argtypes = [vp]
+    L.gfsdc_nlm.argtypes = [vp]
+    L.gfsdc_griddata.argtypes = [vp] + [vp] * 7
+    for n in ("gfsdc_spectogrd", "gfsdc_grdtospec"):
+        getattr(L, n).argtypes = [vp, vp, vp, i, i, i]
+    L.gfsdc_getuv.argtypes = [vp, vp, vp, vp, vp, i, d, i, i]
+    L.gfsdc_getvrtdivspec.argtypes = [vp, vp, vp, vp, vp, i, d, i, i]
+    L.gfsdc_getgrad.argtypes = [vp, vp, vp, vp, i, d, i, i]
+    L.gfsdc_set_state.argtypes = [vp, vp, vp, vp, vp, vp, i, i]
+    L.gfsdc_get_state.argtypes = [vp, vp, vp, vp, vp, vp, i, i]
+    L.gfsdc_set_topo.argtypes = [vp, vp, i, i]
+    L.gfsdc_set_pdryini.argtypes = [vp, vp]
+    L.gfsdc_get_pdryini.argtypes = [vp, vp]
+    L.gfsdc_getdyntend.argtypes = [vp, i, i, vp, vp, vp, vp, vp, i, i]
+    L.gfsdc_advance.argtypes = [vp, i, i]
+    L.gfsdc_hs_phytend.argtypes = [vp, vp, vp, vp, i, i]
+    L.gfsdc_get_grid.argtypes = [vp, i, vp, i, i]
+    L.gfsdc_get_semimp.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.gfsdc_get_damp.argtypes = [vp, vp, vp, vp]
+    L.gfsdc_step_diag.argtypes = [vp, vp]
+    L.gfsdc_init_rest_state.argtypes = [vp, d, d, d, i, i, C.c_ulonglong]
+    L.gfsdc_sync.argtypes = [vp]
+    L.gfsdc_stream.argtypes = [vp]
+    L.gfsdc_stream.restype = vp
+    L.gfsdc_profile_step.argtypes = [vp, vp, vp, i]
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise GfsdcError(f"gfsdc error {rc}: {load().gfsdc_last_error().decode()}")
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def cfg_defaults():
+    c = Cfg()
+    load().gfsdc_cfg_defaults(C.byref(c))
+    return c
+
+
+class Context:
+    """One gfsdc context: a transform plan (and, with nlev > 0, a dycore state)."""
+
+    GRID = {"ug": 0, "vg": 1, "vrtg": 2, "divg": 3, "virtempg": 4, "dlnpdtg": 5, "etadot": 6,
+            "psg": 7, "dlnpsdt": 8, "prs": 9, "dlnpsdx": 14, "dlnpsdy": 15}
+
+    def __init__(self, nlon, nlat, ntrunc, precision=FP64, nlev=0, ak=None, bk=None, dt=0.0,
+                 ntrac=0, nmem=1, **kw):
+        L = load()
+        c = cfg_defaults()
+        c.nlon, c.nlat, c.ntrunc, c.nlev, c.ntrac, c.nmem = nlon, nlat, ntrunc, nlev, ntrac, nmem
+        c.precision, c.dt = precision, dt
+        for k, v in kw.items():
+            setattr(c, k, v)
+        self._ak = self._bk = None
+        if nlev > 0:
+            self._ak = np.ascontiguousarray(ak, np.float64)
+            self._bk = np.ascontiguousarray(bk, np.float64)
+            assert self._ak.size == nlev + 1 and self._bk.size == nlev + 1
+            c.ak = self._ak.ctypes.data_as(C.POINTER(C.c_double))
+            c.bk = self._bk.ctypes.data_as(C.POINTER(C.c_double))
+        self.cfg = c
+        h = C.c_void_p()
+        _check(L.gfsdc_create(C.byref(c), C.byref(h)))
+        self.h = h
+        self.nlon, self.nlat, self.ntrunc = nlon, nlat, ntrunc
+        self.nlev, self.ntrac, self.nmem, self.precision = nlev, ntrac, nmem, precision
+        self.nlm = L.gfsdc_nlm(self.h)
+        self.lats = np.zeros(nlat)
+        self.gauwts = np.zeros(nlat)
+        self.areawts = np.zeros((nlat, nlon))
+        self.degree = np.zeros(self.nlm, np.int32)
+        self.order = np.zeros(self.nlm, np.int32)
+        self.lap = np.zeros(self.nlm)
+        self.invlap = np.zeros(self.nlm)
+        _check(L.gfsdc_griddata(self.h, _p(self.lats), _p(self.gauwts), _p(self.areawts),
+                                _p(self.degree), _p(self.order), _p(self.lap), _p(self.invlap)))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _LIB is not None:
+            _LIB.gfsdc_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    # -- dtype plumbing: host numpy arrays in FP64 or FP32 ------------------
+    @staticmethod
+    def _dt(a):
+        return DT_F32 if a.dtype in (np.float32, np.complex64) else DT_F64
+
+    def _spec_in(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=np.complex64 if dtype == DT_F32 else np.complex128)
+        return a.reshape(-1, self.nlm), a.shape[:-1]
+
+    def _grid_in(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=np.float32 if dtype == DT_F32 else np.float64)
+        return a.reshape(-1, self.nlat, self.nlon), a.shape[:-2]
+
+    def spectogrd(self, spec, dtype=DT_F64):
+        s, lead = self._spec_in(spec, dtype)
+        g = np.zeros((s.shape[0], self.nlat, self.nlon), np.float32 if dtype else np.float64)
+        _check(load().gfsdc_spectogrd(self.h, _p(s), _p(g), s.shape[0], dtype, HOST))
+        return g.reshape(lead + g.shape[1:])
+
+    def grdtospec(self, grid, dtype=DT_F64):
+        g, lead = self._grid_in(grid, dtype)
+        s = np.zeros((g.shape[0], self.nlm), np.complex64 if dtype else np.complex128)
+        _check(load().gfsdc_grdtospec(self.h, _p(g), _p(s), g.shape[0], dtype, HOST))
+        return s.reshape(lead + (self.nlm,))
+
+    def getuv(self, vrt, div, rsphere, dtype=DT_F64):
+        a, lead = self._spec_in(vrt, dtype)
+        b, _ = self._spec_in(div, dtype)
+        u = np.zeros((a.shape[0], self.nlat, self.nlon), np.float32 if dtype else np.float64)
+        v = np.zeros_like(u)
+        _check(load().gfsdc_getuv(self.h, _p(a), _p(b), _p(u), _p(v), a.shape[0], rsphere, dtype,
+                                  HOST))
+        return u.reshape(lead + u.shape[1:]), v.reshape(lead + v.shape[1:])
+
+    def getvrtdivspec(self, u, v, rsphere, dtype=DT_F64):
+        a, lead = self._grid_in(u, dtype)
+        b, _ = self._grid_in(v, dtype)
+        vr = np.zeros((a.shape[0], self.nlm), np.complex64 if dtype else np.complex128)
+        dv = np.zeros_like(vr)
+        _check(load().gfsdc_getvrtdivspec(self.h, _p(a), _p(b), _p(vr), _p(dv), a.shape[0],
+                                          rsphere, dtype, HOST))
+        return vr.reshape(lead + (self.nlm,)), dv.reshape(lead + (self.nlm,))
+
+    def getgrad(self, spec, rsphere, dtype=DT_F64):
+        a, lead = self._spec_in(spec, dtype)
+        gx = np.zeros((a.shape[0], self.nlat, self.nlon), np.float32 if dtype else np.float64)
+        gy = np.zeros_like(gx)
+        _check(load().gfsdc_getgrad(self.h, _p(a), _p(gx), _p(gy), a.shape[0], rsphere, dtype,
+                                    HOST))
+        return gx.reshape(lead + gx.shape[1:]), gy.reshape(lead + gy.shape[1:])
+
+    # -- dycore -------------------------------------------------------------
+    def _s3(self, n=None):
+        return np.zeros((self.nmem, n if n is not None else self.nlev, self.nlm), np.complex128)
+
+    def set_state(self, vrt=None, div=None, tv=None, lnps=None, trac=None):
+        arrs = [None if x is None else np.ascontiguousarray(x, np.complex128)
+                for x in (vrt, div, tv, trac, lnps)]
+        _check(load().gfsdc_set_state(self.h, *[_p(x) for x in arrs], DT_F64, HOST))
+
+    def get_state(self):
+        vrt, div, tv = self._s3(), self._s3(), self._s3()
+        trac = np.zeros((self.nmem, max(self.ntrac, 1), self.nlev, self.nlm), np.complex128)
+        lnps = np.zeros((self.nmem, self.nlm), np.complex128)
+        _check(load().gfsdc_get_state(self.h, _p(vrt), _p(div), _p(tv),
+                                      _p(trac) if self.ntrac else None, _p(lnps), DT_F64, HOST))
+        return dict(vrt=vrt, div=div, tv=tv, trac=trac[:, :self.ntrac], lnps=lnps)
+
+    def set_topo(self, topo):
+        t = np.ascontiguousarray(topo, np.complex128)
+        _check(load().gfsdc_set_topo(self.h, _p(t), DT_F64, HOST))
+
+    @property
+    def pdryini(self):
+        out = np.zeros(self.nmem)
+        _check(load().gfsdc_get_pdryini(self.h, _p(out)))
+        return out
+
+    @pdryini.setter
+    def pdryini(self, v):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.nmem,)))
+        _check(load().gfsdc_set_pdryini(self.h, _p(a)))
+
+    def getdyntend(self, rkstage=1, inverse_only=False):
+        out = dict(dvrt=self._s3(), ddiv=self._s3(), dtv=self._s3(),
+                   dtrac=np.zeros((self.nmem, max(self.ntrac, 1), self.nlev, self.nlm),
+                                  np.complex128),
+                   dlnps=np.zeros((self.nmem, self.nlm), np.complex128))
+        if inverse_only:
+            _check(load().gfsdc_getdyntend(self.h, rkstage, 1, None, None, None, None, None,
+                                           DT_F64, HOST))
+            return None
+        _check(load().gfsdc_getdyntend(self.h, rkstage, 0, _p(out["dvrt"]), _p(out["ddiv"]),
+                                       _p(out["dtv"]), _p(out["dtrac"]) if self.ntrac else None,
+                                       _p(out["dlnps"]), DT_F64, HOST))
+        return out
+
+    def advance(self, nsteps=1, use_graph=False):
+        _check(load().gfsdc_advance(self.h, nsteps, int(use_graph)))
+
+    def hs_phytend(self):
+        a, b, c = self._s3(), self._s3(), self._s3()
+        _check(load().gfsdc_hs_phytend(self.h, _p(a), _p(b), _p(c), DT_F64, HOST))
+        return dict(dvrt=a, ddiv=b, dtv=c)
+
+    def grid(self, name):
+        nk = {"etadot": self.nlev + 1, "psg": 1, "dlnpsdt": 1, "dlnpsdx": 1,
+              "dlnpsdy": 1}.get(name, self.nlev)
+        out = np.zeros((self.nmem, nk, self.nlat, self.nlon))
+        _check(load().gfsdc_get_grid(self.h, self.GRID[name], _p(out), DT_F64, HOST))
+        return out
+
+    def semimp(self):
+        n = self.nlev
+        A = np.zeros(n * n); B = np.zeros(n * n); sv = np.zeros(n); tor = np.zeros(n)
+        D = np.zeros(n * n * (self.ntrunc + 1) * 3)
+        _check(load().gfsdc_get_semimp(self.h, _p(A), _p(B), _p(sv), _p(tor), _p(D)))
+        return dict(amhyb=A.reshape(n, n).T.copy(), bmhyb=B.reshape(n, n).T.copy(), svhyb=sv,
+                    tor_hyb=tor,
+                    d_hyb_m=D.reshape(3, self.ntrunc + 1, n, n).transpose(0, 1, 3, 2).copy())
+
+    def damp(self):
+        d = np.zeros(self.nlm); f = np.zeros(self.nlev); p = np.zeros(self.nlev)
+        _check(load().gfsdc_get_damp(self.h, _p(d), _p(f), _p(p)))
+        return dict(disspec=d, diff_prof=f, dmp_prof=p)
+
+    def step_diag(self):
+        out = np.zeros(4 * self.nmem)
+        _check(load().gfsdc_step_diag(self.h, _p(out)))
+        o = out.reshape(self.nmem, 4)
+        return dict(spdmax=o[:, 0], psmin=o[:, 1], psmax=o[:, 2], pstend=o[:, 3])
+
+    def init_rest_state(self, t0=300.0, ps0=1.0e5, vort_rms=1.0e-6, nmin=1, nmax=21, seed=0):
+        _check(load().gfsdc_init_rest_state(self.h, t0, ps0, vort_rms, nmin, nmax, seed))
+
+    def sync(self):
+        _check(load().gfsdc_sync(self.h))
+
+    def stream(self):
+        return load().gfsdc_stream(self.h)
+
+    def profile_step(self, nmax=64):
+        names = (C.c_char * 48 * nmax)()
+        ms = np.zeros(nmax)
+        n = load().gfsdc_profile_step(self.h, names, _p(ms), nmax)
+        if n < 0:
+            _check(n)
+        return [(bytes(names[k]).split(b"\0")[0].decode(), float(ms[k])) for k in range(n)]

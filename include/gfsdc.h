@@ -1,0 +1,183 @@
+/*
+ * gfsdc -- MI355X (gfx950) spectral transform-and-tendency step of the GFS
+ * hydrostatic spectral dynamical core.
+ *
+ * Plain C ABI (extern "C", pointers and sizes, no torch or HIP types). Two
+ * tiers:
+ *
+ *  1. SHTns-compatible symbols. src/shtns.f90 binds eleven Fortran-callable
+ *     C symbols of the SHTns library (all arguments by reference). Linking
+ *     the unchanged shtns.f90 against this library instead of
+ *     "-lshtns -lfftw3" routes every per-field transform of the reference
+ *     to the GPU. Arrays are host FP64, one field per call, synchronous.
+ *
+ *  2. gfsdc_* handle API. Device-resident, batched, precision-selectable
+ *     (FP64 / FP32 MFMA Legendre), used by a Fortran host's getdyntend /
+ *     advance replacement so that the state never leaves HBM inside a step.
+ *
+ * Conventions (as src/shtns.f90, src/spectral_data.f90, src/grid_data.f90):
+ *   spectral: complex, (ndimspec, nfield) column-major, m-major index
+ *             k(m,n) = m(2T+3-m)/2 + (n-m), orthonormal, no Condon-Shortley
+ *             phase; real field f = sum q_n0 Y_n0 + 2 Re sum_{m>0} q_nm Y_nm
+ *   grid:     real, (nlon, nlat, nfield) column-major, lon fastest,
+ *             latitudes north -> south on Gauss nodes, lon_i = 2 pi i / nlon
+ *   levels:   model levels bottom -> top; ak/bk interfaces top -> bottom
+ *
+ * Errors: every gfsdc_* call returns 0 on success and a negative GFSDC_E*
+ * code otherwise; gfsdc_last_error() returns the text of the calling
+ * thread's last error. Calls on one context are not thread-safe; use one
+ * context per host thread or serialise.
+ */
+#ifndef GFSDC_H
+#define GFSDC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Tier 1: SHTns symbols bound by src/shtns.f90                        */
+/* ------------------------------------------------------------------ */
+/* replaces SHTns shtns_use_threads (shtns.f90:109); the count is recorded, GPU work ignores it */
+void shtns_use_threads_(int *nthreads);
+/* replaces SHTns shtns_set_size (shtns.f90:110): lmax, mmax, mres, norm flags */
+void shtns_set_size_(int *lmax, int *mmax, int *mres, int *norm);
+/* replaces SHTns shtns_precompute (shtns.f90:111): grid type, layout, polar threshold, nlat, nphi */
+void shtns_precompute_(int *type, int *layout, double *eps, int *nlat, int *nphi);
+/* replaces SHTns shtns_calc_nlm (shtns.f90:112) */
+void shtns_calc_nlm_(int *nlm, int *lmax, int *mmax, int *mres);
+/* replaces SHTns shtns_cos_array (shtns.f90:125): cos(colatitude), nlat values, N->S */
+void shtns_cos_array_(double *costh);
+/* replaces SHTns shtns_gauss_wts (shtns.f90:126): nlat/2 Gauss weights (full set sums to 2) */
+void shtns_gauss_wts_(double *wts);
+/* replaces SHTns shtns_spat_to_sh (shtns.f90:195,199) */
+void shtns_spat_to_sh_(double *vr, double *qlm /* complex */);
+/* replaces SHTns shtns_sh_to_spat (shtns.f90:225,229) */
+void shtns_sh_to_spat_(double *qlm /* complex */, double *vr);
+/* replaces SHTns shtns_sphtor_to_spat (shtns.f90:261,270,341,348) */
+void shtns_sphtor_to_spat_(double *slm, double *tlm, double *vt, double *vp);
+/* replaces SHTns shtns_spat_to_sphtor (shtns.f90:303,308) */
+void shtns_spat_to_sphtor_(double *vt, double *vp, double *slm, double *tlm);
+/* replaces SHTns shtns_reset (shtns.f90:160) */
+void shtns_reset_(void);
+
+/* ------------------------------------------------------------------ */
+/* Tier 2: gfsdc handle API                                            */
+/* ------------------------------------------------------------------ */
+enum {
+    GFSDC_OK = 0,
+    GFSDC_EINVAL = -1,   /* bad argument / shape */
+    GFSDC_EHIP = -2,     /* HIP runtime error */
+    GFSDC_ENOMEM = -3,   /* device allocation failed */
+    GFSDC_ESTATE = -4,   /* call out of sequence */
+    GFSDC_ENODEV = -5    /* no gfx950 device */
+};
+
+enum { GFSDC_FP64 = 0, GFSDC_FP32 = 1 };             /* compute precision of the context */
+enum { GFSDC_HOST = 0, GFSDC_DEVICE = 1 };           /* location of a caller buffer */
+enum { GFSDC_DT_F64 = 0, GFSDC_DT_F32 = 1 };         /* element type of a caller buffer */
+enum { GFSDC_PHYS_NONE = 0, GFSDC_PHYS_HS = 1 };     /* physics plug-in (Makefile:50-72) */
+
+typedef struct gfsdc_cfg {
+    int nlon, nlat, ntrunc, nlev, ntrac, nmem;  /* nmem: ensemble members folded into one launch */
+    int precision;                              /* GFSDC_FP64 or GFSDC_FP32 */
+    double dt;                                  /* seconds (params.f90 deltim) */
+    const double *ak, *bk;                      /* nlev+1 interfaces, top -> bottom, Pa / unitless */
+    int explicit_;                              /* params.f90 explicit (0 = IMEX semi-implicit) */
+    int physics;                                /* GFSDC_PHYS_* */
+    int massfix;                                /* params.f90 massfix */
+    int ndiss;                                  /* 0 -> 8 (dyn_init.f90:461) */
+    double efold;                               /* <= 0 -> GFS default by truncation */
+    double hdif_fac, hdif_fac2, fshk, taustratdamp;
+    /* physical constants (physcons.f90); gfsdc_cfg_defaults fills them */
+    double rerth, grav, omega, rd, rv, cp, cvap;
+    int device;                                 /* HIP device ordinal */
+} gfsdc_cfg;
+
+typedef struct gfsdc_ctx gfsdc_ctx;
+
+/* fill defaults of params.f90 / physcons.f90 (sizes left 0) */
+void gfsdc_cfg_defaults(gfsdc_cfg *cfg);
+/* replaces init_dyn's shtns_init (dyn_init.f90:43), setdampspec (:86) and
+ * init_semimpdata (:88): builds Gauss grid, Legendre tables, damping and
+ * semi-implicit operators on the device */
+int gfsdc_create(const gfsdc_cfg *cfg, gfsdc_ctx **out);
+int gfsdc_destroy(gfsdc_ctx *ctx);
+const char *gfsdc_last_error(void);
+/* number of spectral coefficients (ndimspec) and the shtns.f90 module data */
+int gfsdc_nlm(const gfsdc_ctx *ctx);
+int gfsdc_griddata(const gfsdc_ctx *ctx, double *lats, double *gauwts, double *areawts,
+                   int *degree, int *order, double *lap, double *invlap);
+
+/* batched per-field transforms (shtns.f90 API over nfield fields; one call
+ * replaces an OpenMP loop over levels). spec: (ndimspec, nfield) complex of
+ * dtype; grid: (nlon, nlat, nfield) of dtype; loc GFSDC_HOST / GFSDC_DEVICE. */
+/* replaces shtns.f90:203 spectogrd */
+int gfsdc_spectogrd(gfsdc_ctx *ctx, const void *spec, void *grid, int nfield, int dtype, int loc);
+/* replaces shtns.f90:171 grdtospec */
+int gfsdc_grdtospec(gfsdc_ctx *ctx, const void *grid, void *spec, int nfield, int dtype, int loc);
+/* replaces shtns.f90:233 getuv */
+int gfsdc_getuv(gfsdc_ctx *ctx, const void *vrt, const void *div, void *u, void *v, int nfield,
+                double rsphere, int dtype, int loc);
+/* replaces shtns.f90:275 getvrtdivspec */
+int gfsdc_getvrtdivspec(gfsdc_ctx *ctx, const void *u, const void *v, void *vrt, void *div,
+                        int nfield, double rsphere, int dtype, int loc);
+/* replaces shtns.f90:313 getgrad */
+int gfsdc_getgrad(gfsdc_ctx *ctx, const void *spec, void *gx, void *gy, int nfield,
+                  double rsphere, int dtype, int loc);
+
+/* model state, member-major: vrt/div/tv (ndimspec, nlev, nmem), trac
+ * (ndimspec, nlev, ntrac, nmem), lnps (ndimspec, nmem) complex of dtype.
+ * Any pointer may be NULL to leave that field unchanged / unread. */
+int gfsdc_set_state(gfsdc_ctx *ctx, const void *vrt, const void *div, const void *tv,
+                    const void *trac, const void *lnps, int dtype, int loc);
+int gfsdc_get_state(gfsdc_ctx *ctx, void *vrt, void *div, void *tv, void *trac, void *lnps,
+                    int dtype, int loc);
+/* spectral orography (ndimspec, nmem) in metres; grad(g*topo) as dyn_init.f90:84 */
+int gfsdc_set_topo(gfsdc_ctx *ctx, const void *topo, int dtype, int loc);
+/* params.f90 pdryini (Pa) per member; 0 means "take from the first stage-1 call" */
+int gfsdc_set_pdryini(gfsdc_ctx *ctx, const double *pdryini);
+int gfsdc_get_pdryini(gfsdc_ctx *ctx, double *pdryini);
+
+/* replaces dyn_run.f90:33 getdyntend on the context state; tendencies in the
+ * state layout (dtrac may be NULL when ntrac == 0, all may be NULL when
+ * inverse_only, which then only refreshes the grid fields) */
+int gfsdc_getdyntend(gfsdc_ctx *ctx, int rkstage, int inverse_only, void *dvrt, void *ddiv,
+                     void *dtv, void *dtrac, void *dlnps, int dtype, int loc);
+/* replaces run.f90:194 advance, nsteps times (device resident; captured in a
+ * HIP graph after the first step when use_graph != 0) */
+int gfsdc_advance(gfsdc_ctx *ctx, int nsteps, int use_graph);
+/* Held-Suarez tendencies on the current grid fields (phy_run_hs.f90:22) */
+int gfsdc_hs_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
+/* grid fields after the last getdyntend, names of grid_data.f90/pressure_data.f90:
+ * 0 ug,1 vg,2 vrtg,3 divg,4 virtempg,5 dlnpdtg,6 etadot (nlev+1),7 psg,8 dlnpsdt,
+ * 9 prs,14 dlnpsdx,15 dlnpsdy (2-D fields have one level); per member */
+int gfsdc_get_grid(gfsdc_ctx *ctx, int which, void *out, int dtype, int loc);
+/* semimp_data.f90 operators (FP64 host arrays, column-major as the reference):
+ * amhyb,bmhyb (nlev,nlev), svhyb,tor_hyb (nlev), d_hyb_m (nlev,nlev,ntrunc+1,3) */
+int gfsdc_get_semimp(gfsdc_ctx *ctx, double *amhyb, double *bmhyb, double *svhyb,
+                     double *tor_hyb, double *d_hyb_m);
+/* dyn_init.f90:450 setdampspec outputs (FP64 host): disspec (ndimspec), diff_prof, dmp_prof (nlev) */
+int gfsdc_get_damp(gfsdc_ctx *ctx, double *disspec, double *diff_prof, double *dmp_prof);
+/* run.f90:159-162 per-step diagnostics per member: spdmax, min ps, max ps (Pa), pstend */
+int gfsdc_step_diag(gfsdc_ctx *ctx, double *out /* 4 * nmem */);
+/* synthetic state: resting isothermal atmosphere (t0 K, ps0 Pa) plus seeded
+ * random vorticity in total wavenumbers [nmin, nmax] scaled to an rms grid
+ * vorticity of vort_rms (s^-1), member i seeded with seed + i */
+int gfsdc_init_rest_state(gfsdc_ctx *ctx, double t0, double ps0, double vort_rms, int nmin,
+                          int nmax, unsigned long long seed);
+/* wait for all work on the context's stream */
+int gfsdc_sync(gfsdc_ctx *ctx);
+/* HIP stream the context launches on (hipStream_t as void*) */
+void *gfsdc_stream(gfsdc_ctx *ctx);
+
+/* per-kernel timing of one step via HIP events on the context stream:
+ * fills up to nmax (name, milliseconds) pairs, returns the count */
+int gfsdc_profile_step(gfsdc_ctx *ctx, char names[][48], double *ms, int nmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

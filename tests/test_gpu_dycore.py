@@ -1,0 +1,147 @@
+"""GPU parity of the dynamical core (libgfsdc) against the FP64 oracle.
+
+Cases: T42L20 sigma (the CPU-runnable configuration) and a T42L64 hybrid
+column set (the L64 coordinate of the larger configurations at a size the
+oracle finishes in seconds). Tolerances are relative max-norm errors per
+field: FP64 path 1e-9; FP32 path 2e-4 for tendencies (single-step
+cancellation), and the multi-step FP32 drift is bounded in test_advance.
+"""
+import numpy as np
+import pytest
+
+import cases
+import levels
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NLON, NLAT, T = 128, 64, 42
+
+
+def coords(kind, nlev):
+    return levels.sigma_levels(nlev) if kind == "sigma" else levels.hybrid_levels(nlev)
+
+
+def make_pair(kind, nlev, precision, dt=1200.0, **kw):
+    import gfsdc
+    ak, bk = coords(kind, nlev)
+    o = O.Model(NLON, NLAT, T, nlev, ak, bk, dt, **kw)
+    g = gfsdc.Context(NLON, NLAT, T, precision, nlev=nlev, ak=ak, bk=bk, dt=dt, **kw)
+    return o, g
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("kind,nlev", [("sigma", 20), ("hybrid", 64)])
+def test_operators_match(kind, nlev):
+    import gfsdc
+    o, g = make_pair(kind, nlev, gfsdc.FP64)
+    so, sg = o.semimp(), g.semimp()
+    for k in so:
+        assert rel(sg[k], so[k]) < 1e-12, k
+    do, dg = o.damp(), g.damp()
+    for k in do:
+        assert np.allclose(dg[k], do[k], rtol=1e-13, atol=0), k
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("kind,nlev", [("sigma", 20), ("hybrid", 64)])
+def test_getdyntend(kind, nlev, prec):
+    import gfsdc
+    p = gfsdc.FP64 if prec == "fp64" else gfsdc.FP32
+    o, g = make_pair(kind, nlev, p)
+    vrt, div, tv, lnps = cases.perturbed_state(o.sht.degree, o.sht.order, nlev, seed=5)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.set_state(vrt, div, tv, lnps)
+    to = o.getdyntend(1)
+    tg = g.getdyntend(1)
+    tol = 1e-9 if prec == "fp64" else 2e-4
+    for k in ("dvrt", "ddiv", "dtv", "dlnps"):
+        assert rel(tg[k][0], to[k]) < tol, (k, rel(tg[k][0], to[k]))
+    assert abs(g.pdryini[0] - o.pdryini) < 1e-9 * o.pdryini
+    for name in ("ug", "vg", "virtempg", "dlnpdtg", "etadot", "psg", "dlnpsdt"):
+        a, b = g.grid(name)[0], o.grid(name)
+        assert rel(a, b) < (1e-9 if prec == "fp64" else 2e-4), (name, rel(a, b))
+
+
+def test_rest_state_matches_numpy():
+    import gfsdc
+    o, g = make_pair("sigma", 20, gfsdc.FP64)
+    g.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=7)
+    st = g.get_state()
+    vrt, div, tv, lnps = cases.rest_state(o.sht.degree, o.sht.order, 20, seed=7)
+    assert rel(st["vrt"], vrt) < 1e-14
+    assert np.abs(st["div"]).max() == 0
+    assert rel(st["tv"], tv) < 1e-15 and rel(st["lnps"], lnps) < 1e-15
+
+
+@pytest.mark.parametrize("kind,nlev", [("sigma", 20), ("hybrid", 64)])
+def test_advance_fp64(kind, nlev):
+    import gfsdc
+    o, g = make_pair(kind, nlev, gfsdc.FP64)
+    vrt, div, tv, lnps = cases.perturbed_state(o.sht.degree, o.sht.order, nlev, seed=9, amp=0.3)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.set_state(vrt, div, tv, lnps)
+    o.advance(3)
+    g.advance(3)
+    so, sg = o.get_state(), g.get_state()
+    for k in ("vrt", "div", "tv", "lnps"):
+        assert rel(sg[k][0], so[k]) < 1e-9, (k, rel(sg[k][0], so[k]))
+    assert abs(g.pdryini[0] - o.pdryini) < 1e-9 * o.pdryini
+    dg, do = g.step_diag(), o.step_diag()
+    for k in do:
+        assert abs(dg[k][0] - do[k]) <= 1e-8 * abs(do[k]) + 1e-12, k
+
+
+def test_advance_fp32_one_day_t42():
+    """C0 recipe (T42L20 HS, dt 1200 s) for 72 steps: FP32 path vs FP64 oracle."""
+    import gfsdc
+    o, g = make_pair("sigma", 20, gfsdc.FP32)
+    vrt, div, tv, lnps = cases.rest_state(o.sht.degree, o.sht.order, 20, seed=0)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=0)
+    o.advance(72)
+    g.advance(72)
+    so, sg = o.get_state(), g.get_state()
+    errs = {k: np.linalg.norm(sg[k][0] - so[k]) / np.linalg.norm(so[k])
+            for k in ("vrt", "div", "tv", "lnps")}
+    print("FP32 one-day spectral relative error (T42L20):", errs)
+    assert errs["tv"] < 1e-5 and errs["lnps"] < 1e-5
+    assert errs["vrt"] < 1e-3 and errs["div"] < 1e-2
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_graph_replay_matches_eager(prec):
+    import gfsdc
+    p = gfsdc.FP64 if prec == "fp64" else gfsdc.FP32
+    ak, bk = coords("hybrid", 64)
+    mk = lambda: gfsdc.Context(NLON, NLAT, T, p, nlev=64, ak=ak, bk=bk, dt=600.0)
+    a, b = mk(), mk()
+    for c in (a, b):
+        c.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=3)
+    a.advance(4, use_graph=False)
+    b.advance(1, use_graph=True)
+    b.sync()
+    b.step_diag()  # eager work between replays
+    b.advance(3, use_graph=True)
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("vrt", "div", "tv", "lnps"):
+        assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_ensemble_members_independent():
+    import gfsdc
+    ak, bk = coords("sigma", 20)
+    ens = gfsdc.Context(NLON, NLAT, T, gfsdc.FP64, nlev=20, ak=ak, bk=bk, dt=1200.0, nmem=3)
+    ens.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=100)
+    ens.advance(2)
+    se = ens.get_state()
+    for mem in range(3):
+        one = gfsdc.Context(NLON, NLAT, T, gfsdc.FP64, nlev=20, ak=ak, bk=bk, dt=1200.0)
+        one.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=100 + mem)
+        one.advance(2)
+        s1 = one.get_state()
+        for k in ("vrt", "div", "tv", "lnps"):
+            assert rel(se[k][mem], s1[k][0]) < 1e-12, (mem, k)

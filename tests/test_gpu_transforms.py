@@ -1,0 +1,81 @@
+"""GPU parity of the shtns.f90 transform API (libgfsdc) against the FP64 oracle.
+
+Tolerances: FP64 path 1e-11 relative (max-norm over the field batch);
+FP32 path 2e-6 relative for scalar transforms and 5e-6 for vector ones. The
+FP32 vorticity/divergence analysis error is measured against the larger of
+the two output fields: the wind is dominated by its rotational part, so FP32
+rounding of u, v is of that size in both outputs.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(128, 64, 42), (384, 190, 126), (768, 384, 254)]
+A = 6.3712e6
+
+
+def rand_spec(sht, nf, seed, slope=1.5):
+    rng = np.random.default_rng(seed)
+    n = np.maximum(sht.degree, 1).astype(float)
+    q = n ** (-slope) * np.exp(2j * np.pi * rng.random((nf, sht.nlm)))
+    q[:, sht.order == 0] = q[:, sht.order == 0].real
+    return q
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module", params=GRIDS, ids=lambda g: f"T{g[2]}")
+def pair(request):
+    import gfsdc
+    nlon, nlat, T = request.param
+    return O.Sht(nlon, nlat, T), gfsdc.Context(nlon, nlat, T, gfsdc.FP64), \
+        gfsdc.Context(nlon, nlat, T, gfsdc.FP32)
+
+
+def test_griddata(pair):
+    o, g64, g32 = pair
+    for g in (g64, g32):
+        assert np.abs(g.lats - o.lats).max() < 1e-14
+        assert np.abs(g.gauwts - o.gauwts).max() < 1e-14
+        assert np.array_equal(g.degree, o.degree) and np.array_equal(g.order, o.order)
+
+
+@pytest.mark.parametrize("nf", [1, 7, 70])
+def test_spectogrd_grdtospec(pair, nf):
+    o, g64, g32 = pair
+    q = rand_spec(o, nf, 11 + nf)
+    ref = o.spectogrd(q)
+    got = g64.spectogrd(q)
+    assert rel(got, ref) < 1e-11
+    back = g64.grdtospec(got)
+    assert rel(back, q) < 1e-11
+    assert rel(g64.grdtospec(ref), o.grdtospec(ref)) < 1e-11
+    got32 = g32.spectogrd(q.astype(np.complex64), dtype=1)
+    assert rel(got32, ref) < 2e-6
+    assert rel(g32.grdtospec(ref.astype(np.float32), dtype=1), o.grdtospec(ref)) < 2e-6
+
+
+@pytest.mark.parametrize("nf", [1, 9])
+def test_vector_transforms(pair, nf):
+    o, g64, g32 = pair
+    vrt = rand_spec(o, nf, 3) * 1e-5
+    div = rand_spec(o, nf, 4) * 1e-6
+    u0, v0 = o.getuv(vrt, div, A)
+    u, v = g64.getuv(vrt, div, A)
+    assert rel(u, u0) < 1e-11 and rel(v, v0) < 1e-11
+    z0, d0 = o.getvrtdivspec(u0, v0, A)
+    z, d = g64.getvrtdivspec(u0, v0, A)
+    assert rel(z, z0) < 1e-11 and rel(d, d0) < 1e-11
+    gx0, gy0 = o.getgrad(div, A)
+    gx, gy = g64.getgrad(div, A)
+    assert rel(gx, gx0) < 1e-11 and rel(gy, gy0) < 1e-11
+    u32, v32 = g32.getuv(vrt.astype(np.complex64), div.astype(np.complex64), A, dtype=1)
+    assert rel(u32, u0) < 5e-6 and rel(v32, v0) < 5e-6
+    z32, d32 = g32.getvrtdivspec(u0.astype(np.float32), v0.astype(np.float32), A, dtype=1)
+    scale = max(np.abs(z0).max(), np.abs(d0).max())
+    assert np.abs(z32 - z0).max() / scale < 5e-6 and np.abs(d32 - d0).max() / scale < 5e-6
