@@ -43,36 +43,39 @@ template <> struct MF<double> {
 // Table generation (FP64 arithmetic on the device)
 
 // Gauss-Legendre nodes mu (cos colatitude, north -> south) and weights (sum 2)
-// by Newton iteration on P_nlat, one thread per root pair.
+// by Newton iteration on P_nlat(cos theta) in the colatitude, one thread per
+// root pair; w = 2 sin^2(theta) / (n (P_{n-1} - x P_n))^2 keeps polar weights exact.
 __global__ void k_gauss(double *mu, double *w, int nlat) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (nlat + 1) / 2) return;
-    double x = cos(M_PI * (i + 0.75) / (nlat + 0.5));
-    double p0, p1, dp;
+    const int n = nlat;
+    double th = M_PI * (i + 0.75) / (n + 0.5);
+    double x, p0, p1;
     for (int it = 0; it < 100; it++) {
+        x = cos(th);
         p0 = 1.0;
         p1 = x;
-        for (int k = 2; k <= nlat; k++) {
+        for (int k = 2; k <= n; k++) {
             double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
             p0 = p1;
             p1 = p2;
         }
-        dp = nlat * (x * p1 - p0) / (x * x - 1.0);
-        double dx = p1 / dp;
-        x -= dx;
-        if (fabs(dx) < 1e-16) break;
+        double dth = -p1 * sin(th) / (n * (p0 - x * p1));
+        th -= dth;
+        if (fabs(dth) < 1e-16) break;
     }
+    x = cos(th);
     p0 = 1.0;
     p1 = x;
-    for (int k = 2; k <= nlat; k++) {
+    for (int k = 2; k <= n; k++) {
         double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
         p0 = p1;
         p1 = p2;
     }
-    dp = nlat * (x * p1 - p0) / (x * x - 1.0);
+    double st = sin(th), d = n * (p0 - x * p1);
     mu[i] = x;
     mu[nlat - 1 - i] = -x;
-    w[i] = w[nlat - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
+    w[i] = w[nlat - 1 - i] = 2.0 * st * st / (d * d);
 }
 
 __device__ inline double eps_nm_d(int n, int m) {

@@ -24,7 +24,7 @@ def sigma_levels(nlev):
 
 def hybrid_levels(nlev, eta_c=0.2, expo=1.5, p0=1.0e5):
     eta = np.linspace(0.0, 1.0, nlev + 1)
-    bk = np.where(eta > eta_c, ((eta - eta_c) / (1.0 - eta_c)) ** expo, 0.0)
+    bk = np.clip((eta - eta_c) / (1.0 - eta_c), 0.0, None) ** expo
     bk[-1] = 1.0
     ak = p0 * (eta - bk)
     ak[-1] = 0.0

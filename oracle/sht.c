@@ -45,36 +45,41 @@ struct orc_sht {
 };
 
 /* ------------------------------------------------------------------ */
-/* Gauss-Legendre nodes by Newton iteration on P_nlat (FP64).           */
+/* Gauss-Legendre nodes by Newton iteration on P_nlat(cos theta) in the
+ * colatitude theta (FP64); sin(theta) is then exact to rounding, which keeps
+ * the polar weights w = 2 sin^2 / (n (P_{n-1} - x P_n))^2 accurate. */
 void orc_gauss(int nlat, double *mu, double *w) {
+    int n = nlat;
     for (int i = 0; i < (nlat + 1) / 2; i++) {
-        double x = cos(M_PI * (i + 0.75) / (nlat + 0.5));
-        double dp = 0.0;
+        double th = M_PI * (i + 0.75) / (n + 0.5);
+        double x = cos(th), p0 = 1.0, p1 = x;
         for (int it = 0; it < 100; it++) {
-            double p0 = 1.0, p1 = x;
-            for (int k = 2; k <= nlat; k++) {
+            x = cos(th);
+            p0 = 1.0;
+            p1 = x;
+            for (int k = 2; k <= n; k++) {
                 double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
                 p0 = p1;
                 p1 = p2;
             }
-            if (nlat == 1) { p1 = x; p0 = 1.0; }
-            dp = nlat * (x * p1 - p0) / (x * x - 1.0);
-            double dx = p1 / dp;
-            x -= dx;
-            if (fabs(dx) < 1e-16) break;
+            /* dP_n(cos th)/dth = -n (P_{n-1} - x P_n) / sin th */
+            double st = sin(th);
+            double dth = -p1 * st / (n * (p0 - x * p1));
+            th -= dth;
+            if (fabs(dth) < 1e-16) break;
         }
-        {
-            double p0 = 1.0, p1 = x;
-            for (int k = 2; k <= nlat; k++) {
-                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-                p0 = p1;
-                p1 = p2;
-            }
-            dp = nlat * (x * p1 - p0) / (x * x - 1.0);
+        x = cos(th);
+        p0 = 1.0;
+        p1 = x;
+        for (int k = 2; k <= n; k++) {
+            double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
         }
+        double st = sin(th), d = n * (p0 - x * p1);
         mu[i] = x;
         mu[nlat - 1 - i] = -x;
-        w[i] = w[nlat - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
+        w[i] = w[nlat - 1 - i] = 2.0 * st * st / (d * d);
     }
 }
 
