@@ -1,0 +1,79 @@
+! module gfsdc_dycore: iso_c_binding interface of the device-resident
+! dycore tier of libgfsdc (include/gfsdc.h). A Fortran host keeps the
+! reference's call sequence (run.f90 advance -> dyn_run.f90 getdyntend ->
+! semi-implicit solve -> damping -> physics) and calls these entry points
+! instead of looping over levels on the host.
+module gfsdc_dycore
+  use iso_c_binding
+  implicit none
+  private
+  public :: gfsdc_cfg, dyc_defaults, dyc_create, dyc_destroy, dyc_set_state, dyc_get_state, &
+            dyc_getdyntend, dyc_advance, dyc_step_diag, dyc_init_rest_state, dyc_sync, &
+            GFSDC_FP64, GFSDC_FP32, GFSDC_HOST, DT_F64, DT_F32, PHYS_NONE, PHYS_HS
+
+  integer(c_int), parameter :: GFSDC_FP64 = 0, GFSDC_FP32 = 1, GFSDC_HOST = 0, DT_F64 = 0, DT_F32 = 1
+  integer(c_int), parameter :: PHYS_NONE = 0, PHYS_HS = 1
+
+  type, bind(c) :: gfsdc_cfg
+    integer(c_int) :: nlon, nlat, ntrunc, nlev, ntrac, nmem, precision
+    real(c_double) :: dt
+    type(c_ptr) :: ak, bk
+    integer(c_int) :: explicit_, physics, massfix, ndiss
+    real(c_double) :: efold, hdif_fac, hdif_fac2, fshk, taustratdamp
+    real(c_double) :: rerth, grav, omega, rd, rv, cp, cvap
+    integer(c_int) :: device
+  end type
+
+  interface
+    subroutine dyc_defaults(cfg) bind(c, name="gfsdc_cfg_defaults")
+      import :: gfsdc_cfg
+      type(gfsdc_cfg), intent(inout) :: cfg
+    end subroutine
+    integer(c_int) function dyc_create(cfg, out) bind(c, name="gfsdc_create")
+      import :: gfsdc_cfg, c_ptr, c_int
+      type(gfsdc_cfg), intent(in) :: cfg
+      type(c_ptr), intent(out) :: out
+    end function
+    integer(c_int) function dyc_destroy(h) bind(c, name="gfsdc_destroy")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function dyc_set_state(h, vrt, div, tv, trac, lnps, dtype, loc) bind(c, name="gfsdc_set_state")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, vrt, div, tv, trac, lnps
+      integer(c_int), value :: dtype, loc
+    end function
+    integer(c_int) function dyc_get_state(h, vrt, div, tv, trac, lnps, dtype, loc) bind(c, name="gfsdc_get_state")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, vrt, div, tv, trac, lnps
+      integer(c_int), value :: dtype, loc
+    end function
+    integer(c_int) function dyc_getdyntend(h, stage, inv, dvrt, ddiv, dtv, dtrac, dlnps, dtype, loc) &
+        bind(c, name="gfsdc_getdyntend")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, dvrt, ddiv, dtv, dtrac, dlnps
+      integer(c_int), value :: stage, inv, dtype, loc
+    end function
+    integer(c_int) function dyc_advance(h, nsteps, use_graph) bind(c, name="gfsdc_advance")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps, use_graph
+    end function
+    integer(c_int) function dyc_step_diag(h, out) bind(c, name="gfsdc_step_diag")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, out
+    end function
+    integer(c_int) function dyc_init_rest_state(h, t0, ps0, vrms, nmin, nmax, seed) &
+        bind(c, name="gfsdc_init_rest_state")
+      import :: c_ptr, c_int, c_double, c_long_long
+      type(c_ptr), value :: h
+      real(c_double), value :: t0, ps0, vrms
+      integer(c_int), value :: nmin, nmax
+      integer(c_long_long), value :: seed
+    end function
+    integer(c_int) function dyc_sync(h) bind(c, name="gfsdc_sync")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+    end function
+  end interface
+end module gfsdc_dycore

@@ -1,0 +1,77 @@
+! Held-Suarez driver with the Fortran host in charge: sets up the T42L20
+! (or T254L64) configuration, runs run.f90's time loop on the GPU through
+! libgfsdc and prints run.f90's per-step diagnostic line (run.f90:159-163).
+! Also exercises module shtns (the transform wrapper) with a round trip.
+! Usage: gfsdc_hs_driver [nsteps] [T42L20|T254L64] [fp64|fp32]
+program hs_driver
+  use iso_c_binding
+  use kinds, only: r_kind
+  use shtns, only: shtns_init, spectogrd, grdtospec, nlm
+  use gfsdc_dycore
+  implicit none
+  type(gfsdc_cfg) :: cfg
+  type(c_ptr) :: h
+  integer :: nsteps, nt, k, nlev
+  character(len=16) :: arg, cname, prec
+  real(c_double), allocatable, target :: ak(:), bk(:), diag(:)
+  complex(r_kind), allocatable :: spec(:), spec2(:)
+  real(r_kind), allocatable :: grid(:,:)
+  real(c_double) :: t, eta, b, err
+  real :: tc0, tc1
+  nsteps = 72
+  cname = 'T42L20'
+  prec = 'fp64'
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, arg)
+    read (arg, *) nsteps
+  end if
+  if (command_argument_count() >= 2) call get_command_argument(2, cname)
+  if (command_argument_count() >= 3) call get_command_argument(3, prec)
+  call dyc_defaults(cfg)
+  if (cname == 'T254L64') then
+    cfg%nlon = 768; cfg%nlat = 384; cfg%ntrunc = 254; cfg%nlev = 64; cfg%dt = 600.d0
+  else
+    cfg%nlon = 128; cfg%nlat = 64; cfg%ntrunc = 42; cfg%nlev = 20; cfg%dt = 1200.d0
+  end if
+  nlev = cfg%nlev
+  allocate(ak(nlev + 1), bk(nlev + 1), diag(4))
+  do k = 0, nlev
+    eta = real(k, c_double) / nlev
+    if (nlev == 20) then          ! equally spaced sigma
+      ak(k + 1) = 0.d0; bk(k + 1) = eta
+    else                          ! hybrid: B = ((eta-0.2)/0.8)**1.5, A = p0 (eta - B)
+      b = max(0.d0, (eta - 0.2d0) / 0.8d0) ** 1.5d0
+      if (k == nlev) b = 1.d0
+      bk(k + 1) = b; ak(k + 1) = 1.d5 * (eta - b)
+      if (k == nlev) ak(k + 1) = 0.d0
+    end if
+  end do
+  cfg%ak = c_loc(ak)
+  cfg%bk = c_loc(bk)
+  cfg%physics = PHYS_HS
+  cfg%precision = GFSDC_FP64
+  if (prec == 'fp32') cfg%precision = GFSDC_FP32
+  if (dyc_create(cfg, h) /= 0) stop 'gfsdc_create failed'
+  if (dyc_init_rest_state(h, 300.d0, 1.d5, 1.d-6, 1, 21, 0_c_long_long) /= 0) stop 'init failed'
+  ! transform wrapper round trip on the same grid (module shtns)
+  call shtns_init(cfg%nlon, cfg%nlat, cfg%ntrunc)
+  allocate(spec(nlm), spec2(nlm), grid(cfg%nlon, cfg%nlat))
+  spec = 0
+  spec(2) = (1.0, 0.0)
+  spec(cfg%ntrunc + 3) = (0.5, -0.25)
+  call spectogrd(spec, grid)
+  call grdtospec(grid, spec2)
+  err = maxval(abs(spec2 - spec))
+  print '(a,es10.3)', 'shtns round trip max error = ', err
+  t = 0.d0
+  do nt = 1, nsteps
+    call cpu_time(tc0)
+    if (dyc_advance(h, 1, 1) /= 0) stop 'advance failed'
+    if (dyc_step_diag(h, c_loc(diag)) /= 0) stop 'diag failed'
+    call cpu_time(tc1)
+    t = t + cfg%dt
+    write (6, 8998) t / 3600.d0, diag(1), diag(2) / 100., diag(3) / 100., diag(4), tc1 - tc0
+  end do
+8998 format('t = ',f0.3,' hrs, spdmax = ',f7.3,', min/max ps = ',f7.2,'/',f7.2,', pstend = ',f0.3,', cpu time = ',f0.3)
+  if (dyc_destroy(h) /= 0) stop 'destroy failed'
+end program hs_driver
