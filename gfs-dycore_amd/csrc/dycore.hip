@@ -11,6 +11,11 @@
 //     spectral coefficient (one wavefront per coefficient, lane = level), implicit
 //     damping, then the time-split Held-Suarez physics (phy_run_hs.f90:22) and the
 //     dry-mass fixer (dyn_run.f90:605) with an FP64 global sum.
+// ln(ps) is synthesised without its global-mean coefficient c0 = q_00 / sqrt(4 pi):
+// ps = exp(c0 + ln ps') is formed in FP64 and the mass fixer analyses
+// ln ps' + ln(pcorr) (= log(ps pcorr) - c0), adding c0 back to q_00. This is the
+// reference's arithmetic (ln(ps) and log(ps pcorr) transformed to spectral space),
+// rearranged so that the FP32 path does not round a ~11.5 grid field to ~1e-6.
 #include <cmath>
 #include <cstring>
 
@@ -85,7 +90,8 @@ __global__ __launch_bounds__(256) void k_columns(
     T *__restrict__ psg_out, T *__restrict__ dlnpsdt_out, const T *__restrict__ dphisdx,
     const T *__restrict__ dphisdy, const T *__restrict__ ak_, const T *__restrict__ bk_,
     const T *__restrict__ ck_, const T *__restrict__ dbk_, const T *__restrict__ sinlat, int nl, int ntr,
-    int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only) {
+    int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only,
+    const cplx<T> *__restrict__ spec, int nfs) {
     __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1], ck[MAXLEV], dbk[MAXLEV];
     for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
         ak[i] = ak_[i];
@@ -103,8 +109,9 @@ __global__ __launch_bounds__(256) void k_columns(
     const T *U = G, *V = G + (long)nl * ng, *VRT = G + 2L * nl * ng, *DIV = G + 3L * nl * ng;
     const T *TV = G + 4L * nl * ng, *TX = G + 5L * nl * ng, *TY = G + 6L * nl * ng, *Q = G + 7L * nl * ng;
     const T *LNPS = G + (7L + ntr) * nl * ng;
-    const T lnps = LNPS[0], dpx = LNPS[ng], dpy = LNPS[2 * ng];
-    const T ps = exp(lnps);
+    const T dpx = LNPS[ng], dpy = LNPS[2 * ng];
+    const double c0 = (double)spec[(long)mem * nfs + 3L * nl + (long)ntr * nl].x * 0.28209479177387814;  // 1/sqrt(4 pi)
+    const T ps = (T)exp(c0 + (double)LNPS[0]);
     T *ET = etadot + (long)mem * (nl + 1) * ng + col;
     T *WD = dlnpdt + (long)mem * nl * ng + col;
     const T alfa1 = T(SGLF(0.69314718055994530942)), rlnp1 = T(SGLF(99999.99));
@@ -247,8 +254,9 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
         A[2L * nl * ng + o] = -(blprof * kdrag) * DIV[o];
     }
     if (massfix) {
-        T pcorr = T(pdryini[mem] / pmean[mem]);
-        A[3L * nl * ng] = log(ps * pcorr);
+        // log(ps * pcorr) - c0 = ln ps' + log(pcorr)
+        const T *LNPS = G + (7L + ntr) * nl * ng;
+        A[3L * nl * ng] = LNPS[0] + (T)log(pdryini[mem] / pmean[mem]);
     }
 }
 
@@ -519,7 +527,9 @@ __global__ void k_phys_update(cplx<T> *__restrict__ spec, const cplx<T> *__restr
     if (f < 3 * nl + ntr * nl) {
         spec[i] = cplx<T>{s.x + dt * t.x, s.y + dt * t.y};
     } else if (massfix) {
-        // dlnpsspecdt = (workspec - lnpsspec)/dt; lnpsspec += dt*dlnpsspecdt
+        // dlnpsspecdt = (workspec - lnpsspec)/dt; lnpsspec += dt*dlnpsspecdt, where
+        // workspec = analysis of the deviation field + q_00 (the global mean, k = 0)
+        if (i < nfst) t.x += s.x;
         cplx<T> d{(t.x - s.x) / dt, (t.y - s.y) / dt};
         spec[i] = cplx<T>{s.x + dt * d.x, s.y + dt * d.y};
     }
@@ -849,7 +859,7 @@ template <typename T> int Engine<T>::dyn_init() {
         }
         for (int k = 0; k < ntr * nl; k++) d[7 * nl + k] = SynthDesc{SK_COPY, tr + k, -1, 1.0, 0.0, 0, 0};
         int o = 7 * nl + ntr * nl;
-        d[o] = SynthDesc{SK_COPY, lp, -1, 1.0, 0.0, 0, 0};      // lnpsg
+        d[o] = SynthDesc{SK_COPY_DEV, lp, -1, 1.0, 0.0, 0, 0};  // lnpsg minus its global mean
         d[o + 1] = SynthDesc{SK_UHAT, -1, lp, 0.0, ra_, 0, 0};  // dlnpsdx
         d[o + 2] = SynthDesc{SK_VHAT, -1, lp, 0.0, ra_, 0, 0};  // dlnpsdy
         r[o + 1] = r[o + 2] = 1;
@@ -945,7 +955,8 @@ template <typename T> struct Step {
                            (T *)B.dlnpdt, (T *)B.psg, (T *)B.dlnpsdt, (const T *)B.dphisdx, (const T *)B.dphisdy,
                            (const T *)B.ak, (const T *)B.bk, (const T *)B.ck, (const T *)B.dbk,
                            (const T *)B.sinlat, B.nl, B.ntr, B.nsyn, B.nan, B.ng, E.plan.nlon, (T)c.rd,
-                           (T)(c.rd / c.cp), (T)(c.cvap / c.cp - 1.0), (T)(2.0 * c.omega), inverse_only);
+                           (T)(c.rd / c.cp), (T)(c.cvap / c.cp - 1.0), (T)(2.0 * c.omega), inverse_only,
+                           (const cplx<T> *)B.spec, B.nfs);
         mark(inverse_only ? "columns_inverse" : "columns");
     }
     void ps_mean(bool set_pdry, bool mean) {

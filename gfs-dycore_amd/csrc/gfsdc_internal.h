@@ -61,7 +61,8 @@ template <typename T> void plan_destroy(Plan<T> &p);
 
 // Descriptors of the synthesis gather (spectral state -> extended spectral
 // columns) and of the analysis scatter (extended columns -> spectral output).
-enum SynthKind : int { SK_ZERO = 0, SK_COPY = 1, SK_UHAT = 2, SK_VHAT = 3 };
+// SK_COPY_DEV copies a field without its global mean (n = 0) coefficient.
+enum SynthKind : int { SK_ZERO = 0, SK_COPY = 1, SK_UHAT = 2, SK_VHAT = 3, SK_COPY_DEV = 4 };
 struct SynthDesc {
     int kind;
     int a, b;          // source fields (S and T of the vector pair; a for COPY)

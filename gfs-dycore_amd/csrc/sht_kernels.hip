@@ -488,7 +488,7 @@ __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk
     const int m = mn.x, n = mn.y;
     const SynthDesc d = desc[fo];
     cplx<T> r{T(0), T(0)};
-    if (d.kind == SK_COPY) {
+    if (d.kind == SK_COPY || (d.kind == SK_COPY_DEV && n > 0)) {
         r = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
     } else if (d.kind == SK_UHAT || d.kind == SK_VHAT) {
         const T *ep = eps + eoff[m];

@@ -60,7 +60,7 @@ def test_getdyntend(kind, nlev, prec):
     tol = 1e-9 if prec == "fp64" else 2e-4
     for k in ("dvrt", "ddiv", "dtv", "dlnps"):
         assert rel(tg[k][0], to[k]) < tol, (k, rel(tg[k][0], to[k]))
-    assert abs(g.pdryini[0] - o.pdryini) < 1e-9 * o.pdryini
+    assert abs(g.pdryini[0] - o.pdryini) < (1e-9 if prec == "fp64" else 2e-6) * o.pdryini
     for name in ("ug", "vg", "virtempg", "dlnpdtg", "etadot", "psg", "dlnpsdt"):
         a, b = g.grid(name)[0], o.grid(name)
         assert rel(a, b) < (1e-9 if prec == "fp64" else 2e-4), (name, rel(a, b))
@@ -109,7 +109,7 @@ def test_advance_fp32_one_day_t42():
             for k in ("vrt", "div", "tv", "lnps")}
     print("FP32 one-day spectral relative error (T42L20):", errs)
     assert errs["tv"] < 1e-5 and errs["lnps"] < 1e-5
-    assert errs["vrt"] < 1e-3 and errs["div"] < 1e-2
+    assert errs["vrt"] < 3e-5 and errs["div"] < 5e-5
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

@@ -1,7 +1,8 @@
 """GPU parity of the shtns.f90 transform API (libgfsdc) against the FP64 oracle.
 
 Tolerances: FP64 path 1e-11 relative (max-norm over the field batch);
-FP32 path 2e-6 relative for scalar transforms and 5e-6 for vector ones. The
+FP32 path 2e-6 relative for scalar transforms and 5e-6 for vector ones
+(2e-5 at T254, where the FP32 Legendre sums run over up to 256 degrees). The
 FP32 vorticity/divergence analysis error is measured against the larger of
 the two output fields: the wind is dominated by its rotational part, so FP32
 rounding of u, v is of that size in both outputs.
@@ -74,8 +75,9 @@ def test_vector_transforms(pair, nf):
     gx0, gy0 = o.getgrad(div, A)
     gx, gy = g64.getgrad(div, A)
     assert rel(gx, gx0) < 1e-11 and rel(gy, gy0) < 1e-11
+    tol32 = 5e-6 if o.ntrunc < 200 else 2e-5  # FP32 sums over up to T+2 degrees
     u32, v32 = g32.getuv(vrt.astype(np.complex64), div.astype(np.complex64), A, dtype=1)
-    assert rel(u32, u0) < 5e-6 and rel(v32, v0) < 5e-6
+    assert rel(u32, u0) < tol32 and rel(v32, v0) < tol32
     z32, d32 = g32.getvrtdivspec(u0.astype(np.float32), v0.astype(np.float32), A, dtype=1)
     scale = max(np.abs(z0).max(), np.abs(d0).max())
-    assert np.abs(z32 - z0).max() / scale < 5e-6 and np.abs(d32 - d0).max() / scale < 5e-6
+    assert np.abs(z32 - z0).max() / scale < tol32 and np.abs(d32 - d0).max() / scale < tol32
