@@ -63,6 +63,7 @@ struct DycoreBuffers {
     int *kbydeg = nullptr;                       // coefficients sorted by degree
     int *degree = nullptr;                       // degree of coefficient k
     double *pdryini = nullptr, *pmean = nullptr; // FP64 per member
+    double *pspart = nullptr;                    // FP64 partial sums (PS_BLOCKS per member)
     double *diag = nullptr;                      // FP64 4 per member
     SynthDesc *sdesc = nullptr;                  // nm*nsyn
     AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr;
@@ -261,19 +262,35 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
 }
 
 // area-weighted global mean of ps per member (FP64); optionally sets pdryini
-// where it is still 0 (dyn_run.f90:146-149).
+// where it is still 0 (dyn_run.f90:146-149). Two deterministic stages:
+// PS_BLOCKS partial sums per member, then a fixed-order final sum.
+constexpr int PS_BLOCKS = 128;
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_ps_mean(const T *__restrict__ psg, const double *__restrict__ areaw,
-                                                  double *__restrict__ out, double *__restrict__ pdryini,
-                                                  long ng, int nlon) {
-    const int mem = blockIdx.x;
+__global__ __launch_bounds__(256) void k_ps_partial(const T *__restrict__ psg, const double *__restrict__ areaw,
+                                                    double *__restrict__ part, long ng, int nlon) {
+    const int mem = blockIdx.y;
     const T *p = psg + (long)mem * ng;
     double s = 0.0;
-    for (long i = threadIdx.x; i < ng; i += blockDim.x) s += areaw[i / nlon] * (double)p[i];
-    __shared__ double red[1024];
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < ng; i += (long)PS_BLOCKS * 256)
+        s += areaw[i / nlon] * (double)p[i];
+    __shared__ double red[256];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[mem * PS_BLOCKS + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(PS_BLOCKS) void k_ps_final(const double *__restrict__ part, double *__restrict__ out,
+                                                         double *__restrict__ pdryini) {
+    const int mem = blockIdx.x;
+    __shared__ double red[PS_BLOCKS];
+    red[threadIdx.x] = part[mem * PS_BLOCKS + threadIdx.x];
+    __syncthreads();
+    for (int w = PS_BLOCKS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -820,10 +837,11 @@ template <typename T> int Engine<T>::dyn_init() {
     B->dphisdy = dalloc<T>((size_t)nm * ng);
     B->pdryini = dalloc<double>(nm);
     B->pmean = dalloc<double>(nm);
+    B->pspart = dalloc<double>((size_t)nm * PS_BLOCKS);
     B->diag = dalloc<double>(4 * nm);
     void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
                     B->tphys, B->topo, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
-                    B->dphisdy, B->pdryini, B->pmean, B->diag};
+                    B->dphisdy, B->pdryini, B->pmean, B->diag, B->pspart};
     for (void *p : ptrs)
         if (!p) {
             set_error("device allocation failed for the dycore state");
@@ -911,7 +929,7 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->tphys, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
                     B->dphisdy, B->topo, B->at, B->bt, B->dt3, B->sv, B->tor, B->disspec, B->diffp, B->dmpp,
                     B->ak, B->bk, B->ck, B->dbk, B->sinlat, B->coslat, B->areaw, B->kbydeg, B->degree,
-                    B->pdryini, B->pmean, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->rflag_syn,
+                    B->pdryini, B->pmean, B->pspart, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->rflag_syn,
                     B->rflag_an, B->rflag_hs};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -960,9 +978,10 @@ template <typename T> struct Step {
         mark(inverse_only ? "columns_inverse" : "columns");
     }
     void ps_mean(bool set_pdry, bool mean) {
-        hipLaunchKernelGGL(k_ps_mean<T>, dim3(B.nm), dim3(1024), 0, s, (const T *)B.psg, (const double *)B.areaw,
-                           mean ? B.pmean : (double *)nullptr, set_pdry ? B.pdryini : (double *)nullptr, B.ng,
-                           E.plan.nlon);
+        hipLaunchKernelGGL(k_ps_partial<T>, dim3(PS_BLOCKS, B.nm), dim3(256), 0, s, (const T *)B.psg,
+                           (const double *)B.areaw, B.pspart, B.ng, E.plan.nlon);
+        hipLaunchKernelGGL(k_ps_final, dim3(B.nm), dim3(PS_BLOCKS), 0, s, (const double *)B.pspart,
+                           mean ? B.pmean : (double *)nullptr, set_pdry ? B.pdryini : (double *)nullptr);
         mark("ps_mean");
     }
     void analyse(const AnalDesc *desc, const unsigned char *rflag, int ngf, cplx<T> *dst) {

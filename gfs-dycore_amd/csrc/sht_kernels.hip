@@ -12,6 +12,7 @@
 // The arithmetic follows the transforms src/shtns.f90 requests from SHTns:
 // orthonormal harmonics without Condon-Shortley phase on a Gauss grid.
 #include <cmath>
+#include <cstdlib>
 
 #include "gfsdc_internal.h"
 
@@ -133,167 +134,283 @@ __global__ void k_legendre_table(T *plm, const long *poff, const double *mu, int
 }
 
 // ---------------------------------------------------------------------------
-// Legendre synthesis GEMM. One workgroup = one order m, 64 latitude rows, 128
-// columns; 4 waves in a 2x2 arrangement, each wave 32 x 64 via 2 x 4 MFMA
-// 16x16x4 blocks for the even and for the odd degrees.
-// fs[m][j][c] = E + O (north row j), fs[m][nlat-1-j][c] = E - O (south row).
+// Legendre GEMMs. One workgroup = one order m, a 64-row x 128-column output tile
+// for both parities of n - m; 4 waves in a 2x2 arrangement, each wave 32 x 64
+// via 2 x 4 MFMA 16x16x4 blocks per parity. Global loads are 16-byte vectors,
+// prefetched into registers for chunk k+1 while chunk k is computed from LDS.
+// FP32 swaps the MFMA operands (C^T = B^T A^T) so the four accumulators of a lane
+// are four consecutive columns of one output row and the epilogue stores 16-byte
+// vectors; FP64 keeps rows in the accumulator and consecutive lanes store
+// consecutive columns.
 constexpr int LDS_PAD = 16;
 
+template <typename T> struct Vec;
+template <> struct Vec<float> {
+    typedef float4 t;
+    static constexpr int n = 4;
+    static constexpr bool swap = true;
+};
+template <> struct Vec<double> {
+    typedef double2 t;
+    static constexpr int n = 2;
+    static constexpr bool swap = false;
+};
+
+// output columns per workgroup: 128 for FP32, 64 for FP64 (whose accumulators
+// take twice the registers); each wave covers half of them
+template <typename T> struct GT { static constexpr int BN = GEMM_BN, WB = 4; };
+template <> struct GT<double> { static constexpr int BN = 64, WB = 2; };
+
+template <typename T>
+__device__ inline void vstore_lds(T *dst, const typename Vec<T>::t &v) {
+    *(typename Vec<T>::t *)dst = v;
+}
+
+template <typename T, typename Acc>
+__device__ inline void gemm_chunk(T (*As)[GEMM_BK][GEMM_BM + LDS_PAD], T (*Bs)[GEMM_BK][GT<T>::BN + LDS_PAD],
+                                  Acc (&E)[2][GT<T>::WB], Acc (&O)[2][GT<T>::WB], int lane, int wm, int wn) {
+#pragma unroll
+    for (int ks = 0; ks < GEMM_BK; ks += 4) {
+        const int kr = ks + (lane >> 4), li = lane & 15;
+        T ae[2], ao[2];
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++) {
+            ae[bi] = As[0][kr][wm + bi * 16 + li];
+            ao[bi] = As[1][kr][wm + bi * 16 + li];
+        }
+#pragma unroll
+        for (int bj = 0; bj < GT<T>::WB; bj++) {
+            T be = Bs[0][kr][wn + bj * 16 + li];
+            T bo = Bs[1][kr][wn + bj * 16 + li];
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) {
+                if (Vec<T>::swap) {
+                    E[bi][bj] = MF<T>::mma(be, ae[bi], E[bi][bj]);
+                    O[bi][bj] = MF<T>::mma(bo, ao[bi], O[bi][bj]);
+                } else {
+                    E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
+                    O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
+                }
+            }
+        }
+    }
+}
+
+// Synthesis: fs[m][j][c] = E + O (north row j), fs[m][nlat-1-j][c] = E - O.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, const long *__restrict__ poff,
                                                    const int *__restrict__ keoff, const T *__restrict__ sx,
                                                    int ldx, T *__restrict__ fs, int ldf, int nhalf, int nhp,
                                                    int nlat, int ntrunc, int ntj) {
     typedef typename MF<T>::acc acc_t;
+    typedef typename Vec<T>::t V;
+    constexpr int VW = Vec<T>::n;
+    constexpr int BN = GT<T>::BN, WB = GT<T>::WB;
+    constexpr int NA = GEMM_BK * GEMM_BM / VW / 256, NB = GEMM_BK * BN / VW / 256;
     const int tc = blockIdx.x;
     const int mt = blockIdx.y;
     const int m = mt / ntj, tj = mt - m * ntj;
     const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
-    const T *Pe = plm + poff[m];
-    const T *Po = Pe + (long)Ke * nhp;
+    const T *P[2] = {plm + poff[m], plm + poff[m] + (long)Ke * nhp};
+    const int Kp[2] = {Ke, Ko};
     const T *S = sx + (long)keoff[m] * ldx;
-    const int j0 = tj * GEMM_BM, c0 = tc * GEMM_BN;
+    const int j0 = tj * GEMM_BM, c0 = tc * BN;
     __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
-    __shared__ T Bs[2][GEMM_BK][GEMM_BN + LDS_PAD];
+    __shared__ T Bs[2][GEMM_BK][BN + LDS_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
-    acc_t E[2][4], O[2][4];
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * (BN / 2);
+    acc_t E[2][WB], O[2][WB];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
+        for (int b = 0; b < WB; b++) {
             E[a][b] = acc_t{0, 0, 0, 0};
             O[a][b] = acc_t{0, 0, 0, 0};
         }
+    V ra[2][NA], rb[2][NB];
+    const V zero{};
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW, i = k0 + kk;
+                ra[par][u] = (i < Kp[par]) ? *(const V *)(P[par] + (long)i * nhp + j0 + jj) : zero;
+            }
+#pragma unroll
+            for (int u = 0; u < NB; u++) {
+                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW, i = k0 + kk;
+                rb[par][u] = (i < Kp[par]) ? *(const V *)(S + (long)(2 * i + par) * ldx + c0 + cc) : zero;
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW;
+                vstore_lds<T>(&As[par][kk][jj], ra[par][u]);
+            }
+#pragma unroll
+            for (int u = 0; u < NB; u++) {
+                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
+                vstore_lds<T>(&Bs[par][kk][cc], rb[par][u]);
+            }
+        }
+    };
+    // FP32 prefetches chunk k+1 during chunk k; FP64 (twice the registers per
+    // element) loads each chunk just before storing it to keep occupancy
+    constexpr bool PREFETCH = sizeof(T) == 4;
+    if (PREFETCH) load(0);
     for (int k0 = 0; k0 < Ke; k0 += GEMM_BK) {
-#pragma unroll
-        for (int t = tid; t < GEMM_BK * GEMM_BM; t += 256) {
-            int kk = t / GEMM_BM, jj = t - kk * GEMM_BM, i = k0 + kk;
-            As[0][kk][jj] = (i < Ke) ? Pe[(long)i * nhp + j0 + jj] : T(0);
-            As[1][kk][jj] = (i < Ko) ? Po[(long)i * nhp + j0 + jj] : T(0);
-        }
-#pragma unroll
-        for (int t = tid; t < GEMM_BK * GEMM_BN; t += 256) {
-            int kk = t / GEMM_BN, cc = t - kk * GEMM_BN, i = k0 + kk;
-            Bs[0][kk][cc] = (i < Ke) ? S[(long)(2 * i) * ldx + c0 + cc] : T(0);
-            Bs[1][kk][cc] = (i < Ko) ? S[(long)(2 * i + 1) * ldx + c0 + cc] : T(0);
-        }
+        if (!PREFETCH) load(k0);
+        store();
         __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < GEMM_BK; ks += 4) {
-            const int kr = ks + (lane >> 4), li = lane & 15;
-            T ae[2], ao[2];
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                ae[bi] = As[0][kr][wm + bi * 16 + li];
-                ao[bi] = As[1][kr][wm + bi * 16 + li];
-            }
-#pragma unroll
-            for (int bj = 0; bj < 4; bj++) {
-                T be = Bs[0][kr][wn + bj * 16 + li];
-                T bo = Bs[1][kr][wn + bj * 16 + li];
-#pragma unroll
-                for (int bi = 0; bi < 2; bi++) {
-                    E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
-                    O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
-                }
-            }
-        }
+        if (PREFETCH && k0 + GEMM_BK < Ke) load(k0 + GEMM_BK);
+        gemm_chunk<T>(As, Bs, E, O, lane, wm, wn);
         __syncthreads();
     }
+    T *FN = fs + (long)m * nlat * ldf;
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                int j = j0 + wm + bi * 16 + MF<T>::row(lane, r);
-                int c = c0 + wn + bj * 16 + (lane & 15);
+        for (int bj = 0; bj < WB; bj++) {
+            if (Vec<T>::swap) {
+                int j = j0 + wm + bi * 16 + (lane & 15);
+                int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
                 if (j < nhalf) {
-                    T e = E[bi][bj][r], o = O[bi][bj][r];
-                    fs[((long)m * nlat + j) * ldf + c] = e + o;
-                    fs[((long)m * nlat + (nlat - 1 - j)) * ldf + c] = e - o;
+                    acc_t e = E[bi][bj], o = O[bi][bj];
+                    V vn, vs;
+                    T *pn = (T *)&vn, *ps_ = (T *)&vs;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        pn[r] = e[r] + o[r];
+                        ps_[r] = e[r] - o[r];
+                    }
+                    *(V *)(FN + (long)j * ldf + c) = vn;
+                    *(V *)(FN + (long)(nlat - 1 - j) * ldf + c) = vs;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int j = j0 + wm + bi * 16 + MF<T>::row(lane, r);
+                    int c = c0 + wn + bj * 16 + (lane & 15);
+                    if (j < nhalf) {
+                        T e = E[bi][bj][r], o = O[bi][bj][r];
+                        FN[(long)j * ldf + c] = e + o;
+                        FN[(long)(nlat - 1 - j) * ldf + c] = e - o;
+                    }
                 }
             }
+        }
 }
 
-// Legendre analysis GEMM. One workgroup = one order m, 64 output degrees of
-// each parity, 128 columns; K runs over the (zero-padded) latitude pairs.
-// fa[m][0][j][c] holds w_j (F_north + F_south), fa[m][1][j][c] w_j (F_north - F_south).
+// Analysis: output degrees i' of each parity, K over the (zero-padded) latitude
+// pairs. fa[m][0][j][c] = w_j (F_N + F_S), fa[m][1][j][c] = w_j (F_N - F_S).
 template <typename T>
 __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, const long *__restrict__ poff,
                                                   const int *__restrict__ keoff, const int *__restrict__ mtile_m,
                                                   const int *__restrict__ mtile_i, const T *__restrict__ fa,
                                                   int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc) {
     typedef typename MF<T>::acc acc_t;
+    typedef typename Vec<T>::t V;
+    constexpr int VW = Vec<T>::n;
+    constexpr int BN = GT<T>::BN, WB = GT<T>::WB;
+    constexpr int NA = GEMM_BK * GEMM_BM / VW / 256, NB = GEMM_BK * BN / VW / 256;
     const int tc = blockIdx.x;
     const int m = mtile_m[blockIdx.y], i0 = mtile_i[blockIdx.y];
     const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
-    const T *Pe = plm + poff[m];
-    const T *Po = Pe + (long)Ke * nhp;
-    const T *Fe = fa + (long)(2 * m) * nhp * ldf;
-    const T *Fo = Fe + (long)nhp * ldf;
-    const int c0 = tc * GEMM_BN;
+    const T *P[2] = {plm + poff[m], plm + poff[m] + (long)Ke * nhp};
+    const int Kp[2] = {Ke, Ko};
+    const T *F[2] = {fa + (long)(2 * m) * nhp * ldf, fa + (long)(2 * m + 1) * nhp * ldf};
+    const int c0 = tc * BN;
     __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
-    __shared__ T Bs[2][GEMM_BK][GEMM_BN + LDS_PAD];
+    __shared__ T Bs[2][GEMM_BK][BN + LDS_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
-    acc_t E[2][4], O[2][4];
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * (BN / 2);
+    acc_t E[2][WB], O[2][WB];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
+        for (int b = 0; b < WB; b++) {
             E[a][b] = acc_t{0, 0, 0, 0};
             O[a][b] = acc_t{0, 0, 0, 0};
         }
+    V ra[2][NA], rb[2][NB];
+    const V zero{};
+    auto load = [&](int j0) {
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW, i = i0 + ii;
+                ra[par][u] = (i < Kp[par]) ? *(const V *)(P[par] + (long)i * nhp + j0 + kk) : zero;
+            }
+#pragma unroll
+            for (int u = 0; u < NB; u++) {
+                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
+                rb[par][u] = *(const V *)(F[par] + (long)(j0 + kk) * ldf + c0 + cc);
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW;
+                const T *pv = (const T *)&ra[par][u];
+#pragma unroll
+                for (int q = 0; q < VW; q++) As[par][kk + q][ii] = pv[q];
+            }
+#pragma unroll
+            for (int u = 0; u < NB; u++) {
+                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
+                vstore_lds<T>(&Bs[par][kk][cc], rb[par][u]);
+            }
+        }
+    };
+    // FP32 prefetches chunk k+1 during chunk k; FP64 (twice the registers per
+    // element) loads each chunk just before storing it to keep occupancy
+    constexpr bool PREFETCH = sizeof(T) == 4;
+    if (PREFETCH) load(0);
     for (int j0 = 0; j0 < nhp; j0 += GEMM_BK) {
-#pragma unroll
-        for (int t = tid; t < GEMM_BK * GEMM_BM; t += 256) {
-            int ii = t / GEMM_BK, kk = t - ii * GEMM_BK, i = i0 + ii;
-            As[0][kk][ii] = (i < Ke) ? Pe[(long)i * nhp + j0 + kk] : T(0);
-            As[1][kk][ii] = (i < Ko) ? Po[(long)i * nhp + j0 + kk] : T(0);
-        }
-#pragma unroll
-        for (int t = tid; t < GEMM_BK * GEMM_BN; t += 256) {
-            int kk = t / GEMM_BN, cc = t - kk * GEMM_BN;
-            Bs[0][kk][cc] = Fe[(long)(j0 + kk) * ldf + c0 + cc];
-            Bs[1][kk][cc] = Fo[(long)(j0 + kk) * ldf + c0 + cc];
-        }
+        if (!PREFETCH) load(j0);
+        store();
         __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < GEMM_BK; ks += 4) {
-            const int kr = ks + (lane >> 4), li = lane & 15;
-            T ae[2], ao[2];
-#pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                ae[bi] = As[0][kr][wm + bi * 16 + li];
-                ao[bi] = As[1][kr][wm + bi * 16 + li];
-            }
-#pragma unroll
-            for (int bj = 0; bj < 4; bj++) {
-                T be = Bs[0][kr][wn + bj * 16 + li];
-                T bo = Bs[1][kr][wn + bj * 16 + li];
-#pragma unroll
-                for (int bi = 0; bi < 2; bi++) {
-                    E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
-                    O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
-                }
-            }
-        }
+        if (PREFETCH && j0 + GEMM_BK < nhp) load(j0 + GEMM_BK);
+        gemm_chunk<T>(As, Bs, E, O, lane, wm, wn);
         __syncthreads();
     }
     T *X = sx + (long)keoff[m] * ldx;
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 4; bj++)
+        for (int bj = 0; bj < WB; bj++) {
+            if (Vec<T>::swap) {
+                int i = i0 + wm + bi * 16 + (lane & 15);
+                int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
+                acc_t e = E[bi][bj], o = O[bi][bj];
+                V ve, vo;
+                T *pe = (T *)&ve, *po = (T *)&vo;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                int i = i0 + wm + bi * 16 + MF<T>::row(lane, r);
-                int c = c0 + wn + bj * 16 + (lane & 15);
-                if (i < Ke) X[(long)(2 * i) * ldx + c] = E[bi][bj][r];
-                if (i < Ko) X[(long)(2 * i + 1) * ldx + c] = O[bi][bj][r];
+                for (int r = 0; r < 4; r++) {
+                    pe[r] = e[r];
+                    po[r] = o[r];
+                }
+                if (i < Ke) *(V *)(X + (long)(2 * i) * ldx + c) = ve;
+                if (i < Ko) *(V *)(X + (long)(2 * i + 1) * ldx + c) = vo;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int i = i0 + wm + bi * 16 + MF<T>::row(lane, r);
+                    int c = c0 + wn + bj * 16 + (lane & 15);
+                    if (i < Ke) X[(long)(2 * i) * ldx + c] = E[bi][bj][r];
+                    if (i < Ko) X[(long)(2 * i + 1) * ldx + c] = O[bi][bj][r];
+                }
             }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -386,35 +503,61 @@ struct Radices {
 };
 
 template <typename T> struct FftCfg;
-template <> struct FftCfg<float> { static constexpr int NFFT = 8, MAXN = 768; };
+template <> struct FftCfg<float> { static constexpr int NFFT = 4, MAXN = 768; };
 template <> struct FftCfg<double> { static constexpr int NFFT = 4, MAXN = 768; };
+static_assert(FftCfg<double>::MAXN * 8 * 16 + FftCfg<double>::MAXN * 16 <= 160 * 1024, "FFT LDS budget");
 
-// Fourier synthesis: one workgroup per latitude pair and NFFT fields.
+// Fourier synthesis: one workgroup per latitude pair and NFFT fields. The
+// twiddle table is staged in LDS once, the Fourier coefficients of both
+// hemispheres are loaded in one batch per thread before any LDS write, and
+// only the spectral gap T < k < N-T is zero-filled.
 template <typename T, int NFFT, int MAXN>
 __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int ldf, T *__restrict__ grid,
                                                    long gstride, int nf, const unsigned char *__restrict__ rflag,
                                                    const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
                                                    int nlon, int nlat, int ntrunc, Radices rd) {
     __shared__ cplx<T> buf[NFFT * MAXN];
+    __shared__ cplx<T> tws[MAXN];
     const int jp = blockIdx.x, f0 = blockIdx.y * NFFT;
     const int j = jp, j2 = nlat - 1 - jp, N = nlon, tid = threadIdx.x;
-    for (int t = tid; t < NFFT * N; t += 256) buf[t] = cplx<T>{T(0), T(0)};
-    __syncthreads();
-    for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
-        int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
-        if (f >= nf) continue;
-        const T *rn = fs + ((long)m * nlat + j) * ldf + 2 * f;
-        const T *rs = fs + ((long)m * nlat + j2) * ldf + 2 * f;
-        T xr = rn[0], xi = rn[1], yr = rs[0], yi = rs[1];
-        if (m == 0) {
-            xi = T(0);
-            yi = T(0);
+    constexpr int MAXI = (NFFT * (MAXN / 2) + 255) / 256;
+    const int nitems = (ntrunc + 1) * NFFT;
+    cplx<T> xn[MAXI], xs[MAXI];
+#pragma unroll
+    for (int c = 0; c < MAXI; c++) {
+        int idx = tid + c * 256;
+        xn[c] = cplx<T>{T(0), T(0)};
+        xs[c] = xn[c];
+        if (idx < nitems) {
+            int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
+            if (f < nf) {
+                xn[c] = *(const cplx<T> *)(fs + ((long)m * nlat + j) * ldf + 2 * f);
+                xs[c] = *(const cplx<T> *)(fs + ((long)m * nlat + j2) * ldf + 2 * f);
+            }
         }
-        buf[q * N + m] = cplx<T>{xr - yi, xi + yr};
-        if (m > 0) buf[q * N + N - m] = cplx<T>{xr + yi, -xi + yr};
+    }
+    for (int t = tid; t < N; t += 256) tws[t] = tw[t];
+    const int gap = N - 2 * ntrunc - 1;
+    for (int t = tid; t < NFFT * gap; t += 256) {
+        int q = t / gap;
+        buf[q * N + ntrunc + 1 + (t - q * gap)] = cplx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int c = 0; c < MAXI; c++) {
+        int idx = tid + c * 256;
+        if (idx < nitems) {
+            int m = idx / NFFT, q = idx - m * NFFT;
+            T xr = xn[c].x, xi = xn[c].y, yr = xs[c].x, yi = xs[c].y;
+            if (m == 0) {
+                xi = T(0);
+                yi = T(0);
+            }
+            buf[q * N + m] = cplx<T>{xr - yi, xi + yr};
+            if (m > 0) buf[q * N + N - m] = cplx<T>{xr + yi, -xi + yr};
+        }
     }
     __syncthreads();
-    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tw, +1);
+    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, +1);
     const T rc = rcos[j];
     for (int t = tid; t < NFFT * N; t += 256) {
         int q = t / N, i = t - q * N, f = f0 + q;
@@ -426,7 +569,7 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
 }
 
 // Fourier analysis: rows of a latitude pair -> weighted symmetric / antisymmetric
-// Fourier coefficients, m = 0..T.
+// Fourier coefficients, m = 0..T. Grid rows are loaded in one batch per thread.
 template <typename T, int NFFT, int MAXN>
 __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, long gstride, int nf,
                                                   const unsigned char *__restrict__ rflag,
@@ -434,19 +577,32 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
                                                   const cplx<T> *__restrict__ tw, T *__restrict__ fa, int ldf,
                                                   int nlon, int nlat, int nhp, int ntrunc, Radices rd) {
     __shared__ cplx<T> buf[NFFT * MAXN];
+    __shared__ cplx<T> tws[MAXN];
     const int jp = blockIdx.x, f0 = blockIdx.y * NFFT;
     const int j = jp, j2 = nlat - 1 - jp, N = nlon, tid = threadIdx.x;
-    for (int t = tid; t < NFFT * N; t += 256) {
-        int q = t / N, i = t - q * N, f = f0 + q;
-        T x = T(0), y = T(0);
-        if (f < nf) {
-            x = grid[f * gstride + (long)j * nlon + i];
-            y = grid[f * gstride + (long)j2 * nlon + i];
+    constexpr int MAXG = (NFFT * MAXN + 255) / 256;
+    T xv[MAXG], yv[MAXG];
+#pragma unroll
+    for (int c = 0; c < MAXG; c++) {
+        int t = tid + c * 256;
+        xv[c] = T(0);
+        yv[c] = T(0);
+        if (t < NFFT * N) {
+            int q = t / N, i = t - q * N, f = f0 + q;
+            if (f < nf) {
+                xv[c] = grid[f * gstride + (long)j * nlon + i];
+                yv[c] = grid[f * gstride + (long)j2 * nlon + i];
+            }
         }
-        buf[t] = cplx<T>{x, y};
+    }
+    for (int t = tid; t < N; t += 256) tws[t] = tw[t];
+#pragma unroll
+    for (int c = 0; c < MAXG; c++) {
+        int t = tid + c * 256;
+        if (t < NFFT * N) buf[t] = cplx<T>{xv[c], yv[c]};
     }
     __syncthreads();
-    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tw, -1);
+    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, -1);
     const T wj = wfac[j], rc = rcos[j];
     for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
         int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
@@ -457,10 +613,8 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
         T wf = rflag[f] ? wj * rc : wj;
         T *pe = fa + ((long)(2 * m) * nhp + jp) * ldf + 2 * f;
         T *po = fa + ((long)(2 * m + 1) * nhp + jp) * ldf + 2 * f;
-        pe[0] = wf * (xr + yr);
-        pe[1] = wf * (xi + yi);
-        po[0] = wf * (xr - yr);
-        po[1] = wf * (xi - yi);
+        *(cplx<T> *)pe = cplx<T>{wf * (xr + yr), wf * (xi + yi)};
+        *(cplx<T> *)po = cplx<T>{wf * (xr - yr), wf * (xi - yi)};
     }
 }
 
@@ -613,6 +767,11 @@ template <typename T> int plan_create(Plan<T> &p, int nlon, int nlat, int ntrunc
     p.nlme = p.nlm + ntrunc + 1;
     p.nrad = rd.n;
     for (int i = 0; i < rd.n; i++) p.rad[i] = rd.r[i];
+    p.fft_nf = FftCfg<T>::NFFT;
+    if (const char *e = getenv("GFSDC_FFT_NF")) {
+        int v = atoi(e);
+        if (v == 2 || v == 4 || v == 8) p.fft_nf = v;
+    }
     const int T_ = ntrunc;
     // offsets
     p.poff_h.resize(T_ + 2);
@@ -744,14 +903,14 @@ void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk,
 
 template <typename T>
 void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s) {
-    int ntj = p.nhp / GEMM_BM, ntc = (ncol + GEMM_BN - 1) / GEMM_BN;
+    int ntj = p.nhp / GEMM_BM, ntc = (ncol + GT<T>::BN - 1) / GT<T>::BN;
     hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc, (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff, p.keoff,
                        sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj);
 }
 
 template <typename T>
 void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s) {
-    int ntc = (ncol + GEMM_BN - 1) / GEMM_BN;
+    int ntc = (ncol + GT<T>::BN - 1) / GT<T>::BN;
     hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc, p.n_mtiles), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
                        p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc);
 }
@@ -763,23 +922,46 @@ static Radices radices_of(int n, const int *r) {
     return rd;
 }
 
-template <typename T>
-void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                      const unsigned char *rflag, hipStream_t s) {
-    constexpr int NF = FftCfg<T>::NFFT, MN = FftCfg<T>::MAXN;
+template <typename T, int NF>
+static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
+                         const unsigned char *rflag, hipStream_t s) {
+    constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf, (nf + NF - 1) / NF);
     hipLaunchKernelGGL((k_fft_synth<T, NF, MN>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, p.rcos,
                        (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad));
 }
 
-template <typename T>
-void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
-                     int ldf, hipStream_t s) {
-    constexpr int NF = FftCfg<T>::NFFT, MN = FftCfg<T>::MAXN;
+template <typename T, int NF>
+static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
+                        int ldf, hipStream_t s) {
+    constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf, (nf + NF - 1) / NF);
     hipLaunchKernelGGL((k_fft_anal<T, NF, MN>), g, dim3(256), 0, s, grid, gstride, nf, rflag, p.rcos, p.wfac,
                        (const cplx<T> *)p.tw, fa, ldf, p.nlon, p.nlat, p.nhp, p.ntrunc,
                        radices_of(p.nrad, p.rad));
+}
+
+// fields per workgroup of the FFT kernels (p.fft_nf, chosen by the tile autotuner or
+// GFSDC_FFT_NF): more fields give longer contiguous Fourier segments, fewer give
+// more workgroups per CU (LDS = fields * nlon * sizeof(complex))
+template <typename T>
+void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
+                      const unsigned char *rflag, hipStream_t s) {
+    switch (p.fft_nf) {
+    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, s); break;
+    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, s); break;
+    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, s); break;
+    }
+}
+
+template <typename T>
+void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
+                     int ldf, hipStream_t s) {
+    switch (p.fft_nf) {
+    case 2: fft_anal_nf<T, 2>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
+    case 4: fft_anal_nf<T, 4>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
+    default: fft_anal_nf<T, 8>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
+    }
 }
 
 template <typename T>
