@@ -43,6 +43,8 @@ void gfsdc_cfg_defaults(gfsdc_cfg *c) {
     c->rv = (double)4.6150e+2f;
     c->cp = (double)1.0046e+3f;
     c->cvap = (double)1.8460e+3f;
+    c->simple_test = 0;
+    c->turbflux = 1;
 }
 
 int gfsdc_create(const gfsdc_cfg *cfg, gfsdc_ctx **out) {
@@ -159,6 +161,9 @@ int gfsdc_getdyntend(gfsdc_ctx *c, int rkstage, int inverse_only, void *dvrt, vo
 int gfsdc_advance(gfsdc_ctx *c, int nsteps, int use_graph) { DISPATCH(c, advance(nsteps, use_graph)); }
 int gfsdc_hs_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, int dtype, int loc) {
     DISPATCH(c, hs_phytend(dvrt, ddiv, dtv, dtype, loc));
+}
+int gfsdc_simple_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc) {
+    DISPATCH(c, simple_phytend(dvrt, ddiv, dtv, dtrac, dtype, loc));
 }
 int gfsdc_get_grid(gfsdc_ctx *c, int which, void *out, int dtype, int loc) {
     DISPATCH(c, get_grid(which, out, dtype, loc));

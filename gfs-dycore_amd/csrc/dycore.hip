@@ -9,8 +9,11 @@
 //     batched analysis scattered straight into tendency fields.
 //   advance (run.f90:194-565): IMEX RK3 stages with the semi-implicit solve per
 //     spectral coefficient (one wavefront per coefficient, lane = level), implicit
-//     damping, then the time-split Held-Suarez physics (phy_run_hs.f90:22) and the
-//     dry-mass fixer (dyn_run.f90:605) with an FP64 global sum.
+//     damping, then the time-split physics -- Held-Suarez (phy_run_hs.f90:22) or
+//     the simple moist physics (phy_run_simple.f90:28, simple_physics_v5_1.f90) --
+//     and the dry-mass fixer (dyn_run.f90:605) with FP64 global sums.
+// Grid fields synthesised per member (nsyn): u, v, vrt, div, tv, dtv/dx, dtv/dy
+// (nlev each), tracers q, dq/dx, dq/dy (ntrac*nlev each), ln ps', dlnps/dx, dlnps/dy.
 // ln(ps) is synthesised without its global-mean coefficient c0 = q_00 / sqrt(4 pi):
 // ps = exp(c0 + ln ps') is formed in FP64 and the mass fixer analyses
 // ln ps' + ln(pcorr) (= log(ps pcorr) - c0), adding c0 back to q_00. This is the
@@ -44,6 +47,7 @@ struct DycoreBuffers {
     int nsyn = 0;   // synthesised grid fields per member
     int nan = 0;    // analysed grid fields per member
     int nhs = 0;    // Held-Suarez forcing fields per member (3 nlev + 1)
+    int nsp = 0;    // simple-physics grid fields per member: dT, du, dv, dq (nlev each), mass fixer
     long ng = 0;
     void *spec = nullptr, *spec0 = nullptr, *ten[3] = {nullptr, nullptr, nullptr};
     void *lin[3] = {nullptr, nullptr, nullptr};  // linear terms: div[nl], tv[nl], lnps per member
@@ -63,11 +67,16 @@ struct DycoreBuffers {
     int *kbydeg = nullptr;                       // coefficients sorted by degree
     int *degree = nullptr;                       // degree of coefficient k
     double *pdryini = nullptr, *pmean = nullptr; // FP64 per member
-    double *pspart = nullptr;                    // FP64 partial sums (PS_BLOCKS per member)
+    double *pspart = nullptr;                    // FP64 partial sums (2 x PS_BLOCKS per member)
+    double *pwsum = nullptr;                     // FP64 area sum of pwat per member
+    void *colq = nullptr;                        // T per member ng: column tracer mass sum(dpk q)
+    void *pwat = nullptr, *precip = nullptr;     // T per member ng (simple physics)
+    void *pblscr = nullptr;                      // T per member 2 nlev ng: PBL sweep coefficients
+    double *latd = nullptr;                      // FP64 latitude (radians), nlat
     double *diag = nullptr;                      // FP64 4 per member
     SynthDesc *sdesc = nullptr;                  // nm*nsyn
-    AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr;
-    unsigned char *rflag_syn = nullptr, *rflag_an = nullptr, *rflag_hs = nullptr;
+    AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr, *adesc_sp = nullptr;
+    unsigned char *rflag_syn = nullptr, *rflag_an = nullptr, *rflag_hs = nullptr, *rflag_sp = nullptr;
     // host copies of the FP64 operators
     std::vector<double> amhyb, bmhyb, svhyb, tor_hyb, d_hyb_m, disspec_h, diff_h, dmp_h, sl;
     hipGraph_t graph = nullptr;
@@ -92,7 +101,7 @@ __global__ __launch_bounds__(256) void k_columns(
     const T *__restrict__ dphisdy, const T *__restrict__ ak_, const T *__restrict__ bk_,
     const T *__restrict__ ck_, const T *__restrict__ dbk_, const T *__restrict__ sinlat, int nl, int ntr,
     int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only,
-    const cplx<T> *__restrict__ spec, int nfs) {
+    const cplx<T> *__restrict__ spec, int nfs, T *__restrict__ colq) {
     __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1], ck[MAXLEV], dbk[MAXLEV];
     for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
         ak[i] = ak_[i];
@@ -109,7 +118,8 @@ __global__ __launch_bounds__(256) void k_columns(
     const T *G = gsyn + (long)mem * nsyn * ng + col;
     const T *U = G, *V = G + (long)nl * ng, *VRT = G + 2L * nl * ng, *DIV = G + 3L * nl * ng;
     const T *TV = G + 4L * nl * ng, *TX = G + 5L * nl * ng, *TY = G + 6L * nl * ng, *Q = G + 7L * nl * ng;
-    const T *LNPS = G + (7L + ntr) * nl * ng;
+    const T *QX = Q + (long)ntr * nl * ng, *QY = QX + (long)ntr * nl * ng;
+    const T *LNPS = G + (7L + 3L * ntr) * nl * ng;
     const T dpx = LNPS[ng], dpy = LNPS[2 * ng];
     const double c0 = (double)spec[(long)mem * nfs + 3L * nl + (long)ntr * nl].x * 0.28209479177387814;  // 1/sqrt(4 pi)
     const T ps = (T)exp(c0 + (double)LNPS[0]);
@@ -154,6 +164,16 @@ __global__ __launch_bounds__(256) void k_columns(
     }
     psg_out[(long)mem * ng + col] = ps;
     dlnpsdt_out[(long)mem * ng + col] = dlnpsdt;
+    if (colq) {
+        // column tracer mass of the dry-mass diagnostic (dyn_run.f90:138-145)
+        T s = 0;
+        for (int K = 1; K <= nl; K++) {
+            T dpk = pk(K + 1) - pk(K), qs = 0;
+            for (int nt = 0; nt < ntr; nt++) qs += Q[((long)nt * nl + (nl - K)) * ng];
+            s += dpk * qs;
+        }
+        colq[(long)mem * ng + col] = s;
+    }
     if (inverse_only) return;
     T *A = gan + (long)mem * nan * ng + col;
     T *TT = A, *FX = A + (long)nl * ng, *FY = A + 2L * nl * ng, *KE = A + 3L * nl * ng;
@@ -208,6 +228,53 @@ __global__ __launch_bounds__(256) void k_columns(
         px3u -= rd * rlnp * tx;
         px3v -= rd * rlnp * ty;
     }
+    // pass 4: tracers (dyn_run.f90:279-291): horizontal advection with the
+    // synthesised gradient and the Thuburn (1993) flux-limited vertical
+    // advection of getvadv_tracers (dyn_run.f90:478-603), top -> bottom with a
+    // sliding window of layer values D_K, K = 1..nl (D_K = q at L = nl+1-K).
+    const T tiny = std::numeric_limits<T>::min();
+    auto guard = [&](T d) { return fabs(d) < tiny ? copysign(tiny, d) : d; };
+    for (int nt = 0; nt < ntr; nt++) {
+        const T *Qn = Q + (long)nt * nl * ng, *QXn = QX + (long)nt * nl * ng, *QYn = QY + (long)nt * nl * ng;
+        T *DQ = A + (4L * nl + 1 + (long)nt * nl) * ng;
+        T Dk = LV(Qn, nl), Dk1 = LV(Qn, nl - 1);
+        T mx = T(2) * Dk - Dk1;
+        T ddprev = (Dk >= T(0)) ? Dk - fmax(T(0), mx) : Dk - fmin(T(0), mx);  // dd_0 (unguarded)
+        T ddk = guard(Dk1 - Dk);                                                    // dd_1
+        T hprev = Dk;                                                                // half_0
+        for (int K = 1; K <= nl; K++) {
+            const T etop = ET[(long)(K - 1) * ng], ebot = ET[(long)K * ng];
+            const T dpk = pk(K + 1) - pk(K);
+            T hk, Dk2 = T(0), ddn = T(0);
+            if (K <= nl - 1) {
+                if (K + 1 <= nl - 1) {
+                    Dk2 = LV(Qn, nl - 1 - K);  // D_{K+2}
+                    ddn = guard(Dk2 - Dk1);
+                } else {  // dd_nl from the two lowest layers (unguarded)
+                    T mb = T(2) * Dk1 - Dk;
+                    ddn = (Dk1 >= T(0)) ? fmax(T(0), mb) - Dk1 : fmin(T(0), mb) - Dk1;
+                }
+                T h = T(0.5) * (Dk1 + Dk);
+                if (ebot > T(0)) {
+                    T phi = ddprev / ddk;
+                    hk = Dk + (phi + fabs(phi)) / (T(1) + fabs(phi)) * (h - Dk);
+                } else {
+                    T phi = ddn / ddk;
+                    hk = Dk1 + (phi + fabs(phi)) / (T(1) + fabs(phi)) * (h - Dk1);
+                }
+            } else {
+                hk = Dk;  // half_nl = lowest layer value
+            }
+            T vadv = (T(1) / dpk) * ((hk * ebot - hprev * etop) + (Dk * (etop - ebot)));
+            const int L = nl + 1 - K;
+            LV(DQ, L) = -LV(U, L) * LV(QXn, L) - LV(V, L) * LV(QYn, L) - vadv;
+            hprev = hk;
+            ddprev = ddk;
+            ddk = ddn;
+            Dk = Dk1;
+            Dk1 = Dk2;
+        }
+    }
 #undef LV
 }
 
@@ -256,48 +323,289 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
     }
     if (massfix) {
         // log(ps * pcorr) - c0 = ln ps' + log(pcorr)
-        const T *LNPS = G + (7L + ntr) * nl * ng;
+        const T *LNPS = G + (7L + 3L * ntr) * nl * ng;
         A[3L * nl * ng] = LNPS[0] + (T)log(pdryini[mem] / pmean[mem]);
     }
 }
 
-// area-weighted global mean of ps per member (FP64); optionally sets pdryini
-// where it is still 0 (dyn_run.f90:146-149). Two deterministic stages:
-// PS_BLOCKS partial sums per member, then a fixed-order final sum.
+// Area-weighted global sums per member in FP64 (areawts sum to 1, so these are
+// global means): S_a of field a (ps) and, when b is given, S_b of field b (the
+// column tracer mass, or pwat). Two deterministic stages: PS_BLOCKS partial
+// sums per member, then a fixed-order final sum. The final stage optionally
+// sets pdryini = S_a - S_b where it is still 0 (dyn_run.f90:135-149).
 constexpr int PS_BLOCKS = 128;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_ps_partial(const T *__restrict__ psg, const double *__restrict__ areaw,
-                                                    double *__restrict__ part, long ng, int nlon) {
+__global__ __launch_bounds__(256) void k_area_partial(const T *__restrict__ a, const T *__restrict__ b,
+                                                      const double *__restrict__ areaw, double *__restrict__ part,
+                                                      long ng, int nlon) {
     const int mem = blockIdx.y;
-    const T *p = psg + (long)mem * ng;
-    double s = 0.0;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < ng; i += (long)PS_BLOCKS * 256)
-        s += areaw[i / nlon] * (double)p[i];
-    __shared__ double red[256];
-    red[threadIdx.x] = s;
+    const T *pa = a + (long)mem * ng;
+    const T *pb = b ? b + (long)mem * ng : nullptr;
+    double sa = 0.0, sb = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < ng; i += (long)PS_BLOCKS * 256) {
+        const double w = areaw[i / nlon];
+        sa += w * (double)pa[i];
+        if (pb) sb += w * (double)pb[i];
+    }
+    __shared__ double ra[256], rb[256];
+    ra[threadIdx.x] = sa;
+    rb[threadIdx.x] = sb;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[mem * PS_BLOCKS + blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(PS_BLOCKS) void k_ps_final(const double *__restrict__ part, double *__restrict__ out,
-                                                         double *__restrict__ pdryini) {
-    const int mem = blockIdx.x;
-    __shared__ double red[PS_BLOCKS];
-    red[threadIdx.x] = part[mem * PS_BLOCKS + threadIdx.x];
-    __syncthreads();
-    for (int w = PS_BLOCKS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        if ((int)threadIdx.x < w) {
+            ra[threadIdx.x] += ra[threadIdx.x + w];
+            rb[threadIdx.x] += rb[threadIdx.x + w];
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (out) out[mem] = red[0];
-        if (pdryini && pdryini[mem] == 0.0) pdryini[mem] = red[0];
+        part[2 * (mem * PS_BLOCKS + blockIdx.x)] = ra[0];
+        part[2 * (mem * PS_BLOCKS + blockIdx.x) + 1] = rb[0];
     }
+}
+
+__global__ __launch_bounds__(PS_BLOCKS) void k_area_final(const double *__restrict__ part, double *__restrict__ out_a,
+                                                           double *__restrict__ out_b, double *__restrict__ pdryini) {
+    const int mem = blockIdx.x;
+    __shared__ double ra[PS_BLOCKS], rb[PS_BLOCKS];
+    ra[threadIdx.x] = part[2 * (mem * PS_BLOCKS + threadIdx.x)];
+    rb[threadIdx.x] = part[2 * (mem * PS_BLOCKS + threadIdx.x) + 1];
+    __syncthreads();
+    for (int w = PS_BLOCKS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ra[threadIdx.x] += ra[threadIdx.x + w];
+            rb[threadIdx.x] += rb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (out_a) out_a[mem] = ra[0];
+        if (out_b) out_b[mem] = rb[0];
+        if (pdryini && pdryini[mem] == 0.0) pdryini[mem] = ra[0] - rb[0];
+    }
+}
+
+// Simple moist physics on the post-dynamics grid (phy_run_simple.f90:50-107
+// around simple_physics_v5_1.f90), one thread per column and member, column
+// arithmetic in FP64 as the reference's r8 routine. The column runs top ->
+// bottom (c = 1 model top .. nl lowest level; grid level L = nl + 1 - c).
+// Large-scale condensation (265-292) is evaluated level by level inside the
+// upward (bottom -> top) elimination sweep of the implicit boundary-layer
+// solve (375-431); the sweep's coefficients CE/CEm go to pbl scratch and
+// CFu/CFv/CFt/CFq to the output slots, which the downward back-substitution
+// (432-447) overwrites with the final grid changes. Output per member
+// (nsp fields): dT(Tv), du, dv, dq (nlev each; changes over the step, the
+// 1/dt is applied in the analysis) and the mass-fixer slot.
+struct SimpleCfg {
+    double dt, fv, qmin, grav;
+    int sst_mode, turbflux;
+};
+
+__device__ inline double simple_sst(int mode, double lat) {
+    if (mode == 1) {
+        // simple_physics_v5_1.f90:231-239 (moist baroclinic wave)
+        const double pi = 3.14159265358979323846;
+        const double rair = 287.0, a = 6371220.0, om = 7.29212e-5, zvir = (461.5 / 287.0) - 1.0;
+        const double T00 = 288.0, u0 = 35.0, latw = 2.0 * pi / 9.0, eta0 = 0.252;
+        const double etav = (1.0 - eta0) * 0.5 * pi, q0 = (double)0.021f;
+        const double s = sin(lat), c = cos(lat);
+        const double num = T00 + pi * u0 / rair * 1.5 * sin(etav) * pow(cos(etav), 0.5) *
+                                     ((-2.0 * pow(s, 6.0) * (c * c + 1.0 / 3.0) + 10.0 / 63.0) * u0 * pow(cos(etav), 1.5) +
+                                      (8.0 / 5.0 * c * c * c * (s * s + 2.0 / 3.0) - pi / 4.0) * a * om * 0.5);
+        return num / (1.0 + zvir * q0 * exp(-pow(lat / latw, 4.0)));
+    }
+    if (mode == 2) {
+        const double s = sin(lat);
+        return 302.15 - 30.0 * s * s;  // build-defined latitude falloff
+    }
+    return 302.15;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_simple_columns(
+    const T *__restrict__ gsyn, const T *__restrict__ psg, T *__restrict__ gan, T *__restrict__ pbl,
+    T *__restrict__ pwat_out, T *__restrict__ precip_out, const T *__restrict__ ak_, const T *__restrict__ bk_,
+    const double *__restrict__ latd, int nl, int ntr, int nsyn, int nsp, long ng, int nlon, T kappa, SimpleCfg sc) {
+    __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1];
+    for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
+        ak[i] = ak_[i];
+        bk[i] = bk_[i];
+    }
+    __syncthreads();
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = blockIdx.y;
+    if (col >= ng) return;
+    // simple_physics_v5_1.f90:175-222
+    const double gravit = 9.80616, rair = 287.0, cpair = 1.0045e3, latvap = 2.5e6, rh2o = 461.5;
+    const double epsilo = rair / rh2o, zvir = (rh2o / rair) - 1.0, Csfc = 0.0011, T0 = 273.16;
+    const double e0 = 610.78, rhow = 1000.0, Cd0 = 0.0007, Cd1 = 0.000065, Cm = 0.002, v20 = 20.0;
+    const double p0 = 100000.0, pbltop = 85000.0, pblconst = 10000.0, rk = rair / cpair;
+    const double dt = sc.dt;
+    const T *G = gsyn + (long)mem * nsyn * ng + col;
+    const T *U = G, *V = G + (long)nl * ng, *TV = G + 4L * nl * ng, *Q = G + 7L * nl * ng;
+    T *A = gan + (long)mem * nsp * ng + col;
+    T *DT = A, *DU = A + (long)nl * ng, *DV = A + 2L * nl * ng, *DQ = A + 3L * nl * ng;
+    T *CE = pbl + (long)mem * 2 * nl * ng + col, *CEM = CE + (long)nl * ng;
+    const T ps = psg[(long)mem * ng + col];
+    const T fvT = (T)sc.fv, qminT = (T)sc.qmin;
+    // column accessors, c = 1..nl top -> bottom
+    auto lvl = [&](int c) { return (long)(nl - c) * ng; };  // grid offset of column position c
+    auto pint = [&](int c) { return (double)(ak[c - 1] + bk[c - 1] * ps); };
+    auto pmid = [&](int c) {
+        // Phillips layer pressure of pressure_data.f90:87-88 in the grid precision
+        const T p1 = ak[c - 1] + bk[c - 1] * ps, p2 = ak[c] + bk[c] * ps;
+        return (double)pow((pow(p2, kappa) * p2 - pow(p1, kappa) * p1) / ((kappa + T(1)) * (p2 - p1)), T(1) / kappa);
+    };
+    auto tin = [&](int c) {  // T = Tv / (1 + fv max(qmin, q)) (phy_run_simple.f90:56)
+        const long o = lvl(c);
+        return (double)(TV[o] / (T(1) + fvT * fmax(qminT, Q[o])));
+    };
+    // large-scale condensation at one level: (t, q) -> (t, q), precipitation added
+    double precl = 0.0;
+    auto condense = [&](int c, double &t, double &q) {
+        const double pm = pmid(c);
+        const double qsat = epsilo * e0 / pm * exp(-latvap / rh2o * ((1.0 / t) - 1.0 / T0));
+        if (q > qsat) {
+            const double tmp = 1.0 / dt * (q - qsat) / (1.0 + (latvap / cpair) * (epsilo * latvap * qsat / (rair * (t * t))));
+            t = t + (latvap / cpair * tmp) * dt;
+            q = q + (-tmp) * dt;
+            precl = precl + tmp * (pint(c + 1) - pint(c)) / (gravit * rhow);
+        }
+    };
+    const double lat = latd[col / nlon];
+    // za from the lowest level before any update (225-229)
+    const double zab = rair / gravit * tin(nl) * (1.0 + zvir * (double)Q[lvl(nl)]) * 0.5 * (log((double)ps) - log(pint(nl)));
+    double pw = 0.0;
+    auto finish = [&](int c, double t, double q, double u, double v) {
+        // Tv, changes and column water (phy_run_simple.f90:92-107)
+        const long o = lvl(c);
+        const double tv = t * (1.0 + sc.fv * fmax(sc.qmin, q));
+        DT[o] = (T)(tv - (double)TV[o]);
+        const T dq = (T)(q - (double)Q[o]);
+        DQ[o] = dq;
+        DU[o] = (T)(u - (double)U[o]);
+        DV[o] = (T)(v - (double)V[o]);
+        pw += (pint(c + 1) - pint(c)) * (double)(dq + Q[o]);
+    };
+    if (!sc.turbflux) {
+        for (int c = 1; c <= nl; c++) {
+            double t = tin(c), q = (double)Q[lvl(c)];
+            condense(c, t, q);
+            finish(c, t, q, (double)U[lvl(c)], (double)V[lvl(c)]);
+        }
+    } else {
+        // drag and eddy diffusivities from the lowest-level wind (300-335)
+        const double ub = (double)U[lvl(nl)], vb = (double)V[lvl(nl)];
+        const double wind = sqrt(ub * ub + vb * vb);
+        const double kes = Csfc * wind * zab;
+        double cd, kms;
+        if (wind < v20) {
+            cd = Cd0 + Cd1 * wind;
+            kms = cd * wind * zab;
+        } else {
+            cd = Cm;
+            kms = Cm * wind * zab;
+        }
+        auto prof = [&](int c) {  // taper above pbltop; c = nl + 1 is the surface
+            if (c == nl + 1) return 1.0;
+            const double pi_ = pint(c);
+            if (pi_ >= pbltop) return 1.0;
+            return exp(-((pbltop - pi_) * (pbltop - pi_)) / (pblconst * pblconst));
+        };
+        const double tsurf = simple_sst(sc.sst_mode, lat);
+        const double qsats = epsilo * e0 / (double)ps * exp(-latvap / rh2o * ((1.0 / tsurf) - 1.0 / T0));
+        // state after condensation (and, on the lowest level, the surface fluxes)
+        auto state = [&](int c, double &t, double &q, double &u, double &v) {
+            t = tin(c);
+            q = (double)Q[lvl(c)];
+            u = (double)U[lvl(c)];
+            v = (double)V[lvl(c)];
+            condense(c, t, q);
+            if (c == nl) {  // 344-366
+                u = u / (1.0 + cd * wind * dt / zab);
+                v = v / (1.0 + cd * wind * dt / zab);
+                t = (t + Csfc * wind * tsurf * dt / zab) / (1.0 + Csfc * wind * dt / zab);
+                q = (q + Csfc * wind * qsats * dt / zab) / (1.0 + Csfc * wind * dt / zab);
+            }
+        };
+        // upward elimination sweep c = nl..1 (407-431)
+        double tc, qc, uc, vc, ta, qa, ua, va;  // level c and level c-1 (above)
+        state(nl, tc, qc, uc, vc);
+        double ce1 = 0.0, cem1 = 0.0, cfu1 = 0.0, cfv1 = 0.0, cft1 = 0.0, cfq1 = 0.0;  // values at c + 1
+        double ca_c = 0.0, cam_c = 0.0;  // CA(c), CAm(c) from the interface below (0 at c = nl)
+        for (int c = nl; c >= 1; c--) {
+            // interface c (between c-1 and c): CC(c), CCm(c) and CA(c-1), CAm(c-1)
+            double cc_c = 0.0, ccm_c = 0.0, ca_up = 0.0, cam_up = 0.0;
+            if (c >= 2) {
+                state(c - 1, ta, qa, ua, va);
+                const double rho = (pint(c) / (rair * (tc + ta) / 2.0));
+                const double dpm = pmid(c) - pmid(c - 1);
+                const double km = kms * prof(c), ke = kes * prof(c);
+                const double rpu = 1.0 / (pint(c) - pint(c - 1)), rpc = 1.0 / (pint(c + 1) - pint(c));
+                cam_up = rpu * dt * gravit * gravit * km * rho * rho / dpm;
+                ccm_c = rpc * dt * gravit * gravit * km * rho * rho / dpm;
+                ca_up = rpu * dt * gravit * gravit * ke * rho * rho / dpm;
+                cc_c = rpc * dt * gravit * gravit * ke * rho * rho / dpm;
+            }
+            const double dh = 1.0 + ca_c + cc_c - ca_c * ce1;
+            const double dm = 1.0 + cam_c + ccm_c - cam_c * cem1;
+            const double ce = cc_c / dh, cem = ccm_c / dm;
+            const double cfu = (uc + cam_c * cfu1) / dm, cfv = (vc + cam_c * cfv1) / dm;
+            const double cft = (pow(p0 / pmid(c), rk) * tc + ca_c * cft1) / dh;
+            const double cfq = (qc + ca_c * cfq1) / dh;
+            const long o = lvl(c);
+            CE[o] = (T)ce;
+            CEM[o] = (T)cem;
+            DU[o] = (T)cfu;
+            DV[o] = (T)cfv;
+            DT[o] = (T)cft;
+            DQ[o] = (T)cfq;
+            ce1 = ce; cem1 = cem; cfu1 = cfu; cfv1 = cfv; cft1 = cft; cfq1 = cfq;
+            ca_c = ca_up; cam_c = cam_up;
+            tc = ta; qc = qa; uc = ua; vc = va;
+        }
+        // downward back-substitution c = 1..nl (432-447)
+        double tp = 0.0, qp = 0.0, up = 0.0, vp = 0.0, pmp = 0.0;
+        for (int c = 1; c <= nl; c++) {
+            const long o = lvl(c);
+            const double cfu = (double)DU[o], cfv = (double)DV[o], cft = (double)DT[o], cfq = (double)DQ[o];
+            const double pm = pmid(c);
+            double u, v, t, q;
+            if (c == 1) {
+                u = cfu;
+                v = cfv;
+                t = cft * pow(pm / p0, rk);
+                q = cfq;
+            } else {
+                const double ce = (double)CE[o], cem = (double)CEM[o];
+                u = cem * up + cfu;
+                v = cem * vp + cfv;
+                t = (ce * tp * pow(p0 / pmp, rk) + cft) * pow(pm / p0, rk);
+                q = ce * qp + cfq;
+            }
+            finish(c, t, q, u, v);
+            tp = t; qp = q; up = u; vp = v; pmp = pm;
+        }
+    }
+    pwat_out[(long)mem * ng + col] = (T)(pw / sc.grav);
+    precip_out[(long)mem * ng + col] = (T)(precl * 1000.0);
+}
+
+// mass-fixer field of the simple physics: ln ps' + log(pcorr) with
+// pcorr = (pdryini + g pwatg) / pmean (dyn_run.f90:618-631)
+template <typename T>
+__global__ __launch_bounds__(256) void k_massfix_field(const T *__restrict__ gsyn, T *__restrict__ gan,
+                                                       const double *__restrict__ pdryini,
+                                                       const double *__restrict__ pmean,
+                                                       const double *__restrict__ pwsum, int nl, int ntr, int nsyn,
+                                                       int nsp, long ng, double grav) {
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = blockIdx.y;
+    if (col >= ng) return;
+    const T *LNPS = gsyn + (long)mem * nsyn * ng + (7L + 3L * ntr) * nl * ng + col;
+    gan[(long)mem * nsp * ng + 4L * nl * ng + col] =
+        LNPS[0] + (T)log((pdryini[mem] + grav * pwsum[mem]) / pmean[mem]);
 }
 
 // Per-step diagnostics of run.f90:159-162 (spdmax, min/max ps, pstend) per member.
@@ -653,12 +961,20 @@ template <typename T> int Engine<T>::dyn_init() {
         set_error("nlev must be in [2, 64]");
         return GFSDC_EINVAL;
     }
-    if (ntr != 0) {
-        set_error("tracers (ntrac > 0) are not supported by this build of the dycore");
+    if (ntr < 0) {
+        set_error("ntrac must be >= 0");
         return GFSDC_EINVAL;
     }
-    if (cfg.physics != GFSDC_PHYS_NONE && cfg.physics != GFSDC_PHYS_HS) {
-        set_error("physics must be GFSDC_PHYS_NONE or GFSDC_PHYS_HS");
+    if (cfg.physics != GFSDC_PHYS_NONE && cfg.physics != GFSDC_PHYS_HS && cfg.physics != GFSDC_PHYS_SIMPLE) {
+        set_error("physics must be GFSDC_PHYS_NONE, GFSDC_PHYS_HS or GFSDC_PHYS_SIMPLE");
+        return GFSDC_EINVAL;
+    }
+    if (cfg.physics == GFSDC_PHYS_SIMPLE && ntr < 1) {
+        set_error("simple moist physics needs ntrac >= 1 (tracer 1 is specific humidity)");
+        return GFSDC_EINVAL;
+    }
+    if (cfg.simple_test < 0 || cfg.simple_test > 2) {
+        set_error("simple_test must be 0, 1 or 2");
         return GFSDC_EINVAL;
     }
     DycoreBuffers *B = new DycoreBuffers();
@@ -667,9 +983,10 @@ template <typename T> int Engine<T>::dyn_init() {
     B->ntr = ntr;
     B->nm = nm;
     B->nfs = 3 * nl + ntr * nl + 1;
-    B->nsyn = 7 * nl + ntr * nl + 3;
+    B->nsyn = 7 * nl + 3 * ntr * nl + 3;
     B->nan = 4 * nl + 1 + ntr * nl;
     B->nhs = 3 * nl + 1;
+    B->nsp = 4 * nl + 1;
     B->ng = (long)plan.nlat * plan.nlon;
     const long ng = B->ng, NFS = (long)B->nfs * nm;
     // ---- one-off operators in FP64 on the host (semimp_data.f90:127-256, dyn_init.f90:450-494)
@@ -803,6 +1120,11 @@ template <typename T> int Engine<T>::dyn_init() {
         }
         if ((rc = dupload<T>(&B->sinlat, castv<T>(sl), stream))) return rc;
         if ((rc = dupload<double>(&B->areaw, area, stream))) return rc;
+        std::vector<double> la(plan.nlat);
+        for (int j = 0; j < plan.nlat; j++) la[j] = std::asin(plan.mu[j]);  // shtns.f90:127
+        void *p;
+        if ((rc = dupload<double>(&p, la, stream))) return rc;
+        B->latd = (double *)p;
     }
     {
         std::vector<int> deg(nlm), byd;
@@ -837,11 +1159,17 @@ template <typename T> int Engine<T>::dyn_init() {
     B->dphisdy = dalloc<T>((size_t)nm * ng);
     B->pdryini = dalloc<double>(nm);
     B->pmean = dalloc<double>(nm);
-    B->pspart = dalloc<double>((size_t)nm * PS_BLOCKS);
+    B->pspart = dalloc<double>((size_t)nm * PS_BLOCKS * 2);
+    B->pwsum = dalloc<double>(nm);
     B->diag = dalloc<double>(4 * nm);
+    B->colq = dalloc<T>((size_t)nm * ng);
+    B->pwat = dalloc<T>((size_t)nm * ng);
+    B->precip = dalloc<T>((size_t)nm * ng);
+    B->pblscr = dalloc<T>(cfg.physics == GFSDC_PHYS_SIMPLE ? (size_t)nm * 2 * nl * ng : 1);
     void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
                     B->tphys, B->topo, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
-                    B->dphisdy, B->pdryini, B->pmean, B->diag, B->pspart};
+                    B->dphisdy, B->pdryini, B->pmean, B->diag, B->pspart, B->pwsum, B->colq, B->pwat,
+                    B->precip, B->pblscr};
     for (void *p : ptrs)
         if (!p) {
             set_error("device allocation failed for the dycore state");
@@ -852,13 +1180,18 @@ template <typename T> int Engine<T>::dyn_init() {
     GFSDC_CHECK(hipMemsetAsync(B->dphisdx, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->dphisdy, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->pdryini, 0, sizeof(double) * nm, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->pwat, 0, sizeof(T) * nm * ng, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->precip, 0, sizeof(T) * nm * ng, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->pwsum, 0, sizeof(double) * nm, stream));
     Work<T> *W = new Work<T>();
     B->Work = W;
     if ((rc = work_alloc(*W, plan, nm * std::max(B->nsyn, B->nan)))) return rc;
     // ---- descriptors
     std::vector<SynthDesc> sd((size_t)nm * B->nsyn);
     std::vector<unsigned char> rs((size_t)nm * B->nsyn, 0), ra((size_t)nm * B->nan, 0), rh((size_t)nm * B->nan, 0);
-    std::vector<AnalDesc> ad((size_t)nm * B->nfs), ah((size_t)nm * B->nfs);
+    std::vector<AnalDesc> ad((size_t)nm * B->nfs), ah((size_t)nm * B->nfs), ap((size_t)nm * B->nfs);
+    std::vector<unsigned char> rp((size_t)nm * B->nsp, 0);
+    const AnalDesc zero{AK_ZERO, -1, -1, 0.0, 0, 0, -1, 0.0};
     const double a = rerth, ra_ = 1.0 / rerth;
     for (int mem = 0; mem < nm; mem++) {
         SynthDesc *d = &sd[(size_t)mem * B->nsyn];
@@ -875,8 +1208,13 @@ template <typename T> int Engine<T>::dyn_init() {
             d[6 * nl + k] = SynthDesc{SK_VHAT, -1, tv + k, 0.0, ra_, 0, 0};    // dvirtempdy
             r[k] = r[nl + k] = r[5 * nl + k] = r[6 * nl + k] = 1;
         }
-        for (int k = 0; k < ntr * nl; k++) d[7 * nl + k] = SynthDesc{SK_COPY, tr + k, -1, 1.0, 0.0, 0, 0};
-        int o = 7 * nl + ntr * nl;
+        for (int k = 0; k < ntr * nl; k++) {
+            d[7 * nl + k] = SynthDesc{SK_COPY, tr + k, -1, 1.0, 0.0, 0, 0};                   // tracerg
+            d[7 * nl + ntr * nl + k] = SynthDesc{SK_UHAT, -1, tr + k, 0.0, ra_, 0, 0};       // dq/dx (getgrad)
+            d[7 * nl + 2 * ntr * nl + k] = SynthDesc{SK_VHAT, -1, tr + k, 0.0, ra_, 0, 0};   // dq/dy
+            r[7 * nl + ntr * nl + k] = r[7 * nl + 2 * ntr * nl + k] = 1;
+        }
+        int o = 7 * nl + 3 * ntr * nl;
         d[o] = SynthDesc{SK_COPY_DEV, lp, -1, 1.0, 0.0, 0, 0};  // lnpsg minus its global mean
         d[o + 1] = SynthDesc{SK_UHAT, -1, lp, 0.0, ra_, 0, 0};  // dlnpsdx
         d[o + 2] = SynthDesc{SK_VHAT, -1, lp, 0.0, ra_, 0, 0};  // dlnpsdy
@@ -897,10 +1235,26 @@ template <typename T> int Engine<T>::dyn_init() {
             hh[nl + k] = AnalDesc{AK_COPY, h0 + 2 * nl + k, -1, 1.0, 0, 0, -1, 0.0};
             hh[2 * nl + k] = AnalDesc{AK_COPY, h0 + k, -1, 1.0, 0, 0, -1, 0.0};
         }
-        for (int k = 0; k < ntr * nl; k++)
+        for (int k = 0; k < ntr * nl; k++) {
             e[3 * nl + k] = AnalDesc{AK_COPY, g0 + 4 * nl + 1 + k, -1, 1.0, 0, 0, -1, 0.0};
+            hh[3 * nl + k] = zero;  // Held-Suarez leaves tracers unchanged
+        }
         e[3 * nl + ntr * nl] = AnalDesc{AK_COPY, g0 + 4 * nl, -1, 1.0, 0, 0, -1, 0.0};
-        hh[3 * nl + ntr * nl] = AnalDesc{AK_COPY, h0 + 3 * nl, -1, 1.0, 0, 0, -1, 0.0};
+        hh[3 * nl + ntr * nl] = cfg.massfix ? AnalDesc{AK_COPY, h0 + 3 * nl, -1, 1.0, 0, 0, -1, 0.0} : zero;
+        // simple physics (phy_run_simple.f90:160-171): grid changes / dt ->
+        // tv, (vrt, div) = getvrtdivspec(du, dv), tracer 1; mass-fixer field
+        int p0 = mem * B->nsp;
+        AnalDesc *pp = &ap[(size_t)mem * B->nfs];
+        const double rdt = 1.0 / cfg.dt;
+        for (int k = 0; k < nl; k++) {
+            pp[k] = AnalDesc{AK_VRT, p0 + nl + k, p0 + 2 * nl + k, rdt, 0, 0, -1, 0.0};
+            pp[nl + k] = AnalDesc{AK_DIV, p0 + nl + k, p0 + 2 * nl + k, rdt, 0, 0, -1, 0.0};
+            pp[2 * nl + k] = AnalDesc{AK_COPY, p0 + k, -1, rdt, 0, 0, -1, 0.0};
+            rp[p0 + nl + k] = rp[p0 + 2 * nl + k] = 1;
+        }
+        for (int k = 0; k < ntr * nl; k++)
+            pp[3 * nl + k] = k < nl ? AnalDesc{AK_COPY, p0 + 3 * nl + k, -1, rdt, 0, 0, -1, 0.0} : zero;
+        pp[3 * nl + ntr * nl] = cfg.massfix ? AnalDesc{AK_COPY, p0 + 4 * nl, -1, 1.0, 0, 0, -1, 0.0} : zero;
     }
     void *p;
     if ((rc = dupload<SynthDesc>(&p, sd, stream))) return rc;
@@ -909,6 +1263,10 @@ template <typename T> int Engine<T>::dyn_init() {
     B->adesc_dyn = (AnalDesc *)p;
     if ((rc = dupload<AnalDesc>(&p, ah, stream))) return rc;
     B->adesc_hs = (AnalDesc *)p;
+    if ((rc = dupload<AnalDesc>(&p, ap, stream))) return rc;
+    B->adesc_sp = (AnalDesc *)p;
+    if ((rc = dupload<unsigned char>(&p, rp, stream))) return rc;
+    B->rflag_sp = (unsigned char *)p;
     if ((rc = dupload<unsigned char>(&p, rs, stream))) return rc;
     B->rflag_syn = (unsigned char *)p;
     if ((rc = dupload<unsigned char>(&p, ra, stream))) return rc;
@@ -930,7 +1288,8 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->dphisdy, B->topo, B->at, B->bt, B->dt3, B->sv, B->tor, B->disspec, B->diffp, B->dmpp,
                     B->ak, B->bk, B->ck, B->dbk, B->sinlat, B->coslat, B->areaw, B->kbydeg, B->degree,
                     B->pdryini, B->pmean, B->pspart, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->rflag_syn,
-                    B->rflag_an, B->rflag_hs};
+                    B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
+                    B->adesc_sp, B->rflag_sp};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (B->Work) {
@@ -966,7 +1325,7 @@ template <typename T> struct Step {
         launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_syn, s);
         mark("fft_synth");
     }
-    void columns(int inverse_only) {
+    void columns(int inverse_only, bool colq = false) {
         const gfsdc_cfg &c = E.cfg;
         dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
         hipLaunchKernelGGL(k_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (T *)B.gan, (T *)B.etadot,
@@ -974,14 +1333,15 @@ template <typename T> struct Step {
                            (const T *)B.ak, (const T *)B.bk, (const T *)B.ck, (const T *)B.dbk,
                            (const T *)B.sinlat, B.nl, B.ntr, B.nsyn, B.nan, B.ng, E.plan.nlon, (T)c.rd,
                            (T)(c.rd / c.cp), (T)(c.cvap / c.cp - 1.0), (T)(2.0 * c.omega), inverse_only,
-                           (const cplx<T> *)B.spec, B.nfs);
+                           (const cplx<T> *)B.spec, B.nfs, colq ? (T *)B.colq : (T *)nullptr);
         mark(inverse_only ? "columns_inverse" : "columns");
     }
-    void ps_mean(bool set_pdry, bool mean) {
-        hipLaunchKernelGGL(k_ps_partial<T>, dim3(PS_BLOCKS, B.nm), dim3(256), 0, s, (const T *)B.psg,
-                           (const double *)B.areaw, B.pspart, B.ng, E.plan.nlon);
-        hipLaunchKernelGGL(k_ps_final, dim3(B.nm), dim3(PS_BLOCKS), 0, s, (const double *)B.pspart,
-                           mean ? B.pmean : (double *)nullptr, set_pdry ? B.pdryini : (double *)nullptr);
+    // FP64 area sums: S(psg) -> pmean, S(b) -> bsum; pdryini = S(psg) - S(b) where 0
+    void area_sums(const void *b, double *pmean, double *bsum, bool set_pdry) {
+        hipLaunchKernelGGL(k_area_partial<T>, dim3(PS_BLOCKS, B.nm), dim3(256), 0, s, (const T *)B.psg,
+                           (const T *)b, (const double *)B.areaw, B.pspart, B.ng, E.plan.nlon);
+        hipLaunchKernelGGL(k_area_final, dim3(B.nm), dim3(PS_BLOCKS), 0, s, (const double *)B.pspart, pmean, bsum,
+                           set_pdry ? B.pdryini : (double *)nullptr);
         mark("ps_mean");
     }
     void analyse(const AnalDesc *desc, const unsigned char *rflag, int ngf, cplx<T> *dst) {
@@ -995,8 +1355,9 @@ template <typename T> struct Step {
     }
     void getdyntend(int stage) {
         synth_all();
-        columns(0);
-        if (stage == 0) ps_mean(true, false);  // dyn_run.f90:135-153 on this stage's psg
+        columns(0, stage == 0 && B.ntr > 0);
+        // dyn_run.f90:135-153 on this stage's psg (and column tracer mass)
+        if (stage == 0) area_sums(B.ntr > 0 ? B.colq : nullptr, nullptr, nullptr, true);
         analyse(B.adesc_dyn, B.rflag_an, B.nm * B.nan, (cplx<T> *)B.ten[stage]);
     }
     void si_stage(int stage, bool damp) {
@@ -1021,8 +1382,21 @@ template <typename T> struct Step {
         const gfsdc_cfg &c = E.cfg;
         synth_all();
         columns(1);
+        if (c.physics == GFSDC_PHYS_SIMPLE) {
+            simple_columns();
+            if (c.massfix) {
+                area_sums(B.pwat, B.pmean, B.pwsum, false);
+                dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+                hipLaunchKernelGGL(k_massfix_field<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (T *)B.gan,
+                                   (const double *)B.pdryini, (const double *)B.pmean, (const double *)B.pwsum,
+                                   B.nl, B.ntr, B.nsyn, B.nsp, B.ng, c.grav);
+                mark("massfix_field");
+            }
+            analyse(B.adesc_sp, B.rflag_sp, B.nm * B.nsp, (cplx<T> *)B.tphys);
+            phys_update();
+        }
         if (c.physics == GFSDC_PHYS_HS) {
-            ps_mean(false, c.massfix != 0);
+            area_sums(nullptr, c.massfix ? B.pmean : (double *)nullptr, nullptr, false);
             dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
             hipLaunchKernelGGL(k_hs_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg, (T *)B.gan,
                                (const T *)B.ak, (const T *)B.bk, (const T *)B.sinlat, B.pdryini, B.pmean, B.nl,
@@ -1030,12 +1404,31 @@ template <typename T> struct Step {
             mark("hs_columns");
             // HS fields per member: T, vrt, div forcing and the mass-fixer field
             analyse(B.adesc_hs, B.rflag_hs, B.nm * B.nhs, (cplx<T> *)B.tphys);
-            long n = (long)E.plan.nlm * B.nfs * B.nm;
-            hipLaunchKernelGGL(k_phys_update<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                               (cplx<T> *)B.spec, (const cplx<T> *)B.tphys, B.nl, B.ntr, B.nfs, B.nm, E.plan.nlm,
-                               (T)c.dt, c.massfix);
-            mark("phys_update");
+            phys_update();
         }
+    }
+    void simple_columns() {
+        const gfsdc_cfg &c = E.cfg;
+        SimpleCfg sc;
+        sc.dt = c.dt;
+        sc.fv = (double)(T)(c.rv / c.rd - 1.0);  // physcons.f90 con_fvirt
+        sc.qmin = (double)1.e-10f;               // phy_run_simple.f90:37
+        sc.grav = c.grav;
+        sc.sst_mode = c.simple_test;
+        sc.turbflux = c.turbflux;
+        dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+        hipLaunchKernelGGL(k_simple_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg, (T *)B.gan,
+                           (T *)B.pblscr, (T *)B.pwat, (T *)B.precip, (const T *)B.ak, (const T *)B.bk,
+                           (const double *)B.latd, B.nl, B.ntr, B.nsyn, B.nsp, B.ng, E.plan.nlon, (T)(c.rd / c.cp),
+                           sc);
+        mark("simple_columns");
+    }
+    void phys_update() {
+        const gfsdc_cfg &c = E.cfg;
+        long n = (long)E.plan.nlm * B.nfs * B.nm;
+        hipLaunchKernelGGL(k_phys_update<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (cplx<T> *)B.spec,
+                           (const cplx<T> *)B.tphys, B.nl, B.ntr, B.nfs, B.nm, E.plan.nlm, (T)c.dt, c.massfix);
+        mark("phys_update");
     }
     void step() {
         long n = 2L * E.plan.nlm * B.nfs * B.nm;
@@ -1151,15 +1544,16 @@ int Engine<T>::getdyntend(int rkstage, int inverse_only, void *dvrt, void *ddiv,
     Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
     if (inverse_only) {
         st.synth_all();
-        st.columns(1);
-        if (rkstage == 1) st.ps_mean(true, false);
+        st.columns(1, rkstage == 1 && B->ntr > 0);
+        if (rkstage == 1) st.area_sums(B->ntr > 0 ? B->colq : nullptr, nullptr, nullptr, true);
         GFSDC_CHECK(hipGetLastError());
         GFSDC_CHECK(hipStreamSynchronize(stream));
         return 0;
     }
     st.synth_all();
-    st.columns(0);
-    if (rkstage == 1) st.ps_mean(true, false);  // dyn_run.f90:135-153
+    st.columns(0, rkstage == 1 && B->ntr > 0);
+    // dyn_run.f90:135-153
+    if (rkstage == 1) st.area_sums(B->ntr > 0 ? B->colq : nullptr, nullptr, nullptr, true);
     st.analyse(B->adesc_dyn, B->rflag_an, B->nm * B->nan, (cplx<T> *)B->ten[0]);
     GFSDC_CHECK(hipGetLastError());
     const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
@@ -1203,6 +1597,29 @@ template <typename T> int Engine<T>::advance(int nsteps, int use_graph) {
     return 0;
 }
 
+// tendencies of a physics package on the current grid fields, without the
+// state update; tphys in the state layout -> caller buffers
+template <typename T>
+int Engine<T>::pack_tphys(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    GFSDC_CHECK(hipGetLastError());
+    const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
+    struct Part {
+        void *p;
+        int nf, off;
+    } parts[] = {{dvrt, nl, 0}, {ddiv, nl, nl}, {dtv, nl, 2 * nl}, {dtrac, B->ntr * nl, 3 * nl}};
+    for (auto &pt : parts) {
+        if (!pt.p || pt.nf == 0) continue;
+        size_t n = (size_t)pt.nf * nm * nlm;
+        int rc = ensure_stage(3, n * sizeof(cplx<T>));
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pack<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, (cplx<T> *)B->tphys,
+                           (const cplx<T> *)stage[3], pt.nf, pt.off, B->nfs, nm, nlm, 0);
+        if ((rc = output((const T *)stage[3], pt.p, n * 2, dtype, loc, 1))) return rc;
+    }
+    return 0;
+}
+
 template <typename T> int Engine<T>::hs_phytend(void *dvrt, void *ddiv, void *dtv, int dtype, int loc) {
     DycoreBuffers *B = dyn;
     if (!B) {
@@ -1215,22 +1632,24 @@ template <typename T> int Engine<T>::hs_phytend(void *dvrt, void *ddiv, void *dt
                        (T *)B->gan, (const T *)B->ak, (const T *)B->bk, (const T *)B->sinlat, B->pdryini, B->pmean,
                        B->nl, B->ntr, B->nsyn, B->nhs, B->ng, plan.nlon, (T)(cfg.rd / cfg.cp), 0);
     st.analyse(B->adesc_hs, B->rflag_hs, B->nm * B->nhs, (cplx<T> *)B->tphys);
-    GFSDC_CHECK(hipGetLastError());
-    const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
-    struct Part {
-        void *p;
-        int off;
-    } parts[] = {{dvrt, 0}, {ddiv, nl}, {dtv, 2 * nl}};
-    for (auto &pt : parts) {
-        if (!pt.p) continue;
-        size_t n = (size_t)nl * nm * nlm;
-        int rc = ensure_stage(3, n * sizeof(cplx<T>));
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_pack<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, (cplx<T> *)B->tphys,
-                           (const cplx<T> *)stage[3], nl, pt.off, B->nfs, nm, nlm, 0);
-        if ((rc = output((const T *)stage[3], pt.p, n * 2, dtype, loc, 1))) return rc;
+    return pack_tphys(dvrt, ddiv, dtv, nullptr, dtype, loc);
+}
+
+template <typename T>
+int Engine<T>::simple_phytend(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
     }
-    return 0;
+    if (cfg.physics != GFSDC_PHYS_SIMPLE) {
+        set_error("simple_phytend needs a context created with physics = GFSDC_PHYS_SIMPLE");
+        return GFSDC_ESTATE;
+    }
+    Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    st.simple_columns();
+    st.analyse(B->adesc_sp, B->rflag_sp, B->nm * B->nsp, (cplx<T> *)B->tphys);
+    return pack_tphys(dvrt, ddiv, dtv, dtrac, dtype, loc);
 }
 
 template <typename T> int Engine<T>::get_grid(int which, void *out, int dtype, int loc) {
@@ -1253,8 +1672,16 @@ template <typename T> int Engine<T>::get_grid(int which, void *out, int dtype, i
     case GF_ETADOT: src = (const T *)B->etadot; per = (nl + 1) * ng; mstride = per; break;
     case GF_PSG: src = (const T *)B->psg; per = ng; mstride = ng; break;
     case GF_DLNPSDT: src = (const T *)B->dlnpsdt; per = ng; mstride = ng; break;
-    case GF_DLNPSDX: src = (const T *)B->gsyn + (7L + B->ntr) * nl * ng + ng; per = ng; mstride = (long)B->nsyn * ng; break;
-    case GF_DLNPSDY: src = (const T *)B->gsyn + (7L + B->ntr) * nl * ng + 2 * ng; per = ng; mstride = (long)B->nsyn * ng; break;
+    case GF_DLNPSDX: src = (const T *)B->gsyn + (7L + 3L * B->ntr) * nl * ng + ng; per = ng; mstride = (long)B->nsyn * ng; break;
+    case GF_DLNPSDY: src = (const T *)B->gsyn + (7L + 3L * B->ntr) * nl * ng + 2 * ng; per = ng; mstride = (long)B->nsyn * ng; break;
+    case GF_TRACER:
+        if (B->ntr == 0) {
+            set_error("no tracers (ntrac == 0)");
+            return GFSDC_EINVAL;
+        }
+        src = (const T *)B->gsyn + 7L * nl * ng; per = (long)B->ntr * nl * ng; mstride = (long)B->nsyn * ng; break;
+    case GF_PRECIP: src = (const T *)B->precip; per = ng; mstride = ng; break;
+    case GF_PWAT: src = (const T *)B->pwat; per = ng; mstride = ng; break;
     default:
         set_error("unknown grid field");
         return GFSDC_EINVAL;

@@ -10,7 +10,7 @@ namespace gfsdc {
 // Grid-space fields held by the dycore after getdyntend (grid_data.f90 names).
 enum GridField : int {
     GF_UG = 0, GF_VG, GF_VRT, GF_DIV, GF_TV, GF_DLNPDT, GF_ETADOT, GF_PSG, GF_DLNPSDT, GF_PRS,
-    GF_DLNPSDX = 14, GF_DLNPSDY = 15
+    GF_DLNPSDX = 14, GF_DLNPSDY = 15, GF_TRACER = 16, GF_PRECIP = 17, GF_PWAT = 18
 };
 
 struct DycoreBuffers;  // defined in dycore.hip
@@ -61,6 +61,8 @@ template <typename T> struct Engine {
                    void *dlnps, int dtype, int loc);
     int advance(int nsteps, int use_graph);
     int hs_phytend(void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
+    int simple_phytend(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
+    int pack_tphys(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
     int get_grid(int which, void *out, int dtype, int loc);
     int get_semimp(double *amhyb, double *bmhyb, double *svhyb, double *tor_hyb, double *d_hyb_m);
     int get_damp(double *disspec, double *diff_prof, double *dmp_prof);

@@ -70,7 +70,9 @@ struct SynthDesc {
     double sa, sb;     // multipliers of S and T
     int inv_a, inv_b;  // multiply S / T by invlap(n)
 };
-enum AnalKind : int { AK_COPY = 0, AK_VRT = 1, AK_DIV = 2 };
+// AK_ZERO writes a zero coefficient (an output with no source, e.g. a tracer a
+// physics package leaves unchanged).
+enum AnalKind : int { AK_COPY = 0, AK_VRT = 1, AK_DIV = 2, AK_ZERO = 3 };
 struct AnalDesc {
     int kind;
     int a, b;      // analysed columns (field indices): COPY uses a; VRT/DIV use (U~=a, V~=b)

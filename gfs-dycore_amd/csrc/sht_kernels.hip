@@ -682,6 +682,11 @@ __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc
     const AnalDesc d = desc[fo];
     const T *X = sx + (long)(keoff[m] + n - m) * ldx;
     cplx<T> r;
+    if (d.kind == AK_ZERO) {
+        cplx<T> *o = dst + fo * df + (long)k * dk;
+        if (!d.accum) *o = cplx<T>{T(0), T(0)};
+        return;
+    }
     if (d.kind == AK_COPY) {
         r = cplx<T>{X[2 * d.a], X[2 * d.a + 1]};
     } else {

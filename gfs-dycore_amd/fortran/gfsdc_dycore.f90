@@ -22,6 +22,7 @@ module gfsdc_dycore
     real(c_double) :: efold, hdif_fac, hdif_fac2, fshk, taustratdamp
     real(c_double) :: rerth, grav, omega, rd, rv, cp, cvap
     integer(c_int) :: device
+    integer(c_int) :: simple_test, turbflux
   end type
 
   interface

@@ -21,7 +21,7 @@ LIBPATH = os.path.join(_HERE, "libgfsdc.so")
 FP64, FP32 = 0, 1
 HOST, DEVICE = 0, 1
 DT_F64, DT_F32 = 0, 1
-PHYS_NONE, PHYS_HS = 0, 1
+PHYS_NONE, PHYS_HS, PHYS_SIMPLE = 0, 1, 2
 
 
 class GfsdcError(RuntimeError):
@@ -38,6 +38,7 @@ class Cfg(C.Structure):
         ("fshk", C.c_double), ("taustratdamp", C.c_double),
         ("rerth", C.c_double), ("grav", C.c_double), ("omega", C.c_double), ("rd", C.c_double),
         ("rv", C.c_double), ("cp", C.c_double), ("cvap", C.c_double), ("device", C.c_int),
+        ("simple_test", C.c_int), ("turbflux", C.c_int),
     ]
 
 
@@ -49,7 +50,7 @@ SYMBOLS = [
     "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_grdtospec", "gfsdc_getuv",
     "gfsdc_getvrtdivspec", "gfsdc_getgrad", "gfsdc_set_state", "gfsdc_get_state",
     "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
-    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_get_grid", "gfsdc_get_semimp",
+    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_grid", "gfsdc_get_semimp",
     "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_sync",
     "gfsdc_stream", "gfsdc_profile_step",
 ]
@@ -86,6 +87,7 @@ def load():
     L.gfsdc_getdyntend.argtypes = [vp, i, i, vp, vp, vp, vp, vp, i, i]
     L.gfsdc_advance.argtypes = [vp, i, i]
     L.gfsdc_hs_phytend.argtypes = [vp, vp, vp, vp, i, i]
+    L.gfsdc_simple_phytend.argtypes = [vp, vp, vp, vp, vp, i, i]
     L.gfsdc_get_grid.argtypes = [vp, i, vp, i, i]
     L.gfsdc_get_semimp.argtypes = [vp, vp, vp, vp, vp, vp]
     L.gfsdc_get_damp.argtypes = [vp, vp, vp, vp]
@@ -118,7 +120,8 @@ class Context:
     """One gfsdc context: a transform plan (and, with nlev > 0, a dycore state)."""
 
     GRID = {"ug": 0, "vg": 1, "vrtg": 2, "divg": 3, "virtempg": 4, "dlnpdtg": 5, "etadot": 6,
-            "psg": 7, "dlnpsdt": 8, "prs": 9, "dlnpsdx": 14, "dlnpsdy": 15}
+            "psg": 7, "dlnpsdt": 8, "prs": 9, "dlnpsdx": 14, "dlnpsdy": 15, "tracerg": 16,
+            "precip": 17, "pwat": 18}
 
     def __init__(self, nlon, nlat, ntrunc, precision=FP64, nlev=0, ak=None, bk=None, dt=0.0,
                  ntrac=0, nmem=1, **kw):
@@ -264,9 +267,15 @@ class Context:
         _check(load().gfsdc_hs_phytend(self.h, _p(a), _p(b), _p(c), DT_F64, HOST))
         return dict(dvrt=a, ddiv=b, dtv=c)
 
+    def simple_phytend(self):
+        a, b, c = self._s3(), self._s3(), self._s3()
+        q = np.zeros((self.nmem, max(self.ntrac, 1), self.nlev, self.nlm), np.complex128)
+        _check(load().gfsdc_simple_phytend(self.h, _p(a), _p(b), _p(c), _p(q), DT_F64, HOST))
+        return dict(dvrt=a, ddiv=b, dtv=c, dtrac=q[:, :self.ntrac])
+
     def grid(self, name):
-        nk = {"etadot": self.nlev + 1, "psg": 1, "dlnpsdt": 1, "dlnpsdx": 1,
-              "dlnpsdy": 1}.get(name, self.nlev)
+        nk = {"etadot": self.nlev + 1, "psg": 1, "dlnpsdt": 1, "dlnpsdx": 1, "dlnpsdy": 1,
+              "tracerg": self.nlev * self.ntrac, "precip": 1, "pwat": 1}.get(name, self.nlev)
         out = np.zeros((self.nmem, nk, self.nlat, self.nlon))
         _check(load().gfsdc_get_grid(self.h, self.GRID[name], _p(out), DT_F64, HOST))
         return out

@@ -78,7 +78,7 @@ enum {
 enum { GFSDC_FP64 = 0, GFSDC_FP32 = 1 };             /* compute precision of the context */
 enum { GFSDC_HOST = 0, GFSDC_DEVICE = 1 };           /* location of a caller buffer */
 enum { GFSDC_DT_F64 = 0, GFSDC_DT_F32 = 1 };         /* element type of a caller buffer */
-enum { GFSDC_PHYS_NONE = 0, GFSDC_PHYS_HS = 1 };     /* physics plug-in (Makefile:50-72) */
+enum { GFSDC_PHYS_NONE = 0, GFSDC_PHYS_HS = 1, GFSDC_PHYS_SIMPLE = 2 }; /* physics plug-in (Makefile:50-72) */
 
 typedef struct gfsdc_cfg {
     int nlon, nlat, ntrunc, nlev, ntrac, nmem;  /* nmem: ensemble members folded into one launch */
@@ -94,6 +94,12 @@ typedef struct gfsdc_cfg {
     /* physical constants (physcons.f90); gfsdc_cfg_defaults fills them */
     double rerth, grav, omega, rd, rv, cp, cvap;
     int device;                                 /* HIP device ordinal */
+    /* simple moist physics (GFSDC_PHYS_SIMPLE, needs ntrac >= 1, tracer 1 = q):
+     * simple_test selects the sea-surface temperature -- 0 constant 302.15 K
+     * (reference test 0, dcmip 5x), 1 moist baroclinic-wave formula (test 1,
+     * dcmip 4x), 2 302.15 K - 30 K sin^2(lat) (this library's own);
+     * turbflux: surface fluxes + boundary-layer diffusion (dcmip 43 and 5x) */
+    int simple_test, turbflux;
 } gfsdc_cfg;
 
 typedef struct gfsdc_ctx gfsdc_ctx;
@@ -151,9 +157,16 @@ int gfsdc_getdyntend(gfsdc_ctx *ctx, int rkstage, int inverse_only, void *dvrt, 
 int gfsdc_advance(gfsdc_ctx *ctx, int nsteps, int use_graph);
 /* Held-Suarez tendencies on the current grid fields (phy_run_hs.f90:22) */
 int gfsdc_hs_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
+/* replaces phy_run_simple.f90:28 getphytend (simple moist physics on the current
+ * grid fields, i.e. after gfsdc_getdyntend(ctx, 3, 1, ...)): tendencies of
+ * vrt, div, tv (ndimspec, nlev, nmem) and trac (ndimspec, nlev, ntrac, nmem);
+ * also refreshes the precip and pwat grids. The state is not changed. */
+int gfsdc_simple_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype,
+                         int loc);
 /* grid fields after the last getdyntend, names of grid_data.f90/pressure_data.f90:
  * 0 ug,1 vg,2 vrtg,3 divg,4 virtempg,5 dlnpdtg,6 etadot (nlev+1),7 psg,8 dlnpsdt,
- * 9 prs,14 dlnpsdx,15 dlnpsdy (2-D fields have one level); per member */
+ * 9 prs,14 dlnpsdx,15 dlnpsdy, 16 tracerg (nlev*ntrac), 17 precip (mm/s),
+ * 18 pwat (kg/m^2) (phy_data_simple.f90:13; 2-D fields have one level); per member */
 int gfsdc_get_grid(gfsdc_ctx *ctx, int which, void *out, int dtype, int loc);
 /* semimp_data.f90 operators (FP64 host arrays, column-major as the reference):
  * amhyb,bmhyb (nlev,nlev), svhyb,tor_hyb (nlev), d_hyb_m (nlev,nlev,ntrunc+1,3) */

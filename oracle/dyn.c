@@ -60,6 +60,7 @@ struct orc_model {
     /* grid fields */
     double *ug, *vg, *vrtg, *divg, *tvg, *tracg, *dlnpdtg, *etadot, *lnpsg, *psg, *dlnpsdt;
     double *dphisdx, *dphisdy, *dlnpsdx, *dlnpsdy;
+    double *precip, *pwat; /* simple physics: precipitation rate (mm/s), column water (kg/m^2) */
     double *pk, *dpk, *alfa, *rlnp, *prs;
     double *w[10]; /* work grids (nlev) */
     double *cb, *db, *cg, *wk1;
@@ -92,6 +93,8 @@ void orc_cfg_defaults(orc_cfg *c) {
     c->rv = SGL(4.6150e+2);
     c->cp = SGL(1.0046e+3);
     c->cvap = SGL(1.8460e+3);
+    c->simple_test = 0;
+    c->turbflux = 1;
 }
 
 /* counter-based RNG: splitmix64 finaliser over a mixed (seed, stream, index) key */
@@ -301,6 +304,7 @@ orc_model *orc_model_create(const orc_cfg *c) {
     m->lnpsg = malloc(g2); m->psg = malloc(g2); m->dlnpsdt = malloc(g2);
     m->dphisdx = calloc(ng, sizeof(double)); m->dphisdy = calloc(ng, sizeof(double));
     m->dlnpsdx = malloc(g2); m->dlnpsdy = malloc(g2);
+    m->precip = calloc(ng, sizeof(double)); m->pwat = calloc(ng, sizeof(double));
     m->amhyb = malloc(sizeof(double) * nl * nl); m->bmhyb = malloc(sizeof(double) * nl * nl);
     m->svhyb = malloc(sizeof(double) * nl); m->tor_hyb = malloc(sizeof(double) * nl);
     m->d_hyb_m = malloc(sizeof(double) * nl * nl * (m->T + 1) * 3);
@@ -323,7 +327,7 @@ void orc_model_destroy(orc_model *m) {
                     m->etadot, m->prs, m->pk, m->dpk, m->alfa, m->rlnp, m->cb, m->db, m->cg,
                     m->wk1, m->lnpsg, m->psg, m->dlnpsdt, m->dphisdx, m->dphisdy, m->dlnpsdx,
                     m->dlnpsdy, m->amhyb, m->bmhyb, m->svhyb, m->tor_hyb, m->d_hyb_m,
-                    m->disspec, m->diff_prof, m->dmp_prof};
+                    m->disspec, m->diff_prof, m->dmp_prof, m->precip, m->pwat};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int i = 0; i < 10; i++) free(m->w[i]);
     for (int s = 0; s < 3; s++) { free(m->lin_d[s]); free(m->lin_t[s]); free(m->lin_p[s]); }
@@ -460,18 +464,19 @@ static void getvadv(orc_model *m, const double *datag, double *vadv) {
     }
 }
 
-/* ---- dyn_run.f90:478-603 getvadv_tracers (Thuburn 1993 limiter) ----- */
-static void getvadv_tracers(orc_model *m, const double *datag, double *vadv) {
-    int nl = m->nlev;
-    long ng = m->ng;
-    const double *et = m->etadot, *dpk = m->dpk;
+/* ---- dyn_run.f90:478-603 getvadv_tracers (Thuburn 1993 limiter) -----
+ * Columns-fastest arrays: datag (nl levels bottom -> top), etadot (nl+1
+ * interfaces top -> bottom), dpk (nl layers top -> bottom) -> vadv (bottom -> top). */
+void orc_vadv_tracers_cols(int nl, long ng, const double *datag, const double *et, const double *dpk,
+                           double *vadv) {
     const double epstiny = 2.2250738585072014e-308; /* tiny(real(8)) */
+#define LL(X, k) ((X) + (long)((k) - 1) * ng)
 #pragma omp parallel
     {
         double *half = malloc(sizeof(double) * (nl + 1)), *dd = malloc(sizeof(double) * (nl + 1));
 #pragma omp for
         for (long i = 0; i < ng; i++) {
-#define D(k) L(datag, k)[i]
+#define D(k) LL(datag, k)[i]
             for (int k = 1; k <= nl - 1; k++) {
                 half[k] = 0.5 * (D(nl - k) + D(nl + 1 - k));
                 dd[k] = D(nl - k) - D(nl + 1 - k);
@@ -486,7 +491,7 @@ static void getvadv_tracers(orc_model *m, const double *datag, double *vadv) {
                 if (fabs(dd[k]) < epstiny) dd[k] = copysign(epstiny, dd[k]);
             for (int k = 1; k <= nl - 1; k++) {
                 double phi;
-                if (L(et, k + 1)[i] > 0.) {
+                if (LL(et, k + 1)[i] > 0.) {
                     phi = dd[k - 1] / dd[k];
                     half[k] = D(nl + 1 - k) + (phi + fabs(phi)) / (1. + fabs(phi)) * (half[k] - D(nl + 1 - k));
                 } else {
@@ -495,13 +500,18 @@ static void getvadv_tracers(orc_model *m, const double *datag, double *vadv) {
                 }
             }
             for (int k = 1; k <= nl; k++)
-                L(vadv, nl + 1 - k)[i] = (1. / L(dpk, k)[i]) *
-                                         ((half[k] * L(et, k + 1)[i] - half[k - 1] * L(et, k)[i]) +
-                                          (D(nl + 1 - k) * (L(et, k)[i] - L(et, k + 1)[i])));
+                LL(vadv, nl + 1 - k)[i] = (1. / LL(dpk, k)[i]) *
+                                          ((half[k] * LL(et, k + 1)[i] - half[k - 1] * LL(et, k)[i]) +
+                                           (D(nl + 1 - k) * (LL(et, k)[i] - LL(et, k + 1)[i])));
 #undef D
         }
         free(half); free(dd);
     }
+#undef LL
+}
+
+static void getvadv_tracers(orc_model *m, const double *datag, double *vadv) {
+    orc_vadv_tracers_cols(m->nlev, m->ng, datag, m->etadot, m->dpk, vadv);
 }
 
 /* ---- dyn_run.f90:33-305 getdyntend --------------------------------- */
@@ -620,6 +630,68 @@ void orc_hs_phytend(orc_model *m, ocplx *dvrt, ocplx *ddiv, ocplx *dtv) {
     orc_grdtospec(m->sht, fd, ddiv, nl);
 }
 
+/* ---- phy_run_simple.f90:28-196 getphytend (simple moist physics) --------
+ * On the model's current grid fields (after getdyntend(3, inverse_only)):
+ * each column is handed to simple_physics top -> bottom with T = Tv/(1+fv q)
+ * (50-63), the updated column is turned back into Tv (92-94), grid tendencies
+ * are the changes over the step (95-101), analysed and divided by dt
+ * (160-171). Tracer 1 is specific humidity; other tracers get no tendency. */
+void orc_simple_phytend(orc_model *m, ocplx *dvrt, ocplx *ddiv, ocplx *dtv, ocplx *dtrac) {
+    int nl = m->nlev, nlm = m->nlm;
+    long ng = m->ng;
+    double dt = m->c.dt;
+    const double qmin = SGL(1.e-10), fv = m->c.rv / m->c.rd - 1.0;
+    double *dT = m->w[0], *du = m->w[1], *dv = m->w[2], *dq = m->w[3];
+    const double *q0 = m->tracg; /* tracer 1, levels bottom -> top */
+#pragma omp parallel
+    {
+        double *gt = malloc(sizeof(double) * nl), *gq = malloc(sizeof(double) * nl);
+        double *gu = malloc(sizeof(double) * nl), *gv = malloc(sizeof(double) * nl);
+        double *prsl = malloc(sizeof(double) * nl), *prsi = malloc(sizeof(double) * (nl + 1));
+        double *delp = malloc(sizeof(double) * nl), *rdelp = malloc(sizeof(double) * nl);
+#pragma omp for
+        for (long i = 0; i < ng; i++) {
+            for (int k = 1; k <= nl; k++) {
+                int c = nl - k; /* top -> bottom position of level k */
+                double q = L(q0, k)[i];
+                gq[c] = q;
+                gu[c] = L(m->ug, k)[i];
+                gv[c] = L(m->vg, k)[i];
+                gt[c] = L(m->tvg, k)[i] / (1. + fv * fmax(qmin, q));
+                prsl[c] = L(m->prs, k)[i];
+            }
+            for (int k = 1; k <= nl + 1; k++) prsi[k - 1] = L(m->pk, k)[i];
+            for (int k = 0; k < nl; k++) {
+                delp[k] = prsi[k + 1] - prsi[k];
+                rdelp[k] = 1. / delp[k];
+            }
+            double precl;
+            orc_simple_column(nl, dt, m->lats[i / m->nlon], gt, gq, gu, gv, prsl, prsi, delp, rdelp,
+                              m->psg[i], &precl, m->c.simple_test, m->c.turbflux);
+            m->precip[i] = precl * 1000.;
+            double pw = 0.0;
+            for (int k = 1; k <= nl; k++) {
+                int c = nl - k;
+                double tv = gt[c] * (1. + fv * fmax(qmin, gq[c]));
+                L(dT, k)[i] = tv - L(m->tvg, k)[i];
+                L(dq, k)[i] = gq[c] - L(q0, k)[i];
+                L(du, k)[i] = gu[c] - L(m->ug, k)[i];
+                L(dv, k)[i] = gv[c] - L(m->vg, k)[i];
+                pw += L(m->dpk, nl - k + 1)[i] * (L(dq, k)[i] + L(q0, k)[i]);
+            }
+            m->pwat[i] = pw / m->c.grav;
+        }
+        free(gt); free(gq); free(gu); free(gv); free(prsl); free(prsi); free(delp); free(rdelp);
+    }
+    orc_grdtospec(m->sht, dT, dtv, nl);
+    orc_getvrtdivspec(m->sht, du, dv, dvrt, ddiv, nl, m->c.rerth);
+    memset(dtrac, 0, sizeof(ocplx) * nlm * nl * m->ntrac);
+    orc_grdtospec(m->sht, dq, dtrac, nl);
+    for (long i = 0; i < (long)nlm * nl; i++) {
+        dtv[i] /= dt; dvrt[i] /= dt; ddiv[i] /= dt; dtrac[i] /= dt;
+    }
+}
+
 /* ---- run.f90:194-565 advance ---------------------------------------- */
 static void semi_implicit(orc_model *m, int stage, ocplx **dd, ocplx **dT, ocplx **dp,
                           const ocplx *div0, const ocplx *tv0, const ocplx *lnps0,
@@ -694,11 +766,22 @@ void orc_advance(orc_model *m, int nsteps) {
                     m->tv[i] += dt * dT[1][i];
                 }
                 for (int k = 0; k < nlm; k++) dp[1][k] = 0.0;
+            } else if (m->c.physics == 2) {
+                orc_simple_phytend(m, dv[1], dd[1], dT[1], dq[1]);
+                for (long i = 0; i < (long)nlm * nl; i++) {
+                    m->vrt[i] += dt * dv[1][i];
+                    m->div[i] += dt * dd[1][i];
+                    m->tv[i] += dt * dT[1][i];
+                }
+                for (long i = 0; i < (long)nlm * nl * ntrac; i++) m->trac[i] += dt * dq[1][i];
+                for (int k = 0; k < nlm; k++) dp[1][k] = 0.0;
             }
             if (m->c.massfix) {
-                /* dyn_run.f90:605-634 with pwat = 0 (stub/HS physics data) */
+                /* dyn_run.f90:605-634; pwat is 0 for Held-Suarez (phy_data_stub.f90:17)
+                 * and the post-physics column water for the simple physics */
+                double pwatg = m->c.physics == 2 ? area_sum(m, m->pwat) : 0.0;
                 double pmean = area_sum(m, m->psg);
-                double pcorr = (m->pdryini + m->c.grav * 0.0) / pmean;
+                double pcorr = (m->pdryini + m->c.grav * pwatg) / pmean;
                 double *g = m->w[3];
                 for (long i = 0; i < m->ng; i++) g[i] = log(m->psg[i] * pcorr);
                 ocplx *ws = malloc(s1);
@@ -821,6 +904,8 @@ void orc_get_grid(const orc_model *m, int which, double *out) {
     case 14: src = m->dlnpsdx; n = ng; break;
     case 15: src = m->dlnpsdy; n = ng; break;
     case 16: src = m->tracg; n = n3 * (m->ntrac > 0 ? m->ntrac : 0); break;
+    case 17: src = m->precip; n = ng; break;
+    case 18: src = m->pwat; n = ng; break;
     default: return;
     }
     if (n > 0) memcpy(out, src, sizeof(double) * n);

@@ -74,13 +74,17 @@ typedef struct {
     double dt;
     const double *ak, *bk; /* nlev+1, top -> bottom (as pressure_data.f90) */
     int explicit_;         /* params.f90 explicit */
-    int physics;           /* 0 adiabatic (stub), 1 Held-Suarez */
+    int physics;           /* 0 adiabatic (stub), 1 Held-Suarez, 2 simple moist (ntrac >= 1) */
     int massfix;
     int ndiss;             /* 0 -> 8 */
     double efold;          /* <=0 -> GFS default by truncation */
     double hdif_fac, hdif_fac2, fshk, taustratdamp;
     /* physical constants (physcons.f90); orc_cfg_defaults fills them */
     double rerth, grav, omega, rd, rv, cp, cvap;
+    /* simple physics (physics = 2): surface temperature mode (0 constant
+     * 302.15 K, 1 baroclinic-wave formula, 2 302.15 K - 30 K sin^2 lat) and
+     * the turbulent surface/boundary-layer fluxes switch (phy_run_simple.f90:47-49) */
+    int simple_test, turbflux;
 } orc_cfg;
 
 typedef struct orc_model orc_model;
@@ -105,11 +109,20 @@ void orc_getdyntend(orc_model *m, int rkstage, int inverse_only, ocplx *dvrt, oc
 void orc_advance(orc_model *m, int nsteps);
 /* phy_run_hs.f90:22 getphytend on the model's current grid fields */
 void orc_hs_phytend(orc_model *m, ocplx *dvrt, ocplx *ddiv, ocplx *dtv);
+/* phy_run_simple.f90:28 getphytend on the model's current grid fields; also
+ * fills the precip (mm/s) and pwat (kg/m^2) grids */
+void orc_simple_phytend(orc_model *m, ocplx *dvrt, ocplx *ddiv, ocplx *dtv, ocplx *dtrac);
+/* simple_physics_v5_1.f90 on one column, top -> bottom (t, q, u, v in place) */
+void orc_simple_column(int nl, double dtime, double lat, double *t, double *q, double *u, double *v,
+                       const double *pmid, const double *pint, const double *pdel, const double *rpdel,
+                       double ps, double *precl, int sst_mode, int turbflux);
+double orc_simple_sst(int sst_mode, double lat);
 
 /* grid diagnostics held by the model after getdyntend (grid_data.f90 names):
  * which: 0 ug,1 vg,2 vrtg,3 divg,4 virtempg,5 dlnpdtg,6 etadot(nlev+1),
  * 7 psg(2d),8 dlnpsdt(2d),9 prs,10 pk(nlev+1),11 dpk,12 alfa,13 rlnp,
- * 14 dlnpsdx(2d), 15 dlnpsdy(2d), 16 tracerg (nlev*ntrac) */
+ * 14 dlnpsdx(2d), 15 dlnpsdy(2d), 16 tracerg (nlev*ntrac), 17 precip(2d),
+ * 18 pwat(2d) */
 void orc_get_grid(const orc_model *m, int which, double *out);
 /* one-off operators: amhyb,bmhyb (nlev*nlev col-major), svhyb,tor_hyb (nlev),
  * d_hyb_m (nlev,nlev,ntrunc+1,3) col-major */
@@ -125,6 +138,11 @@ void orc_step_diag(const orc_model *m, double *out4);
 void orc_calc_pressdata(int nlev, int ncol, const double *ak, const double *bk, double rk,
                         const double *lnpsg, double *pk, double *dpk, double *alfa,
                         double *rlnp, double *prs);
+
+/* dyn_run.f90:478 getvadv_tracers on standalone columns (columns fastest):
+ * datag (nlev, bottom->top), etadot (nlev+1, top->bottom), dpk (nlev, top->bottom) */
+void orc_vadv_tracers_cols(int nlev, long ncol, const double *datag, const double *etadot,
+                           const double *dpk, double *vadv);
 
 /* counter-based RNG shared with the product: splitmix64 of (seed, stream, index) */
 uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index);

@@ -31,6 +31,7 @@ class Cfg(C.Structure):
         ("fshk", C.c_double), ("taustratdamp", C.c_double),
         ("rerth", C.c_double), ("grav", C.c_double), ("omega", C.c_double), ("rd", C.c_double),
         ("rv", C.c_double), ("cp", C.c_double), ("cvap", C.c_double),
+        ("simple_test", C.c_int), ("turbflux", C.c_int),
     ]
 
 
@@ -74,12 +75,18 @@ def lib():
         L.orc_getdyntend.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.orc_advance.argtypes = [vp, C.c_int]
         L.orc_hs_phytend.argtypes = [vp, _cp, _cp, _cp]
+        L.orc_simple_phytend.argtypes = [vp, _cp, _cp, _cp, _cp]
+        L.orc_simple_column.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        _dp, _dp, C.c_double, C.POINTER(C.c_double), C.c_int, C.c_int]
+        L.orc_simple_sst.argtypes = [C.c_int, C.c_double]
+        L.orc_simple_sst.restype = C.c_double
         L.orc_get_grid.argtypes = [vp, C.c_int, _dp]
         L.orc_get_semimp.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_get_damp.argtypes = [vp, vp, vp, vp]
         L.orc_step_diag.argtypes = [vp, _dp]
         L.orc_calc_pressdata.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp,
                                          _dp, _dp, _dp]
+        L.orc_vadv_tracers_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
         L.orc_rng_u64.restype = C.c_uint64
         L.orc_rng_u64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_rng_uniform.restype = C.c_double
@@ -187,7 +194,7 @@ class Model:
 
     GRID = {"ug": 0, "vg": 1, "vrtg": 2, "divg": 3, "virtempg": 4, "dlnpdtg": 5, "etadot": 6,
             "psg": 7, "dlnpsdt": 8, "prs": 9, "pk": 10, "dpk": 11, "alfa": 12, "rlnp": 13,
-            "dlnpsdx": 14, "dlnpsdy": 15, "tracerg": 16}
+            "dlnpsdx": 14, "dlnpsdy": 15, "tracerg": 16, "precip": 17, "pwat": 18}
 
     def __init__(self, nlon, nlat, ntrunc, nlev, ak, bk, dt, ntrac=0, **kw):
         self.ak = np.ascontiguousarray(ak, np.float64)
@@ -250,10 +257,18 @@ class Model:
         lib().orc_hs_phytend(self.h, a, b, c)
         return dict(dvrt=a, ddiv=b, dtv=c)
 
+    def simple_phytend(self):
+        s3 = (self.nlev, self.nlm)
+        a, b, c = (np.zeros(s3, np.complex128) for _ in range(3))
+        q = np.zeros((max(self.ntrac, 1), self.nlev, self.nlm), np.complex128)
+        lib().orc_simple_phytend(self.h, a, b, c, q)
+        return dict(dvrt=a, ddiv=b, dtv=c, dtrac=q)
+
     def grid(self, name):
         n2 = self.nlat * self.nlon
         nk = {"etadot": self.nlev + 1, "pk": self.nlev + 1, "psg": 1, "dlnpsdt": 1,
-              "dlnpsdx": 1, "dlnpsdy": 1, "tracerg": self.nlev * self.ntrac}.get(name, self.nlev)
+              "dlnpsdx": 1, "dlnpsdy": 1, "tracerg": self.nlev * self.ntrac, "precip": 1,
+              "pwat": 1}.get(name, self.nlev)
         out = np.zeros(nk * n2)
         lib().orc_get_grid(self.h, self.GRID[name], out)
         return out.reshape(nk, self.nlat, self.nlon)
@@ -288,6 +303,31 @@ def calc_pressdata(ak, bk, rk, lnpsg):
     rlnp = np.zeros((nlev, ncol)); prs = np.zeros((nlev, ncol))
     lib().orc_calc_pressdata(nlev, ncol, ak, bk, rk, lnpsg, pk, dpk, alfa, rlnp, prs)
     return dict(pk=pk, dpk=dpk, alfa=alfa, rlnp=rlnp, prs=prs)
+
+
+def vadv_tracers(datag, etadot, dpk):
+    """getvadv_tracers on columns: datag (nlev, ncol) bottom -> top, etadot
+    (nlev+1, ncol) and dpk (nlev, ncol) top -> bottom."""
+    datag, etadot, dpk = (np.ascontiguousarray(x, np.float64) for x in (datag, etadot, dpk))
+    nl, n = datag.shape
+    out = np.zeros_like(datag)
+    lib().orc_vadv_tracers_cols(nl, n, datag, etadot, dpk, out)
+    return out
+
+
+def simple_column(dtime, lat, t, q, u, v, pmid, pint, pdel, rpdel, ps, sst_mode=0, turbflux=True):
+    """simple_physics_v5_1.f90 on one column (top -> bottom); returns the
+    updated (t, q, u, v) and the precipitation rate (m/s)."""
+    t, q, u, v = (np.array(x, np.float64) for x in (t, q, u, v))
+    pmid, pint, pdel, rpdel = (np.ascontiguousarray(x, np.float64) for x in (pmid, pint, pdel, rpdel))
+    pr = C.c_double(0.0)
+    lib().orc_simple_column(t.size, dtime, lat, t, q, u, v, pmid, pint, pdel, rpdel, ps, C.byref(pr),
+                            int(sst_mode), int(bool(turbflux)))
+    return t, q, u, v, pr.value
+
+
+def simple_sst(sst_mode, lat):
+    return lib().orc_simple_sst(int(sst_mode), float(lat))
 
 
 def rng_uniform(seed, stream, index):
