@@ -45,6 +45,10 @@ void gfsdc_cfg_defaults(gfsdc_cfg *c) {
     c->cvap = (double)1.8460e+3f;
     c->simple_test = 0;
     c->turbflux = 1;
+    c->tpat_amp = 0.0;
+    c->tpat_tau = 6.0 * 3600.0;
+    c->tpat_lscale = 500.0e3;
+    c->tpat_seed = 1;
 }
 
 int gfsdc_create(const gfsdc_cfg *cfg, gfsdc_ctx **out) {
@@ -165,6 +169,7 @@ int gfsdc_hs_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, int dtype,
 int gfsdc_simple_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc) {
     DISPATCH(c, simple_phytend(dvrt, ddiv, dtv, dtrac, dtype, loc));
 }
+int gfsdc_get_pattern(gfsdc_ctx *c, void *spec, int dtype, int loc) { DISPATCH(c, get_pattern(spec, dtype, loc)); }
 int gfsdc_get_grid(gfsdc_ctx *c, int which, void *out, int dtype, int loc) {
     DISPATCH(c, get_grid(which, out, dtype, loc));
 }

@@ -37,6 +37,15 @@ static const double TAB_IMP[3] = {SGLF(1. / 6.), SGLF(1. / 3.), SGLF(1. / 4.)};
 
 constexpr int MAXLEV = 64;  // levels map onto the 64 lanes of a wavefront in the solve
 
+// counter-based RNG: splitmix64 finaliser over a mixed (seed, stream, index) key
+__host__ __device__ inline uint64_t rng_mix(uint64_t seed, uint64_t stream, uint64_t index) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ULL ^ (stream + 0x632BE59BD9B4E019ULL) * 0xBF58476D1CE4E5B9ULL;
+    z += index * 0x94D049BB133111EBULL + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
 struct Coeffs3 {
     double a[3], aa[3];
 };
@@ -73,6 +82,12 @@ struct DycoreBuffers {
     void *pwat = nullptr, *precip = nullptr;     // T per member ng (simple physics)
     void *pblscr = nullptr;                      // T per member 2 nlev ng: PBL sweep coefficients
     double *latd = nullptr;                      // FP64 latitude (radians), nlat
+    // AR(1) temperature pattern (patterngenerator.f90), per member
+    void *tpat = nullptr;                        // cplx<T> nm * nlm
+    double *tpat_var = nullptr;                  // FP64 nlm, normalised variance spectrum
+    void *tpat_vf = nullptr;                     // T nlev, vertical taper (bottom -> top)
+    unsigned long long *tpat_ctr = nullptr;      // advances so far (random stream index)
+    double tpat_phi = 0.0;
     double *diag = nullptr;                      // FP64 4 per member
     SynthDesc *sdesc = nullptr;                  // nm*nsyn
     AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr, *adesc_sp = nullptr;
@@ -608,6 +623,60 @@ __global__ __launch_bounds__(256) void k_massfix_field(const T *__restrict__ gsy
         LNPS[0] + (T)log((pdryini[mem] + grav * pwsum[mem]) / pmean[mem]);
 }
 
+// AR(1) pattern advance (patterngenerator.f90:134-143 with getnoise 115-132):
+// pat = phi pat + f var noise, one thread per (coefficient, member). Noise is
+// Box-Muller on the counter RNG: seed + member, stream 2^40 + advance count
+// (read from the device counter so graph replays continue the stream),
+// index 2k / 2k+1. sc = sqrt(normfact / ntrunc) with normfact = 2 pi.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pattern_advance(cplx<T> *__restrict__ pat, const double *__restrict__ var,
+                                                         const int2 *__restrict__ kmn, int nlm,
+                                                         const unsigned long long *__restrict__ ctr,
+                                                         unsigned long long seed, double phi, double f, double sc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = blockIdx.y;
+    if (k >= nlm) return;
+    double nr = 0.0, ni = 0.0;
+    if (k > 0) {
+        const uint64_t stream = (1ULL << 40) + ctr[0], sd = seed + (uint64_t)mem;
+        const double u1 = (double)(rng_mix(sd, stream, 2ULL * k) >> 11) * 0x1p-53;
+        const double u2 = (double)(rng_mix(sd, stream, 2ULL * k + 1) >> 11) * 0x1p-53;
+        const double r = sqrt(-2.0 * log(1.0 - u1));
+        const double gr = r * cos(2.0 * M_PI * u2), gi = r * sin(2.0 * M_PI * u2);
+        const int2 mn = kmn[k];
+        const double w = 1.0 / sqrt(2.0 * mn.y + 1.0);
+        if (mn.x != 0) {
+            nr = gr * w * sc;
+            ni = gi * w * sc;
+        } else {
+            nr = sqrt(2.0) * gr * w * sc;
+        }
+    }
+    cplx<T> *p = pat + (long)mem * nlm + k;
+    const double v = f * var[k];
+    *p = cplx<T>{(T)(phi * (double)p->x + v * nr), (T)(phi * (double)p->y + v * ni)};
+}
+
+__global__ void k_counter_inc(unsigned long long *ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] += 1ULL;
+}
+
+// temperature forcing pattern / tau on every level (vertical taper vf)
+template <typename T>
+__global__ void k_pattern_apply(cplx<T> *__restrict__ spec, const cplx<T> *__restrict__ pat, const T *__restrict__ vf,
+                                int nl, int nfs, int nm, int nlm, T dt_tau) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)nlm * nm * nl;
+    if (i >= tot) return;
+    const int lev = (int)(i % nl);
+    const long r = i / nl;
+    const int mem = (int)(r % nm), k = (int)(r / nm);
+    const cplx<T> q = pat[(long)mem * nlm + k];
+    const T a = dt_tau * vf[lev];
+    cplx<T> *o = spec + ((long)k * nm + mem) * nfs + 2 * nl + lev;
+    *o = cplx<T>{o->x + a * q.x, o->y + a * q.y};
+}
+
 // Per-step diagnostics of run.f90:159-162 (spdmax, min/max ps, pstend) per member.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_diag(const T *__restrict__ gsyn, const T *__restrict__ psg,
@@ -947,13 +1016,7 @@ template <typename T> static std::vector<T> castv(const std::vector<double> &v) 
     return std::vector<T>(v.begin(), v.end());
 }
 
-uint64_t rng_u64(uint64_t seed, uint64_t stream, uint64_t index) {
-    uint64_t z = seed * 0x9E3779B97F4A7C15ULL ^ (stream + 0x632BE59BD9B4E019ULL) * 0xBF58476D1CE4E5B9ULL;
-    z += index * 0x94D049BB133111EBULL + 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
+uint64_t rng_u64(uint64_t seed, uint64_t stream, uint64_t index) { return rng_mix(seed, stream, index); }
 
 template <typename T> int Engine<T>::dyn_init() {
     const int nl = cfg.nlev, ntr = cfg.ntrac, nm = cfg.nmem, T_ = plan.ntrunc, nlm = plan.nlm;
@@ -1186,6 +1249,44 @@ template <typename T> int Engine<T>::dyn_init() {
     Work<T> *W = new Work<T>();
     B->Work = W;
     if ((rc = work_alloc(*W, plan, nm * std::max(B->nsyn, B->nan)))) return rc;
+    // ---- AR(1) temperature pattern (patterngenerator.f90:33-177, stoch_data.f90 spin-up)
+    if (cfg.tpat_amp > 0) {
+        if (!(cfg.tpat_tau > 0) || cfg.tpat_lscale < 0) {
+            set_error("tpat_tau must be > 0 and tpat_lscale >= 0 when tpat_amp > 0");
+            return GFSDC_EINVAL;
+        }
+        std::vector<double> var(nlm), vf(nl);
+        const double L = cfg.tpat_lscale, a = cfg.rerth, twopi = 2.0 * M_PI;
+        double v = 0.0;
+        for (int m = 0, k = 0; m <= T_; m++)
+            for (int n = m; n <= T_; n++, k++) {
+                var[k] = std::sqrt(T_ * std::exp(L * L * (-(double)n * (n + 1.0)) / (4.0 * a * a)));
+                double nz2 = (k == 0) ? 0.0 : 2.0 / (2.0 * n + 1.0) * (twopi / T_);
+                v += (m != 0 ? 1.0 : 0.5) * var[k] * var[k] * nz2;
+            }
+        v /= twopi;
+        for (double &x : var) x /= std::sqrt(v);
+        for (int k = 0; k < nl; k++) {
+            const double sl = B->sl[k];
+            vf[k] = 1.0;
+            if (sl < 0.10 && sl > 0.05) vf[k] = (sl - 0.05) / (0.10 - 0.05);
+            else if (sl <= 0.05) vf[k] = 0.0;
+        }
+        void *p;
+        if ((rc = dupload<double>(&p, var, stream))) return rc;
+        B->tpat_var = (double *)p;
+        if ((rc = dupload<T>(&B->tpat_vf, castv<T>(vf), stream))) return rc;
+        B->tpat = dalloc<cplx<T>>((size_t)nm * nlm);
+        B->tpat_ctr = dalloc<unsigned long long>(1);
+        if (!B->tpat || !B->tpat_ctr) {
+            set_error("device allocation failed for the pattern");
+            return GFSDC_ENOMEM;
+        }
+        GFSDC_CHECK(hipMemsetAsync(B->tpat, 0, sizeof(cplx<T>) * nm * nlm, stream));
+        GFSDC_CHECK(hipMemsetAsync(B->tpat_ctr, 0, sizeof(unsigned long long), stream));
+        B->tpat_phi = std::exp(-cfg.dt / cfg.tpat_tau);
+        if ((rc = pattern_spinup())) return rc;
+    }
     // ---- descriptors
     std::vector<SynthDesc> sd((size_t)nm * B->nsyn);
     std::vector<unsigned char> rs((size_t)nm * B->nsyn, 0), ra((size_t)nm * B->nan, 0), rh((size_t)nm * B->nan, 0);
@@ -1289,7 +1390,7 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->ak, B->bk, B->ck, B->dbk, B->sinlat, B->coslat, B->areaw, B->kbydeg, B->degree,
                     B->pdryini, B->pmean, B->pspart, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->rflag_syn,
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
-                    B->adesc_sp, B->rflag_sp};
+                    B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (B->Work) {
@@ -1430,7 +1531,28 @@ template <typename T> struct Step {
                            (const cplx<T> *)B.tphys, B.nl, B.ntr, B.nfs, B.nm, E.plan.nlm, (T)c.dt, c.massfix);
         mark("phys_update");
     }
+    // one AR(1) advance of every member's pattern (phi = 0 and f = amp start it)
+    void pattern_advance(double phi, double f) {
+        const Plan<T> &p = E.plan;
+        hipLaunchKernelGGL(k_pattern_advance<T>, dim3((unsigned)((p.nlm + 255) / 256), B.nm), dim3(256), 0, s,
+                           (cplx<T> *)B.tpat, (const double *)B.tpat_var, (const int2 *)p.kmn, p.nlm,
+                           (const unsigned long long *)B.tpat_ctr,
+                           (unsigned long long)(long long)E.cfg.tpat_seed, phi, f,
+                           std::sqrt(2.0 * M_PI / p.ntrunc));
+        hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, s, B.tpat_ctr);
+        mark("pattern_advance");
+    }
+    void pattern_apply() {
+        long n = (long)E.plan.nlm * B.nm * B.nl;
+        hipLaunchKernelGGL(k_pattern_apply<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (cplx<T> *)B.spec,
+                           (const cplx<T> *)B.tpat, (const T *)B.tpat_vf, B.nl, B.nfs, B.nm, E.plan.nlm,
+                           (T)(E.cfg.dt / E.cfg.tpat_tau));
+        mark("pattern_apply");
+    }
     void step() {
+        const bool pat = B.tpat != nullptr;
+        // run.f90:272-292: the random pattern used this step
+        if (pat) pattern_advance(B.tpat_phi, E.cfg.tpat_amp * std::sqrt(1.0 - B.tpat_phi * B.tpat_phi));
         long n = 2L * E.plan.nlm * B.nfs * B.nm;
         hipLaunchKernelGGL(k_copy<T>, dim3(2048), dim3(256), 0, s, (T *)B.spec0, (const T *)B.spec, n);
         mark("save_state");
@@ -1439,8 +1561,32 @@ template <typename T> struct Step {
             si_stage(st, st == 2);
         }
         if (E.cfg.physics != GFSDC_PHYS_NONE) physics();
+        // stochastic temperature forcing, where run.f90:539-551 adds its tvforcing
+        if (pat) pattern_apply();
     }
 };
+
+// init_stochdata (stoch_data.f90): start from stdev * varspectrum * noise and
+// spin up 5 e-folding times
+template <typename T> int Engine<T>::pattern_spinup() {
+    DycoreBuffers *B = dyn;
+    Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    st.pattern_advance(0.0, cfg.tpat_amp);
+    const int nspinup = (int)(5.0 * cfg.tpat_tau / cfg.dt);
+    for (int i = 0; i < nspinup; i++)
+        st.pattern_advance(B->tpat_phi, cfg.tpat_amp * std::sqrt(1.0 - B->tpat_phi * B->tpat_phi));
+    GFSDC_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T> int Engine<T>::get_pattern(void *out, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B || !B->tpat) {
+        set_error("context has no stochastic pattern (tpat_amp == 0)");
+        return GFSDC_ESTATE;
+    }
+    return output((const T *)B->tpat, out, (size_t)plan.nlm * B->nm * 2, dtype, loc, 1);
+}
 
 template <typename T>
 int Engine<T>::set_state(const void *vrt, const void *div, const void *tv, const void *trac, const void *lnps,

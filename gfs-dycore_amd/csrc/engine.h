@@ -63,6 +63,8 @@ template <typename T> struct Engine {
     int hs_phytend(void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
     int simple_phytend(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
     int pack_tphys(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
+    int pattern_spinup();
+    int get_pattern(void *out, int dtype, int loc);
     int get_grid(int which, void *out, int dtype, int loc);
     int get_semimp(double *amhyb, double *bmhyb, double *svhyb, double *tor_hyb, double *d_hyb_m);
     int get_damp(double *disspec, double *diff_prof, double *dmp_prof);

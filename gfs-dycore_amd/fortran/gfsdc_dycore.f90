@@ -23,6 +23,8 @@ module gfsdc_dycore
     real(c_double) :: rerth, grav, omega, rd, rv, cp, cvap
     integer(c_int) :: device
     integer(c_int) :: simple_test, turbflux
+    real(c_double) :: tpat_amp, tpat_tau, tpat_lscale
+    integer(c_int) :: tpat_seed
   end type
 
   interface

@@ -39,6 +39,8 @@ class Cfg(C.Structure):
         ("rerth", C.c_double), ("grav", C.c_double), ("omega", C.c_double), ("rd", C.c_double),
         ("rv", C.c_double), ("cp", C.c_double), ("cvap", C.c_double), ("device", C.c_int),
         ("simple_test", C.c_int), ("turbflux", C.c_int),
+        ("tpat_amp", C.c_double), ("tpat_tau", C.c_double), ("tpat_lscale", C.c_double),
+        ("tpat_seed", C.c_int),
     ]
 
 
@@ -50,7 +52,7 @@ SYMBOLS = [
     "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_grdtospec", "gfsdc_getuv",
     "gfsdc_getvrtdivspec", "gfsdc_getgrad", "gfsdc_set_state", "gfsdc_get_state",
     "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
-    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_grid", "gfsdc_get_semimp",
+    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_get_grid", "gfsdc_get_semimp",
     "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_sync",
     "gfsdc_stream", "gfsdc_profile_step",
 ]
@@ -88,6 +90,7 @@ def load():
     L.gfsdc_advance.argtypes = [vp, i, i]
     L.gfsdc_hs_phytend.argtypes = [vp, vp, vp, vp, i, i]
     L.gfsdc_simple_phytend.argtypes = [vp, vp, vp, vp, vp, i, i]
+    L.gfsdc_get_pattern.argtypes = [vp, vp, i, i]
     L.gfsdc_get_grid.argtypes = [vp, i, vp, i, i]
     L.gfsdc_get_semimp.argtypes = [vp, vp, vp, vp, vp, vp]
     L.gfsdc_get_damp.argtypes = [vp, vp, vp, vp]
@@ -272,6 +275,12 @@ class Context:
         q = np.zeros((self.nmem, max(self.ntrac, 1), self.nlev, self.nlm), np.complex128)
         _check(load().gfsdc_simple_phytend(self.h, _p(a), _p(b), _p(c), _p(q), DT_F64, HOST))
         return dict(dvrt=a, ddiv=b, dtv=c, dtrac=q[:, :self.ntrac])
+
+    def pattern(self):
+        """current AR(1) temperature pattern, (nmem, ndimspec) complex"""
+        out = np.zeros((self.nmem, self.nlm), np.complex128)
+        _check(load().gfsdc_get_pattern(self.h, _p(out), DT_F64, HOST))
+        return out
 
     def grid(self, name):
         nk = {"etadot": self.nlev + 1, "psg": 1, "dlnpsdt": 1, "dlnpsdx": 1, "dlnpsdy": 1,

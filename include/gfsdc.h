@@ -100,6 +100,13 @@ typedef struct gfsdc_cfg {
      * dcmip 4x), 2 302.15 K - 30 K sin^2(lat) (this library's own);
      * turbflux: surface fluxes + boundary-layer diffusion (dcmip 43 and 5x) */
     int simple_test, turbflux;
+    /* AR(1) spectral temperature pattern per member (patterngenerator.f90,
+     * advanced at the start of each step as run.f90:272-292) applied as the
+     * temperature forcing pattern/tau at the end of the step, tapered to 0
+     * above sigma 0.05 (stoch_data.f90). tpat_amp (K) = 0 disables it;
+     * tpat_tau (s), tpat_lscale (m); member i uses seed tpat_seed + i. */
+    double tpat_amp, tpat_tau, tpat_lscale;
+    int tpat_seed;
 } gfsdc_cfg;
 
 typedef struct gfsdc_ctx gfsdc_ctx;
@@ -163,6 +170,8 @@ int gfsdc_hs_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, int dtyp
  * also refreshes the precip and pwat grids. The state is not changed. */
 int gfsdc_simple_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype,
                          int loc);
+/* current AR(1) pattern, (ndimspec, nmem) complex of dtype (error if tpat_amp == 0) */
+int gfsdc_get_pattern(gfsdc_ctx *ctx, void *spec, int dtype, int loc);
 /* grid fields after the last getdyntend, names of grid_data.f90/pressure_data.f90:
  * 0 ug,1 vg,2 vrtg,3 divg,4 virtempg,5 dlnpdtg,6 etadot (nlev+1),7 psg,8 dlnpsdt,
  * 9 prs,14 dlnpsdx,15 dlnpsdy, 16 tracerg (nlev*ntrac), 17 precip (mm/s),

@@ -70,6 +70,11 @@ struct orc_model {
     double pdryini, kappa;
     ocplx *lin_d[3], *lin_t[3], *lin_p[3]; /* linear terms of the RK stages */
     int have_grid;
+    /* AR(1) temperature pattern */
+    ocplx *tpat;
+    double *tpat_var, *tpat_vfact, tpat_phi;
+    uint64_t tpat_step;
+    int *order;
 };
 
 #define L(X, k) ((X) + (long)((k) - 1) * m->ng)
@@ -256,6 +261,74 @@ static void setdampspec(orc_model *m) {
     }
 }
 
+/* ---- patterngenerator.f90:33-177 AR(1) spectral pattern --------------
+ * Gaussian noise from the counter RNG (Box-Muller on two uniforms per
+ * coefficient) in place of the reference's Mersenne twister stream:
+ * member seed = tpat_seed (+ member), stream = 2^40 + advance count,
+ * index = 2k, 2k+1. */
+static void pattern_noise(const orc_model *m, uint64_t step, ocplx *noise) {
+    const double normfact = 2.0 * M_PI; /* stoch_data.f90 passes 2 pi */
+    const double sc = sqrt(normfact / m->T);
+    const uint64_t seed = (uint64_t)(int64_t)m->c.tpat_seed, stream = (1ULL << 40) + step;
+    for (int k = 0; k < m->nlm; k++) {
+        if (k == 0) { noise[0] = 0.0; continue; } /* no global mean (115-116) */
+        double u1 = orc_rng_uniform(seed, stream, 2 * (uint64_t)k);
+        double u2 = orc_rng_uniform(seed, stream, 2 * (uint64_t)k + 1);
+        double r = sqrt(-2.0 * log(1.0 - u1));
+        double gr = r * cos(2.0 * M_PI * u2), gi = r * sin(2.0 * M_PI * u2);
+        double n = m->degree[k], w = 1.0 / sqrt(2.0 * n + 1.0);
+        noise[k] = (m->order[k] != 0) ? (gr + I * gi) * w * sc : sqrt(2.0) * gr * w * sc;
+    }
+}
+
+void orc_pattern_advance(orc_model *m) {
+    /* patterngenerator.f90:134-143 */
+    ocplx *noise = malloc(sizeof(ocplx) * m->nlm);
+    pattern_noise(m, m->tpat_step++, noise);
+    double f = m->c.tpat_amp * sqrt(1.0 - m->tpat_phi * m->tpat_phi);
+    for (int k = 0; k < m->nlm; k++) m->tpat[k] = m->tpat_phi * m->tpat[k] + f * m->tpat_var[k] * noise[k];
+    free(noise);
+}
+
+static void pattern_init(orc_model *m) {
+    int nlm = m->nlm, T = m->T;
+    double L = m->c.tpat_lscale, a = m->c.rerth;
+    /* gaussian_spect (145-177): sqrt(T exp(L^2 lap / 4a^2)), normalised so
+     * that the deterministic reference noise has unit grid variance */
+    double var = 0.0;
+    for (int k = 0; k < nlm; k++) {
+        double n = m->degree[k], lap = -n * (n + 1.0);
+        m->tpat_var[k] = sqrt(T * exp(L * L * lap / (4.0 * a * a)));
+        double nz2 = (k == 0) ? 0.0 : 2.0 / (2.0 * n + 1.0); /* |(1+i)|^2 or |sqrt 2|^2 over 2n+1 */
+        nz2 *= (2.0 * M_PI) / T;
+        double v = m->tpat_var[k] * m->tpat_var[k] * nz2;
+        var += (m->order[k] != 0) ? v : 0.5 * v;
+    }
+    var /= 2.0 * M_PI;
+    for (int k = 0; k < nlm; k++) m->tpat_var[k] /= sqrt(var);
+    m->tpat_phi = exp(-m->c.dt / m->c.tpat_tau);
+    /* vertical taper of stoch_data.f90 (sigma 0.05 - 0.10) */
+    for (int k = 0; k < m->nlev; k++) {
+        double sl = m->sl[k];
+        m->tpat_vfact[k] = 1.0;
+        if (sl < 0.10 && sl > 0.05) m->tpat_vfact[k] = (sl - 0.05) / (0.10 - 0.05);
+        else if (sl <= 0.05) m->tpat_vfact[k] = 0.0;
+    }
+    /* init_stochdata: start from scaled noise, spin up 5 e-folding times */
+    m->tpat_step = 0;
+    pattern_noise(m, m->tpat_step++, m->tpat);
+    for (int k = 0; k < nlm; k++) m->tpat[k] *= m->c.tpat_amp * m->tpat_var[k];
+    int nspinup = (int)(5.0 * m->c.tpat_tau / m->c.dt);
+    for (int i = 0; i < nspinup; i++) orc_pattern_advance(m);
+}
+
+void orc_get_pattern(const orc_model *m, ocplx *pattern, double *varspectrum, double *phi, double *vfact) {
+    if (pattern) memcpy(pattern, m->tpat, sizeof(ocplx) * m->nlm);
+    if (varspectrum) memcpy(varspectrum, m->tpat_var, sizeof(double) * m->nlm);
+    if (phi) *phi = m->tpat_phi;
+    if (vfact) memcpy(vfact, m->tpat_vfact, sizeof(double) * m->nlev);
+}
+
 orc_model *orc_model_create(const orc_cfg *c) {
     orc_model *m = calloc(1, sizeof(orc_model));
     m->c = *c;
@@ -315,6 +388,12 @@ orc_model *orc_model_create(const orc_cfg *c) {
     }
     init_semimp(m);
     setdampspec(m);
+    m->order = malloc(sizeof(int) * nlm);
+    orc_sht_griddata(m->sht, NULL, NULL, NULL, NULL, m->order, NULL, NULL);
+    m->tpat = calloc(nlm, sizeof(ocplx));
+    m->tpat_var = calloc(nlm, sizeof(double));
+    m->tpat_vfact = calloc(nl, sizeof(double));
+    if (c->tpat_amp > 0) pattern_init(m);
     return m;
 }
 
@@ -327,7 +406,8 @@ void orc_model_destroy(orc_model *m) {
                     m->etadot, m->prs, m->pk, m->dpk, m->alfa, m->rlnp, m->cb, m->db, m->cg,
                     m->wk1, m->lnpsg, m->psg, m->dlnpsdt, m->dphisdx, m->dphisdy, m->dlnpsdx,
                     m->dlnpsdy, m->amhyb, m->bmhyb, m->svhyb, m->tor_hyb, m->d_hyb_m,
-                    m->disspec, m->diff_prof, m->dmp_prof, m->precip, m->pwat};
+                    m->disspec, m->diff_prof, m->dmp_prof, m->precip, m->pwat, m->order,
+                    m->tpat, m->tpat_var, m->tpat_vfact};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int i = 0; i < 10; i++) free(m->w[i]);
     for (int s = 0; s < 3; s++) { free(m->lin_d[s]); free(m->lin_t[s]); free(m->lin_p[s]); }
@@ -711,6 +791,8 @@ void orc_advance(orc_model *m, int nsteps) {
     const double caa[3][3] = {{AA21, 0, 0}, {AA31, AA32, 0}, {BB1, BB2, BB3}};
     const double cimp[3] = {AA22, AA33, BB4};
     for (int step = 0; step < nsteps; step++) {
+        /* run.f90:272-292: advance the random pattern used this step */
+        if (m->c.tpat_amp > 0) orc_pattern_advance(m);
         memcpy(vrt0, m->vrt, s3); memcpy(div0, m->div, s3); memcpy(tv0, m->tv, s3);
         if (ntrac > 0) memcpy(tr0, m->trac, st);
         memcpy(lp0, m->lnps, s1);
@@ -791,6 +873,12 @@ void orc_advance(orc_model *m, int nsteps) {
             }
             for (int k = 0; k < nlm; k++) m->lnps[k] += dt * dp[1][k];
         }
+        /* stochastic temperature forcing pattern/tau, in the place of the
+         * additive-noise tvforcing of run.f90:539-551 */
+        if (m->c.tpat_amp > 0)
+            for (int k = 0; k < nl; k++)
+                for (int n = 0; n < nlm; n++)
+                    m->tv[(long)k * nlm + n] += dt * m->tpat_vfact[k] * m->tpat[n] / m->c.tpat_tau;
     }
     free(vrt0); free(div0); free(tv0); free(tr0); free(lp0);
     for (int s = 0; s < 3; s++) {

@@ -85,6 +85,10 @@ typedef struct {
      * 302.15 K, 1 baroclinic-wave formula, 2 302.15 K - 30 K sin^2 lat) and
      * the turbulent surface/boundary-layer fluxes switch (phy_run_simple.f90:47-49) */
     int simple_test, turbflux;
+    /* AR(1) spectral temperature pattern (patterngenerator.f90) added as a
+     * temperature forcing pattern/tau at the end of each step; amp (K) = 0: off */
+    double tpat_amp, tpat_tau, tpat_lscale;
+    int tpat_seed;
 } orc_cfg;
 
 typedef struct orc_model orc_model;
@@ -143,6 +147,14 @@ void orc_calc_pressdata(int nlev, int ncol, const double *ak, const double *bk, 
  * datag (nlev, bottom->top), etadot (nlev+1, top->bottom), dpk (nlev, top->bottom) */
 void orc_vadv_tracers_cols(int nlev, long ncol, const double *datag, const double *etadot,
                            const double *dpk, double *vadv);
+
+/* AR(1) pattern of the model (patterngenerator.f90): current spectral pattern
+ * (ndimspec complex), the normalised variance spectrum (ndimspec), phi and the
+ * level factor vfact (nlev, bottom -> top); any pointer may be NULL */
+void orc_get_pattern(const orc_model *m, ocplx *pattern, double *varspectrum, double *phi,
+                     double *vfact);
+/* one patterngenerator_advance of the model's pattern */
+void orc_pattern_advance(orc_model *m);
 
 /* counter-based RNG shared with the product: splitmix64 of (seed, stream, index) */
 uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index);

@@ -32,6 +32,8 @@ class Cfg(C.Structure):
         ("rerth", C.c_double), ("grav", C.c_double), ("omega", C.c_double), ("rd", C.c_double),
         ("rv", C.c_double), ("cp", C.c_double), ("cvap", C.c_double),
         ("simple_test", C.c_int), ("turbflux", C.c_int),
+        ("tpat_amp", C.c_double), ("tpat_tau", C.c_double), ("tpat_lscale", C.c_double),
+        ("tpat_seed", C.c_int),
     ]
 
 
@@ -86,6 +88,8 @@ def lib():
         L.orc_step_diag.argtypes = [vp, _dp]
         L.orc_calc_pressdata.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp,
                                          _dp, _dp, _dp]
+        L.orc_get_pattern.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_pattern_advance.argtypes = [vp]
         L.orc_vadv_tracers_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
         L.orc_rng_u64.restype = C.c_uint64
         L.orc_rng_u64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
@@ -263,6 +267,18 @@ class Model:
         q = np.zeros((max(self.ntrac, 1), self.nlev, self.nlm), np.complex128)
         lib().orc_simple_phytend(self.h, a, b, c, q)
         return dict(dvrt=a, ddiv=b, dtv=c, dtrac=q)
+
+    def pattern(self):
+        """AR(1) temperature pattern: dict(spec, varspectrum, phi, vfact)"""
+        spec = np.zeros(self.nlm, np.complex128)
+        var = np.zeros(self.nlm)
+        phi = C.c_double(0.0)
+        vf = np.zeros(self.nlev)
+        lib().orc_get_pattern(self.h, _ptr(spec), _ptr(var), C.byref(phi), _ptr(vf))
+        return dict(spec=spec, varspectrum=var, phi=phi.value, vfact=vf)
+
+    def pattern_advance(self):
+        lib().orc_pattern_advance(self.h)
 
     def grid(self, name):
         n2 = self.nlat * self.nlon
