@@ -170,6 +170,35 @@ int gfsdc_simple_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, void *
     DISPATCH(c, simple_phytend(dvrt, ddiv, dtv, dtrac, dtype, loc));
 }
 int gfsdc_get_pattern(gfsdc_ctx *c, void *spec, int dtype, int loc) { DISPATCH(c, get_pattern(spec, dtype, loc)); }
+int gfsdc_mlp_init(gfsdc_ctx *c, int hidden, int nlayers, int maxbatch, long capacity, unsigned long long seed) {
+    DISPATCH(c, mlp_init(hidden, nlayers, maxbatch, capacity, seed));
+}
+int gfsdc_mlp_collect(gfsdc_ctx *c, int nsteps, int stride, long *ncollected) {
+    DISPATCH(c, mlp_collect(nsteps, stride, ncollected));
+}
+int gfsdc_mlp_normalize(gfsdc_ctx *c) { DISPATCH(c, mlp_normalize()); }
+int gfsdc_mlp_train(gfsdc_ctx *c, int nsteps, int batch, double lr, double wd, long ntrain, double *losses) {
+    DISPATCH(c, mlp_train(nsteps, batch, lr, wd, ntrain, losses));
+}
+int gfsdc_mlp_eval(gfsdc_ctx *c, long first, long count, double *err, double *base) {
+    DISPATCH(c, mlp_eval(first, count, err, base));
+}
+int gfsdc_mlp_enable(gfsdc_ctx *c, int on, int judge_stride, double threshold) {
+    DISPATCH(c, mlp_enable(on, judge_stride, threshold));
+}
+int gfsdc_mlp_judge_stats(gfsdc_ctx *c, double *out) { DISPATCH(c, mlp_judge_stats(out)); }
+long gfsdc_mlp_nparams(gfsdc_ctx *c) {
+    if (!c) return 0;
+    return c->ef ? c->ef->mlp_nparams() : (c->ed ? c->ed->mlp_nparams() : 0);
+}
+int gfsdc_mlp_params(gfsdc_ctx *c, float *get, const float *set) { DISPATCH(c, mlp_params(get, set)); }
+int gfsdc_mlp_forward(gfsdc_ctx *c, const float *x, long n, float *y) { DISPATCH(c, mlp_forward_host(x, n, y)); }
+int gfsdc_mlp_grad(gfsdc_ctx *c, const float *x, const float *t, int n, float *grad, double *loss) {
+    DISPATCH(c, mlp_grad_host(x, t, n, grad, loss));
+}
+int gfsdc_mlp_adamw(gfsdc_ctx *c, const float *grad, double lr, double wd) {
+    DISPATCH(c, mlp_adamw_host(grad, lr, wd));
+}
 int gfsdc_get_grid(gfsdc_ctx *c, int which, void *out, int dtype, int loc) {
     DISPATCH(c, get_grid(which, out, dtype, loc));
 }

@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "engine.h"
+#include "mlp.h"
 
 namespace gfsdc {
 
@@ -88,6 +89,21 @@ struct DycoreBuffers {
     void *tpat_vf = nullptr;                     // T nlev, vertical taper (bottom -> top)
     unsigned long long *tpat_ctr = nullptr;      // advances so far (random stream index)
     double tpat_phi = 0.0;
+    // learned column physics (replaces the simple-physics column call when on)
+    MlpNet *mlp = nullptr;
+    float *ds_x = nullptr, *ds_y = nullptr;      // dataset: raw features / targets [cap][nin|nout]
+    long ds_cap = 0, ds_n = 0;
+    float *nrm = nullptr;                        // [mean_x | std_x | mean_y | std_y]
+    bool nrm_ready = false, mlp_on = false;
+    float *xin = nullptr, *yout = nullptr;       // inference rows [nm*ng][nin], [nm*ng][nout]
+    float *tbx = nullptr, *tby = nullptr;        // training batch (normalised)
+    int judge_stride = 50;
+    double judge_thr = 0.5;
+    void *ganj = nullptr, *pwatj = nullptr, *precipj = nullptr;  // plain physics on the judged columns
+    double *jpart = nullptr, *jstat = nullptr;   // judge partial sums; [last error, fallbacks, judged steps]
+    int *jflag = nullptr;                        // 1: this step falls back to the plain physics
+    unsigned long long *jctr = nullptr;          // judge / sampling step counter
+    unsigned long long bctr = 0;                 // training batch counter
     double *diag = nullptr;                      // FP64 4 per member
     SynthDesc *sdesc = nullptr;                  // nm*nsyn
     AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr, *adesc_sp = nullptr;
@@ -417,6 +433,24 @@ struct SimpleCfg {
     int sst_mode, turbflux;
 };
 
+// column selection of the column kernels: every column, or (judge and data
+// sampling) columns i * stride + off with off drawn from the counter RNG by a
+// device step counter; gate: when set and 0, the launch does nothing (the
+// plain-physics fallback of the learned physics)
+struct ColSel {
+    int stride;
+    const unsigned long long *ctr;
+    const int *gate;
+};
+
+__device__ inline long sel_col(const ColSel &cs, long i, long ng) {
+    if (cs.stride <= 0) return i < ng ? i : -1;
+    const long nsel = ng / cs.stride;
+    if (i >= nsel) return -1;
+    const long off = (long)(rng_mix(0x4A554447ULL, cs.ctr ? cs.ctr[0] : 0ULL, 0) % (uint64_t)cs.stride);
+    return i * cs.stride + off;
+}
+
 __device__ inline double simple_sst(int mode, double lat) {
     if (mode == 1) {
         // simple_physics_v5_1.f90:231-239 (moist baroclinic wave)
@@ -441,16 +475,18 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_simple_columns(
     const T *__restrict__ gsyn, const T *__restrict__ psg, T *__restrict__ gan, T *__restrict__ pbl,
     T *__restrict__ pwat_out, T *__restrict__ precip_out, const T *__restrict__ ak_, const T *__restrict__ bk_,
-    const double *__restrict__ latd, int nl, int ntr, int nsyn, int nsp, long ng, int nlon, T kappa, SimpleCfg sc) {
+    const double *__restrict__ latd, int nl, int ntr, int nsyn, int nsp, long ng, int nlon, T kappa, SimpleCfg sc,
+    ColSel cs) {
+    if (cs.gate && cs.gate[0] == 0) return;
     __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1];
     for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
         ak[i] = ak_[i];
         bk[i] = bk_[i];
     }
     __syncthreads();
-    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long col = sel_col(cs, (long)blockIdx.x * blockDim.x + threadIdx.x, ng);
     const int mem = blockIdx.y;
-    if (col >= ng) return;
+    if (col < 0) return;
     // simple_physics_v5_1.f90:175-222
     const double gravit = 9.80616, rair = 287.0, cpair = 1.0045e3, latvap = 2.5e6, rh2o = 461.5;
     const double epsilo = rair / rh2o, zvir = (rh2o / rair) - 1.0, Csfc = 0.0011, T0 = 273.16;
@@ -605,6 +641,225 @@ __global__ __launch_bounds__(256) void k_simple_columns(
     }
     pwat_out[(long)mem * ng + col] = (T)(pw / sc.grav);
     precip_out[(long)mem * ng + col] = (T)(precl * 1000.0);
+}
+
+// ---- learned column physics: features, targets, application, judge --------
+// Column features (top -> bottom, the plain physics' inputs): T (= Tv / (1 + fv
+// max(qmin, q))), q, u, v (nlev each), ps, sea-surface temperature, sin(lat);
+// targets: the plain physics' changes of Tv, q, u, v over the step. Optional
+// normalisation nrm = [mean_x | std_x | mean_y | std_y].
+template <typename T>
+__global__ __launch_bounds__(256) void k_mlp_features(const T *__restrict__ gsyn, const T *__restrict__ psg,
+                                                      const double *__restrict__ latd, int nl, int nsyn, long ng,
+                                                      int nlon, SimpleCfg sc, ColSel cs, long nsel,
+                                                      const float *__restrict__ nrm, float *__restrict__ x,
+                                                      long row0) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long col = sel_col(cs, i, ng);
+    const int mem = blockIdx.y;
+    if (col < 0) return;
+    const int nin = 4 * nl + 3;
+    const T *G = gsyn + (long)mem * nsyn * ng + col;
+    const T *U = G, *V = G + (long)nl * ng, *TV = G + 4L * nl * ng, *Q = G + 7L * nl * ng;
+    float *xr = x + (row0 + (long)mem * nsel + i) * nin;
+    const T fvT = (T)sc.fv, qminT = (T)sc.qmin;
+    for (int c = 1; c <= nl; c++) {
+        const long o = (long)(nl - c) * ng;
+        xr[c - 1] = (float)(TV[o] / (T(1) + fvT * fmax(qminT, Q[o])));
+        xr[nl + c - 1] = (float)Q[o];
+        xr[2 * nl + c - 1] = (float)U[o];
+        xr[3 * nl + c - 1] = (float)V[o];
+    }
+    const double lat = latd[col / nlon];
+    xr[4 * nl] = (float)psg[(long)mem * ng + col];
+    xr[4 * nl + 1] = (float)simple_sst(sc.sst_mode, lat);
+    xr[4 * nl + 2] = (float)sin(lat);
+    if (nrm)
+        for (int f = 0; f < nin; f++) xr[f] = (xr[f] - nrm[f]) / nrm[nin + f];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mlp_targets(const T *__restrict__ gan, int nl, int nsp, long ng, ColSel cs,
+                                                     long nsel, float *__restrict__ y, long row0) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long col = sel_col(cs, i, ng);
+    const int mem = blockIdx.y;
+    if (col < 0) return;
+    const T *A = gan + (long)mem * nsp * ng + col;
+    float *yr = y + (row0 + (long)mem * nsel + i) * (4L * nl);
+    for (int c = 1; c <= nl; c++) {
+        const long o = (long)(nl - c) * ng;
+        yr[c - 1] = (float)A[o];                         // Tv change
+        yr[nl + c - 1] = (float)A[3L * nl * ng + o];     // q change
+        yr[2 * nl + c - 1] = (float)A[(long)nl * ng + o]; // u change
+        yr[3 * nl + c - 1] = (float)A[2L * nl * ng + o];  // v change
+    }
+}
+
+// network output (normalised, rows = member * ng + column) -> the physics slots
+// of gan and the column water pwat; no precipitation diagnostic
+template <typename T>
+__global__ __launch_bounds__(256) void k_mlp_apply(const float *__restrict__ y, const float *__restrict__ nrm,
+                                                   const T *__restrict__ gsyn, const T *__restrict__ psg,
+                                                   const T *__restrict__ ak, const T *__restrict__ bk,
+                                                   T *__restrict__ gan, T *__restrict__ pwat_out,
+                                                   T *__restrict__ precip_out, int nl, int nsyn, int nsp, long ng,
+                                                   double grav) {
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = blockIdx.y;
+    if (col >= ng) return;
+    const int nin = 4 * nl + 3, nout = 4 * nl;
+    const float *my = nrm + 2 * nin, *sy = my + nout;
+    const float *yr = y + ((long)mem * ng + col) * nout;
+    const T *Q = gsyn + (long)mem * nsyn * ng + 7L * nl * ng + col;
+    T *A = gan + (long)mem * nsp * ng + col;
+    const T ps = psg[(long)mem * ng + col];
+    double pw = 0.0;
+    for (int c = 1; c <= nl; c++) {
+        const long o = (long)(nl - c) * ng;
+        auto val = [&](int f) { return my[f] + sy[f] * yr[f]; };
+        A[o] = (T)val(c - 1);
+        const T dq = (T)val(nl + c - 1);
+        A[3L * nl * ng + o] = dq;
+        A[(long)nl * ng + o] = (T)val(2 * nl + c - 1);
+        A[2L * nl * ng + o] = (T)val(3 * nl + c - 1);
+        const double dp = (double)((ak[c] + bk[c] * ps) - (ak[c - 1] + bk[c - 1] * ps));
+        pw += dp * (double)(dq + Q[o]);
+    }
+    pwat_out[(long)mem * ng + col] = (T)(pw / grav);
+    precip_out[(long)mem * ng + col] = T(0);
+}
+
+// judge: normalised squared difference between the network's and the plain
+// physics' changes on the sampled columns (block partial sums)
+template <typename T>
+__global__ __launch_bounds__(256) void k_judge_partial(const T *__restrict__ gan, const T *__restrict__ ganj,
+                                                       const float *__restrict__ nrm, int nl, int nsp, long ng,
+                                                       ColSel cs, double *__restrict__ part) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long col = sel_col(cs, i, ng);
+    const int mem = blockIdx.y;
+    double s = 0.0;
+    if (col >= 0) {
+        const int nin = 4 * nl + 3, nout = 4 * nl;
+        const float *sy = nrm + 2 * nin + nout;
+        const long base = (long)mem * nsp * ng + col;
+        // slot order in gan: Tv, u, v, q; target order: Tv, q, u, v
+        const int tslot[4] = {0, 2, 3, 1};
+        for (int sl = 0; sl < 4; sl++)
+            for (int c = 1; c <= nl; c++) {
+                const long o = base + ((long)sl * nl + (nl - c)) * ng;
+                const double d = ((double)gan[o] - (double)ganj[o]) / sy[tslot[sl] * nl + c - 1];
+                s += d * d;
+            }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// rms error in units of the targets' standard deviations; sets the fallback
+// flag above the threshold; stats = [last error, fallbacks, judged steps]
+__global__ __launch_bounds__(256) void k_judge_final(const double *__restrict__ part, int npart, double count,
+                                                     double thr, int *__restrict__ flag,
+                                                     double *__restrict__ stats) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < npart; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double err = sqrt(red[0] / count);
+        const int fb = !(err <= thr);  // NaN falls back too
+        flag[0] = fb;
+        stats[0] = err;
+        stats[1] += fb;
+        stats[2] += 1.0;
+    }
+}
+
+// per-feature mean and standard deviation of a row-major [n][nf] array (FP64
+// sums, one block per feature); std floored at 1e-10
+__global__ __launch_bounds__(256) void k_colstats(const float *__restrict__ a, long n, int nf,
+                                                  float *__restrict__ mean, float *__restrict__ sd) {
+    const int f = blockIdx.x;
+    double s = 0.0, s2 = 0.0;
+    for (long r = threadIdx.x; r < n; r += 256) {
+        const double v = a[r * nf + f];
+        s += v;
+        s2 += v * v;
+    }
+    __shared__ double r1[256], r2[256];
+    r1[threadIdx.x] = s;
+    r2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+            r2[threadIdx.x] += r2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = r1[0] / n, var = fmax(r2[0] / n - m * m, 0.0);
+        mean[f] = (float)m;
+        sd[f] = (float)fmax(sqrt(var), 1e-10);
+    }
+}
+
+// a normalised training batch: rows drawn uniformly (counter RNG) from [0, ntrain)
+__global__ void k_mlp_batch(const float *__restrict__ dx, const float *__restrict__ dy, long ntrain, int nin,
+                            int nout, const float *__restrict__ nrm, unsigned long long seed,
+                            unsigned long long ctr, int nb, float *__restrict__ bx, float *__restrict__ by) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nf = nin + nout;
+    if (i >= (long)nb * nf) return;
+    const int r = (int)(i / nf), f = (int)(i % nf);
+    const long src = (long)(rng_mix(seed, ctr, (uint64_t)r) % (uint64_t)ntrain);
+    if (f < nin)
+        bx[(long)r * nin + f] = (dx[src * nin + f] - nrm[f]) / nrm[nin + f];
+    else
+        by[(long)r * nout + f - nin] = (dy[src * nout + f - nin] - nrm[2 * nin + f - nin]) / nrm[2 * nin + nout + f - nin];
+}
+
+// normalised rms error of predictions against targets (rows [0, n)) and of the
+// mean predictor (the normalised targets themselves): partial sums
+__global__ __launch_bounds__(256) void k_eval_partial(const float *__restrict__ yp, const float *__restrict__ dy,
+                                                      long n, int nout, const float *__restrict__ nrm_y,
+                                                      double *__restrict__ part) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0, b = 0.0;
+    if (i < n * nout) {
+        const int f = (int)(i % nout);
+        const double t = (dy[i] - nrm_y[f]) / nrm_y[nout + f];
+        const double d = yp[i] - t;
+        e = d * d;
+        b = t * t;
+    }
+    __shared__ double r1[256], r2[256];
+    r1[threadIdx.x] = e;
+    r2[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+            r2[threadIdx.x] += r2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r1[0];
+        part[2 * blockIdx.x + 1] = r2[0];
+    }
 }
 
 // mass-fixer field of the simple physics: ln ps' + log(pcorr) with
@@ -1390,7 +1645,10 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->ak, B->bk, B->ck, B->dbk, B->sinlat, B->coslat, B->areaw, B->kbydeg, B->degree,
                     B->pdryini, B->pmean, B->pspart, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->rflag_syn,
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
-                    B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr};
+                    B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
+                    B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
+                    B->jflag, B->jctr};
+    if (B->mlp) mlp_destroy(B->mlp);
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (B->Work) {
@@ -1484,7 +1742,10 @@ template <typename T> struct Step {
         synth_all();
         columns(1);
         if (c.physics == GFSDC_PHYS_SIMPLE) {
-            simple_columns();
+            if (B.mlp_on)
+                mlp_columns();
+            else
+                simple_columns();
             if (c.massfix) {
                 area_sums(B.pwat, B.pmean, B.pwsum, false);
                 dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
@@ -1508,7 +1769,7 @@ template <typename T> struct Step {
             phys_update();
         }
     }
-    void simple_columns() {
+    SimpleCfg simple_cfg() const {
         const gfsdc_cfg &c = E.cfg;
         SimpleCfg sc;
         sc.dt = c.dt;
@@ -1517,12 +1778,49 @@ template <typename T> struct Step {
         sc.grav = c.grav;
         sc.sst_mode = c.simple_test;
         sc.turbflux = c.turbflux;
-        dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
-        hipLaunchKernelGGL(k_simple_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg, (T *)B.gan,
-                           (T *)B.pblscr, (T *)B.pwat, (T *)B.precip, (const T *)B.ak, (const T *)B.bk,
+        return sc;
+    }
+    // plain simple physics on all columns (or the selection cs) into (gan, pwat, precip)
+    void simple_columns(ColSel cs = ColSel{0, nullptr, nullptr}, void *gan = nullptr, void *pwat = nullptr,
+                        void *precip = nullptr, const char *name = "simple_columns") {
+        const gfsdc_cfg &c = E.cfg;
+        long nthreads = cs.stride > 0 ? B.ng / cs.stride : B.ng;
+        dim3 g((unsigned)((nthreads + 255) / 256), B.nm);
+        hipLaunchKernelGGL(k_simple_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg,
+                           (T *)(gan ? gan : B.gan), (T *)B.pblscr, (T *)(pwat ? pwat : B.pwat),
+                           (T *)(precip ? precip : B.precip), (const T *)B.ak, (const T *)B.bk,
                            (const double *)B.latd, B.nl, B.ntr, B.nsyn, B.nsp, B.ng, E.plan.nlon, (T)(c.rd / c.cp),
-                           sc);
-        mark("simple_columns");
+                           simple_cfg(), cs);
+        mark(name);
+    }
+    // learned column physics with the plain physics as judge on every
+    // judge_stride-th column and as the fallback for the whole step
+    void mlp_columns() {
+        const gfsdc_cfg &c = E.cfg;
+        const long rows = (long)B.nm * B.ng;
+        dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+        const int nin = 4 * B.nl + 3, nout = 4 * B.nl;
+        hipLaunchKernelGGL(k_mlp_features<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg,
+                           (const double *)B.latd, B.nl, B.nsyn, B.ng, E.plan.nlon, simple_cfg(),
+                           ColSel{0, nullptr, nullptr}, B.ng, (const float *)B.nrm, B.xin, 0L);
+        mark("mlp_features");
+        mlp_forward(B.mlp, B.xin, nin, (int)rows, B.yout, nout, s);
+        mark("mlp_forward");
+        hipLaunchKernelGGL(k_mlp_apply<T>, g, dim3(256), 0, s, (const float *)B.yout, (const float *)B.nrm,
+                           (const T *)B.gsyn, (const T *)B.psg, (const T *)B.ak, (const T *)B.bk, (T *)B.gan,
+                           (T *)B.pwat, (T *)B.precip, B.nl, B.nsyn, B.nsp, B.ng, c.grav);
+        mark("mlp_apply");
+        const ColSel js{B.judge_stride, B.jctr, nullptr};
+        simple_columns(js, B.ganj, B.pwatj, B.precipj, "judge_columns");
+        const long nsel = B.ng / B.judge_stride;
+        dim3 gj((unsigned)((nsel + 255) / 256), B.nm);
+        hipLaunchKernelGGL(k_judge_partial<T>, gj, dim3(256), 0, s, (const T *)B.gan, (const T *)B.ganj,
+                           (const float *)B.nrm, B.nl, B.nsp, B.ng, js, B.jpart);
+        hipLaunchKernelGGL(k_judge_final, dim3(1), dim3(256), 0, s, (const double *)B.jpart, (int)(gj.x * gj.y),
+                           (double)nsel * B.nm * nout, B.judge_thr, B.jflag, B.jstat);
+        hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, s, B.jctr);
+        mark("judge");
+        simple_columns(ColSel{0, nullptr, B.jflag}, nullptr, nullptr, nullptr, "fallback_columns");
     }
     void phys_update() {
         const gfsdc_cfg &c = E.cfg;
@@ -1586,6 +1884,293 @@ template <typename T> int Engine<T>::get_pattern(void *out, int dtype, int loc) 
         return GFSDC_ESTATE;
     }
     return output((const T *)B->tpat, out, (size_t)plan.nlm * B->nm * 2, dtype, loc, 1);
+}
+
+// ---- learned column physics: data, training, judge ---------------------------
+
+__global__ void k_norm_rows(const float *__restrict__ src, long n, int nf, const float *__restrict__ mean,
+                            const float *__restrict__ sd, float *__restrict__ dst) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * nf) return;
+    const int f = (int)(i % nf);
+    dst[i] = (src[i] - mean[f]) / sd[f];
+}
+
+template <typename T> static int mlp_need(DycoreBuffers *B, bool norm) {
+    if (!B || !B->mlp) {
+        set_error("no column MLP: call gfsdc_mlp_init first");
+        return GFSDC_ESTATE;
+    }
+    if (norm && !B->nrm_ready) {
+        set_error("column MLP has no normalisation: collect data and call gfsdc_mlp_normalize first");
+        return GFSDC_ESTATE;
+    }
+    return 0;
+}
+
+template <typename T>
+int Engine<T>::mlp_init(int hidden, int nlayers, int maxbatch, long capacity, unsigned long long seed) {
+    DycoreBuffers *B = dyn;
+    if (!B || cfg.physics != GFSDC_PHYS_SIMPLE) {
+        set_error("the column MLP needs a dycore context with physics = GFSDC_PHYS_SIMPLE (its teacher and judge)");
+        return GFSDC_ESTATE;
+    }
+    if (B->mlp) {
+        set_error("column MLP already initialised");
+        return GFSDC_ESTATE;
+    }
+    if (capacity < 1 || maxbatch < 1) {
+        set_error("capacity and maxbatch must be >= 1");
+        return GFSDC_EINVAL;
+    }
+    const int nl = B->nl, nin = 4 * nl + 3, nout = 4 * nl, nm = B->nm;
+    const long ng = B->ng;
+    int rc = mlp_create(&B->mlp, nin, nout, hidden, nlayers, maxbatch, seed, stream);
+    if (rc) return rc;
+    B->ds_cap = capacity;
+    B->ds_n = 0;
+    B->ds_x = dalloc<float>((size_t)capacity * nin);
+    B->ds_y = dalloc<float>((size_t)capacity * nout);
+    B->nrm = dalloc<float>(2 * (size_t)(nin + nout));
+    B->xin = dalloc<float>((size_t)nm * ng * nin);
+    B->yout = dalloc<float>((size_t)nm * ng * nout);
+    B->tbx = dalloc<float>((size_t)B->mlp->Bp * nin);
+    B->tby = dalloc<float>((size_t)B->mlp->Bp * nout);
+    B->ganj = dalloc<T>((size_t)nm * B->nsp * ng);
+    B->pwatj = dalloc<T>((size_t)nm * ng);
+    B->precipj = dalloc<T>((size_t)nm * ng);
+    B->jpart = dalloc<double>((size_t)nm * (ng / 256 + 2));
+    B->jstat = dalloc<double>(3);
+    B->jflag = dalloc<int>(1);
+    B->jctr = dalloc<unsigned long long>(1);
+    void *ptrs[] = {B->ds_x, B->ds_y, B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj,
+                    B->precipj, B->jpart, B->jstat, B->jflag, B->jctr};
+    for (void *q : ptrs)
+        if (!q) {
+            set_error("device allocation failed for the column MLP");
+            return GFSDC_ENOMEM;
+        }
+    GFSDC_CHECK(hipMemsetAsync(B->jstat, 0, sizeof(double) * 3, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->jflag, 0, sizeof(int), stream));
+    GFSDC_CHECK(hipMemsetAsync(B->jctr, 0, sizeof(unsigned long long), stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> int Engine<T>::mlp_collect(int nsteps, int stride, long *ncollected) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    if (stride < 1) {
+        set_error("stride must be >= 1");
+        return GFSDC_EINVAL;
+    }
+    const bool was_on = B->mlp_on;
+    B->mlp_on = false;  // the plain physics produces the training data
+    Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    const long nsel = B->ng / stride, per = nsel * B->nm;
+    for (int i = 0; i < nsteps && B->ds_n + per <= B->ds_cap; i++) {
+        st.step();
+        const ColSel cs{stride, B->jctr, nullptr};
+        dim3 g((unsigned)((nsel + 255) / 256), B->nm);
+        hipLaunchKernelGGL(k_mlp_features<T>, g, dim3(256), 0, stream, (const T *)B->gsyn, (const T *)B->psg,
+                           (const double *)B->latd, B->nl, B->nsyn, B->ng, plan.nlon, st.simple_cfg(), cs, nsel,
+                           (const float *)nullptr, B->ds_x, B->ds_n);
+        hipLaunchKernelGGL(k_mlp_targets<T>, g, dim3(256), 0, stream, (const T *)B->gan, B->nl, B->nsp, B->ng, cs,
+                           nsel, B->ds_y, B->ds_n);
+        hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, stream, B->jctr);
+        GFSDC_CHECK(hipGetLastError());
+        B->ds_n += per;
+    }
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    B->mlp_on = was_on;
+    if (ncollected) *ncollected = B->ds_n;
+    return 0;
+}
+
+template <typename T> int Engine<T>::mlp_normalize() {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    if (B->ds_n < 2) {
+        set_error("collect at least two columns before gfsdc_mlp_normalize");
+        return GFSDC_ESTATE;
+    }
+    const int nin = B->mlp->nin, nout = B->mlp->nout;
+    hipLaunchKernelGGL(k_colstats, dim3(nin), dim3(256), 0, stream, (const float *)B->ds_x, B->ds_n, nin, B->nrm,
+                       B->nrm + nin);
+    hipLaunchKernelGGL(k_colstats, dim3(nout), dim3(256), 0, stream, (const float *)B->ds_y, B->ds_n, nout,
+                       B->nrm + 2 * nin, B->nrm + 2 * nin + nout);
+    GFSDC_CHECK(hipGetLastError());
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    B->nrm_ready = true;
+    return 0;
+}
+
+template <typename T>
+int Engine<T>::mlp_train(int nsteps, int batch, double lr, double wd, long ntrain, double *losses) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, true);
+    if (rc) return rc;
+    if (ntrain <= 0 || ntrain > B->ds_n) ntrain = B->ds_n;
+    if (batch < 1 || batch > B->mlp->Bp) {
+        set_error("batch must be in [1, maxbatch]");
+        return GFSDC_EINVAL;
+    }
+    const int nin = B->mlp->nin, nout = B->mlp->nout;
+    double *dl = dalloc<double>(nsteps > 0 ? nsteps : 1);
+    if (!dl) {
+        set_error("device allocation failed");
+        return GFSDC_ENOMEM;
+    }
+    const long nb = (long)batch * (nin + nout);
+    for (int it = 0; it < nsteps; it++) {
+        hipLaunchKernelGGL(k_mlp_batch, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream,
+                           (const float *)B->ds_x, (const float *)B->ds_y, ntrain, nin, nout, (const float *)B->nrm,
+                           (unsigned long long)B->mlp->seed ^ 0xBA7C4ULL, B->bctr++, batch, B->tbx, B->tby);
+        if ((rc = mlp_backward(B->mlp, B->tbx, nin, B->tby, nout, batch, stream))) break;
+        if ((rc = mlp_adamw(B->mlp, lr, wd, nullptr, stream))) break;
+        GFSDC_CHECK(hipMemcpyAsync(dl + it, mlp_loss_dev(B->mlp), sizeof(double), hipMemcpyDeviceToDevice, stream));
+    }
+    if (!rc && losses && nsteps > 0)
+        GFSDC_CHECK(hipMemcpyAsync(losses, dl, sizeof(double) * nsteps, hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    (void)hipFree(dl);
+    return rc;
+}
+
+template <typename T> int Engine<T>::mlp_eval(long first, long count, double *err, double *base) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, true);
+    if (rc) return rc;
+    if (first < 0 || count < 1 || first + count > B->ds_n) {
+        set_error("eval rows outside the collected data");
+        return GFSDC_EINVAL;
+    }
+    const int nin = B->mlp->nin, nout = B->mlp->nout;
+    const long cap = (long)B->nm * B->ng;  // rows of the inference buffers
+    double se = 0.0, sb = 0.0;
+    std::vector<double> part;
+    for (long r0 = first; r0 < first + count; r0 += cap) {
+        const long n = std::min(cap, first + count - r0);
+        hipLaunchKernelGGL(k_norm_rows, dim3((unsigned)((n * nin + 255) / 256)), dim3(256), 0, stream,
+                           (const float *)B->ds_x + r0 * nin, n, nin, (const float *)B->nrm,
+                           (const float *)B->nrm + nin, B->xin);
+        if ((rc = mlp_forward(B->mlp, B->xin, nin, (int)n, B->yout, nout, stream))) return rc;
+        const long nblk = (n * nout + 255) / 256;
+        double *dp = (double *)B->jpart;  // reuse when large enough, else a temporary
+        bool tmp = (size_t)(2 * nblk) > (size_t)B->nm * (B->ng / 256 + 2);
+        if (tmp) dp = dalloc<double>(2 * nblk);
+        if (!dp) {
+            set_error("device allocation failed");
+            return GFSDC_ENOMEM;
+        }
+        hipLaunchKernelGGL(k_eval_partial, dim3((unsigned)nblk), dim3(256), 0, stream, (const float *)B->yout,
+                           (const float *)B->ds_y + r0 * nout, n, nout, (const float *)B->nrm + 2 * nin, dp);
+        part.resize(2 * nblk);
+        GFSDC_CHECK(hipMemcpyAsync(part.data(), dp, sizeof(double) * 2 * nblk, hipMemcpyDeviceToHost, stream));
+        GFSDC_CHECK(hipStreamSynchronize(stream));
+        if (tmp) (void)hipFree(dp);
+        for (long i = 0; i < nblk; i++) {
+            se += part[2 * i];
+            sb += part[2 * i + 1];
+        }
+    }
+    if (err) *err = std::sqrt(se / ((double)count * nout));
+    if (base) *base = std::sqrt(sb / ((double)count * nout));
+    return 0;
+}
+
+template <typename T> int Engine<T>::mlp_enable(int on, int judge_stride, double threshold) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, on != 0);
+    if (rc) return rc;
+    if (judge_stride < 1 || judge_stride > B->ng) {
+        set_error("judge_stride must be in [1, nlon*nlat]");
+        return GFSDC_EINVAL;
+    }
+    B->mlp_on = on != 0;
+    B->judge_stride = judge_stride;
+    B->judge_thr = threshold;
+    GFSDC_CHECK(hipMemsetAsync(B->jstat, 0, sizeof(double) * 3, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    // the captured step changes: capture again on the next graph advance
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->gexec = nullptr;
+    B->graph = nullptr;
+    return 0;
+}
+
+template <typename T> int Engine<T>::mlp_judge_stats(double *out) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    GFSDC_CHECK(hipMemcpyAsync(out, B->jstat, sizeof(double) * 3, hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> long Engine<T>::mlp_nparams() {
+    DycoreBuffers *B = dyn;
+    return (B && B->mlp) ? (long)gfsdc::mlp_nparams(B->mlp) : 0;
+}
+
+template <typename T> int Engine<T>::mlp_params(float *get, const float *set) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    if (set && (rc = mlp_set_params(B->mlp, set, stream))) return rc;
+    if (get && (rc = mlp_get_params(B->mlp, get, stream))) return rc;
+    return 0;
+}
+
+template <typename T> int Engine<T>::mlp_forward_host(const float *x, long n, float *y) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    const int nin = B->mlp->nin, nout = B->mlp->nout;
+    float *dx = dalloc<float>((size_t)n * nin), *dy = dalloc<float>((size_t)n * nout);
+    if (!dx || !dy) {
+        if (dx) (void)hipFree(dx);
+        if (dy) (void)hipFree(dy);
+        set_error("device allocation failed");
+        return GFSDC_ENOMEM;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(dx, x, sizeof(float) * n * nin, hipMemcpyHostToDevice, stream));
+    rc = mlp_forward(B->mlp, dx, nin, (int)n, dy, nout, stream);
+    if (!rc) GFSDC_CHECK(hipMemcpyAsync(y, dy, sizeof(float) * n * nout, hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return rc;
+}
+
+template <typename T> int Engine<T>::mlp_grad_host(const float *x, const float *t, int n, float *grad, double *loss) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    const int nin = B->mlp->nin, nout = B->mlp->nout;
+    if (n < 1 || n > B->mlp->Bp) {
+        set_error("n must be in [1, maxbatch]");
+        return GFSDC_EINVAL;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(B->tbx, x, sizeof(float) * n * nin, hipMemcpyHostToDevice, stream));
+    GFSDC_CHECK(hipMemcpyAsync(B->tby, t, sizeof(float) * n * nout, hipMemcpyHostToDevice, stream));
+    if ((rc = mlp_backward(B->mlp, B->tbx, nin, B->tby, nout, n, stream))) return rc;
+    if (loss) GFSDC_CHECK(hipMemcpyAsync(loss, mlp_loss_dev(B->mlp), sizeof(double), hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    if (grad && (rc = mlp_get_grad(B->mlp, grad, stream))) return rc;
+    return 0;
+}
+
+template <typename T> int Engine<T>::mlp_adamw_host(const float *grad, double lr, double wd) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    rc = mlp_adamw(B->mlp, lr, wd, grad, stream);
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return rc;
 }
 
 template <typename T>

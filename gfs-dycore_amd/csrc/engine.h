@@ -65,6 +65,19 @@ template <typename T> struct Engine {
     int pack_tphys(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
     int pattern_spinup();
     int get_pattern(void *out, int dtype, int loc);
+    // learned column physics
+    int mlp_init(int hidden, int nlayers, int maxbatch, long capacity, unsigned long long seed);
+    int mlp_collect(int nsteps, int stride, long *ncollected);
+    int mlp_normalize();
+    int mlp_train(int nsteps, int batch, double lr, double wd, long ntrain, double *losses);
+    int mlp_eval(long first, long count, double *err, double *base);
+    int mlp_enable(int on, int judge_stride, double threshold);
+    int mlp_judge_stats(double *out);
+    long mlp_nparams();
+    int mlp_params(float *get, const float *set);
+    int mlp_forward_host(const float *x, long n, float *y);
+    int mlp_grad_host(const float *x, const float *t, int n, float *grad, double *loss);
+    int mlp_adamw_host(const float *grad, double lr, double wd);
     int get_grid(int which, void *out, int dtype, int loc);
     int get_semimp(double *amhyb, double *bmhyb, double *svhyb, double *tor_hyb, double *d_hyb_m);
     int get_damp(double *disspec, double *diff_prof, double *dmp_prof);

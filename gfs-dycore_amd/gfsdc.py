@@ -52,7 +52,10 @@ SYMBOLS = [
     "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_grdtospec", "gfsdc_getuv",
     "gfsdc_getvrtdivspec", "gfsdc_getgrad", "gfsdc_set_state", "gfsdc_get_state",
     "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
-    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_get_grid", "gfsdc_get_semimp",
+    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_mlp_init", "gfsdc_mlp_collect",
+    "gfsdc_mlp_normalize", "gfsdc_mlp_train", "gfsdc_mlp_eval", "gfsdc_mlp_enable", "gfsdc_mlp_judge_stats",
+    "gfsdc_mlp_nparams", "gfsdc_mlp_params", "gfsdc_mlp_forward", "gfsdc_mlp_grad", "gfsdc_mlp_adamw",
+    "gfsdc_get_grid", "gfsdc_get_semimp",
     "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_sync",
     "gfsdc_stream", "gfsdc_profile_step",
 ]
@@ -91,6 +94,20 @@ def load():
     L.gfsdc_hs_phytend.argtypes = [vp, vp, vp, vp, i, i]
     L.gfsdc_simple_phytend.argtypes = [vp, vp, vp, vp, vp, i, i]
     L.gfsdc_get_pattern.argtypes = [vp, vp, i, i]
+    lg = C.c_long
+    L.gfsdc_mlp_init.argtypes = [vp, i, i, i, lg, C.c_ulonglong]
+    L.gfsdc_mlp_collect.argtypes = [vp, i, i, C.POINTER(lg)]
+    L.gfsdc_mlp_normalize.argtypes = [vp]
+    L.gfsdc_mlp_train.argtypes = [vp, i, i, d, d, lg, vp]
+    L.gfsdc_mlp_eval.argtypes = [vp, lg, lg, C.POINTER(d), C.POINTER(d)]
+    L.gfsdc_mlp_enable.argtypes = [vp, i, i, d]
+    L.gfsdc_mlp_judge_stats.argtypes = [vp, vp]
+    L.gfsdc_mlp_nparams.argtypes = [vp]
+    L.gfsdc_mlp_nparams.restype = lg
+    L.gfsdc_mlp_params.argtypes = [vp, vp, vp]
+    L.gfsdc_mlp_forward.argtypes = [vp, vp, lg, vp]
+    L.gfsdc_mlp_grad.argtypes = [vp, vp, vp, i, vp, C.POINTER(d)]
+    L.gfsdc_mlp_adamw.argtypes = [vp, vp, d, d]
     L.gfsdc_get_grid.argtypes = [vp, i, vp, i, i]
     L.gfsdc_get_semimp.argtypes = [vp, vp, vp, vp, vp, vp]
     L.gfsdc_get_damp.argtypes = [vp, vp, vp, vp]
@@ -281,6 +298,62 @@ class Context:
         out = np.zeros((self.nmem, self.nlm), np.complex128)
         _check(load().gfsdc_get_pattern(self.h, _p(out), DT_F64, HOST))
         return out
+
+    # -- learned column physics ------------------------------------------------
+    def mlp_init(self, hidden=512, nlayers=3, maxbatch=4096, capacity=1 << 20, seed=1):
+        _check(load().gfsdc_mlp_init(self.h, hidden, nlayers, maxbatch, capacity, seed))
+        self.mlp_nin, self.mlp_nout = 4 * self.nlev + 3, 4 * self.nlev
+
+    def mlp_collect(self, nsteps, stride):
+        n = C.c_long(0)
+        _check(load().gfsdc_mlp_collect(self.h, nsteps, stride, C.byref(n)))
+        return n.value
+
+    def mlp_normalize(self):
+        _check(load().gfsdc_mlp_normalize(self.h))
+
+    def mlp_train(self, nsteps, batch, lr, wd=0.0, ntrain=0):
+        losses = np.zeros(max(nsteps, 1))
+        _check(load().gfsdc_mlp_train(self.h, nsteps, batch, lr, wd, ntrain, _p(losses)))
+        return losses[:nsteps]
+
+    def mlp_eval(self, first, count):
+        e, b = C.c_double(0), C.c_double(0)
+        _check(load().gfsdc_mlp_eval(self.h, first, count, C.byref(e), C.byref(b)))
+        return e.value, b.value
+
+    def mlp_enable(self, on=True, judge_stride=50, threshold=0.5):
+        _check(load().gfsdc_mlp_enable(self.h, int(on), judge_stride, threshold))
+
+    def mlp_judge_stats(self):
+        out = np.zeros(3)
+        _check(load().gfsdc_mlp_judge_stats(self.h, _p(out)))
+        return dict(error=out[0], fallbacks=int(out[1]), judged=int(out[2]))
+
+    def mlp_params(self, set=None):
+        n = load().gfsdc_mlp_nparams(self.h)
+        out = np.zeros(n, np.float32)
+        src = None if set is None else np.ascontiguousarray(set, np.float32)
+        _check(load().gfsdc_mlp_params(self.h, _p(out), _p(src) if src is not None else None))
+        return out
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.mlp_nout), np.float32)
+        _check(load().gfsdc_mlp_forward(self.h, _p(x), x.shape[0], _p(y)))
+        return y
+
+    def mlp_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(load().gfsdc_mlp_nparams(self.h), np.float32)
+        loss = C.c_double(0)
+        _check(load().gfsdc_mlp_grad(self.h, _p(x), _p(t), x.shape[0], _p(g), C.byref(loss)))
+        return g, loss.value
+
+    def mlp_adamw(self, grad, lr, wd=0.0):
+        g = None if grad is None else np.ascontiguousarray(grad, np.float32)
+        _check(load().gfsdc_mlp_adamw(self.h, _p(g) if g is not None else None, lr, wd))
 
     def grid(self, name):
         nk = {"etadot": self.nlev + 1, "psg": 1, "dlnpsdt": 1, "dlnpsdx": 1, "dlnpsdy": 1,

@@ -172,6 +172,45 @@ int gfsdc_simple_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, void
                          int loc);
 /* current AR(1) pattern, (ndimspec, nmem) complex of dtype (error if tpat_amp == 0) */
 int gfsdc_get_pattern(gfsdc_ctx *ctx, void *spec, int dtype, int loc);
+/* ---- learned column physics (no counterpart in the reference) --------------
+ * A column MLP stands in for the simple-physics column call of
+ * phy_run_simple.f90:66-67: inputs T, q, u, v (nlev each, top -> bottom), ps,
+ * sea-surface temperature, sin(lat); outputs the step's changes of Tv, q, u,
+ * v; GELU hidden layers; bf16 weights on the matrix cores with FP32
+ * accumulation, FP32 master weights, AdamW. The plain physics of the context
+ * (physics = GFSDC_PHYS_SIMPLE) generates the training data, judges every
+ * judge_stride-th column each step and replaces the whole step's physics when
+ * the judged rms error (in units of each output's standard deviation)
+ * exceeds the threshold. */
+int gfsdc_mlp_init(gfsdc_ctx *ctx, int hidden, int nlayers, int maxbatch, long capacity,
+                   unsigned long long seed);
+/* advance nsteps with the plain physics, keeping every stride-th column of
+ * each step (random offset) as (features, targets); returns the total count */
+int gfsdc_mlp_collect(gfsdc_ctx *ctx, int nsteps, int stride, long *ncollected);
+/* per-feature mean / standard deviation of the collected data */
+int gfsdc_mlp_normalize(gfsdc_ctx *ctx);
+/* nsteps AdamW steps on random batches from the first ntrain collected rows
+ * (0: all); losses (nsteps, may be NULL) = batch mean squared normalised error */
+int gfsdc_mlp_train(gfsdc_ctx *ctx, int nsteps, int batch, double lr, double wd, long ntrain,
+                    double *losses);
+/* rms normalised error of the network on collected rows [first, first+count)
+ * and of the mean predictor (no physics change beyond the mean) on them */
+int gfsdc_mlp_eval(gfsdc_ctx *ctx, long first, long count, double *err, double *base);
+/* use the network in advance (on != 0) with its judge and fallback */
+int gfsdc_mlp_enable(gfsdc_ctx *ctx, int on, int judge_stride, double threshold);
+/* judge statistics since enable: last error, fallback steps, judged steps */
+int gfsdc_mlp_judge_stats(gfsdc_ctx *ctx, double *out3);
+/* parameters in the layout W_1[nin][hidden], b_1[hidden], ..., W_L[hidden][nout], b_L[nout] */
+long gfsdc_mlp_nparams(gfsdc_ctx *ctx);
+int gfsdc_mlp_params(gfsdc_ctx *ctx, float *get, const float *set);
+/* network on n normalised host rows x[n][nin] -> y[n][nout] */
+int gfsdc_mlp_forward(gfsdc_ctx *ctx, const float *x, long n, float *y);
+/* mean-squared-error loss and its parameter gradient on n <= maxbatch normalised host rows */
+int gfsdc_mlp_grad(gfsdc_ctx *ctx, const float *x, const float *t, int n, float *grad, double *loss);
+/* one AdamW step (beta 0.9/0.999, eps 1e-8, decay on weights only) with a host
+ * gradient in the parameter layout, or the last gfsdc_mlp_grad when grad is NULL */
+int gfsdc_mlp_adamw(gfsdc_ctx *ctx, const float *grad, double lr, double wd);
+
 /* grid fields after the last getdyntend, names of grid_data.f90/pressure_data.f90:
  * 0 ug,1 vg,2 vrtg,3 divg,4 virtempg,5 dlnpdtg,6 etadot (nlev+1),7 psg,8 dlnpsdt,
  * 9 prs,14 dlnpsdx,15 dlnpsdy, 16 tracerg (nlev*ntrac), 17 precip (mm/s),

@@ -156,6 +156,13 @@ void orc_get_pattern(const orc_model *m, ocplx *pattern, double *varspectrum, do
 /* one patterngenerator_advance of the model's pattern */
 void orc_pattern_advance(orc_model *m);
 
+/* column MLP of the learned physics (FP64): parameters per layer W[K][N], b[N] */
+long orc_mlp_nparams(int nin, int nout, int hidden, int nlayers);
+double orc_mlp_loss_grad(int nin, int nout, int hidden, int nlayers, const double *params, int n,
+                         const double *x, const double *t, double *grad, double *y);
+void orc_mlp_adamw(int nin, int nout, int hidden, int nlayers, double *params, double *m, double *v,
+                   const double *grad, int step, double lr, double wd);
+
 /* counter-based RNG shared with the product: splitmix64 of (seed, stream, index) */
 uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index);
 double orc_rng_uniform(uint64_t seed, uint64_t stream, uint64_t index);

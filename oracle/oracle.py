@@ -89,6 +89,11 @@ def lib():
         L.orc_calc_pressdata.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp,
                                          _dp, _dp, _dp]
         L.orc_get_pattern.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_mlp_nparams.restype = C.c_long
+        L.orc_mlp_nparams.argtypes = [C.c_int] * 4
+        L.orc_mlp_loss_grad.restype = C.c_double
+        L.orc_mlp_loss_grad.argtypes = [C.c_int] * 4 + [_dp, C.c_int, _dp, _dp, vp, vp]
+        L.orc_mlp_adamw.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double]
         L.orc_pattern_advance.argtypes = [vp]
         L.orc_vadv_tracers_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
         L.orc_rng_u64.restype = C.c_uint64
@@ -329,6 +334,29 @@ def vadv_tracers(datag, etadot, dpk):
     out = np.zeros_like(datag)
     lib().orc_vadv_tracers_cols(nl, n, datag, etadot, dpk, out)
     return out
+
+
+class Mlp:
+    """FP64 column MLP: nin -> hidden x nlayers (GELU) -> nout; flat
+    parameters per layer W[K][N] then b[N]."""
+
+    def __init__(self, nin, nout, hidden, nlayers):
+        self.dims = (nin, nout, hidden, nlayers)
+        self.nparams = lib().orc_mlp_nparams(*self.dims)
+
+    def loss_grad(self, params, x, t, grad=True):
+        params = np.ascontiguousarray(params, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        t = np.ascontiguousarray(t, np.float64)
+        g = np.zeros(self.nparams) if grad else None
+        y = np.zeros((x.shape[0], self.dims[1]))
+        loss = lib().orc_mlp_loss_grad(*self.dims, params, x.shape[0], x, t, _ptr(g), _ptr(y))
+        return loss, g, y
+
+    def adamw(self, params, m, v, grad, step, lr, wd):
+        for a in (params, m, v):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        lib().orc_mlp_adamw(*self.dims, params, m, v, np.ascontiguousarray(grad, np.float64), step, lr, wd)
 
 
 def simple_column(dtime, lat, t, q, u, v, pmid, pint, pdel, rpdel, ps, sst_mode=0, turbflux=True):
