@@ -60,14 +60,14 @@ int gfsdc_create(const gfsdc_cfg *cfg, gfsdc_ctx **out) {
     gfsdc_ctx *c = new gfsdc_ctx();
     c->precision = cfg->precision;
     int rc;
-    if (cfg->precision == GFSDC_FP32) {
+    if (cfg->precision == GFSDC_FP32 || cfg->precision == GFSDC_BF16X3) {
         c->ef = new gfsdc::Engine<float>();
         rc = c->ef->init(*cfg);
     } else if (cfg->precision == GFSDC_FP64) {
         c->ed = new gfsdc::Engine<double>();
         rc = c->ed->init(*cfg);
     } else {
-        set_error("precision must be GFSDC_FP64 or GFSDC_FP32");
+        set_error("precision must be GFSDC_FP64, GFSDC_FP32 or GFSDC_BF16X3");
         rc = GFSDC_EINVAL;
     }
     if (rc) {

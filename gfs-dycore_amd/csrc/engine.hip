@@ -43,6 +43,7 @@ template <typename T> int Engine<T>::init(const gfsdc_cfg &c) {
     }
     GFSDC_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     int rc = plan_create(plan, c.nlon, c.nlat, c.ntrunc, stream);
+    plan.leg_bf16x3 = (c.precision == GFSDC_BF16X3) ? 1 : 0;
     if (rc) return rc;
     if (c.nlev > 0) {
         rc = dyn_init();

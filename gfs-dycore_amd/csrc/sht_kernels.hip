@@ -165,6 +165,15 @@ template <typename T>
 __device__ inline void vstore_lds(T *dst, const typename Vec<T>::t &v) {
     *(typename Vec<T>::t *)dst = v;
 }
+// vector component q (a constant after unrolling) without taking the vector's address
+__device__ inline float vget(const float4 &v, int q) { return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w; }
+__device__ inline double vget(const double2 &v, int q) { return q == 0 ? v.x : v.y; }
+// x, or zero when keep is false, selected component by component (a select of
+// whole vector structs is lowered to a select of addresses through scratch)
+__device__ inline float4 vkeep(const float4 &x, bool keep) {
+    return float4{keep ? x.x : 0.f, keep ? x.y : 0.f, keep ? x.z : 0.f, keep ? x.w : 0.f};
+}
+__device__ inline double2 vkeep(const double2 &x, bool keep) { return double2{keep ? x.x : 0.0, keep ? x.y : 0.0}; }
 
 template <typename T, typename Acc>
 __device__ inline void gemm_chunk(T (*As)[GEMM_BK][GEMM_BM + LDS_PAD], T (*Bs)[GEMM_BK][GT<T>::BN + LDS_PAD],
@@ -211,8 +220,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     const int mt = blockIdx.y;
     const int m = mt / ntj, tj = mt - m * ntj;
     const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
-    const T *P[2] = {plm + poff[m], plm + poff[m] + (long)Ke * nhp};
-    const int Kp[2] = {Ke, Ko};
+    const T *P0 = plm + poff[m], *P1 = P0 + (long)Ke * nhp;
     const T *S = sx + (long)keoff[m] * ldx;
     const int j0 = tj * GEMM_BM, c0 = tc * BN;
     __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
@@ -235,12 +243,15 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
 #pragma unroll
             for (int u = 0; u < NA; u++) {
                 int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW, i = k0 + kk;
-                ra[par][u] = (i < Kp[par]) ? *(const V *)(P[par] + (long)i * nhp + j0 + jj) : zero;
+                // clamped in-bounds load, zeroed by value (no select between pointers)
+                V x = *(const V *)((par ? P1 : P0) + (long)min(i, (par ? Ko : Ke) - 1) * nhp + j0 + jj);
+                ra[par][u] = vkeep(x, i < (par ? Ko : Ke));
             }
 #pragma unroll
             for (int u = 0; u < NB; u++) {
                 int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW, i = k0 + kk;
-                rb[par][u] = (i < Kp[par]) ? *(const V *)(S + (long)(2 * i + par) * ldx + c0 + cc) : zero;
+                V x = *(const V *)(S + (long)(2 * min(i, (par ? Ko : Ke) - 1) + par) * ldx + c0 + cc);
+                rb[par][u] = vkeep(x, i < (par ? Ko : Ke));
             }
         }
     };
@@ -276,20 +287,14 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
         for (int bj = 0; bj < WB; bj++) {
-            if (Vec<T>::swap) {
+            if constexpr (Vec<T>::swap) {
                 int j = j0 + wm + bi * 16 + (lane & 15);
                 int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
                 if (j < nhalf) {
                     acc_t e = E[bi][bj], o = O[bi][bj];
-                    V vn, vs;
-                    T *pn = (T *)&vn, *ps_ = (T *)&vs;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        pn[r] = e[r] + o[r];
-                        ps_[r] = e[r] - o[r];
-                    }
-                    *(V *)(FN + (long)j * ldf + c) = vn;
-                    *(V *)(FN + (long)(nlat - 1 - j) * ldf + c) = vs;
+                    *(float4 *)(FN + (long)j * ldf + c) = float4{e[0] + o[0], e[1] + o[1], e[2] + o[2], e[3] + o[3]};
+                    *(float4 *)(FN + (long)(nlat - 1 - j) * ldf + c) =
+                        float4{e[0] - o[0], e[1] - o[1], e[2] - o[2], e[3] - o[3]};
                 }
             } else {
 #pragma unroll
@@ -321,9 +326,8 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
     const int tc = blockIdx.x;
     const int m = mtile_m[blockIdx.y], i0 = mtile_i[blockIdx.y];
     const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
-    const T *P[2] = {plm + poff[m], plm + poff[m] + (long)Ke * nhp};
-    const int Kp[2] = {Ke, Ko};
-    const T *F[2] = {fa + (long)(2 * m) * nhp * ldf, fa + (long)(2 * m + 1) * nhp * ldf};
+    const T *P0 = plm + poff[m], *P1 = P0 + (long)Ke * nhp;
+    const T *F0 = fa + (long)(2 * m) * nhp * ldf, *F1 = F0 + (long)nhp * ldf;
     const int c0 = tc * BN;
     __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
     __shared__ T Bs[2][GEMM_BK][BN + LDS_PAD];
@@ -345,12 +349,13 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
 #pragma unroll
             for (int u = 0; u < NA; u++) {
                 int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW, i = i0 + ii;
-                ra[par][u] = (i < Kp[par]) ? *(const V *)(P[par] + (long)i * nhp + j0 + kk) : zero;
+                V x = *(const V *)((par ? P1 : P0) + (long)min(i, (par ? Ko : Ke) - 1) * nhp + j0 + kk);
+                ra[par][u] = vkeep(x, i < (par ? Ko : Ke));
             }
 #pragma unroll
             for (int u = 0; u < NB; u++) {
                 int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
-                rb[par][u] = *(const V *)(F[par] + (long)(j0 + kk) * ldf + c0 + cc);
+                rb[par][u] = vkeep(*(const V *)((par ? F1 : F0) + (long)(j0 + kk) * ldf + c0 + cc), true);
             }
         }
     };
@@ -360,9 +365,8 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
 #pragma unroll
             for (int u = 0; u < NA; u++) {
                 int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW;
-                const T *pv = (const T *)&ra[par][u];
 #pragma unroll
-                for (int q = 0; q < VW; q++) As[par][kk + q][ii] = pv[q];
+                for (int q = 0; q < VW; q++) As[par][kk + q][ii] = vget(ra[par][u], q);
             }
 #pragma unroll
             for (int u = 0; u < NB; u++) {
@@ -388,19 +392,12 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
         for (int bj = 0; bj < WB; bj++) {
-            if (Vec<T>::swap) {
+            if constexpr (Vec<T>::swap) {
                 int i = i0 + wm + bi * 16 + (lane & 15);
                 int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
                 acc_t e = E[bi][bj], o = O[bi][bj];
-                V ve, vo;
-                T *pe = (T *)&ve, *po = (T *)&vo;
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    pe[r] = e[r];
-                    po[r] = o[r];
-                }
-                if (i < Ke) *(V *)(X + (long)(2 * i) * ldx + c) = ve;
-                if (i < Ko) *(V *)(X + (long)(2 * i + 1) * ldx + c) = vo;
+                if (i < Ke) *(float4 *)(X + (long)(2 * i) * ldx + c) = float4{e[0], e[1], e[2], e[3]};
+                if (i < Ko) *(float4 *)(X + (long)(2 * i + 1) * ldx + c) = float4{o[0], o[1], o[2], o[3]};
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
@@ -410,6 +407,180 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
                     if (i < Ko) X[(long)(2 * i + 1) * ldx + c] = O[bi][bj][r];
                 }
             }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 Legendre GEMMs (FP32 plan option): each FP32 operand x is split into
+// bf16 hi = rn(x) and lo = rn(x - hi); a product is hi.hi + hi.lo + lo.hi
+// (the lo.lo term, ~2^-16 relative, is dropped) on the bf16 matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation). Same tiling as the FP32
+// kernels (64 x 128 per workgroup, both parities); operands are staged in LDS
+// k-contiguous ([row][k], 8 consecutive k per lane and MFMA) and the MFMA
+// operands are swapped so a lane's four accumulators are four consecutive
+// output columns (16-byte stores).
+typedef unsigned short b3u8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b3bf8 __attribute__((ext_vector_type(8)));
+typedef float b3f4 __attribute__((ext_vector_type(4)));
+constexpr int B3K = 32, B3P = B3K + 8;
+
+__device__ inline unsigned short b3_rn(float f) {
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline void b3_split(float x, unsigned short &hi, unsigned short &lo) {
+    hi = b3_rn(x);
+    lo = b3_rn(x - __builtin_bit_cast(float, (unsigned int)hi << 16));
+}
+__device__ inline b3f4 b3_mma3(const unsigned short *bh, const unsigned short *bl, const unsigned short *ah,
+                               const unsigned short *al, b3f4 c) {
+    const b3bf8 Bh = __builtin_bit_cast(b3bf8, *(const b3u8 *)bh), Bl = __builtin_bit_cast(b3bf8, *(const b3u8 *)bl);
+    const b3bf8 Ah = __builtin_bit_cast(b3bf8, *(const b3u8 *)ah), Al = __builtin_bit_cast(b3bf8, *(const b3u8 *)al);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bl, Ah, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bh, Al, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bh, Ah, c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_leg_synth_b3(const float *__restrict__ plm, const long *__restrict__ poff,
+                                                      const int *__restrict__ keoff, const float *__restrict__ sx,
+                                                      int ldx, float *__restrict__ fs, int ldf, int nhalf, int nhp,
+                                                      int nlat, int ntrunc, int ntj) {
+    const int tc = blockIdx.x, mt = blockIdx.y;
+    const int m = mt / ntj, tj = mt - m * ntj;
+    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
+    const float *P0 = plm + poff[m], *P1 = P0 + (long)Ke * nhp;
+    const float *S = sx + (long)keoff[m] * ldx;
+    const int j0 = tj * GEMM_BM, c0 = tc * GEMM_BN;
+    __shared__ __attribute__((aligned(16))) unsigned short Ah[2][GEMM_BM][B3P], Al[2][GEMM_BM][B3P];
+    __shared__ __attribute__((aligned(16))) unsigned short Bh[2][GEMM_BN][B3P], Bl[2][GEMM_BN][B3P];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    b3f4 E[2][4], O[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) E[a][b] = O[a][b] = b3f4{0.f, 0.f, 0.f, 0.f};
+    const float4 zero{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < Ke; k0 += B3K) {
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+            const float *Pp = par ? P1 : P0;
+            const int Kpp = par ? Ko : Ke;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {  // P: 32 degrees x 64 latitudes
+                const int v = tid + 256 * u, kk = v >> 4, jj = (v & 15) * 4, i = k0 + kk;
+                float4 x = *(const float4 *)(Pp + (long)min(i, Kpp - 1) * nhp + j0 + jj);
+                x = vkeep(x, i < Kpp);
+                b3_split(x.x, Ah[par][jj][kk], Al[par][jj][kk]);
+                b3_split(x.y, Ah[par][jj + 1][kk], Al[par][jj + 1][kk]);
+                b3_split(x.z, Ah[par][jj + 2][kk], Al[par][jj + 2][kk]);
+                b3_split(x.w, Ah[par][jj + 3][kk], Al[par][jj + 3][kk]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {  // S: 32 degrees x 128 columns
+                const int v = tid + 256 * u, kk = v >> 5, cc = (v & 31) * 4, i = k0 + kk;
+                float4 x = *(const float4 *)(S + (long)(2 * min(i, Kpp - 1) + par) * ldx + c0 + cc);
+                x = vkeep(x, i < Kpp);
+                b3_split(x.x, Bh[par][cc][kk], Bl[par][cc][kk]);
+                b3_split(x.y, Bh[par][cc + 1][kk], Bl[par][cc + 1][kk]);
+                b3_split(x.z, Bh[par][cc + 2][kk], Bl[par][cc + 2][kk]);
+                b3_split(x.w, Bh[par][cc + 3][kk], Bl[par][cc + 3][kk]);
+            }
+        }
+        __syncthreads();
+        const int r = lane & 15, kq = 8 * (lane >> 4);
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) {
+                const int jr = wm + bi * 16 + r, cr = wn + bj * 16 + r;
+                E[bi][bj] = b3_mma3(&Bh[0][cr][kq], &Bl[0][cr][kq], &Ah[0][jr][kq], &Al[0][jr][kq], E[bi][bj]);
+                O[bi][bj] = b3_mma3(&Bh[1][cr][kq], &Bl[1][cr][kq], &Ah[1][jr][kq], &Al[1][jr][kq], O[bi][bj]);
+            }
+        __syncthreads();
+    }
+    float *FN = fs + (long)m * nlat * ldf;
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+            const int j = j0 + wm + bi * 16 + (lane & 15), c = c0 + wn + bj * 16 + 4 * (lane >> 4);
+            if (j < nhalf) {
+                const b3f4 e = E[bi][bj], o = O[bi][bj];
+                *(float4 *)(FN + (long)j * ldf + c) = float4{e[0] + o[0], e[1] + o[1], e[2] + o[2], e[3] + o[3]};
+                *(float4 *)(FN + (long)(nlat - 1 - j) * ldf + c) =
+                    float4{e[0] - o[0], e[1] - o[1], e[2] - o[2], e[3] - o[3]};
+            }
+        }
+}
+
+__global__ __launch_bounds__(256) void k_leg_anal_b3(const float *__restrict__ plm, const long *__restrict__ poff,
+                                                     const int *__restrict__ keoff, const int *__restrict__ mtile_m,
+                                                     const int *__restrict__ mtile_i, const float *__restrict__ fa,
+                                                     int ldf, float *__restrict__ sx, int ldx, int nhp, int ntrunc) {
+    const int tc = blockIdx.x;
+    const int m = mtile_m[blockIdx.y], i0 = mtile_i[blockIdx.y];
+    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
+    const float *P0 = plm + poff[m], *P1 = P0 + (long)Ke * nhp;
+    const float *F0 = fa + (long)(2 * m) * nhp * ldf, *F1 = F0 + (long)nhp * ldf;
+    const int c0 = tc * GEMM_BN;
+    __shared__ __attribute__((aligned(16))) unsigned short Ah[2][GEMM_BM][B3P], Al[2][GEMM_BM][B3P];
+    __shared__ __attribute__((aligned(16))) unsigned short Bh[2][GEMM_BN][B3P], Bl[2][GEMM_BN][B3P];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    b3f4 E[2][4], O[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) E[a][b] = O[a][b] = b3f4{0.f, 0.f, 0.f, 0.f};
+    const float4 zero{0.f, 0.f, 0.f, 0.f};
+    for (int j0 = 0; j0 < nhp; j0 += B3K) {
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+            const float *Pp = par ? P1 : P0;
+            const int Kpp = par ? Ko : Ke;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {  // P: 64 degrees x 32 latitudes (k-contiguous already)
+                const int v = tid + 256 * u, ii = v >> 3, jj = (v & 7) * 4, i = i0 + ii;
+                float4 x = *(const float4 *)(Pp + (long)min(i, Kpp - 1) * nhp + j0 + jj);
+                x = vkeep(x, i < Kpp);
+                b3_split(x.x, Ah[par][ii][jj], Al[par][ii][jj]);
+                b3_split(x.y, Ah[par][ii][jj + 1], Al[par][ii][jj + 1]);
+                b3_split(x.z, Ah[par][ii][jj + 2], Al[par][ii][jj + 2]);
+                b3_split(x.w, Ah[par][ii][jj + 3], Al[par][ii][jj + 3]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {  // F: 32 latitudes x 128 columns
+                const int v = tid + 256 * u, kk = v >> 5, cc = (v & 31) * 4;
+                const float4 x = *(const float4 *)((par ? F1 : F0) + (long)(j0 + kk) * ldf + c0 + cc);
+                b3_split(x.x, Bh[par][cc][kk], Bl[par][cc][kk]);
+                b3_split(x.y, Bh[par][cc + 1][kk], Bl[par][cc + 1][kk]);
+                b3_split(x.z, Bh[par][cc + 2][kk], Bl[par][cc + 2][kk]);
+                b3_split(x.w, Bh[par][cc + 3][kk], Bl[par][cc + 3][kk]);
+            }
+        }
+        __syncthreads();
+        const int r = lane & 15, kq = 8 * (lane >> 4);
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bj = 0; bj < 4; bj++) {
+                const int ir = wm + bi * 16 + r, cr = wn + bj * 16 + r;
+                E[bi][bj] = b3_mma3(&Bh[0][cr][kq], &Bl[0][cr][kq], &Ah[0][ir][kq], &Al[0][ir][kq], E[bi][bj]);
+                O[bi][bj] = b3_mma3(&Bh[1][cr][kq], &Bl[1][cr][kq], &Ah[1][ir][kq], &Al[1][ir][kq], O[bi][bj]);
+            }
+        __syncthreads();
+    }
+    float *X = sx + (long)keoff[m] * ldx;
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+            const int i = i0 + wm + bi * 16 + (lane & 15), c = c0 + wn + bj * 16 + 4 * (lane >> 4);
+            const b3f4 e = E[bi][bj], o = O[bi][bj];
+            if (i < Ke) *(float4 *)(X + (long)(2 * i) * ldx + c) = float4{e[0], e[1], e[2], e[3]};
+            if (i < Ko) *(float4 *)(X + (long)(2 * i + 1) * ldx + c) = float4{o[0], o[1], o[2], o[3]};
         }
 }
 
@@ -909,6 +1080,13 @@ void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk,
 template <typename T>
 void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s) {
     int ntj = p.nhp / GEMM_BM, ntc = (ncol + GT<T>::BN - 1) / GT<T>::BN;
+    if constexpr (sizeof(T) == 4) {
+        if (p.leg_bf16x3) {
+            hipLaunchKernelGGL(k_leg_synth_b3, dim3((ncol + GEMM_BN - 1) / GEMM_BN, (p.ntrunc + 1) * ntj), dim3(256), 0,
+                               s, p.plm, p.poff, p.keoff, sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc, (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff, p.keoff,
                        sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj);
 }
@@ -916,6 +1094,13 @@ void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, in
 template <typename T>
 void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s) {
     int ntc = (ncol + GT<T>::BN - 1) / GT<T>::BN;
+    if constexpr (sizeof(T) == 4) {
+        if (p.leg_bf16x3) {
+            hipLaunchKernelGGL(k_leg_anal_b3, dim3((ncol + GEMM_BN - 1) / GEMM_BN, p.n_mtiles), dim3(256), 0, s, p.plm,
+                               p.poff, p.keoff, p.mtile_m, p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc, p.n_mtiles), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
                        p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc);
 }

@@ -18,7 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libgfsdc.so")
 
-FP64, FP32 = 0, 1
+FP64, FP32, BF16X3 = 0, 1, 2
 HOST, DEVICE = 0, 1
 DT_F64, DT_F32 = 0, 1
 PHYS_NONE, PHYS_HS, PHYS_SIMPLE = 0, 1, 2

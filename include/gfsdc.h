@@ -75,7 +75,10 @@ enum {
     GFSDC_ENODEV = -5    /* no gfx950 device */
 };
 
-enum { GFSDC_FP64 = 0, GFSDC_FP32 = 1 };             /* compute precision of the context */
+/* compute precision of the context: FP64 throughout; FP32 (FP64 tables and
+ * global sums); BF16X3 = FP32 with the Legendre GEMMs as three bf16 matrix-core
+ * products per FP32 product (split operands hi + lo, lo.lo dropped) */
+enum { GFSDC_FP64 = 0, GFSDC_FP32 = 1, GFSDC_BF16X3 = 2 };
 enum { GFSDC_HOST = 0, GFSDC_DEVICE = 1 };           /* location of a caller buffer */
 enum { GFSDC_DT_F64 = 0, GFSDC_DT_F32 = 1 };         /* element type of a caller buffer */
 enum { GFSDC_PHYS_NONE = 0, GFSDC_PHYS_HS = 1, GFSDC_PHYS_SIMPLE = 2 }; /* physics plug-in (Makefile:50-72) */

@@ -81,3 +81,33 @@ def test_vector_transforms(pair, nf):
     z32, d32 = g32.getvrtdivspec(u0.astype(np.float32), v0.astype(np.float32), A, dtype=1)
     scale = max(np.abs(z0).max(), np.abs(d0).max())
     assert np.abs(z32 - z0).max() / scale < tol32 and np.abs(d32 - d0).max() / scale < tol32
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: f"T{g[2]}")
+def test_bf16x3_transforms(grid):
+    """BF16X3 precision: the Legendre GEMMs as three bf16 matrix-core products
+    per FP32 product (hi.hi + hi.lo + lo.hi, lo.lo dropped, ~2^-16 relative
+    per product). Tolerance (relative max-norm, T42 to T254): 3e-5 for scalar
+    transforms; 1e-3 for getuv, whose extended-degree recurrence form sums
+    terms that largely cancel (measured 2.6e-4 at T254), against 2e-5 for the
+    FP32 path."""
+    import gfsdc
+    nlon, nlat, T = grid
+    o = O.Sht(nlon, nlat, T)
+    g = gfsdc.Context(nlon, nlat, T, gfsdc.BF16X3)
+    q = rand_spec(o, 16, 5)
+    ref = o.spectogrd(q)
+    got = g.spectogrd(q.astype(np.complex64), dtype=1)
+    e_syn = rel(got, ref)
+    back = g.grdtospec(ref.astype(np.float32), dtype=1)
+    e_ana = rel(back, o.grdtospec(ref))
+    vrt, div = rand_spec(o, 4, 6), rand_spec(o, 4, 7) * 0.1
+    vrt[:, 0] = div[:, 0] = 0
+    u, v = g.getuv(vrt.astype(np.complex64), div.astype(np.complex64), A, dtype=1)
+    uo, vo = o.getuv(vrt, div, A)
+    e_uv = max(rel(u, uo), rel(v, vo))
+    print(f"T{T} bf16x3 relative errors: synthesis {e_syn:.2e} analysis {e_ana:.2e} getuv {e_uv:.2e}")
+    assert e_syn < 3e-5 and e_ana < 3e-5 and e_uv < 1e-3
+    # and it is a genuinely different arithmetic path from FP32 (errors well above FP32 rounding)
+    g32 = gfsdc.Context(nlon, nlat, T, gfsdc.FP32)
+    assert rel(g32.spectogrd(q.astype(np.complex64), dtype=1), ref) < e_syn
