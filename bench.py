@@ -1,12 +1,12 @@
 """Benchmark: simulated days per wall-day of the GFS spectral dycore step.
 
-Workload (default T254L64): 768x384 Gauss grid, triangular truncation 254,
-64 hybrid levels, dry Held-Suarez forcing, IMEX RK3 with dt = 600 s, from a
-resting isothermal atmosphere (300 K, 1000 hPa) plus seeded random vorticity
-(rms 1e-6 s^-1 in total wavenumbers 1-21). One "step" is one call of
-run.f90 advance: three getdyntend passes with semi-implicit solves, implicit
-damping, the inverse-transform pass, Held-Suarez physics and the dry-mass
-fixer, all on the GPU, replayed from a HIP graph.
+Default workload (the headline line) T254L64: 768x384 Gauss grid, triangular
+truncation 254, 64 hybrid levels, dry Held-Suarez forcing, IMEX RK3 with
+dt = 600 s, from a resting isothermal atmosphere (300 K, 1000 hPa) plus seeded
+random vorticity (rms 1e-6 s^-1 in total wavenumbers 1-21). One "step" is one
+call of run.f90 advance: three getdyntend passes with semi-implicit solves,
+implicit damping, the inverse-transform pass, Held-Suarez physics and the
+dry-mass fixer, all on the GPU, replayed from a HIP graph.
 
 value = mixed-precision (FP32 Legendre/FFT/columns, FP64 tables and global
 sums) simulated days per wall-day; the FP64 GPU path is timed beside it and
@@ -14,6 +14,13 @@ the spectral error of the mixed path after --error-steps steps (default one
 model day) against the FP64 path is reported. roofline = the dominant kernel
 of the step, timed with HIP events on the context stream; cpu_baseline = the
 FP64 CPU oracle timed on one step of the same workload on the host cores.
+
+Other workloads (each prints its own JSON line instead of the headline):
+  --members N [--pattern]  ensemble of N members folded into every launch
+                           (optionally with the AR(1) temperature pattern)
+  --micro                  transform microbenchmark: spectogrd + grdtospec +
+                           getuv on 64 levels x 4 fields, FP64 / FP32 / BF16X3
+  --moist {plain,mlp}      moist run with the simple physics or the column MLP
 """
 import argparse
 import json
@@ -29,14 +36,17 @@ sys.path.insert(0, os.path.join(ROOT, "gfs-dycore_amd"))
 import gfsdc  # noqa: E402
 import levels  # noqa: E402
 
-PEAK = {"fp32_mfma": 157.3e12, "fp64_mfma": 78.6e12, "hbm": 8.0e12}
+PEAK = {"fp32_mfma": 157.3e12, "fp64_mfma": 78.6e12, "bf16_mfma": 2.5e15, "hbm": 8.0e12}
+PATTERN = dict(tpat_amp=0.5, tpat_tau=6 * 3600.0, tpat_lscale=500.0e3, tpat_seed=1)
 
 
-def make(cfgname, precision):
+def make(cfgname, precision, nmem=1, pattern=False, **kw):
     c = levels.CONFIGS[cfgname]
     ak, bk = levels.levels_for(cfgname)
+    extra = dict(PATTERN) if pattern else {}
+    extra.update(kw)
     ctx = gfsdc.Context(c["nlon"], c["nlat"], c["ntrunc"], precision, nlev=c["nlev"], ak=ak, bk=bk,
-                        dt=c["dt"], physics=gfsdc.PHYS_HS)
+                        dt=c["dt"], nmem=nmem, physics=extra.pop("physics", gfsdc.PHYS_HS), **extra)
     ctx.init_rest_state(300.0, 1.0e5, 1.0e-6, 1, 21, seed=0)
     return ctx
 
@@ -76,19 +86,34 @@ def algorithmic_work(name, ctx, esz):
     Legendre GEMMs (2 nlat ndimspec per scalar field), bytes otherwise
     (each input / output array read or written once)."""
     nlat, nlon, T, nlm = ctx.nlat, ctx.nlon, ctx.ntrunc, ctx.nlm
-    nl = ctx.nlev
-    nsyn, nan, nhs = 7 * nl + 3, 4 * nl + 1, 3 * nl + 1
+    nl, nm = ctx.nlev, ctx.nmem
+    nsyn = (7 + 3 * ctx.ntrac) * nl + 3
     ng = nlat * nlon
     four = (T + 1) * nlat * 2 * esz  # Fourier coefficients of one field, both hemispheres
     if name == "leg_synth":
-        return "mfma", nsyn * 2.0 * nlat * nlm
+        return "mfma", nm * nsyn * 2.0 * nlat * nlm
     if name == "leg_anal":
         return "mfma", None  # batch size differs between dynamics and physics passes
     if name == "fft_synth":
-        return "hbm", nsyn * (four + ng * esz)
+        return "hbm", nm * nsyn * (four + ng * esz)
     if name == "columns":
-        return "hbm", ((7 * nl + 3) + (4 * nl + 1) + (2 * nl + 1) + 2) * ng * esz
+        return "hbm", nm * ((7 * nl + 3) + (4 * nl + 1) + (2 * nl + 1) + 2) * ng * esz
     return "hbm", None
+
+
+def roofline(tab, ctx, esz):
+    per_step = {k: v[0] * v[1] for k, v in tab.items()}
+    dom = max(per_step, key=per_step.get)
+    kind, work = algorithmic_work(dom, ctx, esz)
+    avg_s = tab[dom][0] * 1e-3
+    roof = {"kernel": dom, "bound": kind, "avg_launch_ms": tab[dom][0], "traffic": None}
+    if work is not None:
+        if kind == "mfma":
+            roof.update(achieved=work / avg_s / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s")
+        else:
+            roof.update(achieved=work / avg_s / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s")
+        roof["frac"] = roof["achieved"] / roof["peak"]
+    return roof, per_step
 
 
 def spectral_error(a, b):
@@ -116,56 +141,39 @@ def cpu_baseline(cfgname):
                       f"oracle at {cfgname}, OpenMP over fields, {t:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="T254L64", choices=sorted(levels.CONFIGS))
-    ap.add_argument("--error-steps", type=int, default=144, help="steps for the mixed-vs-FP64 error (0: skip)")
-    ap.add_argument("--prof-steps", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: the step is timed on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
-        sys.exit(2)
+def base_line(args, value, unit, metric, config, dtype="fp32", **extra):
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic (resting 300 K isothermal state + seeded random vorticity)",
+            "config": config}
+    line.update(extra)
+    return line
+
+
+def run_headline(args):
     cfg = levels.CONFIGS[args.config]
     dt = cfg["dt"]
-
     c32 = make(args.config, gfsdc.FP32)
+    nf_best, nf_ms = c32.autotune()
     t32 = time_steps(c32, args.steps, args.warmup)
-    out = {"fp32_ms_per_step": t32 * 1e3}
+    out = {"fp32_ms_per_step": t32 * 1e3, "autotune_fft_fields": nf_best,
+           "autotune_fft_ms": {str(k): round(float(v), 4) for k, v in zip((2, 4, 8), nf_ms)}}
     if not args.no_fp64:
         c64 = make(args.config, gfsdc.FP64)
+        c64.autotune()
         t64 = time_steps(c64, args.steps, args.warmup)
         out["fp64_ms_per_step"] = t64 * 1e3
         out["fp64_days_per_day"] = dt / t64
         out["speedup_mixed_vs_fp64"] = t64 / t32
         del c64
-
-    # per-kernel timing (HIP events on the context stream) of the mixed path
     tab = kernel_table(c32, args.prof_steps)
-    per_step = {k: v[0] * v[1] for k, v in tab.items()}
-    dom = max(per_step, key=per_step.get)
-    kind, work = algorithmic_work(dom, c32, 4)
-    avg_s = tab[dom][0] * 1e-3
-    roof = {"kernel": dom, "bound": kind, "avg_launch_ms": tab[dom][0], "traffic": None}
-    if work is not None:
-        if kind == "mfma":
-            roof.update(achieved=work / avg_s / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s")
-        else:
-            roof.update(achieved=work / avg_s / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s")
-        roof["frac"] = roof["achieved"] / roof["peak"]
+    roof, per_step = roofline(tab, c32, 4)
     leg = tab.get("leg_synth")
     legendre = None
     if leg:
         _, f = algorithmic_work("leg_synth", c32, 4)
         legendre = {"kernel": "leg_synth", "avg_launch_ms": leg[0], "achieved_tflops": f / (leg[0] * 1e-3) / 1e12,
-                    "peak_tflops": PEAK["fp32_mfma"] / 1e12,
-                    "frac": f / (leg[0] * 1e-3) / PEAK["fp32_mfma"]}
-
+                    "peak_tflops": PEAK["fp32_mfma"] / 1e12, "frac": f / (leg[0] * 1e-3) / PEAK["fp32_mfma"]}
     err = None
     if args.error_steps > 0 and not args.no_fp64:
         a, b = make(args.config, gfsdc.FP32), make(args.config, gfsdc.FP64)
@@ -173,33 +181,160 @@ def main():
         b.advance(args.error_steps, use_graph=True)
         err = spectral_error(a, b)
         del a, b
-
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.config)
+    line = base_line(
+        args, dt / t32, "simulated days/wall-day",
+        "simulated days/wall-day at T254L64, mixed vs FP64 GPU, at stated error",
+        {"workload": f"{args.config} dry Held-Suarez dycore step (IMEX RK3, dt={dt:g} s)",
+         "grid": f"{cfg['nlon']}x{cfg['nlat']}", "ntrunc": cfg["ntrunc"], "nlev": cfg["nlev"],
+         "members": 1, "hip_graph": True},
+        ms_per_step=t32 * 1e3, **out, error_vs_fp64_after_steps=args.error_steps if err else 0,
+        spectral_rel_error_vs_fp64=err, roofline=roof, legendre_gemm=legendre,
+        kernels_ms_per_step={k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
+        cpu_baseline=cpu)
+    return line
 
-    line = {
-        "metric": "simulated days/wall-day at T254L64, mixed vs FP64 GPU, at stated error",
-        "value": dt / t32,
-        "unit": "simulated days/wall-day",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t32 * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (resting 300 K isothermal state + seeded random vorticity)",
-        "config": {"workload": f"{args.config} dry Held-Suarez dycore step (IMEX RK3, dt={dt:g} s)",
-                   "grid": f"{cfg['nlon']}x{cfg['nlat']}", "ntrunc": cfg["ntrunc"], "nlev": cfg["nlev"],
-                   "members": 1, "hip_graph": True},
-        **out,
-        "error_vs_fp64_after_steps": args.error_steps if err else 0,
-        "spectral_rel_error_vs_fp64": err,
-        "roofline": roof,
-        "legendre_gemm": legendre,
-        "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
-        "cpu_baseline": cpu,
-    }
+
+def run_ensemble(args):
+    """config: N members of the dry dycore folded into every launch (GEMM
+    columns, FFT batch, column-kernel grid), optionally each with its own AR(1)
+    temperature pattern; value = member-days per wall-day"""
+    cfg = levels.CONFIGS[args.config]
+    dt, nm = cfg["dt"], args.members
+    ctx = make(args.config, gfsdc.FP32, nmem=nm, pattern=args.pattern)
+    nf_best, _ = ctx.autotune()
+    t = time_steps(ctx, args.steps, args.warmup)
+    one = make(args.config, gfsdc.FP32, nmem=1, pattern=args.pattern)
+    one.autotune()
+    t1 = time_steps(one, args.steps, args.warmup)
+    tab = kernel_table(ctx, 1)
+    roof, per_step = roofline(tab, ctx, 4)
+    return base_line(
+        args, dt * nm / t, "member-days/wall-day",
+        f"ensemble throughput, {nm} members of {args.config}" + (" with AR(1) temperature pattern" if args.pattern else ""),
+        {"workload": f"{nm}-member {args.config} dry Held-Suarez ensemble (IMEX RK3, dt={dt:g} s)"
+                     + (", AR(1) pattern 0.5 K / 6 h / 500 km" if args.pattern else ""),
+         "members": nm, "hip_graph": True},
+        ms_per_step=t * 1e3, per_member_days_per_day=dt / t * nm / nm, single_member_ms_per_step=t1 * 1e3,
+        batching_gain=t1 * nm / t, autotune_fft_fields=nf_best, roofline=roof,
+        kernels_ms_per_step={k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])})
+
+
+def run_micro(args):
+    """transform microbenchmark at T254: spectogrd + grdtospec round trip plus
+    getuv on 64 levels x 4 fields (vrt, div, T, q); spectra with amplitude
+    n^-1.5 and random phase (m = 0 real). Errors are relative max-norm against
+    the FP64 GPU path (itself equal to the FP64 oracle to 1e-11 in the tests)."""
+    nlon, nlat, T = 768, 384, 254
+    ctxs = {p: gfsdc.Context(nlon, nlat, T, getattr(gfsdc, p)) for p in ("FP64", "FP32", "BF16X3")}
+    c64 = ctxs["FP64"]
+    rng = np.random.default_rng(0)
+    n = np.maximum(c64.degree, 1).astype(float)
+    nf = 4 * 64
+    q = n ** -1.5 * np.exp(2j * np.pi * rng.random((nf, c64.nlm)))
+    q[:, c64.order == 0] = q[:, c64.order == 0].real
+    vrt, div = q[:64].copy(), q[64:128].copy() * 0.1
+    vrt[:, 0] = div[:, 0] = 0
+    ref = {"grid": c64.spectogrd(q)}
+    ref["spec"] = c64.grdtospec(ref["grid"])
+    ref["u"], ref["v"] = c64.getuv(vrt, div, 6.3712e6)
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+
+    res = {}
+    for name, c in ctxs.items():
+        dt_ = 0 if name == "FP64" else 1
+        cast = (lambda a: a) if dt_ == 0 else (lambda a: a.astype(np.complex64 if np.iscomplexobj(a) else np.float32))
+        g = c.spectogrd(cast(q), dtype=dt_)
+        s = c.grdtospec(cast(ref["grid"]), dtype=dt_)
+        u, v = c.getuv(cast(vrt), cast(div), 6.3712e6, dtype=dt_)
+        err = {"spectogrd": rel(g, ref["grid"]), "grdtospec": rel(s, ref["spec"]),
+               "getuv": max(rel(u, ref["u"]), rel(v, ref["v"]))}
+        # device-resident timing: one round trip + getuv per repeat, host copies excluded
+        ms = c.bench_transforms(nf, 64, args.steps)
+        res[name] = {"ms_per_iteration": ms, "rel_error_vs_fp64": err}
+    value = 1e3 / res["FP32"]["ms_per_iteration"]
+    return base_line(
+        args, value, "iterations/s",
+        "T254 transform microbenchmark: spectogrd+grdtospec round trip + getuv, 64 levels x 4 fields",
+        {"workload": "T254 (768x384) transforms, 256 scalar fields + 64 vector pairs, device resident"},
+        dtype="fp32", data="synthetic spectra, amplitude n^-1.5, random phase", precisions=res,
+        speedup_fp32_vs_fp64=res["FP64"]["ms_per_iteration"] / res["FP32"]["ms_per_iteration"],
+        speedup_bf16x3_vs_fp64=res["FP64"]["ms_per_iteration"] / res["BF16X3"]["ms_per_iteration"])
+
+
+def run_moist(args):
+    """moist T254L64 (or --config) step with the simple physics, or with the
+    column MLP in its place (512-wide, 3 hidden layers, judge on every 50th
+    column, fallback off for timing: an untrained network is timed, its
+    accuracy is covered by the tests)"""
+    cfg = levels.CONFIGS[args.config]
+    dt = cfg["dt"]
+    ctx = make(args.config, gfsdc.FP32, ntrac=1, physics=gfsdc.PHYS_SIMPLE, simple_test=2, turbflux=1)
+    ak, bk = levels.levels_for(args.config)
+    qg = levels.moist_grid(ctx.lats, ctx.nlon, ak, bk)
+    st = ctx.get_state()
+    ctx.set_state(st["vrt"], st["div"], st["tv"], st["lnps"], trac=ctx.grdtospec(qg)[None, None])
+    ctx.autotune()
+    mlp = None
+    if args.moist == "mlp":
+        ctx.mlp_init(512, 3, 32768, 1 << 17, seed=1)
+        ctx.mlp_collect(1, 8)
+        ctx.mlp_normalize()
+        ctx.mlp_enable(True, judge_stride=50, threshold=1e30)
+    t = time_steps(ctx, args.steps, args.warmup)
+    tab = kernel_table(ctx, 1)
+    roof, per_step = roofline(tab, ctx, 4)
+    if args.moist == "mlp":
+        ng = ctx.nlat * ctx.nlon
+        nin, h, nout = 4 * ctx.nlev + 3, 512, 4 * ctx.nlev
+        flops = 2.0 * ng * (nin * h + 2 * h * h + h * nout)
+        fw = tab.get("mlp_forward", (None,))[0]
+        mlp = {"forward_ms": fw, "flop_per_column": flops / ng,
+               "achieved_tflops": flops / (fw * 1e-3) / 1e12 if fw else None,
+               "peak_tflops_bf16": PEAK["bf16_mfma"] / 1e12,
+               "frac": flops / (fw * 1e-3) / PEAK["bf16_mfma"] if fw else None,
+               "judge": ctx.mlp_judge_stats()}
+    return base_line(
+        args, dt / t, "simulated days/wall-day",
+        f"moist {args.config} step, {'column MLP' if args.moist == 'mlp' else 'simple'} physics",
+        {"workload": f"{args.config} moist dycore + {'column MLP (judge 2 %)' if args.moist == 'mlp' else 'simple physics'}"
+                     f", SST 302.15 K - 30 K sin^2 lat, dt={dt:g} s", "members": 1, "hip_graph": True},
+        ms_per_step=t * 1e3, mlp=mlp, roofline=roof,
+        kernels_ms_per_step={k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default=None, choices=sorted(levels.CONFIGS))
+    ap.add_argument("--error-steps", type=int, default=144, help="steps for the mixed-vs-FP64 error (0: skip)")
+    ap.add_argument("--prof-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--members", type=int, default=0, help="ensemble members (ensemble workload)")
+    ap.add_argument("--pattern", action="store_true", help="AR(1) temperature pattern per member")
+    ap.add_argument("--micro", action="store_true", help="transform microbenchmark")
+    ap.add_argument("--moist", choices=("plain", "mlp"), default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the step is timed on a single MI355X; "
+              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        sys.exit(2)
+    if args.micro:
+        line = run_micro(args)
+    elif args.members:
+        args.config = args.config or "T126L64"
+        line = run_ensemble(args)
+    elif args.moist:
+        args.config = args.config or "T254L64"
+        line = run_moist(args)
+    else:
+        args.config = args.config or "T254L64"
+        line = run_headline(args)
     print(json.dumps(line))
 
 

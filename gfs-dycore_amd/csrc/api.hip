@@ -170,6 +170,10 @@ int gfsdc_simple_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, void *
     DISPATCH(c, simple_phytend(dvrt, ddiv, dtv, dtrac, dtype, loc));
 }
 int gfsdc_get_pattern(gfsdc_ctx *c, void *spec, int dtype, int loc) { DISPATCH(c, get_pattern(spec, dtype, loc)); }
+int gfsdc_autotune(gfsdc_ctx *c, double *ms, int *best) { DISPATCH(c, autotune(ms, best)); }
+int gfsdc_bench_transforms(gfsdc_ctx *c, int nscalar, int nvector, int reps, double *ms) {
+    DISPATCH(c, bench_transforms(nscalar, nvector, reps, ms));
+}
 int gfsdc_mlp_init(gfsdc_ctx *c, int hidden, int nlayers, int maxbatch, long capacity, unsigned long long seed) {
     DISPATCH(c, mlp_init(hidden, nlayers, maxbatch, capacity, seed));
 }

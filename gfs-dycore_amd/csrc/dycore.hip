@@ -1886,6 +1886,53 @@ template <typename T> int Engine<T>::get_pattern(void *out, int dtype, int loc) 
     return output((const T *)B->tpat, out, (size_t)plan.nlm * B->nm * 2, dtype, loc, 1);
 }
 
+// ---- tile autotuner ----------------------------------------------------------
+// Times the step's Fourier transforms (the synthesis of every grid field and
+// the analysis of the dynamics tendencies) with 2, 4 and 8 fields per FFT
+// workgroup on this context's own sizes and keeps the fastest. Leaves the
+// model state untouched; the next graph advance captures the new choice.
+template <typename T> int Engine<T>::autotune(double *ms, int *best) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("autotune needs a dycore context");
+        return GFSDC_ESTATE;
+    }
+    Work<T> &W = *(Work<T> *)B->Work;
+    const int cand[3] = {2, 4, 8};
+    const int keep = plan.fft_nf;
+    hipEvent_t e0, e1;
+    GFSDC_CHECK(hipEventCreate(&e0));
+    GFSDC_CHECK(hipEventCreate(&e1));
+    double bestms = 1e300;
+    int bestnf = keep;
+    for (int c = 0; c < 3; c++) {
+        plan.fft_nf = cand[c];
+        float t = 0.f;
+        for (int rep = 0; rep < 2; rep++) {  // first pass warms up
+            GFSDC_CHECK(hipEventRecord(e0, stream));
+            launch_fft_synth(plan, W.fs, W.ldc, (T *)B->gsyn, B->ng, B->nm * B->nsyn, B->rflag_syn, stream);
+            launch_fft_anal(plan, (const T *)B->gan, B->ng, B->nm * B->nan, B->rflag_an, W.fa, W.ldc, stream);
+            GFSDC_CHECK(hipEventRecord(e1, stream));
+            GFSDC_CHECK(hipEventSynchronize(e1));
+            GFSDC_CHECK(hipEventElapsedTime(&t, e0, e1));
+        }
+        if (ms) ms[c] = t;
+        if (t < bestms) {
+            bestms = t;
+            bestnf = cand[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan.fft_nf = bestnf;
+    if (best) *best = bestnf;
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->gexec = nullptr;
+    B->graph = nullptr;
+    return 0;
+}
+
 // ---- learned column physics: data, training, judge ---------------------------
 
 __global__ void k_norm_rows(const float *__restrict__ src, long n, int nf, const float *__restrict__ mean,

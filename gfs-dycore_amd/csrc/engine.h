@@ -44,6 +44,7 @@ template <typename T> struct Engine {
     int output(const T *d, void *p, size_t n, int dtype, int loc, int slot);
 
     int spectogrd(const void *spec, void *grid, int nf, int dtype, int loc);
+    int bench_transforms(int ns, int nv, int reps, double *ms);
     int grdtospec(const void *grid, void *spec, int nf, int dtype, int loc);
     int sphtor_to_spat(const void *s, const void *t, void *u, void *v, int nf, double sscale, int sinv,
                        double tscale, int tinv, int dtype, int loc);
@@ -64,6 +65,7 @@ template <typename T> struct Engine {
     int simple_phytend(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
     int pack_tphys(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
     int pattern_spinup();
+    int autotune(double *ms, int *best);
     int get_pattern(void *out, int dtype, int loc);
     // learned column physics
     int mlp_init(int hidden, int nlayers, int maxbatch, long capacity, unsigned long long seed);

@@ -207,6 +207,63 @@ template <typename T> int Engine<T>::grdtospec(const void *grid, void *spec, int
     return output(out, spec, (size_t)nf * plan.nlm * 2, dtype, loc, 1);
 }
 
+// Transform microbenchmark with device-resident buffers: per iteration a
+// spectogrd + grdtospec round trip of ns scalar fields and a getuv of nv
+// vector pairs through the same entry points as callers (loc = device).
+template <typename T> __global__ void k_fill_spec(cplx<T> *s, long n, int nlm) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = (int)(i % nlm);
+    T a = T(1) / (T(1) + T(k % 97));
+    s[i] = cplx<T>{a, (k % 7) ? a : T(0)};
+}
+
+template <typename T> int Engine<T>::bench_transforms(int ns, int nv, int reps, double *ms) {
+    if (ns < 1 || nv < 0 || reps < 1) {
+        set_error("bench_transforms: ns >= 1, nv >= 0, reps >= 1");
+        return GFSDC_EINVAL;
+    }
+    const size_t ng = (size_t)plan.nlat * plan.nlon, nlm = plan.nlm;
+    const int dt = sizeof(T) == 8 ? GFSDC_DT_F64 : GFSDC_DT_F32;
+    T *spec = nullptr, *grid = nullptr, *spec2 = nullptr, *uv = nullptr;
+    GFSDC_CHECK(hipMalloc((void **)&spec, sizeof(cplx<T>) * nlm * (ns + 2 * nv)));
+    GFSDC_CHECK(hipMalloc((void **)&spec2, sizeof(cplx<T>) * nlm * ns));
+    GFSDC_CHECK(hipMalloc((void **)&grid, sizeof(T) * ng * ns));
+    GFSDC_CHECK(hipMalloc((void **)&uv, sizeof(T) * ng * 2 * (nv > 0 ? nv : 1)));
+    const long nsp = (long)nlm * (ns + 2 * nv);
+    hipLaunchKernelGGL(k_fill_spec<T>, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, stream, (cplx<T> *)spec,
+                       nsp, (int)nlm);
+    const T *vrt = spec + 2 * nlm * ns, *div = vrt + 2 * nlm * nv;
+    auto iter = [&]() -> int {
+        int rc = spectogrd(spec, grid, ns, dt, GFSDC_DEVICE);
+        if (!rc) rc = grdtospec(grid, spec2, ns, dt, GFSDC_DEVICE);
+        if (!rc && nv > 0)
+            rc = sphtor_to_spat(vrt, div, uv, uv + ng * nv, nv, 6.3712e6, 1, 6.3712e6, 1, dt, GFSDC_DEVICE);
+        return rc;
+    };
+    int rc = iter();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) {
+        GFSDC_CHECK(hipStreamSynchronize(stream));
+        GFSDC_CHECK(hipEventCreate(&e0));
+        GFSDC_CHECK(hipEventCreate(&e1));
+        GFSDC_CHECK(hipEventRecord(e0, stream));
+        for (int r = 0; r < reps && !rc; r++) rc = iter();
+        GFSDC_CHECK(hipEventRecord(e1, stream));
+        GFSDC_CHECK(hipEventSynchronize(e1));
+        float t = 0.f;
+        GFSDC_CHECK(hipEventElapsedTime(&t, e0, e1));
+        if (ms) *ms = t / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(spec);
+    (void)hipFree(spec2);
+    (void)hipFree(grid);
+    (void)hipFree(uv);
+    return rc;
+}
+
 // (S, T) -> (Vt, Vp) of SHTns sphtor_to_spat with S = sscale [invlap] s, T = tscale [invlap] t;
 // s may be null (zero spheroidal part, as getgrad).
 template <typename T>

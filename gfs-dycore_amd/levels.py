@@ -39,6 +39,32 @@ CONFIGS = {
 }
 
 
+def moist_grid(lats, nlon, ak, bk, q0=0.025, seed=0):
+    """Specific humidity on the grid, shape (nlev bottom -> top, nlat, nlon): a
+    moist tropical lower troposphere drying upward (sigma^3) and poleward, with a
+    zonal wavenumber-3 modulation whose phase depends on the seed; supersaturated
+    near the surface in the tropics of a 300 K state, so condensation occurs."""
+    ak = np.asarray(ak, np.float64)
+    bk = np.asarray(bk, np.float64)
+    sig = (0.5 * ((ak[:-1] + ak[1:]) / 1.0e5 + (bk[:-1] + bk[1:])))[::-1]
+    ph = 2 * np.pi * _uniform(seed, 7, 0)
+    lam = 2 * np.pi * np.arange(nlon) / nlon
+    lat = np.asarray(lats, np.float64)
+    return (q0 * np.exp(-(lat / 0.7) ** 4)[None, :, None] * sig[:, None, None] ** 3 *
+            (1.0 + 0.2 * np.cos(3 * lam + ph))[None, None, :])
+
+
+def _uniform(seed, stream, index):
+    """one draw of the library's counter RNG (splitmix64 of seed, stream, index)"""
+    m = (1 << 64) - 1
+    z = ((seed * 0x9E3779B97F4A7C15) & m) ^ (((stream + 0x632BE59BD9B4E019) & m) * 0xBF58476D1CE4E5B9 & m)
+    z = (z + index * 0x94D049BB133111EB + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    return (z >> 11) * 2.0 ** -53
+
+
 def levels_for(name):
     c = CONFIGS[name]
     return sigma_levels(c["nlev"]) if c["levels"] == "sigma" else hybrid_levels(c["nlev"])

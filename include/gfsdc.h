@@ -173,6 +173,15 @@ int gfsdc_hs_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, int dtyp
  * also refreshes the precip and pwat grids. The state is not changed. */
 int gfsdc_simple_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype,
                          int loc);
+/* tile autotuner (no counterpart in the reference): times the step's Fourier
+ * transforms with 2, 4 and 8 fields per FFT workgroup on this context's sizes
+ * and keeps the fastest; ms (3, may be NULL) receives the timings, best the
+ * choice. The model state is unchanged. */
+int gfsdc_autotune(gfsdc_ctx *ctx, double *ms, int *best);
+/* transform microbenchmark (no counterpart in the reference): milliseconds per
+ * iteration of a spectogrd + grdtospec round trip of nscalar fields plus a
+ * getuv of nvector pairs on device-resident buffers, averaged over reps */
+int gfsdc_bench_transforms(gfsdc_ctx *ctx, int nscalar, int nvector, int reps, double *ms);
 /* current AR(1) pattern, (ndimspec, nmem) complex of dtype (error if tpat_amp == 0) */
 int gfsdc_get_pattern(gfsdc_ctx *ctx, void *spec, int dtype, int loc);
 /* ---- learned column physics (no counterpart in the reference) --------------

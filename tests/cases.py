@@ -74,15 +74,7 @@ def perturbed_state(degree, order, nlev, seed, amp=1.0, nmax=None):
 
 
 def moist_grid(lats, nlon, ak, bk, q0=0.025, seed=0):
-    """Specific humidity on the grid, shape (nlev bottom -> top, nlat, nlon):
-    a moist tropical lower troposphere drying upward (sigma^3) and poleward,
-    with a seeded zonal wavenumber-3 modulation; supersaturated near the
-    surface in the tropics of a 300 K state, so condensation occurs."""
-    ak = np.asarray(ak, np.float64)
-    bk = np.asarray(bk, np.float64)
-    sig = (0.5 * ((ak[:-1] + ak[1:]) / 1.0e5 + (bk[:-1] + bk[1:])))[::-1]
-    ph = 2 * np.pi * uniform(seed, 7, 0)
-    lam = 2 * np.pi * np.arange(nlon) / nlon
-    lat = np.asarray(lats, np.float64)
-    return (q0 * np.exp(-(lat / 0.7) ** 4)[None, :, None] * sig[:, None, None] ** 3 *
-            (1.0 + 0.2 * np.cos(3 * lam + ph))[None, None, :])
+    """Specific humidity on the grid (nlev bottom -> top, nlat, nlon); the
+    synthetic moist state of the benchmark configurations (levels.moist_grid)."""
+    import levels
+    return levels.moist_grid(lats, nlon, ak, bk, q0=q0, seed=seed)
