@@ -673,6 +673,51 @@ struct Radices {
     int n, r[16];
 };
 
+// The same radix sequence as factor_nlon, at compile time, for the FFT kernels
+// specialised to a transform length (constant divisors, unrolled stages).
+struct RadPlan {
+    int n, r[16];
+};
+constexpr RadPlan rad_plan(int n) {
+    RadPlan p{0, {}};
+    while (n % 4 == 0 && n > 4) {
+        p.r[p.n++] = 4;
+        n /= 4;
+    }
+    while (n % 3 == 0) {
+        p.r[p.n++] = 3;
+        n /= 3;
+    }
+    while (n % 4 == 0) {
+        p.r[p.n++] = 4;
+        n /= 4;
+    }
+    while (n % 2 == 0) {
+        p.r[p.n++] = 2;
+        n /= 2;
+    }
+    return p;
+}
+
+template <typename T, int NFFT, int N, int S, int Ns>
+__device__ inline void fft_stages(cplx<T> *buf, const cplx<T> *tw, int sign) {
+    constexpr RadPlan P = rad_plan(N);
+    if constexpr (S < P.n) {
+        constexpr int R = P.r[S];
+        fft_stage<T, R, (NFFT * N / R + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
+        fft_stages<T, NFFT, N, S + 1, Ns * R>(buf, tw, sign);
+    }
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8).
+// Give each XCD a contiguous range of logical tiles, so that tiles sharing
+// cache lines (the field groups of one latitude pair) run on the same XCD and
+// meet in its L2 instead of fetching the same lines through eight L2s.
+__device__ inline int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
 template <typename T> struct FftCfg;
 template <> struct FftCfg<float> { static constexpr int NFFT = 4, MAXN = 768; };
 template <> struct FftCfg<double> { static constexpr int NFFT = 4, MAXN = 768; };
@@ -682,16 +727,20 @@ static_assert(FftCfg<double>::MAXN * 8 * 16 + FftCfg<double>::MAXN * 16 <= 160 *
 // twiddle table is staged in LDS once, the Fourier coefficients of both
 // hemispheres are loaded in one batch per thread before any LDS write, and
 // only the spectral gap T < k < N-T is zero-filled.
-template <typename T, int NFFT, int MAXN>
+// NC > 0 specialises the kernel to transform length NC (compile-time radix
+// stages and divisors, LDS sized to NC); NC = 0 handles any length <= MAXN.
+template <typename T, int NFFT, int MAXN, int NC>
 __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int ldf, T *__restrict__ grid,
                                                    long gstride, int nf, const unsigned char *__restrict__ rflag,
                                                    const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
                                                    int nlon, int nlat, int ntrunc, Radices rd) {
-    __shared__ cplx<T> buf[NFFT * MAXN];
-    __shared__ cplx<T> tws[MAXN];
-    const int jp = blockIdx.x, f0 = blockIdx.y * NFFT;
-    const int j = jp, j2 = nlat - 1 - jp, N = nlon, tid = threadIdx.x;
-    constexpr int MAXI = (NFFT * (MAXN / 2) + 255) / 256;
+    constexpr int LN = NC ? NC : MAXN;
+    __shared__ cplx<T> buf[NFFT * LN];
+    __shared__ cplx<T> tws[LN];
+    const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
+    const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, tid = threadIdx.x;
+    constexpr int MAXI = (NFFT * (LN / 2) + 255) / 256;
     const int nitems = (ntrunc + 1) * NFFT;
     cplx<T> xn[MAXI], xs[MAXI];
 #pragma unroll
@@ -728,7 +777,10 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
         }
     }
     __syncthreads();
-    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, +1);
+    if constexpr (NC > 0)
+        fft_stages<T, NFFT, NC, 0, 1>(buf, tws, +1);
+    else
+        fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, +1);
     const T rc = rcos[j];
     for (int t = tid; t < NFFT * N; t += 256) {
         int q = t / N, i = t - q * N, f = f0 + q;
@@ -741,17 +793,19 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
 
 // Fourier analysis: rows of a latitude pair -> weighted symmetric / antisymmetric
 // Fourier coefficients, m = 0..T. Grid rows are loaded in one batch per thread.
-template <typename T, int NFFT, int MAXN>
+template <typename T, int NFFT, int MAXN, int NC>
 __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, long gstride, int nf,
                                                   const unsigned char *__restrict__ rflag,
                                                   const T *__restrict__ rcos, const T *__restrict__ wfac,
                                                   const cplx<T> *__restrict__ tw, T *__restrict__ fa, int ldf,
                                                   int nlon, int nlat, int nhp, int ntrunc, Radices rd) {
-    __shared__ cplx<T> buf[NFFT * MAXN];
-    __shared__ cplx<T> tws[MAXN];
-    const int jp = blockIdx.x, f0 = blockIdx.y * NFFT;
-    const int j = jp, j2 = nlat - 1 - jp, N = nlon, tid = threadIdx.x;
-    constexpr int MAXG = (NFFT * MAXN + 255) / 256;
+    constexpr int LN = NC ? NC : MAXN;
+    __shared__ cplx<T> buf[NFFT * LN];
+    __shared__ cplx<T> tws[LN];
+    const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
+    const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, tid = threadIdx.x;
+    constexpr int MAXG = (NFFT * LN + 255) / 256;
     T xv[MAXG], yv[MAXG];
 #pragma unroll
     for (int c = 0; c < MAXG; c++) {
@@ -773,7 +827,10 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
         if (t < NFFT * N) buf[t] = cplx<T>{xv[c], yv[c]};
     }
     __syncthreads();
-    fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, -1);
+    if constexpr (NC > 0)
+        fft_stages<T, NFFT, NC, 0, 1>(buf, tws, -1);
+    else
+        fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, -1);
     const T wj = wfac[j], rc = rcos[j];
     for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
         int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
@@ -801,43 +858,45 @@ __device__ inline cplx<T> src_coef(const cplx<T> *__restrict__ src, long sf, lon
     return cplx<T>{v.x * s, v.y * s};
 }
 
+constexpr int GATHER_KE = 8;
+
 template <typename T>
 __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk, const SynthDesc *__restrict__ desc,
                                int nfo, T *__restrict__ sx, int ldx, const int2 *__restrict__ kemn,
                                const int *__restrict__ koff, const T *__restrict__ eps, const int *__restrict__ eoff,
-                               int ntrunc) {
-    const int ke = blockIdx.y;
+                               int ntrunc, int nke) {
     const int fo = blockIdx.x * blockDim.x + threadIdx.x;
     if (fo >= nfo) return;
-    const int2 mn = kemn[ke];
-    const int m = mn.x, n = mn.y;
     const SynthDesc d = desc[fo];
-    cplx<T> r{T(0), T(0)};
-    if (d.kind == SK_COPY || (d.kind == SK_COPY_DEV && n > 0)) {
-        r = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
-    } else if (d.kind == SK_UHAT || d.kind == SK_VHAT) {
-        const T *ep = eps + eoff[m];
-        T en = ep[n - m], en1 = ep[n + 1 - m];
-        T cm1 = T(n - 1) * en, cp1 = T(n + 2) * en1, fm = T(m);
-        if (d.kind == SK_UHAT) {
-            // U = (n-1) eps_n S_{n-1} - (n+2) eps_{n+1} S_{n+1} + i m T_n
-            cplx<T> sm = src_coef(src, sf, sk, d.a, koff, m, n - 1, ntrunc, d.sa, d.inv_a);
-            cplx<T> sp = src_coef(src, sf, sk, d.a, koff, m, n + 1, ntrunc, d.sa, d.inv_a);
-            cplx<T> t0 = src_coef(src, sf, sk, d.b, koff, m, n, ntrunc, d.sb, d.inv_b);
-            r.x = cm1 * sm.x - cp1 * sp.x - fm * t0.y;
-            r.y = cm1 * sm.y - cp1 * sp.y + fm * t0.x;
-        } else {
-            // V = i m S_n - (n-1) eps_n T_{n-1} + (n+2) eps_{n+1} T_{n+1}
-            cplx<T> s0 = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
-            cplx<T> tm = src_coef(src, sf, sk, d.b, koff, m, n - 1, ntrunc, d.sb, d.inv_b);
-            cplx<T> tp = src_coef(src, sf, sk, d.b, koff, m, n + 1, ntrunc, d.sb, d.inv_b);
-            r.x = -fm * s0.y - cm1 * tm.x + cp1 * tp.x;
-            r.y = fm * s0.x - cm1 * tm.y + cp1 * tp.y;
+    // GATHER_KE extended coefficients per workgroup: one descriptor load, GATHER_KE rows
+    for (int ke = blockIdx.y * GATHER_KE; ke < min(nke, (blockIdx.y + 1) * GATHER_KE); ke++) {
+        const int2 mn = kemn[ke];
+        const int m = mn.x, n = mn.y;
+        cplx<T> r{T(0), T(0)};
+        if (d.kind == SK_COPY || (d.kind == SK_COPY_DEV && n > 0)) {
+            r = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
+        } else if (d.kind == SK_UHAT || d.kind == SK_VHAT) {
+            const T *ep = eps + eoff[m];
+            T en = ep[n - m], en1 = ep[n + 1 - m];
+            T cm1 = T(n - 1) * en, cp1 = T(n + 2) * en1, fm = T(m);
+            if (d.kind == SK_UHAT) {
+                // U = (n-1) eps_n S_{n-1} - (n+2) eps_{n+1} S_{n+1} + i m T_n
+                cplx<T> sm = src_coef(src, sf, sk, d.a, koff, m, n - 1, ntrunc, d.sa, d.inv_a);
+                cplx<T> sp = src_coef(src, sf, sk, d.a, koff, m, n + 1, ntrunc, d.sa, d.inv_a);
+                cplx<T> t0 = src_coef(src, sf, sk, d.b, koff, m, n, ntrunc, d.sb, d.inv_b);
+                r.x = cm1 * sm.x - cp1 * sp.x - fm * t0.y;
+                r.y = cm1 * sm.y - cp1 * sp.y + fm * t0.x;
+            } else {
+                // V = i m S_n - (n-1) eps_n T_{n-1} + (n+2) eps_{n+1} T_{n+1}
+                cplx<T> s0 = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
+                cplx<T> tm = src_coef(src, sf, sk, d.b, koff, m, n - 1, ntrunc, d.sb, d.inv_b);
+                cplx<T> tp = src_coef(src, sf, sk, d.b, koff, m, n + 1, ntrunc, d.sb, d.inv_b);
+                r.x = -fm * s0.y - cm1 * tm.x + cp1 * tp.x;
+                r.y = fm * s0.x - cm1 * tm.y + cp1 * tp.y;
+            }
         }
+        *(cplx<T> *)(sx + (long)ke * ldx + 2 * fo) = r;
     }
-    T *o = sx + (long)ke * ldx + 2 * fo;
-    o[0] = r.x;
-    o[1] = r.y;
 }
 
 template <typename T>
@@ -1072,9 +1131,9 @@ template <typename T>
 void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc, int nfo,
                          T *sx, int ldx, double rsphere, hipStream_t s) {
     (void)rsphere;
-    dim3 grid((nfo + 63) / 64, p.nlme);
-    hipLaunchKernelGGL(k_synth_gather<T>, grid, dim3(64), 0, s, src, sf, sk, desc, nfo, sx, ldx, p.kemn, p.koff,
-                       p.eps, p.eoff, p.ntrunc);
+    dim3 grid((nfo + 255) / 256, (p.nlme + GATHER_KE - 1) / GATHER_KE);
+    hipLaunchKernelGGL(k_synth_gather<T>, grid, dim3(256), 0, s, src, sf, sk, desc, nfo, sx, ldx, p.kemn, p.koff,
+                       p.eps, p.eoff, p.ntrunc, p.nlme);
 }
 
 template <typename T>
@@ -1116,19 +1175,34 @@ template <typename T, int NF>
 static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
                          const unsigned char *rflag, hipStream_t s) {
     constexpr int MN = FftCfg<T>::MAXN;
-    dim3 g(p.nhalf, (nf + NF - 1) / NF);
-    hipLaunchKernelGGL((k_fft_synth<T, NF, MN>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, p.rcos,
-                       (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad));
+    dim3 g(p.nhalf * ((nf + NF - 1) / NF));
+#define FFT_SYN(NC)                                                                                             \
+    hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, p.rcos, \
+                       (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad))
+    switch (p.nlon) {
+    case 768: FFT_SYN(768); break;
+    case 384: FFT_SYN(384); break;
+    case 128: FFT_SYN(128); break;
+    default: FFT_SYN(0); break;
+    }
+#undef FFT_SYN
 }
 
 template <typename T, int NF>
 static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
                         int ldf, hipStream_t s) {
     constexpr int MN = FftCfg<T>::MAXN;
-    dim3 g(p.nhalf, (nf + NF - 1) / NF);
-    hipLaunchKernelGGL((k_fft_anal<T, NF, MN>), g, dim3(256), 0, s, grid, gstride, nf, rflag, p.rcos, p.wfac,
-                       (const cplx<T> *)p.tw, fa, ldf, p.nlon, p.nlat, p.nhp, p.ntrunc,
-                       radices_of(p.nrad, p.rad));
+    dim3 g(p.nhalf * ((nf + NF - 1) / NF));
+#define FFT_ANA(NC)                                                                                             \
+    hipLaunchKernelGGL((k_fft_anal<T, NF, MN, NC>), g, dim3(256), 0, s, grid, gstride, nf, rflag, p.rcos, p.wfac,   \
+                       (const cplx<T> *)p.tw, fa, ldf, p.nlon, p.nlat, p.nhp, p.ntrunc, radices_of(p.nrad, p.rad))
+    switch (p.nlon) {
+    case 768: FFT_ANA(768); break;
+    case 384: FFT_ANA(384); break;
+    case 128: FFT_ANA(128); break;
+    default: FFT_ANA(0); break;
+    }
+#undef FFT_ANA
 }
 
 // fields per workgroup of the FFT kernels (p.fft_nf, chosen by the tile autotuner or
