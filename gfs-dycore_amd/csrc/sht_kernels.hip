@@ -617,10 +617,18 @@ template <typename T, int R> __device__ inline void small_dft(cplx<T> *v, int si
     }
 }
 
+// LDS layout of the transforms: field q's element x sits at q * fft_pitch(N) +
+// swz(x), where swz permutes x within its block of 16 complex values by XOR
+// with 5 * (x / 16) mod 16. The early Stockham stages write with stride R
+// (Ns = 1) and in runs of Ns = 4; under this permutation both patterns, and the
+// contiguous reads, put 16 consecutive lanes on 16 different bank pairs.
+__host__ __device__ constexpr int fft_pitch(int N) { return (N + 15) & ~15; }
+__device__ inline int fft_pad(int x) { return x ^ (((x >> 4) * 5) & 15); }
+
 template <typename T, int R, int MAXB>
 __device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cplx<T> *__restrict__ tw,
                                  int sign) {
-    const int nbf = N / R, total = nbf * nfft;
+    const int nbf = N / R, total = nbf * nfft, NP = fft_pitch(N);
     const int tstep = N / (Ns * R);
     cplx<T> v[MAXB][R];
 #pragma unroll
@@ -629,7 +637,7 @@ __device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cp
         if (b < total) {
             int q = b / nbf, j = b - q * nbf;
 #pragma unroll
-            for (int r = 0; r < R; r++) v[bb][r] = buf[q * N + j + r * nbf];
+            for (int r = 0; r < R; r++) v[bb][r] = buf[q * NP + fft_pad(j + r * nbf)];
         }
     }
     __syncthreads();
@@ -648,7 +656,7 @@ __device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cp
             small_dft<T, R>(v[bb], sign);
             int idx = (j / Ns) * Ns * R + jm;
 #pragma unroll
-            for (int r = 0; r < R; r++) buf[q * N + idx + r * Ns] = v[bb][r];
+            for (int r = 0; r < R; r++) buf[q * NP + fft_pad(idx + r * Ns)] = v[bb][r];
         }
     }
     __syncthreads();
@@ -721,7 +729,7 @@ __device__ inline int xcd_tile(int b, int nb) {
 template <typename T> struct FftCfg;
 template <> struct FftCfg<float> { static constexpr int NFFT = 4, MAXN = 768; };
 template <> struct FftCfg<double> { static constexpr int NFFT = 4, MAXN = 768; };
-static_assert(FftCfg<double>::MAXN * 8 * 16 + FftCfg<double>::MAXN * 16 <= 160 * 1024, "FFT LDS budget");
+static_assert(fft_pitch(FftCfg<double>::MAXN) * 8 * 16 + FftCfg<double>::MAXN * 16 <= 160 * 1024, "FFT LDS budget");
 
 // Fourier synthesis: one workgroup per latitude pair and NFFT fields. The
 // twiddle table is staged in LDS once, the Fourier coefficients of both
@@ -735,11 +743,11 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
                                                    const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
                                                    int nlon, int nlat, int ntrunc, Radices rd) {
     constexpr int LN = NC ? NC : MAXN;
-    __shared__ cplx<T> buf[NFFT * LN];
+    __shared__ cplx<T> buf[NFFT * fft_pitch(LN)];
     __shared__ cplx<T> tws[LN];
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
-    const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, tid = threadIdx.x;
+    const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
     constexpr int MAXI = (NFFT * (LN / 2) + 255) / 256;
     const int nitems = (ntrunc + 1) * NFFT;
     cplx<T> xn[MAXI], xs[MAXI];
@@ -758,10 +766,9 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
     }
     for (int t = tid; t < N; t += 256) tws[t] = tw[t];
     const int gap = N - 2 * ntrunc - 1;
-    for (int t = tid; t < NFFT * gap; t += 256) {
-        int q = t / gap;
-        buf[q * N + ntrunc + 1 + (t - q * gap)] = cplx<T>{T(0), T(0)};
-    }
+#pragma unroll
+    for (int q = 0; q < NFFT; q++)
+        for (int i = tid; i < gap; i += 256) buf[q * NP + fft_pad(ntrunc + 1 + i)] = cplx<T>{T(0), T(0)};
 #pragma unroll
     for (int c = 0; c < MAXI; c++) {
         int idx = tid + c * 256;
@@ -772,8 +779,8 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
                 xi = T(0);
                 yi = T(0);
             }
-            buf[q * N + m] = cplx<T>{xr - yi, xi + yr};
-            if (m > 0) buf[q * N + N - m] = cplx<T>{xr + yi, -xi + yr};
+            buf[q * NP + fft_pad(m)] = cplx<T>{xr - yi, xi + yr};
+            if (m > 0) buf[q * NP + fft_pad(N - m)] = cplx<T>{xr + yi, -xi + yr};
         }
     }
     __syncthreads();
@@ -782,12 +789,18 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
     else
         fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, +1);
     const T rc = rcos[j];
-    for (int t = tid; t < NFFT * N; t += 256) {
-        int q = t / N, i = t - q * N, f = f0 + q;
-        if (f >= nf) continue;
-        T s = rflag[f] ? rc : T(1);
-        grid[f * gstride + (long)j * nlon + i] = buf[t].x * s;
-        grid[f * gstride + (long)j2 * nlon + i] = buf[t].y * s;
+#pragma unroll
+    for (int q = 0; q < NFFT; q++) {
+        const int f = f0 + q;
+        if (f >= nf) break;
+        const T s = rflag[f] ? rc : T(1);
+        T *gn = grid + f * gstride + (long)j * nlon, *gs = grid + f * gstride + (long)j2 * nlon;
+        const cplx<T> *bq = buf + q * NP;
+        for (int i = tid; i < N; i += 256) {
+            const cplx<T> z = bq[fft_pad(i)];
+            gn[i] = z.x * s;
+            gs[i] = z.y * s;
+        }
     }
 }
 
@@ -800,11 +813,11 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
                                                   const cplx<T> *__restrict__ tw, T *__restrict__ fa, int ldf,
                                                   int nlon, int nlat, int nhp, int ntrunc, Radices rd) {
     constexpr int LN = NC ? NC : MAXN;
-    __shared__ cplx<T> buf[NFFT * LN];
+    __shared__ cplx<T> buf[NFFT * fft_pitch(LN)];
     __shared__ cplx<T> tws[LN];
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
-    const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, tid = threadIdx.x;
+    const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
     constexpr int MAXG = (NFFT * LN + 255) / 256;
     T xv[MAXG], yv[MAXG];
 #pragma unroll
@@ -824,7 +837,10 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
 #pragma unroll
     for (int c = 0; c < MAXG; c++) {
         int t = tid + c * 256;
-        if (t < NFFT * N) buf[t] = cplx<T>{xv[c], yv[c]};
+        if (t < NFFT * N) {
+            const int q = t / N;
+            buf[q * NP + fft_pad(t - q * N)] = cplx<T>{xv[c], yv[c]};
+        }
     }
     __syncthreads();
     if constexpr (NC > 0)
@@ -835,7 +851,7 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
     for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
         int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
         if (f >= nf) continue;
-        cplx<T> zk = buf[q * N + m], zn = buf[q * N + (m == 0 ? 0 : N - m)];
+        cplx<T> zk = buf[q * NP + fft_pad(m)], zn = buf[q * NP + fft_pad(m == 0 ? 0 : N - m)];
         T xr = T(0.5) * (zk.x + zn.x), xi = T(0.5) * (zk.y - zn.y);
         T yr = T(0.5) * (zk.y + zn.y), yi = T(-0.5) * (zk.x - zn.x);
         T wf = rflag[f] ? wj * rc : wj;
