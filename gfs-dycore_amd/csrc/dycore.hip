@@ -63,6 +63,7 @@ struct DycoreBuffers {
     void *lin[3] = {nullptr, nullptr, nullptr};  // linear terms: div[nl], tv[nl], lnps per member
     void *tphys = nullptr;                       // physics tendencies
     void *gsyn = nullptr, *gan = nullptr;        // grid fields (T)
+    void *gmean = nullptr;                       // global means added back by the Fourier synthesis (nm * nsyn)
     void *etadot = nullptr, *dlnpdt = nullptr;   // per member (nl+1)*ng, nl*ng
     void *psg = nullptr, *dlnpsdt = nullptr;     // per member ng
     void *dphisdx = nullptr, *dphisdy = nullptr; // per member ng
@@ -72,7 +73,7 @@ struct DycoreBuffers {
     void *at = nullptr, *bt = nullptr, *dt3 = nullptr;  // transposed amhyb, bmhyb, d_hyb_m
     void *sv = nullptr, *tor = nullptr;
     void *disspec = nullptr, *diffp = nullptr, *dmpp = nullptr;
-    void *ak = nullptr, *bk = nullptr, *ck = nullptr, *dbk = nullptr;  // T
+    void *ak = nullptr, *bk = nullptr, *ck = nullptr, *dbk = nullptr, *dak = nullptr;  // T
     void *sinlat = nullptr, *coslat = nullptr, *areaw = nullptr;      // T, nlat; areaw FP64 nlat
     int *kbydeg = nullptr;                       // coefficients sorted by degree
     int2 *si_tiles = nullptr;                    // (degree, first item) tiles of the matrix-core solve
@@ -123,6 +124,66 @@ struct DycoreBuffers {
 // ---------------------------------------------------------------------------
 // kernels
 
+// Layer coefficients of calc_pressdata / getpresgrad for K >= 2 (pressure_data.f90:
+// 76-82, dyn_run.f90:393-410): rlnp = ln(p2/p1), alfa = 1 - p1/dp rlnp,
+// cofa = -(bk_K p1/p2 - bk_{K-1} + rlnp (bk_{K-1} - p1 dbk/dp)) / dp and the cofb
+// increment bk_K / p2 - bk_{K-1} / p1. With thin layers (dp/p of a few per cent)
+// these expressions cancel to a few per cent of their terms; FP64 evaluates
+// them as written, FP32 uses identities exact in real arithmetic that do not
+// cancel (y = rlnp):
+//   rlnp = ln(1 + x) = 2 atanh(u), x = dp / p1, u = x / (2 + x) <= 1/3 for x <= 1,
+//          = 2 (u + u^3/3 + ... + u^15/15) (truncation 5e-9 at u = 1/3),
+//   alfa = 1 - y / (e^y - 1) = y/2 - y^2/12 + y^4/720 - y^6/30240 + y^8/1209600
+//          - y^10/47900160 (Bernoulli series, truncation 5e-10 at y = 1),
+//   bk_K p1/p2 - bk_{K-1} = -ck / p2 and bk_{K-1} - p1 dbk / dp = ck / dp, so
+//   cofa = (ck / dp) (e^-y - 1 + alfa) / p1 = (ck / dp) (alfa - dp / p2) / p1,
+//   bk_K / p2 - bk_{K-1} / p1 = -ck / (p1 p2),
+// with ck = ak_K bk_{K-1} - ak_{K-1} bk_K formed in FP64 on the host. The layer
+// thickness is dp = dak + dbk ps from the FP64 differences of ak and bk (the
+// difference of the rounded interface pressures would carry their rounding,
+// amplified by p/dp).
+template <typename T> struct LayerCoef {
+    T rlnp, alfa, cofa, dbp;  // dbp = bk_K / p2 - bk_{K-1} / p1
+};
+template <typename T>
+__device__ inline LayerCoef<T> layer_coef(T p1, T p2, T dpk, T bk1, T bk2, T ck, T dbk) {
+    LayerCoef<T> c;
+    if constexpr (sizeof(T) == 4) {
+        // hardware reciprocals (1 ulp): these feed products, never a cancellation
+        const T rp1 = __builtin_amdgcn_rcpf(p1), rp2 = __builtin_amdgcn_rcpf(p2), rdp = __builtin_amdgcn_rcpf(dpk);
+        const T x = dpk * rp1;
+        T y;
+        if (x <= T(1)) {
+            const T u = x * __builtin_amdgcn_rcpf(T(2) + x), u2 = u * u;
+            y = T(2) * u *
+                (T(1) + u2 * (T(1.0 / 3.0) +
+                              u2 * (T(0.2) + u2 * (T(1.0 / 7.0) + u2 * (T(1.0 / 9.0) +
+                                                                       u2 * (T(1.0 / 11.0) +
+                                                                             u2 * (T(1.0 / 13.0) + u2 * T(1.0 / 15.0))))))));
+        } else {
+            y = log1p(x);
+        }
+        c.rlnp = y;
+        if (y <= T(1)) {
+            const T y2 = y * y;
+            c.alfa = y * T(0.5) -
+                     y2 * (T(1.0 / 12.0) -
+                           y2 * (T(1.0 / 720.0) -
+                                 y2 * (T(1.0 / 30240.0) - y2 * (T(1.0 / 1209600.0) - y2 * T(1.0 / 47900160.0)))));
+        } else {
+            c.alfa = T(1) - (p1 * rdp) * y;
+        }
+        c.cofa = ck * rdp * (c.alfa - dpk * rp2) * rp1;
+        c.dbp = -ck * rp1 * rp2;
+    } else {
+        c.rlnp = log(p2 / p1);
+        c.alfa = T(1) - (p1 / dpk) * c.rlnp;
+        c.cofa = -(T(1) / dpk) * (bk2 * p1 / p2 - bk1 + c.rlnp * (bk1 - p1 * dbk / dpk));
+        c.dbp = bk2 / p2 - bk1 / p1;
+    }
+    return c;
+}
+
 // Column kernel: calc_pressdata + getomega (+ getpresgrad, getvadv and the
 // tendency assembly unless inverse_only). One thread per grid column and member.
 // Level indices: K counts interfaces/layers top -> bottom (1-based, as pk, dpk,
@@ -132,16 +193,17 @@ __global__ __launch_bounds__(256) void k_columns(
     const T *__restrict__ gsyn, T *__restrict__ gan, T *__restrict__ etadot, T *__restrict__ dlnpdt,
     T *__restrict__ psg_out, T *__restrict__ dlnpsdt_out, const T *__restrict__ dphisdx,
     const T *__restrict__ dphisdy, const T *__restrict__ ak_, const T *__restrict__ bk_,
-    const T *__restrict__ ck_, const T *__restrict__ dbk_, const T *__restrict__ sinlat, int nl, int ntr,
-    int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only,
+    const T *__restrict__ ck_, const T *__restrict__ dbk_, const T *__restrict__ dak_, const T *__restrict__ sinlat,
+    int nl, int ntr, int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only,
     const cplx<T> *__restrict__ spec, int nfs, T *__restrict__ colq) {
-    __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1], ck[MAXLEV], dbk[MAXLEV];
+    __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1], ck[MAXLEV], dbk[MAXLEV], dak[MAXLEV];
     for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
         ak[i] = ak_[i];
         bk[i] = bk_[i];
         if (i < nl) {
             ck[i] = ck_[i];
             dbk[i] = dbk_[i];
+            dak[i] = dak_[i];
         }
     }
     __syncthreads();
@@ -161,12 +223,13 @@ __global__ __launch_bounds__(256) void k_columns(
     const T alfa1 = T(SGLF(0.69314718055994530942)), rlnp1 = T(SGLF(99999.99));
 #define LV(P, L) (P)[(long)((L) - 1) * ng]
     auto pk = [&](int K) { return ak[K - 1] + bk[K - 1] * ps; };
+    auto dp = [&](int K) { return dak[K - 1] + dbk[K - 1] * ps; };  // pk(K + 1) - pk(K)
     // pass 1: column totals of the continuity integrals (getomega db, cb)
     T db = 0, cb = 0;
     for (int K = 1; K <= nl; K++) {
         int L = nl + 1 - K;
         T cg = LV(U, L) * dpx + LV(V, L) * dpy;
-        db += LV(DIV, L) * (pk(K + 1) - pk(K));
+        db += LV(DIV, L) * dp(K);
         cb += cg * dbk[K - 1];
     }
     const T dlnpsdt = -db / ps - cb;
@@ -177,9 +240,13 @@ __global__ __launch_bounds__(256) void k_columns(
     ET[(long)nl * ng] = T(0);
     for (int K = 1; K <= nl; K++) {
         int L = nl + 1 - K;
-        T p1 = pk(K), p2 = pk(K + 1), dpk = p2 - p1;
-        T rlnp = (K == 1) ? rlnp1 : log(p2 / p1);
-        T alfa = (K == 1) ? alfa1 : T(1) - (p1 / dpk) * rlnp;
+        T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K);
+        T rlnp = rlnp1, alfa = alfa1;
+        if (K > 1) {
+            const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
+            rlnp = lc.rlnp;
+            alfa = lc.alfa;
+        }
         T cg = LV(U, L) * dpx + LV(V, L) * dpy;
         T dv = LV(DIV, L);
         T workb, workc;
@@ -201,7 +268,7 @@ __global__ __launch_bounds__(256) void k_columns(
         // column tracer mass of the dry-mass diagnostic (dyn_run.f90:138-145)
         T s = 0;
         for (int K = 1; K <= nl; K++) {
-            T dpk = pk(K + 1) - pk(K), qs = 0;
+            T dpk = dp(K), qs = 0;
             for (int nt = 0; nt < ntr; nt++) qs += Q[((long)nt * nl + (nl - K)) * ng];
             s += dpk * qs;
         }
@@ -217,11 +284,21 @@ __global__ __launch_bounds__(256) void k_columns(
     T cofb_acc = 0, px3u = 0, px3v = 0;
     for (int K = nl; K >= 1; K--) {
         int L = nl + 1 - K;
-        T p1 = pk(K), p2 = pk(K + 1), dpk = p2 - p1;
-        T rlnp = (K == 1) ? rlnp1 : log(p2 / p1);
-        T alfa = (K == 1) ? alfa1 : T(1) - (p1 / dpk) * rlnp;
+        T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K);
+        T rlnp, alfa, cofa, dbp;
+        if (K == 1) {
+            rlnp = rlnp1;
+            alfa = alfa1;
+            cofa = -(T(1) / dpk) * (bk[K] * p1 / p2 - bk[K - 1] + rlnp * (bk[K - 1] - p1 * dbk[K - 1] / dpk));
+            dbp = bk[K] / p2 - bk[K - 1] / p1;
+        } else {
+            const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
+            rlnp = lc.rlnp;
+            alfa = lc.alfa;
+            cofa = lc.cofa;
+            dbp = lc.dbp;
+        }
         T cofb0 = (K == 1) ? -(T(1) / dpk) * (alfa * dbk[0]) : -(T(1) / dpk) * (bk[K - 1] * rlnp + alfa * dbk[K - 1]);
-        T cofa = -(T(1) / dpk) * (bk[K] * p1 / p2 - bk[K - 1] + rlnp * (bk[K - 1] - p1 * dbk[K - 1] / dpk));
         T tv = LV(TV, L), tx = LV(TX, L), ty = LV(TY, L);
         T prsgx = cofb0 * rd * tv * ps * dpx - dphx + cofb_acc * ps * dpx + px3u - rd * alfa * tx -
                   cofa * rd * tv * ps * dpx;
@@ -257,7 +334,7 @@ __global__ __launch_bounds__(256) void k_columns(
         LV(FY, L) = v * z - vu;
         LV(KE, L) = T(0.5) * (u * u + v * v);
         // accumulations for the layer above (getpresgrad cofb and px3 recursions)
-        cofb_acc -= rd * (bk[K] / p2 - bk[K - 1] / p1) * tv;
+        cofb_acc -= rd * dbp * tv;
         px3u -= rd * rlnp * tx;
         px3v -= rd * rlnp * ty;
     }
@@ -277,7 +354,7 @@ __global__ __launch_bounds__(256) void k_columns(
         T hprev = Dk;                                                                // half_0
         for (int K = 1; K <= nl; K++) {
             const T etop = ET[(long)(K - 1) * ng], ebot = ET[(long)K * ng];
-            const T dpk = pk(K + 1) - pk(K);
+            const T dpk = dp(K);
             T hk, Dk2 = T(0), ddn = T(0);
             if (K <= nl - 1) {
                 if (K + 1 <= nl - 1) {
@@ -311,6 +388,31 @@ __global__ __launch_bounds__(256) void k_columns(
 #undef LV
 }
 
+// Phillips layer pressure ((p2^(k+1) - p1^(k+1)) / ((k+1)(p2 - p1)))^(1/k) of
+// pressure_data.f90:87-88. In FP32 the difference of powers cancels (a layer is
+// a few per cent thick), losing 3e-5 relative; float therefore evaluates the
+// same quantity as sqrt(p1 p2) exp(S(y)), y = ln(p2 / p1), from the series
+// ln((e^z - 1) / z) = z/2 + sum_j B_2j z^2j / (2j (2j)!) taken at z = (k+1) y and
+// z = y (B_2j Bernoulli numbers; five terms, truncation 3e-9 at y = 1), which
+// keeps it to a few ulp. The top layer (p1 = 0) has no cancellation.
+template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa) {
+    if constexpr (sizeof(T) == 4) {
+        if (p1 > T(0)) {
+            const T y = log1p((p2 - p1) / p1);
+            if (y <= T(1)) {
+                const T a = kappa + T(1), a2 = a * a, a4 = a2 * a2, a6 = a4 * a2, a8 = a4 * a4, a10 = a8 * a2;
+                const T y2 = y * y;
+                const T d1 = (a2 - T(1)) / (T(24) * kappa), d2 = -(a4 - T(1)) / (T(2880) * kappa);
+                const T d3 = (a6 - T(1)) / (T(181440) * kappa), d4 = -(a8 - T(1)) / (T(9676800) * kappa);
+                const T d5 = (a10 - T(1)) / (T(479001600) * kappa);
+                const T S = y2 * (d1 + y2 * (d2 + y2 * (d3 + y2 * (d4 + y2 * d5))));
+                return sqrt(p1 * p2) * exp(S);
+            }
+        }
+    }
+    return pow((pow(p2, kappa) * p2 - pow(p1, kappa) * p1) / ((kappa + T(1)) * (p2 - p1)), T(1) / kappa);
+}
+
 // Held-Suarez forcing on the post-dynamics grid fields (phy_run_hs.f90:42-77)
 // and the dry-mass-fixer field log(ps * pcorr) (dyn_run.f90:626-631).
 template <typename T>
@@ -341,9 +443,8 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
     const T expo = T(SGLF(2.f / 7.f));
     for (int L = 1; L <= nl; L++) {
         int K = nl + 1 - L;
-        T p1 = ak[K - 1] + bk[K - 1] * ps, p2 = ak[K] + bk[K] * ps, dpk = p2 - p1;
-        // Phillips layer pressure, pressure_data.f90:87-88
-        T prs = pow((pow(p2, kappa) * p2 - pow(p1, kappa) * p1) / ((kappa + T(1)) * dpk), T(1) / kappa);
+        T p1 = ak[K - 1] + bk[K - 1] * ps, p2 = ak[K] + bk[K] * ps;
+        T prs = phillips_prs(p1, p2, kappa);
         T blprof = prs / ps;
         T teq = pow(prs / p0, expo) * (T(315) - deltmp * sl * sl - delthz * log(prs / p0) * cl2);
         blprof = (blprof - sigbot) / (T(1) - sigbot);
@@ -507,8 +608,7 @@ __global__ __launch_bounds__(256) void k_simple_columns(
     auto pint = [&](int c) { return (double)(ak[c - 1] + bk[c - 1] * ps); };
     auto pmid = [&](int c) {
         // Phillips layer pressure of pressure_data.f90:87-88 in the grid precision
-        const T p1 = ak[c - 1] + bk[c - 1] * ps, p2 = ak[c] + bk[c] * ps;
-        return (double)pow((pow(p2, kappa) * p2 - pow(p1, kappa) * p1) / ((kappa + T(1)) * (p2 - p1)), T(1) / kappa);
+        return (double)phillips_prs(ak[c - 1] + bk[c - 1] * ps, ak[c] + bk[c] * ps, kappa);
     };
     auto tin = [&](int c) {  // T = Tv / (1 + fv max(qmin, q)) (phy_run_simple.f90:56)
         const long o = lvl(c);
@@ -1646,6 +1746,11 @@ template <typename T> int Engine<T>::dyn_init() {
     if ((rc = dupload<T>(&B->ck, castv<T>(ck), stream))) return rc;
     if ((rc = dupload<T>(&B->dbk, castv<T>(dbk), stream))) return rc;
     {
+        std::vector<double> dak(nl);
+        for (int k = 0; k < nl; k++) dak[k] = ak[k + 1] - ak[k];
+        if ((rc = dupload<T>(&B->dak, castv<T>(dak), stream))) return rc;
+    }
+    {
         std::vector<double> sl(plan.nlat), area(plan.nlat);
         double tot = 0;
         for (int j = 0; j < plan.nlat; j++) tot += plan.w[j] * plan.nlon;
@@ -1691,6 +1796,7 @@ template <typename T> int Engine<T>::dyn_init() {
     B->tphys = dalloc<cplx<T>>(nspec);
     B->topo = dalloc<cplx<T>>((size_t)nlm * nm);
     B->gsyn = dalloc<T>((size_t)nm * B->nsyn * ng);
+    B->gmean = dalloc<T>((size_t)nm * B->nsyn);
     B->gan = dalloc<T>((size_t)nm * B->nan * ng);
     B->etadot = dalloc<T>((size_t)nm * (nl + 1) * ng);
     B->dlnpdt = dalloc<T>((size_t)nm * nl * ng);
@@ -1710,7 +1816,7 @@ template <typename T> int Engine<T>::dyn_init() {
     void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
                     B->tphys, B->topo, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
                     B->dphisdy, B->pdryini, B->pmean, B->diag, B->pspart, B->pwsum, B->colq, B->pwat,
-                    B->precip, B->pblscr};
+                    B->precip, B->pblscr, B->gmean};
     for (void *p : ptrs)
         if (!p) {
             set_error("device allocation failed for the dycore state");
@@ -1724,6 +1830,7 @@ template <typename T> int Engine<T>::dyn_init() {
     GFSDC_CHECK(hipMemsetAsync(B->pwat, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->precip, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->pwsum, 0, sizeof(double) * nm, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->gmean, 0, sizeof(T) * nm * B->nsyn, stream));
     Work<T> *W = new Work<T>();
     B->Work = W;
     if ((rc = work_alloc(*W, plan, nm * std::max(B->nsyn, B->nan)))) return rc;
@@ -1782,7 +1889,7 @@ template <typename T> int Engine<T>::dyn_init() {
             d[nl + k] = SynthDesc{SK_VHAT, vrt + k, div + k, a, a, 1, 1};      // vg
             d[2 * nl + k] = SynthDesc{SK_COPY, vrt + k, -1, 1.0, 0.0, 0, 0};   // vrtg
             d[3 * nl + k] = SynthDesc{SK_COPY, div + k, -1, 1.0, 0.0, 0, 0};   // divg
-            d[4 * nl + k] = SynthDesc{SK_COPY, tv + k, -1, 1.0, 0.0, 0, 0};    // virtempg
+            d[4 * nl + k] = SynthDesc{SK_COPY_MEAN, tv + k, -1, 1.0, 0.0, 0, 0};  // virtempg
             d[5 * nl + k] = SynthDesc{SK_UHAT, -1, tv + k, 0.0, ra_, 0, 0};    // dvirtempdx (getgrad)
             d[6 * nl + k] = SynthDesc{SK_VHAT, -1, tv + k, 0.0, ra_, 0, 0};    // dvirtempdy
             r[k] = r[nl + k] = r[5 * nl + k] = r[6 * nl + k] = 1;
@@ -1870,7 +1977,7 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
-                    B->jflag, B->jctr};
+                    B->jflag, B->jctr, B->gmean, B->dak};
     if (B->mlp) mlp_destroy(B->mlp);
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -1900,11 +2007,12 @@ template <typename T> struct Step {
     void synth_all() {
         const Plan<T> &p = E.plan;
         int nf = B.nm * B.nsyn;
-        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc, nf, W.sx, W.ldc, 1.0, s);
+        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc, nf, W.sx, W.ldc, 1.0, s,
+                            (T *)B.gmean);
         mark("synth_gather");
         launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
         mark("leg_synth");
-        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_syn, s);
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_syn, s, (const T *)B.gmean);
         mark("fft_synth");
     }
     void columns(int inverse_only, bool colq = false) {
@@ -1913,7 +2021,7 @@ template <typename T> struct Step {
         hipLaunchKernelGGL(k_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (T *)B.gan, (T *)B.etadot,
                            (T *)B.dlnpdt, (T *)B.psg, (T *)B.dlnpsdt, (const T *)B.dphisdx, (const T *)B.dphisdy,
                            (const T *)B.ak, (const T *)B.bk, (const T *)B.ck, (const T *)B.dbk,
-                           (const T *)B.sinlat, B.nl, B.ntr, B.nsyn, B.nan, B.ng, E.plan.nlon, (T)c.rd,
+                           (const T *)B.dak, (const T *)B.sinlat, B.nl, B.ntr, B.nsyn, B.nan, B.ng, E.plan.nlon, (T)c.rd,
                            (T)(c.rd / c.cp), (T)(c.cvap / c.cp - 1.0), (T)(2.0 * c.omega), inverse_only,
                            (const cplx<T> *)B.spec, B.nfs, colq ? (T *)B.colq : (T *)nullptr);
         mark(inverse_only ? "columns_inverse" : "columns");
@@ -2147,7 +2255,8 @@ template <typename T> int Engine<T>::autotune(double *ms, int *best) {
         float t = 0.f;
         for (int rep = 0; rep < 2; rep++) {  // first pass warms up
             GFSDC_CHECK(hipEventRecord(e0, stream));
-            launch_fft_synth(plan, W.fs, W.ldc, (T *)B->gsyn, B->ng, B->nm * B->nsyn, B->rflag_syn, stream);
+            launch_fft_synth(plan, W.fs, W.ldc, (T *)B->gsyn, B->ng, B->nm * B->nsyn, B->rflag_syn, stream,
+                             (const T *)B->gmean);
             launch_fft_anal(plan, (const T *)B->gan, B->ng, B->nm * B->nan, B->rflag_an, W.fa, W.ldc, stream);
             GFSDC_CHECK(hipEventRecord(e1, stream));
             GFSDC_CHECK(hipEventSynchronize(e1));

@@ -64,7 +64,11 @@ template <typename T> void plan_destroy(Plan<T> &p);
 // Descriptors of the synthesis gather (spectral state -> extended spectral
 // columns) and of the analysis scatter (extended columns -> spectral output).
 // SK_COPY_DEV copies a field without its global mean (n = 0) coefficient.
-enum SynthKind : int { SK_ZERO = 0, SK_COPY = 1, SK_UHAT = 2, SK_VHAT = 3, SK_COPY_DEV = 4 };
+// SK_COPY_MEAN does the same and hands the mean, q_00 / sqrt(4 pi), to the
+// Fourier synthesis, which adds it to every grid point after the transform: the
+// Legendre sums and FFT stages then round relative to the field's deviation
+// (a few K for temperature) instead of its mean (300 K).
+enum SynthKind : int { SK_ZERO = 0, SK_COPY = 1, SK_UHAT = 2, SK_VHAT = 3, SK_COPY_DEV = 4, SK_COPY_MEAN = 5 };
 struct SynthDesc {
     int kind;
     int a, b;          // source fields (S and T of the vector pair; a for COPY)
@@ -88,12 +92,12 @@ struct AnalDesc {
 // Spectral source element (field f, coefficient k) sits at src[f*sf + k*sk] (complex).
 template <typename T>
 void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc,
-                         int nfo, T *sx, int ldx, double rsphere, hipStream_t s);
+                         int nfo, T *sx, int ldx, double rsphere, hipStream_t s, T *mean = nullptr);
 template <typename T>
 void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s);
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                      const unsigned char *rcos_flag, hipStream_t s);
+                      const unsigned char *rcos_flag, hipStream_t s, const T *mean = nullptr);
 template <typename T>
 void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rcos_flag,
                      T *fa, int ldf, hipStream_t s);

@@ -236,7 +236,6 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
             O[a][b] = acc_t{0, 0, 0, 0};
         }
     V ra[2][NA], rb[2][NB];
-    const V zero{};
     auto load = [&](int k0) {
 #pragma unroll
         for (int par = 0; par < 2; par++) {
@@ -342,7 +341,6 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
             O[a][b] = acc_t{0, 0, 0, 0};
         }
     V ra[2][NA], rb[2][NB];
-    const V zero{};
     auto load = [&](int j0) {
 #pragma unroll
         for (int par = 0; par < 2; par++) {
@@ -461,7 +459,6 @@ __global__ __launch_bounds__(256) void k_leg_synth_b3(const float *__restrict__ 
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) E[a][b] = O[a][b] = b3f4{0.f, 0.f, 0.f, 0.f};
-    const float4 zero{0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < Ke; k0 += B3K) {
 #pragma unroll
         for (int par = 0; par < 2; par++) {
@@ -534,7 +531,6 @@ __global__ __launch_bounds__(256) void k_leg_anal_b3(const float *__restrict__ p
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) E[a][b] = O[a][b] = b3f4{0.f, 0.f, 0.f, 0.f};
-    const float4 zero{0.f, 0.f, 0.f, 0.f};
     for (int j0 = 0; j0 < nhp; j0 += B3K) {
 #pragma unroll
         for (int par = 0; par < 2; par++) {
@@ -740,8 +736,9 @@ static_assert(fft_pitch(FftCfg<double>::MAXN) * 8 * 16 + FftCfg<double>::MAXN * 
 template <typename T, int NFFT, int MAXN, int NC>
 __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int ldf, T *__restrict__ grid,
                                                    long gstride, int nf, const unsigned char *__restrict__ rflag,
-                                                   const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
-                                                   int nlon, int nlat, int ntrunc, Radices rd) {
+                                                   const T *__restrict__ mean, const T *__restrict__ rcos,
+                                                   const cplx<T> *__restrict__ tw, int nlon, int nlat, int ntrunc,
+                                                   Radices rd) {
     constexpr int LN = NC ? NC : MAXN;
     __shared__ cplx<T> buf[NFFT * fft_pitch(LN)];
     __shared__ cplx<T> tws[LN];
@@ -794,12 +791,13 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
         const int f = f0 + q;
         if (f >= nf) break;
         const T s = rflag[f] ? rc : T(1);
+        const T c = mean ? mean[f] : T(0);
         T *gn = grid + f * gstride + (long)j * nlon, *gs = grid + f * gstride + (long)j2 * nlon;
         const cplx<T> *bq = buf + q * NP;
         for (int i = tid; i < N; i += 256) {
             const cplx<T> z = bq[fft_pad(i)];
-            gn[i] = z.x * s;
-            gs[i] = z.y * s;
+            gn[i] = z.x * s + c;
+            gs[i] = z.y * s + c;
         }
     }
 }
@@ -880,16 +878,22 @@ template <typename T>
 __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk, const SynthDesc *__restrict__ desc,
                                int nfo, T *__restrict__ sx, int ldx, const int2 *__restrict__ kemn,
                                const int *__restrict__ koff, const T *__restrict__ eps, const int *__restrict__ eoff,
-                               int ntrunc, int nke) {
+                               int ntrunc, int nke, T *__restrict__ mean) {
     const int fo = blockIdx.x * blockDim.x + threadIdx.x;
     if (fo >= nfo) return;
     const SynthDesc d = desc[fo];
+    if (mean && blockIdx.y == 0) {
+        // global mean of an SK_COPY_MEAN field (Y_0^0 = 1 / sqrt(4 pi)); zero for the others
+        T c = T(0);
+        if (d.kind == SK_COPY_MEAN) c = T((double)src[(long)d.a * sf].x * d.sa * 0.28209479177387814);
+        mean[fo] = c;
+    }
     // GATHER_KE extended coefficients per workgroup: one descriptor load, GATHER_KE rows
     for (int ke = blockIdx.y * GATHER_KE; ke < min(nke, (blockIdx.y + 1) * GATHER_KE); ke++) {
         const int2 mn = kemn[ke];
         const int m = mn.x, n = mn.y;
         cplx<T> r{T(0), T(0)};
-        if (d.kind == SK_COPY || (d.kind == SK_COPY_DEV && n > 0)) {
+        if (d.kind == SK_COPY || ((d.kind == SK_COPY_DEV || d.kind == SK_COPY_MEAN) && n > 0)) {
             r = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
         } else if (d.kind == SK_UHAT || d.kind == SK_VHAT) {
             const T *ep = eps + eoff[m];
@@ -1145,11 +1149,11 @@ template <typename T> void work_free(Work<T> &w) {
 
 template <typename T>
 void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc, int nfo,
-                         T *sx, int ldx, double rsphere, hipStream_t s) {
+                         T *sx, int ldx, double rsphere, hipStream_t s, T *mean) {
     (void)rsphere;
     dim3 grid((nfo + 255) / 256, (p.nlme + GATHER_KE - 1) / GATHER_KE);
     hipLaunchKernelGGL(k_synth_gather<T>, grid, dim3(256), 0, s, src, sf, sk, desc, nfo, sx, ldx, p.kemn, p.koff,
-                       p.eps, p.eoff, p.ntrunc, p.nlme);
+                       p.eps, p.eoff, p.ntrunc, p.nlme, mean);
 }
 
 template <typename T>
@@ -1189,12 +1193,12 @@ static Radices radices_of(int n, const int *r) {
 
 template <typename T, int NF>
 static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                         const unsigned char *rflag, hipStream_t s) {
+                         const unsigned char *rflag, const T *mean, hipStream_t s) {
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_SYN(NC)                                                                                             \
-    hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, p.rcos, \
-                       (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad))
+    hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, mean,   \
+                       p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad))
     switch (p.nlon) {
     case 768: FFT_SYN(768); break;
     case 384: FFT_SYN(384); break;
@@ -1226,11 +1230,11 @@ static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, c
 // more workgroups per CU (LDS = fields * nlon * sizeof(complex))
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                      const unsigned char *rflag, hipStream_t s) {
+                      const unsigned char *rflag, hipStream_t s, const T *mean) {
     switch (p.fft_nf) {
-    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, s); break;
-    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, s); break;
-    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, s); break;
+    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, s); break;
+    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, s); break;
+    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, s); break;
     }
 }
 
@@ -1258,11 +1262,11 @@ void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc 
     template int work_alloc<T>(Work<T> &, const Plan<T> &, int);                                               \
     template void work_free<T>(Work<T> &);                                                                     \
     template void launch_synth_gather<T>(const Plan<T> &, const cplx<T> *, long, long, const SynthDesc *, int,  \
-                                         T *, int, double, hipStream_t);                                       \
+                                         T *, int, double, hipStream_t, T *);                                  \
     template void launch_leg_synth<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);            \
     template void launch_leg_anal<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);             \
     template void launch_fft_synth<T>(const Plan<T> &, const T *, int, T *, long, int, const unsigned char *,  \
-                                      hipStream_t);                                                            \
+                                      hipStream_t, const T *);                                                 \
     template void launch_fft_anal<T>(const Plan<T> &, const T *, long, int, const unsigned char *, T *, int,   \
                                      hipStream_t);                                                             \
     template void launch_anal_scatter<T>(const Plan<T> &, const T *, int, const AnalDesc *, int, cplx<T> *,    \
