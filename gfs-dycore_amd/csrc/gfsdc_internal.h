@@ -46,7 +46,7 @@ template <typename T> struct Plan {
     int *eoff = nullptr;     // device, offset of order m in eps (T+3-m entries)
     T *rcos = nullptr;       // device, 1/cos(lat), nlat
     T *wfac = nullptr;       // device, 2 pi w_j / nlon, nlat
-    T *tw = nullptr;         // device, complex twiddles exp(-2 pi i t/nlon), 2*nlon
+    T *tw = nullptr;         // device, complex FFT twiddles in stage order (see plan_create), 2*nlon
     int *mtile_m = nullptr;  // device, analysis m-tile -> m
     int *mtile_i = nullptr;  // device, analysis m-tile -> first row
     int n_mtiles = 0;

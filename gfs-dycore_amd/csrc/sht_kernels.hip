@@ -134,9 +134,11 @@ __global__ void k_legendre_table(T *plm, const long *poff, const double *mu, int
 }
 
 // ---------------------------------------------------------------------------
-// Legendre GEMMs. One workgroup = one order m, a 64-row x 128-column output tile
-// for both parities of n - m; 4 waves in a 2x2 arrangement, each wave 32 x 64
-// via 2 x 4 MFMA 16x16x4 blocks per parity. Global loads are 16-byte vectors,
+// Legendre GEMMs. One workgroup = one order m, a 64-row x 128-column output tile;
+// 4 waves in a 2x2 arrangement, each wave 32 x 64 via 2 x 4 MFMA 16x16x4 blocks.
+// The synthesis gives each parity of n - m its own workgroup (half the
+// accumulators: 7 waves per SIMD instead of 3); the analysis keeps both
+// parities in one workgroup, which measured faster there. Global loads are 16-byte vectors,
 // prefetched into registers for chunk k+1 while chunk k is computed from LDS.
 // FP32 swaps the MFMA operands (C^T = B^T A^T) so the four accumulators of a lane
 // are four consecutive columns of one output row and the epilogue stores 16-byte
@@ -175,110 +177,107 @@ __device__ inline float4 vkeep(const float4 &x, bool keep) {
 }
 __device__ inline double2 vkeep(const double2 &x, bool keep) { return double2{keep ? x.x : 0.0, keep ? x.y : 0.0}; }
 
-template <typename T, typename Acc>
-__device__ inline void gemm_chunk(T (*As)[GEMM_BK][GEMM_BM + LDS_PAD], T (*Bs)[GEMM_BK][GT<T>::BN + LDS_PAD],
-                                  Acc (&E)[2][GT<T>::WB], Acc (&O)[2][GT<T>::WB], int lane, int wm, int wn) {
+// One LDS chunk of a wave's 32 x (16 WB) tile. GUARD skips (wave-uniformly) the
+// k-steps of 4 at or beyond kmax and the 16-column blocks at or beyond nbj, so a
+// partial last chunk or column tile issues only the MFMAs it needs.
+template <typename T, bool GUARD, typename Acc>
+__device__ inline void gemm_chunk(T (*As)[GEMM_BM + LDS_PAD], T (*Bs)[GT<T>::BN + LDS_PAD], Acc (&E)[2][GT<T>::WB],
+                                  int lane, int wm, int wn, int kmax = GEMM_BK, int nbj = GT<T>::WB) {
 #pragma unroll
     for (int ks = 0; ks < GEMM_BK; ks += 4) {
+        if (GUARD && ks >= kmax) break;
         const int kr = ks + (lane >> 4), li = lane & 15;
-        T ae[2], ao[2];
+        T a[2];
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++) {
-            ae[bi] = As[0][kr][wm + bi * 16 + li];
-            ao[bi] = As[1][kr][wm + bi * 16 + li];
-        }
+        for (int bi = 0; bi < 2; bi++) a[bi] = As[kr][wm + bi * 16 + li];
 #pragma unroll
         for (int bj = 0; bj < GT<T>::WB; bj++) {
-            T be = Bs[0][kr][wn + bj * 16 + li];
-            T bo = Bs[1][kr][wn + bj * 16 + li];
+            if (GUARD && bj >= nbj) break;
+            T b = Bs[kr][wn + bj * 16 + li];
 #pragma unroll
             for (int bi = 0; bi < 2; bi++) {
-                if (Vec<T>::swap) {
-                    E[bi][bj] = MF<T>::mma(be, ae[bi], E[bi][bj]);
-                    O[bi][bj] = MF<T>::mma(bo, ao[bi], O[bi][bj]);
-                } else {
-                    E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
-                    O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
-                }
+                if (Vec<T>::swap)
+                    E[bi][bj] = MF<T>::mma(b, a[bi], E[bi][bj]);
+                else
+                    E[bi][bj] = MF<T>::mma(a[bi], b, E[bi][bj]);
             }
         }
     }
 }
 
-// Synthesis: fs[m][j][c] = E + O (north row j), fs[m][nlat-1-j][c] = E - O.
+// Synthesis, one parity per workgroup (blockIdx.y = 2 (m ntj + tj) + parity):
+// the even-degree sum E_j goes to row j of fs[m], the odd-degree sum O_j to row
+// nlat-1-j; the Fourier synthesis forms the hemispheres E + O (north) and E - O
+// (south) as it loads them. Each workgroup holds one parity's accumulators only.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, const long *__restrict__ poff,
                                                    const int *__restrict__ keoff, const T *__restrict__ sx,
                                                    int ldx, T *__restrict__ fs, int ldf, int nhalf, int nhp,
-                                                   int nlat, int ntrunc, int ntj) {
+                                                   int nlat, int ntrunc, int ntj, int ncol) {
     typedef typename MF<T>::acc acc_t;
     typedef typename Vec<T>::t V;
     constexpr int VW = Vec<T>::n;
     constexpr int BN = GT<T>::BN, WB = GT<T>::WB;
     constexpr int NA = GEMM_BK * GEMM_BM / VW / 256, NB = GEMM_BK * BN / VW / 256;
     const int tc = blockIdx.x;
-    const int mt = blockIdx.y;
+    const int par = blockIdx.y & 1, mt = blockIdx.y >> 1;
     const int m = mt / ntj, tj = mt - m * ntj;
-    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
-    const T *P0 = plm + poff[m], *P1 = P0 + (long)Ke * nhp;
-    const T *S = sx + (long)keoff[m] * ldx;
+    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1, Kp = par ? Ko : Ke;
+    const T *P = plm + poff[m] + (par ? (long)Ke * nhp : 0L);
+    const T *S = sx + (long)keoff[m] * ldx + (long)par * ldx;
     const int j0 = tj * GEMM_BM, c0 = tc * BN;
-    __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
-    __shared__ T Bs[2][GEMM_BK][BN + LDS_PAD];
+    __shared__ T As[GEMM_BK][GEMM_BM + LDS_PAD];
+    __shared__ T Bs[GEMM_BK][BN + LDS_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * (BN / 2);
-    acc_t E[2][WB], O[2][WB];
+    acc_t E[2][WB];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < WB; b++) {
-            E[a][b] = acc_t{0, 0, 0, 0};
-            O[a][b] = acc_t{0, 0, 0, 0};
-        }
-    V ra[2][NA], rb[2][NB];
+        for (int b = 0; b < WB; b++) E[a][b] = acc_t{0, 0, 0, 0};
+    V ra[NA], rb[NB];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int par = 0; par < 2; par++) {
+        for (int u = 0; u < NA; u++) {
+            int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW, i = k0 + kk;
+            // clamped in-bounds load, zeroed by value (no select between pointers)
+            V x = *(const V *)(P + (long)min(i, Kp - 1) * nhp + j0 + jj);
+            ra[u] = vkeep(x, i < Kp);
+        }
 #pragma unroll
-            for (int u = 0; u < NA; u++) {
-                int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW, i = k0 + kk;
-                // clamped in-bounds load, zeroed by value (no select between pointers)
-                V x = *(const V *)((par ? P1 : P0) + (long)min(i, (par ? Ko : Ke) - 1) * nhp + j0 + jj);
-                ra[par][u] = vkeep(x, i < (par ? Ko : Ke));
-            }
-#pragma unroll
-            for (int u = 0; u < NB; u++) {
-                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW, i = k0 + kk;
-                V x = *(const V *)(S + (long)(2 * min(i, (par ? Ko : Ke) - 1) + par) * ldx + c0 + cc);
-                rb[par][u] = vkeep(x, i < (par ? Ko : Ke));
-            }
+        for (int u = 0; u < NB; u++) {
+            int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW, i = k0 + kk;
+            V x = *(const V *)(S + (long)(2 * min(i, Kp - 1)) * ldx + c0 + cc);
+            rb[u] = vkeep(x, i < Kp);
         }
     };
     auto store = [&]() {
 #pragma unroll
-        for (int par = 0; par < 2; par++) {
+        for (int u = 0; u < NA; u++) {
+            int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW;
+            vstore_lds<T>(&As[kk][jj], ra[u]);
+        }
 #pragma unroll
-            for (int u = 0; u < NA; u++) {
-                int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW;
-                vstore_lds<T>(&As[par][kk][jj], ra[par][u]);
-            }
-#pragma unroll
-            for (int u = 0; u < NB; u++) {
-                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
-                vstore_lds<T>(&Bs[par][kk][cc], rb[par][u]);
-            }
+        for (int u = 0; u < NB; u++) {
+            int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
+            vstore_lds<T>(&Bs[kk][cc], rb[u]);
         }
     };
     // FP32 prefetches chunk k+1 during chunk k; FP64 (twice the registers per
     // element) loads each chunk just before storing it to keep occupancy
     constexpr bool PREFETCH = sizeof(T) == 4;
-    if (PREFETCH) load(0);
-    for (int k0 = 0; k0 < Ke; k0 += GEMM_BK) {
+    // valid 16-column blocks of this wave (the last column tile is partial)
+    const int nbj = min(WB, max(0, (ncol - c0 - wn + 15) / 16));
+    if (PREFETCH && Kp > 0) load(0);
+    for (int k0 = 0; k0 < Kp; k0 += GEMM_BK) {
         if (!PREFETCH) load(k0);
         store();
         __syncthreads();
-        if (PREFETCH && k0 + GEMM_BK < Ke) load(k0 + GEMM_BK);
-        gemm_chunk<T>(As, Bs, E, O, lane, wm, wn);
+        if (PREFETCH && k0 + GEMM_BK < Kp) load(k0 + GEMM_BK);
+        if (nbj == WB && k0 + GEMM_BK <= Kp)
+            gemm_chunk<T, false>(As, Bs, E, lane, wm, wn);
+        else
+            gemm_chunk<T, true>(As, Bs, E, lane, wm, wn, Kp - k0, nbj);
         __syncthreads();
     }
     T *FN = fs + (long)m * nlat * ldf;
@@ -290,24 +289,54 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
                 int j = j0 + wm + bi * 16 + (lane & 15);
                 int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
                 if (j < nhalf) {
-                    acc_t e = E[bi][bj], o = O[bi][bj];
-                    *(float4 *)(FN + (long)j * ldf + c) = float4{e[0] + o[0], e[1] + o[1], e[2] + o[2], e[3] + o[3]};
-                    *(float4 *)(FN + (long)(nlat - 1 - j) * ldf + c) =
-                        float4{e[0] - o[0], e[1] - o[1], e[2] - o[2], e[3] - o[3]};
+                    acc_t e = E[bi][bj];
+                    *(float4 *)(FN + (long)(par ? nlat - 1 - j : j) * ldf + c) = float4{e[0], e[1], e[2], e[3]};
                 }
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     int j = j0 + wm + bi * 16 + MF<T>::row(lane, r);
                     int c = c0 + wn + bj * 16 + (lane & 15);
-                    if (j < nhalf) {
-                        T e = E[bi][bj][r], o = O[bi][bj][r];
-                        FN[(long)j * ldf + c] = e + o;
-                        FN[(long)(nlat - 1 - j) * ldf + c] = e - o;
-                    }
+                    if (j < nhalf) FN[(long)(par ? nlat - 1 - j : j) * ldf + c] = E[bi][bj][r];
                 }
             }
         }
+}
+
+// Both parities of one LDS chunk (analysis). GUARD skips, wave-uniformly, the
+// 16-row blocks of E (O) at or beyond ne (no) and the 16-column blocks at or
+// beyond nbj: the rows past the last degree of a parity and the columns past
+// the last field of a partial tile.
+template <typename T, bool GUARD, typename Acc>
+__device__ inline void gemm_chunk2(T (*As)[GEMM_BK][GEMM_BM + LDS_PAD], T (*Bs)[GEMM_BK][GT<T>::BN + LDS_PAD],
+                                   Acc (&E)[2][GT<T>::WB], Acc (&O)[2][GT<T>::WB], int lane, int wm, int wn,
+                                   int ne = 2, int no = 2, int nbj = GT<T>::WB) {
+#pragma unroll
+    for (int ks = 0; ks < GEMM_BK; ks += 4) {
+        const int kr = ks + (lane >> 4), li = lane & 15;
+        T ae[2], ao[2];
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++) {
+            ae[bi] = As[0][kr][wm + bi * 16 + li];
+            ao[bi] = As[1][kr][wm + bi * 16 + li];
+        }
+#pragma unroll
+        for (int bj = 0; bj < GT<T>::WB; bj++) {
+            if (GUARD && bj >= nbj) break;
+            T be = Bs[0][kr][wn + bj * 16 + li];
+            T bo = Bs[1][kr][wn + bj * 16 + li];
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) {
+                if (Vec<T>::swap) {
+                    if (!GUARD || bi < ne) E[bi][bj] = MF<T>::mma(be, ae[bi], E[bi][bj]);
+                    if (!GUARD || bi < no) O[bi][bj] = MF<T>::mma(bo, ao[bi], O[bi][bj]);
+                } else {
+                    if (!GUARD || bi < ne) E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
+                    if (!GUARD || bi < no) O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
+                }
+            }
+        }
+    }
 }
 
 // Analysis: output degrees i' of each parity, K over the (zero-padded) latitude
@@ -316,7 +345,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, const long *__restrict__ poff,
                                                   const int *__restrict__ keoff, const int *__restrict__ mtile_m,
                                                   const int *__restrict__ mtile_i, const T *__restrict__ fa,
-                                                  int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc) {
+                                                  int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc, int ncol) {
     typedef typename MF<T>::acc acc_t;
     typedef typename Vec<T>::t V;
     constexpr int VW = Vec<T>::n;
@@ -376,13 +405,20 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
     // FP32 prefetches chunk k+1 during chunk k; FP64 (twice the registers per
     // element) loads each chunk just before storing it to keep occupancy
     constexpr bool PREFETCH = sizeof(T) == 4;
+    // valid 16-row blocks per parity and 16-column blocks of this wave
+    const int ne = min(2, max(0, (Ke - i0 - wm + 15) / 16)), no = min(2, max(0, (Ko - i0 - wm + 15) / 16));
+    const int nbj = min(WB, max(0, (ncol - c0 - wn + 15) / 16));
+    const bool full = ne == 2 && no == 2 && nbj == WB;
     if (PREFETCH) load(0);
     for (int j0 = 0; j0 < nhp; j0 += GEMM_BK) {
         if (!PREFETCH) load(j0);
         store();
         __syncthreads();
         if (PREFETCH && j0 + GEMM_BK < nhp) load(j0 + GEMM_BK);
-        gemm_chunk<T>(As, Bs, E, O, lane, wm, wn);
+        if (full)
+            gemm_chunk2<T, false>(As, Bs, E, O, lane, wm, wn);
+        else
+            gemm_chunk2<T, true>(As, Bs, E, O, lane, wm, wn, ne, no, nbj);
         __syncthreads();
     }
     T *X = sx + (long)keoff[m] * ldx;
@@ -503,11 +539,10 @@ __global__ __launch_bounds__(256) void k_leg_synth_b3(const float *__restrict__ 
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) {
             const int j = j0 + wm + bi * 16 + (lane & 15), c = c0 + wn + bj * 16 + 4 * (lane >> 4);
-            if (j < nhalf) {
+            if (j < nhalf) {  // even sum to row j, odd sum to row nlat-1-j (as k_leg_synth)
                 const b3f4 e = E[bi][bj], o = O[bi][bj];
-                *(float4 *)(FN + (long)j * ldf + c) = float4{e[0] + o[0], e[1] + o[1], e[2] + o[2], e[3] + o[3]};
-                *(float4 *)(FN + (long)(nlat - 1 - j) * ldf + c) =
-                    float4{e[0] - o[0], e[1] - o[1], e[2] - o[2], e[3] - o[3]};
+                *(float4 *)(FN + (long)j * ldf + c) = float4{e[0], e[1], e[2], e[3]};
+                *(float4 *)(FN + (long)(nlat - 1 - j) * ldf + c) = float4{o[0], o[1], o[2], o[3]};
             }
         }
 }
@@ -617,15 +652,17 @@ template <typename T, int R> __device__ inline void small_dft(cplx<T> *v, int si
 // swz(x), where swz permutes x within its block of 16 complex values by XOR
 // with 5 * (x / 16) mod 16. The early Stockham stages write with stride R
 // (Ns = 1) and in runs of Ns = 4; under this permutation both patterns, and the
-// contiguous reads, put 16 consecutive lanes on 16 different bank pairs.
-__host__ __device__ constexpr int fft_pitch(int N) { return (N + 15) & ~15; }
+// contiguous reads, put 16 consecutive lanes on 16 different bank pairs. The
+// field pitch is 8 complex values past a multiple of 16, so the lanes that
+// alternate between two fields (loading and storing the Fourier coefficients)
+// fall on different banks.
+__host__ __device__ constexpr int fft_pitch(int N) { return ((N + 15) & ~15) + 8; }
 __device__ inline int fft_pad(int x) { return x ^ (((x >> 4) * 5) & 15); }
 
 template <typename T, int R, int MAXB>
 __device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cplx<T> *__restrict__ tw,
                                  int sign) {
     const int nbf = N / R, total = nbf * nfft, NP = fft_pitch(N);
-    const int tstep = N / (Ns * R);
     cplx<T> v[MAXB][R];
 #pragma unroll
     for (int bb = 0; bb < MAXB; bb++) {
@@ -645,7 +682,7 @@ __device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cp
             int jm = j % Ns;
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                cplx<T> t = tw[jm * r * tstep];
+                cplx<T> t = tw[Ns - 1 + (r - 1) * Ns + jm];  // = w^(jm r tstep), stage-ordered table
                 if (sign > 0) t.y = -t.y;
                 v[bb][r] = cmul(v[bb][r], t);
             }
@@ -756,8 +793,11 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
         if (idx < nitems) {
             int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
             if (f < nf) {
-                xn[c] = *(const cplx<T> *)(fs + ((long)m * nlat + j) * ldf + 2 * f);
-                xs[c] = *(const cplx<T> *)(fs + ((long)m * nlat + j2) * ldf + 2 * f);
+                // rows j / j2 hold the even / odd Legendre sums: north E + O, south E - O
+                const cplx<T> e = *(const cplx<T> *)(fs + ((long)m * nlat + j) * ldf + 2 * f);
+                const cplx<T> o = *(const cplx<T> *)(fs + ((long)m * nlat + j2) * ldf + 2 * f);
+                xn[c] = cplx<T>{e.x + o.x, e.y + o.y};
+                xs[c] = cplx<T>{e.x - o.x, e.y - o.y};
             }
         }
     }
@@ -1100,9 +1140,21 @@ template <typename T> int plan_create(Plan<T> &p, int nlon, int nlat, int ntrunc
         rcos[j] = (T)(1.0 / std::sqrt((1.0 - p.mu[j]) * (1.0 + p.mu[j])));
         wfac[j] = (T)(2.0 * M_PI * p.w[j] / nlon);
     }
-    for (int t = 0; t < nlon; t++) {
-        tw[2 * t] = (T)std::cos(2.0 * M_PI * t / nlon);
-        tw[2 * t + 1] = (T)(-std::sin(2.0 * M_PI * t / nlon));
+    // twiddles in stage order: the stage with radix R after Ns = (product of the
+    // earlier radices) holds w^(jm r nlon / (Ns R)), w = exp(-2 pi i / nlon), at
+    // Ns - 1 + (r - 1) Ns + jm for r = 1..R-1, jm = 0..Ns-1 (nlon - 1 entries in
+    // all), so the lanes of a stage, which have consecutive jm, read
+    // consecutive entries instead of a stride of r nlon / (Ns R).
+    std::fill(tw.begin(), tw.end(), T(0));
+    for (int st = 0, Ns = 1; st < rd.n; Ns *= rd.r[st], st++) {
+        const int R = rd.r[st], tstep = nlon / (Ns * R);
+        for (int r = 1; r < R; r++)
+            for (int jm = 0; jm < Ns; jm++) {
+                const int e = Ns - 1 + (r - 1) * Ns + jm;
+                const double a = 2.0 * M_PI * (double)((long)jm * r * tstep) / nlon;
+                tw[2 * e] = (T)std::cos(a);
+                tw[2 * e + 1] = (T)(-std::sin(a));
+            }
     }
     if ((rc = upload(&p.rcos, rcos.data(), sizeof(T) * nlat, s))) return rc;
     if ((rc = upload(&p.wfac, wfac.data(), sizeof(T) * nlat, s))) return rc;
@@ -1166,8 +1218,8 @@ void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, in
             return;
         }
     }
-    hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc, (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff, p.keoff,
-                       sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj);
+    hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc, 2 * (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff,
+                       p.keoff, sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj, ncol);
 }
 
 template <typename T>
@@ -1181,7 +1233,7 @@ void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int
         }
     }
     hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc, p.n_mtiles), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
-                       p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc);
+                       p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol);
 }
 
 static Radices radices_of(int n, const int *r) {
