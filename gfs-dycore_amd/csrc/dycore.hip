@@ -224,18 +224,11 @@ __global__ __launch_bounds__(256) void k_columns(
 #define LV(P, L) (P)[(long)((L) - 1) * ng]
     auto pk = [&](int K) { return ak[K - 1] + bk[K - 1] * ps; };
     auto dp = [&](int K) { return dak[K - 1] + dbk[K - 1] * ps; };  // pk(K + 1) - pk(K)
-    // pass 1: column totals of the continuity integrals (getomega db, cb)
+    // pass 1 (top -> bottom): getomega's running sums db, cb, omega/p (dlnpdtg)
+    // on layers, and etadot at interfaces less its term -ps bk_K dlnpsdt, which
+    // needs the column totals and is added where etadot is read (pass 3, or
+    // the loop below when only the inverse is wanted)
     T db = 0, cb = 0;
-    for (int K = 1; K <= nl; K++) {
-        int L = nl + 1 - K;
-        T cg = LV(U, L) * dpx + LV(V, L) * dpy;
-        db += LV(DIV, L) * dp(K);
-        cb += cg * dbk[K - 1];
-    }
-    const T dlnpsdt = -db / ps - cb;
-    // pass 2: etadot at interfaces and omega/p (dlnpdtg) on layers
-    db = 0;
-    cb = 0;
     ET[0] = T(0);
     ET[(long)nl * ng] = T(0);
     for (int K = 1; K <= nl; K++) {
@@ -260,8 +253,10 @@ __global__ __launch_bounds__(256) void k_columns(
         LV(WD, L) = (workc - workb) / dpk;
         db += dv * dpk;
         cb += cg * dbk[K - 1];
-        if (K <= nl - 1) ET[(long)K * ng] = -ps * (bk[K] * dlnpsdt + cb) - db;
+        if (K <= nl - 1) ET[(long)K * ng] = -(ps * cb) - db;
     }
+    const T dlnpsdt = -db / ps - cb;
+    auto et_fix = [&](int K) { return -(ps * bk[K]) * dlnpsdt; };  // etadot - stored value, 0 < K < nl
     psg_out[(long)mem * ng + col] = ps;
     dlnpsdt_out[(long)mem * ng + col] = dlnpsdt;
     if (colq) {
@@ -274,16 +269,35 @@ __global__ __launch_bounds__(256) void k_columns(
         }
         colq[(long)mem * ng + col] = s;
     }
-    if (inverse_only) return;
+    if (inverse_only) {
+        for (int K = 1; K <= nl - 1; K++) ET[(long)K * ng] += et_fix(K);
+        return;
+    }
     T *A = gan + (long)mem * nan * ng + col;
     T *TT = A, *FX = A + (long)nl * ng, *FY = A + 2L * nl * ng, *KE = A + 3L * nl * ng;
     A[4L * nl * ng] = dlnpsdt;
     const T f = twoom * sinlat[col / nlon];
     const T dphx = dphisdx[(long)mem * ng + col], dphy = dphisdy[(long)mem * ng + col];
-    // pass 3: bottom -> top (K = nl..1): getpresgrad accumulations, getvadv, assembly
+    // pass 3: bottom -> top (K = nl..1): getpresgrad accumulations, getvadv, assembly.
+    // u, v, tv of the levels below / at / above L slide through registers; each
+    // interface's etadot is completed (and stored) as the top of the layer below it
     T cofb_acc = 0, px3u = 0, px3v = 0;
+    T um = 0, vm = 0, tm = 0;                          // level L - 1
+    T u = LV(U, 1), v = LV(V, 1), tv = LV(TV, 1);      // level L
+    T eb = T(0);                                       // etadot at the layer's bottom interface
     for (int K = nl; K >= 1; K--) {
         int L = nl + 1 - K;
+        T up = 0, vp = 0, tp = 0;                      // level L + 1
+        if (L < nl) {
+            up = LV(U, L + 1);
+            vp = LV(V, L + 1);
+            tp = LV(TV, L + 1);
+        }
+        T ea = T(0);                                   // etadot at the top interface K - 1
+        if (K > 1) {
+            ea = ET[(long)(K - 1) * ng] + et_fix(K - 1);
+            ET[(long)(K - 1) * ng] = ea;
+        }
         T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K);
         T rlnp, alfa, cofa, dbp;
         if (K == 1) {
@@ -299,30 +313,26 @@ __global__ __launch_bounds__(256) void k_columns(
             dbp = lc.dbp;
         }
         T cofb0 = (K == 1) ? -(T(1) / dpk) * (alfa * dbk[0]) : -(T(1) / dpk) * (bk[K - 1] * rlnp + alfa * dbk[K - 1]);
-        T tv = LV(TV, L), tx = LV(TX, L), ty = LV(TY, L);
+        T tx = LV(TX, L), ty = LV(TY, L);
         T prsgx = cofb0 * rd * tv * ps * dpx - dphx + cofb_acc * ps * dpx + px3u - rd * alfa * tx -
                   cofa * rd * tv * ps * dpx;
         T prsgy = cofb0 * rd * tv * ps * dpy - dphy + cofb_acc * ps * dpy + px3v - rd * alfa * ty -
                   cofa * rd * tv * ps * dpy;
         // getvadv for u, v, tv at level L
-        T u = LV(U, L), v = LV(V, L);
         T vu, vv, vt;
         T h = T(0.5) / dpk;
-        if (L == nl) {
-            T e2 = ET[ng];
-            vu = h * e2 * (LV(U, nl - 1) - u);
-            vv = h * e2 * (LV(V, nl - 1) - v);
-            vt = h * e2 * (LV(TV, nl - 1) - tv);
-        } else if (L == 1) {
-            T en = ET[(long)(nl - 1) * ng];
-            vu = h * en * (u - LV(U, 2));
-            vv = h * en * (v - LV(V, 2));
-            vt = h * en * (tv - LV(TV, 2));
+        if (L == nl) {  // top layer: etadot at interface 1 (its bottom)
+            vu = h * eb * (um - u);
+            vv = h * eb * (vm - v);
+            vt = h * eb * (tm - tv);
+        } else if (L == 1) {  // bottom layer: etadot at interface nl - 1 (its top)
+            vu = h * ea * (u - up);
+            vv = h * ea * (v - vp);
+            vt = h * ea * (tv - tp);
         } else {
-            T eb = ET[(long)K * ng], ea = ET[(long)(K - 1) * ng];
-            vu = h * (eb * (LV(U, L - 1) - u) + ea * (u - LV(U, L + 1)));
-            vv = h * (eb * (LV(V, L - 1) - v) + ea * (v - LV(V, L + 1)));
-            vt = h * (eb * (LV(TV, L - 1) - tv) + ea * (tv - LV(TV, L + 1)));
+            vu = h * (eb * (um - u) + ea * (u - up));
+            vv = h * (eb * (vm - v) + ea * (v - vp));
+            vt = h * (eb * (tm - tv) + ea * (tv - tp));
         }
         T vq = kappa * LV(WD, L) * tv;
         if (ntr > 0) vq = vq / (T(1) + cvcp * LV(Q, L));
@@ -337,6 +347,13 @@ __global__ __launch_bounds__(256) void k_columns(
         cofb_acc -= rd * dbp * tv;
         px3u -= rd * rlnp * tx;
         px3v -= rd * rlnp * ty;
+        um = u;
+        vm = v;
+        tm = tv;
+        u = up;
+        v = vp;
+        tv = tp;
+        eb = ea;
     }
     // pass 4: tracers (dyn_run.f90:279-291): horizontal advection with the
     // synthesised gradient and the Thuburn (1993) flux-limited vertical
