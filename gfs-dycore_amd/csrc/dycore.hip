@@ -767,34 +767,53 @@ __global__ __launch_bounds__(256) void k_simple_columns(
 // max(qmin, q))), q, u, v (nlev each), ps, sea-surface temperature, sin(lat);
 // targets: the plain physics' changes of Tv, q, u, v over the step. Optional
 // normalisation nrm = [mean_x | std_x | mean_y | std_y].
+// Feature rows of the column MLP, one row per selected column: T (= Tv / (1 + fv
+// max(qmin, q)), as the plain physics sees it), q, u, v top -> bottom, then ps,
+// SST and sin(lat), normalised when nrm is given. A workgroup gathers MLP_CB
+// columns with lanes along columns (coalesced level rows), transposes them in
+// LDS, and writes their rows, which are contiguous in x, as one coalesced block.
+constexpr int MLP_CB = 32;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_mlp_features(const T *__restrict__ gsyn, const T *__restrict__ psg,
                                                       const double *__restrict__ latd, int nl, int nsyn, long ng,
                                                       int nlon, SimpleCfg sc, ColSel cs, long nsel,
                                                       const float *__restrict__ nrm, float *__restrict__ x,
                                                       long row0) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long col = sel_col(cs, i, ng);
+    extern __shared__ float tile[];  // MLP_CB rows x pitch
+    const int nin = 4 * nl + 3, pitch = nin | 1;
+    const long i0 = (long)blockIdx.x * MLP_CB;
     const int mem = blockIdx.y;
-    if (col < 0) return;
-    const int nin = 4 * nl + 3;
-    const T *G = gsyn + (long)mem * nsyn * ng + col;
+    const int r = threadIdx.x % MLP_CB, grp = threadIdx.x / MLP_CB, ngrp = blockDim.x / MLP_CB;
+    const long col = sel_col(cs, i0 + r, ng);
+    const long cc = col < 0 ? 0 : col;  // rows past the selection gather column 0 and are not written
+    const T *G = gsyn + (long)mem * nsyn * ng + cc;
     const T *U = G, *V = G + (long)nl * ng, *TV = G + 4L * nl * ng, *Q = G + 7L * nl * ng;
-    float *xr = x + (row0 + (long)mem * nsel + i) * nin;
     const T fvT = (T)sc.fv, qminT = (T)sc.qmin;
-    for (int c = 1; c <= nl; c++) {
+    float *tr = tile + r * pitch;
+    for (int c = 1 + grp; c <= nl; c += ngrp) {
         const long o = (long)(nl - c) * ng;
-        xr[c - 1] = (float)(TV[o] / (T(1) + fvT * fmax(qminT, Q[o])));
-        xr[nl + c - 1] = (float)Q[o];
-        xr[2 * nl + c - 1] = (float)U[o];
-        xr[3 * nl + c - 1] = (float)V[o];
+        const T q = Q[o];
+        tr[c - 1] = (float)(TV[o] / (T(1) + fvT * fmax(qminT, q)));
+        tr[nl + c - 1] = (float)q;
+        tr[2 * nl + c - 1] = (float)U[o];
+        tr[3 * nl + c - 1] = (float)V[o];
     }
-    const double lat = latd[col / nlon];
-    xr[4 * nl] = (float)psg[(long)mem * ng + col];
-    xr[4 * nl + 1] = (float)simple_sst(sc.sst_mode, lat);
-    xr[4 * nl + 2] = (float)sin(lat);
-    if (nrm)
-        for (int f = 0; f < nin; f++) xr[f] = (xr[f] - nrm[f]) / nrm[nin + f];
+    if (grp == 0) {
+        const double lat = latd[cc / nlon];
+        tr[4 * nl] = (float)psg[(long)mem * ng + cc];
+        tr[4 * nl + 1] = (float)simple_sst(sc.sst_mode, lat);
+        tr[4 * nl + 2] = (float)sin(lat);
+    }
+    __syncthreads();
+    const int nrow = (int)min((long)MLP_CB, nsel - i0);
+    float *xb = x + (row0 + (long)mem * nsel + i0) * nin;
+    for (int e = threadIdx.x; e < nrow * nin; e += blockDim.x) {
+        const int rr = e / nin, f = e - rr * nin;
+        float v = tile[rr * pitch + f];
+        if (nrm) v = (v - nrm[f]) / nrm[nin + f];
+        xb[e] = v;
+    }
 }
 
 template <typename T>
@@ -815,8 +834,10 @@ __global__ __launch_bounds__(256) void k_mlp_targets(const T *__restrict__ gan, 
     }
 }
 
-// network output (normalised, rows = member * ng + column) -> the physics slots
-// of gan and the column water pwat; no precipitation diagnostic
+// Network outputs (normalised dT, dq, du, dv per level, top -> bottom) back to
+// the physics grid fields; the column water after the change (pwat) for the mass
+// fixer. Rows of MLP_CB columns are read as one coalesced block and transposed
+// in LDS so that the level rows of the fields are written coalesced.
 template <typename T>
 __global__ __launch_bounds__(256) void k_mlp_apply(const float *__restrict__ y, const float *__restrict__ nrm,
                                                    const T *__restrict__ gsyn, const T *__restrict__ psg,
@@ -824,29 +845,46 @@ __global__ __launch_bounds__(256) void k_mlp_apply(const float *__restrict__ y, 
                                                    T *__restrict__ gan, T *__restrict__ pwat_out,
                                                    T *__restrict__ precip_out, int nl, int nsyn, int nsp, long ng,
                                                    double grav) {
-    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int mem = blockIdx.y;
-    if (col >= ng) return;
-    const int nin = 4 * nl + 3, nout = 4 * nl;
+    extern __shared__ float tile[];  // MLP_CB rows x pitch
+    __shared__ double pws[256 / MLP_CB][MLP_CB];
+    const int nin = 4 * nl + 3, nout = 4 * nl, pitch = nout | 1;
     const float *my = nrm + 2 * nin, *sy = my + nout;
-    const float *yr = y + ((long)mem * ng + col) * nout;
-    const T *Q = gsyn + (long)mem * nsyn * ng + 7L * nl * ng + col;
-    T *A = gan + (long)mem * nsp * ng + col;
-    const T ps = psg[(long)mem * ng + col];
-    double pw = 0.0;
-    for (int c = 1; c <= nl; c++) {
-        const long o = (long)(nl - c) * ng;
-        auto val = [&](int f) { return my[f] + sy[f] * yr[f]; };
-        A[o] = (T)val(c - 1);
-        const T dq = (T)val(nl + c - 1);
-        A[3L * nl * ng + o] = dq;
-        A[(long)nl * ng + o] = (T)val(2 * nl + c - 1);
-        A[2L * nl * ng + o] = (T)val(3 * nl + c - 1);
-        const double dp = (double)((ak[c] + bk[c] * ps) - (ak[c - 1] + bk[c - 1] * ps));
-        pw += dp * (double)(dq + Q[o]);
+    const long c0 = (long)blockIdx.x * MLP_CB;
+    const int mem = blockIdx.y;
+    const int nrow = (int)min((long)MLP_CB, ng - c0);
+    const float *yb = y + ((long)mem * ng + c0) * nout;
+    for (int e = threadIdx.x; e < nrow * nout; e += blockDim.x) {
+        const int rr = e / nout, f = e - rr * nout;
+        tile[rr * pitch + f] = my[f] + sy[f] * yb[e];
     }
-    pwat_out[(long)mem * ng + col] = (T)(pw / grav);
-    precip_out[(long)mem * ng + col] = T(0);
+    __syncthreads();
+    const int r = threadIdx.x % MLP_CB, grp = threadIdx.x / MLP_CB, ngrp = blockDim.x / MLP_CB;
+    const long col = c0 + r;
+    double pw = 0.0;
+    if (r < nrow) {
+        const T *Q = gsyn + (long)mem * nsyn * ng + 7L * nl * ng + col;
+        T *A = gan + (long)mem * nsp * ng + col;
+        const T ps = psg[(long)mem * ng + col];
+        const float *tr = tile + r * pitch;
+        for (int c = 1 + grp; c <= nl; c += ngrp) {
+            const long o = (long)(nl - c) * ng;
+            A[o] = (T)tr[c - 1];
+            const T dq = (T)tr[nl + c - 1];
+            A[3L * nl * ng + o] = dq;
+            A[(long)nl * ng + o] = (T)tr[2 * nl + c - 1];
+            A[2L * nl * ng + o] = (T)tr[3 * nl + c - 1];
+            const double dp = (double)((ak[c] + bk[c] * ps) - (ak[c - 1] + bk[c - 1] * ps));
+            pw += dp * (double)(dq + Q[o]);
+        }
+    }
+    pws[grp][r] = pw;
+    __syncthreads();
+    if (grp == 0 && r < nrow) {
+        double t = 0.0;
+        for (int k = 0; k < ngrp; k++) t += pws[k][r];
+        pwat_out[(long)mem * ng + col] = (T)(t / grav);
+        precip_out[(long)mem * ng + col] = T(0);
+    }
 }
 
 // judge: normalised squared difference between the network's and the plain
@@ -2160,15 +2198,17 @@ template <typename T> struct Step {
     void mlp_columns() {
         const gfsdc_cfg &c = E.cfg;
         const long rows = (long)B.nm * B.ng;
-        dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+        dim3 g((unsigned)((B.ng + MLP_CB - 1) / MLP_CB), B.nm);
         const int nin = 4 * B.nl + 3, nout = 4 * B.nl;
-        hipLaunchKernelGGL(k_mlp_features<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg,
+        hipLaunchKernelGGL(k_mlp_features<T>, g, dim3(256), sizeof(float) * MLP_CB * (nin | 1), s, (const T *)B.gsyn,
+                           (const T *)B.psg,
                            (const double *)B.latd, B.nl, B.nsyn, B.ng, E.plan.nlon, simple_cfg(),
                            ColSel{0, nullptr, nullptr}, B.ng, (const float *)B.nrm, B.xin, 0L);
         mark("mlp_features");
         mlp_forward(B.mlp, B.xin, nin, (int)rows, B.yout, nout, s);
         mark("mlp_forward");
-        hipLaunchKernelGGL(k_mlp_apply<T>, g, dim3(256), 0, s, (const float *)B.yout, (const float *)B.nrm,
+        hipLaunchKernelGGL(k_mlp_apply<T>, g, dim3(256), sizeof(float) * MLP_CB * (nout | 1), s,
+                           (const float *)B.yout, (const float *)B.nrm,
                            (const T *)B.gsyn, (const T *)B.psg, (const T *)B.ak, (const T *)B.bk, (T *)B.gan,
                            (T *)B.pwat, (T *)B.precip, B.nl, B.nsyn, B.nsp, B.ng, c.grav);
         mark("mlp_apply");
@@ -2383,9 +2423,10 @@ template <typename T> int Engine<T>::mlp_collect(int nsteps, int stride, long *n
         st.step();
         const ColSel cs{stride, B->jctr, nullptr};
         dim3 g((unsigned)((nsel + 255) / 256), B->nm);
-        hipLaunchKernelGGL(k_mlp_features<T>, g, dim3(256), 0, stream, (const T *)B->gsyn, (const T *)B->psg,
-                           (const double *)B->latd, B->nl, B->nsyn, B->ng, plan.nlon, st.simple_cfg(), cs, nsel,
-                           (const float *)nullptr, B->ds_x, B->ds_n);
+        dim3 gf((unsigned)((nsel + MLP_CB - 1) / MLP_CB), B->nm);
+        hipLaunchKernelGGL(k_mlp_features<T>, gf, dim3(256), sizeof(float) * MLP_CB * ((4 * B->nl + 3) | 1), stream,
+                           (const T *)B->gsyn, (const T *)B->psg, (const double *)B->latd, B->nl, B->nsyn, B->ng,
+                           plan.nlon, st.simple_cfg(), cs, nsel, (const float *)nullptr, B->ds_x, B->ds_n);
         hipLaunchKernelGGL(k_mlp_targets<T>, g, dim3(256), 0, stream, (const T *)B->gan, B->nl, B->nsp, B->ng, cs,
                            nsel, B->ds_y, B->ds_n);
         hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, stream, B->jctr);

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
                     float v = acc[i][j][r];
                     if (m < M) {
                         if (EPI == EPI_GELU) {
-                            e.z[(long)m * e.ldz + n] = v;
+                            if (e.z) e.z[(long)m * e.ldz + n] = v;  // kept for the backward pass only
                             v = gelu(v);
                         } else {
                             v *= dgelu(e.z[(long)m * e.ldz + n]);
@@ -132,7 +132,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
                     }
                 }
                 // four consecutive rows of one column: one 8-byte store in the transposed copy
-                if (mb + 3 < M) {
+                // (read by the weight-gradient GEMM; inference passes none)
+                if (!e.ot) {
+                } else if (mb + 3 < M) {
                     uint2 pk;
                     pk.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
                     pk.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
@@ -157,7 +159,7 @@ __global__ void k_act_in(const float *__restrict__ x, int ldx, int n, int nin, i
     else if (k < nin && b < n) v = x[(long)b * ldx + k];
     const uint16_t h = f2bf(v);
     a[i] = h;
-    at[(long)k * Bp + b] = h;
+    if (at) at[(long)k * Bp + b] = h;  // transposed copy for the weight gradient (training only)
 }
 
 // constant parts of a hidden activation buffer: bias column = 1, padding = 0
@@ -422,8 +424,10 @@ static int splits(const MlpNet *net, int Mb) {
     return S;
 }
 
-// forward of Mb (multiple of TM) rows already in L[0].a / at; output in net->y
-static void forward_batch(MlpNet *net, int Mb, hipStream_t s) {
+// forward of Mb (multiple of TM) rows already in L[0].a / at; output in net->y.
+// Training keeps the pre-activations and the transposed activations for the
+// backward pass; inference writes only what the next layer reads.
+static void forward_batch(MlpNet *net, int Mb, bool train, hipStream_t s) {
     const int nL = (int)net->L.size(), Bp = net->Bp;
     for (int l = 0; l < nL; l++) {
         MlpLayer &Ly = net->L[l];
@@ -436,11 +440,11 @@ static void forward_batch(MlpNet *net, int Mb, hipStream_t s) {
                                Ly.N, Ly.Kp, e);
         } else {
             MlpLayer &Nx = net->L[l + 1];
-            e.z = Ly.z;
+            e.z = train ? Ly.z : nullptr;
             e.ldz = Ly.N;
             e.o = Nx.a;
             e.ldo = Nx.Kp;
-            e.ot = Nx.at;
+            e.ot = train ? Nx.at : nullptr;
             e.ldot = Bp;
             hipLaunchKernelGGL(k_gemm_nt<EPI_GELU>, g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt, (long)Ly.Kp, Mb,
                                Ly.N, Ly.Kp, e);
@@ -448,20 +452,20 @@ static void forward_batch(MlpNet *net, int Mb, hipStream_t s) {
     }
 }
 
-static int input_batch(MlpNet *net, const float *x, int ldx, int n, hipStream_t s) {
+static int input_batch(MlpNet *net, const float *x, int ldx, int n, bool train, hipStream_t s) {
     MlpLayer &L0 = net->L[0];
     const int Mb = round_up(n, TM);
     const long na = (long)Mb * L0.Kp;
     hipLaunchKernelGGL(k_act_in, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, x, ldx, n, net->nin, L0.Kp,
-                       Mb, net->Bp, L0.a, L0.at);
+                       Mb, net->Bp, L0.a, train ? L0.at : (uint16_t *)nullptr);
     return Mb;
 }
 
 int mlp_forward(MlpNet *net, const float *x, int ldx, int n, float *y, int ldy, hipStream_t s) {
     for (int r0 = 0; r0 < n; r0 += net->Bp) {
         const int nb = std::min(net->Bp, n - r0);
-        const int Mb = input_batch(net, x + (long)r0 * ldx, ldx, nb, s);
-        forward_batch(net, Mb, s);
+        const int Mb = input_batch(net, x + (long)r0 * ldx, ldx, nb, false, s);
+        forward_batch(net, Mb, false, s);
         GFSDC_CHECK(hipMemcpy2DAsync(y + (long)r0 * ldy, sizeof(float) * ldy, net->y, sizeof(float) * net->Np,
                                      sizeof(float) * net->nout, nb, hipMemcpyDeviceToDevice, s));
     }
@@ -475,10 +479,10 @@ int mlp_backward(MlpNet *net, const float *x, int ldx, const float *t, int ldt, 
         return GFSDC_EINVAL;
     }
     const int nL = (int)net->L.size(), Bp = net->Bp;
-    const int Mb = input_batch(net, x, ldx, n, s);
+    const int Mb = input_batch(net, x, ldx, n, true, s);
     const int S = splits(net, Mb);
     net->last_S = S;
-    forward_batch(net, Mb, s);
+    forward_batch(net, Mb, true, s);
     MlpLayer &Lo = net->L[nL - 1];
     const int nblk = (int)(((long)Mb * net->Np + 255) / 256);
     hipLaunchKernelGGL(k_loss, dim3((unsigned)nblk), dim3(256), 0, s, net->y, net->Np, t, ldt, n, net->nout, Mb,
