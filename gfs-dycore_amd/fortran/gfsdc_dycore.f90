@@ -78,5 +78,47 @@ module gfsdc_dycore
       import :: c_ptr, c_int
       type(c_ptr), value :: h
     end function
+    ! AR(1) temperature pattern of every member (ndimspec complex per member)
+    integer(c_int) function dyc_get_pattern(h, spec, dtype, loc) bind(c, name="gfsdc_get_pattern")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, spec
+      integer(c_int), value :: dtype, loc
+    end function
+    ! learned column physics in place of the simple physics' column call
+    integer(c_int) function dyc_mlp_init(h, hidden, nlayers, maxbatch, capacity, seed) &
+        bind(c, name="gfsdc_mlp_init")
+      import :: c_ptr, c_int, c_long, c_long_long
+      type(c_ptr), value :: h
+      integer(c_int), value :: hidden, nlayers, maxbatch
+      integer(c_long), value :: capacity
+      integer(c_long_long), value :: seed
+    end function
+    integer(c_int) function dyc_mlp_collect(h, nsteps, stride, ncollected) bind(c, name="gfsdc_mlp_collect")
+      import :: c_ptr, c_int, c_long
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps, stride
+      integer(c_long) :: ncollected
+    end function
+    integer(c_int) function dyc_mlp_normalize(h) bind(c, name="gfsdc_mlp_normalize")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function dyc_mlp_train(h, nsteps, batch, lr, wd, ntrain, losses) bind(c, name="gfsdc_mlp_train")
+      import :: c_ptr, c_int, c_long, c_double
+      type(c_ptr), value :: h, losses
+      integer(c_int), value :: nsteps, batch
+      real(c_double), value :: lr, wd
+      integer(c_long), value :: ntrain
+    end function
+    integer(c_int) function dyc_mlp_enable(h, on, judge_stride, threshold) bind(c, name="gfsdc_mlp_enable")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: on, judge_stride
+      real(c_double), value :: threshold
+    end function
+    integer(c_int) function dyc_mlp_judge_stats(h, out3) bind(c, name="gfsdc_mlp_judge_stats")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, out3
+    end function
   end interface
 end module gfsdc_dycore
