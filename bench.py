@@ -21,6 +21,9 @@ Other workloads (each prints its own JSON line instead of the headline):
   --micro                  transform microbenchmark: spectogrd + grdtospec +
                            getuv on 64 levels x 4 fields, FP64 / FP32 / BF16X3
   --moist {plain,mlp}      moist run with the simple physics or the column MLP
+  --copy-calibration       (with any workload) also run a 1 GiB device-to-device
+                           copy, the reference kernel of known size for scaling
+                           rocprofv3 FETCH_SIZE / WRITE_SIZE into HBM bytes
 """
 import argparse
 import json
@@ -106,7 +109,7 @@ def roofline(tab, ctx, esz):
     dom = max(per_step, key=per_step.get)
     kind, work = algorithmic_work(dom, ctx, esz)
     avg_s = tab[dom][0] * 1e-3
-    roof = {"kernel": dom, "bound": kind, "avg_launch_ms": tab[dom][0], "traffic": None}
+    roof = {"kernel": dom, "bound": kind, "avg_launch_ms": tab[dom][0], "traffic": measured_traffic(dom)}
     if work is not None:
         if kind == "mfma":
             roof.update(achieved=work / avg_s / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s")
@@ -114,6 +117,36 @@ def roofline(tab, ctx, esz):
             roof.update(achieved=work / avg_s / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s")
         roof["frac"] = roof["achieved"] / roof["peak"]
     return roof, per_step
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_T254L64.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a step kernel from the rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the counter's
+    reading for a copy of known size): the row '| kernel | ... | bytes |' of
+    the summary's traffic table, or None when the file or row is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 6 and cells[0] == kernel:
+                    return float(cells[5])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """a plain device copy of nbytes between two fresh buffers (the library's
+    k_copy_bytes, 1 GiB: beyond every cache), the calibration kernel for the
+    profiler's FETCH_SIZE / WRITE_SIZE counters"""
+    ctx = gfsdc.Context(128, 64, 42, gfsdc.FP32)
+    ms = ctx.bench_copy(nbytes, reps)
+    del ctx
+    return {"kernel": "k_copy_bytes", "bytes_read": nbytes, "bytes_written": nbytes, "launches": reps,
+            "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
 def spectral_error(a, b):
@@ -319,6 +352,8 @@ def main():
     ap.add_argument("--pattern", action="store_true", help="AR(1) temperature pattern per member")
     ap.add_argument("--micro", action="store_true", help="transform microbenchmark")
     ap.add_argument("--moist", choices=("plain", "mlp"), default=None)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the step is timed on a single MI355X; "
@@ -335,6 +370,8 @@ def main():
     else:
         args.config = args.config or "T254L64"
         line = run_headline(args)
+    if args.copy_calibration:
+        line["copy_calibration"] = copy_calibration()
     print(json.dumps(line))
 
 

@@ -174,6 +174,7 @@ int gfsdc_autotune(gfsdc_ctx *c, double *ms, int *best) { DISPATCH(c, autotune(m
 int gfsdc_bench_transforms(gfsdc_ctx *c, int nscalar, int nvector, int reps, double *ms) {
     DISPATCH(c, bench_transforms(nscalar, nvector, reps, ms));
 }
+int gfsdc_bench_copy(gfsdc_ctx *c, long nbytes, int reps, double *ms) { DISPATCH(c, bench_copy(nbytes, reps, ms)); }
 int gfsdc_mlp_init(gfsdc_ctx *c, int hidden, int nlayers, int maxbatch, long capacity, unsigned long long seed) {
     DISPATCH(c, mlp_init(hidden, nlayers, maxbatch, capacity, seed));
 }

@@ -45,6 +45,7 @@ template <typename T> struct Engine {
 
     int spectogrd(const void *spec, void *grid, int nf, int dtype, int loc);
     int bench_transforms(int ns, int nv, int reps, double *ms);
+    int bench_copy(long nbytes, int reps, double *ms);
     int grdtospec(const void *grid, void *spec, int nf, int dtype, int loc);
     int sphtor_to_spat(const void *s, const void *t, void *u, void *v, int nf, double sscale, int sinv,
                        double tscale, int tinv, int dtype, int loc);

@@ -264,6 +264,44 @@ template <typename T> int Engine<T>::bench_transforms(int ns, int nv, int reps, 
     return rc;
 }
 
+// plain 16-byte-vector copy, grid-stride
+__global__ void k_copy_bytes(float4 *__restrict__ dst, const float4 *__restrict__ src, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+template <typename T> int Engine<T>::bench_copy(long nbytes, int reps, double *ms) {
+    if (nbytes < 16 || nbytes % 16 || reps < 1) {
+        set_error("bench_copy: nbytes a positive multiple of 16, reps >= 1");
+        return GFSDC_EINVAL;
+    }
+    void *a = nullptr, *b = nullptr;
+    GFSDC_CHECK(hipMalloc(&a, nbytes));
+    if (hipMalloc(&b, nbytes) != hipSuccess) {
+        (void)hipFree(a);
+        set_error("bench_copy: device allocation failed");
+        return GFSDC_ENOMEM;
+    }
+    const long n = nbytes / 16;
+    GFSDC_CHECK(hipMemsetAsync(a, 0, nbytes, stream));
+    hipEvent_t e0, e1;
+    GFSDC_CHECK(hipEventCreate(&e0));
+    GFSDC_CHECK(hipEventCreate(&e1));
+    GFSDC_CHECK(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy_bytes, dim3(4096), dim3(256), 0, stream, (float4 *)b, (const float4 *)a, n);
+    GFSDC_CHECK(hipEventRecord(e1, stream));
+    GFSDC_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    GFSDC_CHECK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return 0;
+}
+
 // (S, T) -> (Vt, Vp) of SHTns sphtor_to_spat with S = sscale [invlap] s, T = tscale [invlap] t;
 // s may be null (zero spheroidal part, as getgrad).
 template <typename T>

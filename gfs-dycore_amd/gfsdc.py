@@ -52,7 +52,7 @@ SYMBOLS = [
     "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_grdtospec", "gfsdc_getuv",
     "gfsdc_getvrtdivspec", "gfsdc_getgrad", "gfsdc_set_state", "gfsdc_get_state",
     "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
-    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_autotune", "gfsdc_bench_transforms", "gfsdc_mlp_init", "gfsdc_mlp_collect",
+    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_autotune", "gfsdc_bench_transforms", "gfsdc_bench_copy", "gfsdc_mlp_init", "gfsdc_mlp_collect",
     "gfsdc_mlp_normalize", "gfsdc_mlp_train", "gfsdc_mlp_eval", "gfsdc_mlp_enable", "gfsdc_mlp_judge_stats",
     "gfsdc_mlp_nparams", "gfsdc_mlp_params", "gfsdc_mlp_forward", "gfsdc_mlp_grad", "gfsdc_mlp_adamw",
     "gfsdc_get_grid", "gfsdc_get_semimp",
@@ -96,6 +96,7 @@ def load():
     L.gfsdc_get_pattern.argtypes = [vp, vp, i, i]
     L.gfsdc_autotune.argtypes = [vp, vp, C.POINTER(i)]
     L.gfsdc_bench_transforms.argtypes = [vp, i, i, i, C.POINTER(d)]
+    L.gfsdc_bench_copy.argtypes = [vp, C.c_long, i, C.POINTER(d)]
     lg = C.c_long
     L.gfsdc_mlp_init.argtypes = [vp, i, i, i, lg, C.c_ulonglong]
     L.gfsdc_mlp_collect.argtypes = [vp, i, i, C.POINTER(lg)]
@@ -312,6 +313,12 @@ class Context:
         """ms per spectogrd+grdtospec round trip of nscalar fields + getuv of nvector pairs"""
         ms = C.c_double(0)
         _check(load().gfsdc_bench_transforms(self.h, nscalar, nvector, reps, C.byref(ms)))
+        return ms.value
+
+    def bench_copy(self, nbytes, reps):
+        """ms per device copy of nbytes (profiler counter calibration)"""
+        ms = C.c_double(0)
+        _check(load().gfsdc_bench_copy(self.h, nbytes, reps, C.byref(ms)))
         return ms.value
 
     # -- learned column physics ------------------------------------------------

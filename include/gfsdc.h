@@ -182,6 +182,11 @@ int gfsdc_autotune(gfsdc_ctx *ctx, double *ms, int *best);
  * iteration of a spectogrd + grdtospec round trip of nscalar fields plus a
  * getuv of nvector pairs on device-resident buffers, averaged over reps */
 int gfsdc_bench_transforms(gfsdc_ctx *ctx, int nscalar, int nvector, int reps, double *ms);
+/* device-to-device copy of nbytes (a multiple of 16) between two fresh buffers,
+ * reps times (no counterpart in the reference): the kernel of known size against
+ * which profiler memory counters (FETCH_SIZE, WRITE_SIZE) are scaled to bytes;
+ * milliseconds per copy */
+int gfsdc_bench_copy(gfsdc_ctx *ctx, long nbytes, int reps, double *ms);
 /* current AR(1) pattern, (ndimspec, nmem) complex of dtype (error if tpat_amp == 0) */
 int gfsdc_get_pattern(gfsdc_ctx *ctx, void *spec, int dtype, int loc);
 /* ---- learned column physics (no counterpart in the reference) --------------
