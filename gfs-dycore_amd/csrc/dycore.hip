@@ -571,7 +571,7 @@ __device__ inline long sel_col(const ColSel &cs, long i, long ng) {
     return i * cs.stride + off;
 }
 
-__device__ inline double simple_sst(int mode, double lat) {
+__device__ __forceinline__ double simple_sst(int mode, double lat) {
     if (mode == 1) {
         // simple_physics_v5_1.f90:231-239 (moist baroclinic wave)
         const double pi = 3.14159265358979323846;

@@ -618,11 +618,11 @@ __global__ __launch_bounds__(256) void k_leg_anal_b3(const float *__restrict__ p
 // ---------------------------------------------------------------------------
 // Stockham FFT in LDS: NFFT transforms of length N (radices 2, 3, 4), in place
 // with a barrier between the load and store half of every stage.
-template <typename T> __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
     return cplx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
-template <typename T, int R> __device__ inline void small_dft(cplx<T> *v, int sign) {
+template <typename T, int R> __device__ __forceinline__ void small_dft(cplx<T> *v, int sign) {
     if (R == 2) {
         cplx<T> a = v[0], b = v[1];
         v[0] = cplx<T>{a.x + b.x, a.y + b.y};
@@ -660,7 +660,7 @@ __host__ __device__ constexpr int fft_pitch(int N) { return ((N + 15) & ~15) + 8
 __device__ inline int fft_pad(int x) { return x ^ (((x >> 4) * 5) & 15); }
 
 template <typename T, int R, int MAXB>
-__device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cplx<T> *__restrict__ tw,
+__device__ __forceinline__ void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cplx<T> *__restrict__ tw,
                                  int sign) {
     const int nbf = N / R, total = nbf * nfft, NP = fft_pitch(N);
     cplx<T> v[MAXB][R];
@@ -696,7 +696,7 @@ __device__ inline void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cp
 }
 
 template <typename T, int NFFT, int MAXN>
-__device__ inline void fft_run(cplx<T> *buf, int N, const int *rad, int nrad, const cplx<T> *tw, int sign) {
+__device__ __forceinline__ void fft_run(cplx<T> *buf, int N, const int *rad, int nrad, const cplx<T> *tw, int sign) {
     int Ns = 1;
     for (int s = 0; s < nrad; s++) {
         int R = rad[s];
@@ -741,7 +741,7 @@ constexpr RadPlan rad_plan(int n) {
 }
 
 template <typename T, int NFFT, int N, int S, int Ns>
-__device__ inline void fft_stages(cplx<T> *buf, const cplx<T> *tw, int sign) {
+__device__ __forceinline__ void fft_stages(cplx<T> *buf, const cplx<T> *tw, int sign) {
     constexpr RadPlan P = rad_plan(N);
     if constexpr (S < P.n) {
         constexpr int R = P.r[S];
