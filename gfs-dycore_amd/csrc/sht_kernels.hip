@@ -146,6 +146,16 @@ __global__ void k_legendre_table(T *plm, const long *poff, const double *mu, int
 // consecutive columns.
 constexpr int LDS_PAD = 16;
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8).
+// Give each XCD a contiguous range of logical tiles, so that tiles sharing
+// cache lines (the field groups of one latitude pair; the Legendre block and
+// the spectral columns of one order m) run on the same XCD and meet in its L2
+// instead of fetching the same lines through eight L2s.
+__device__ inline int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
 template <typename T> struct Vec;
 template <> struct Vec<float> {
     typedef float4 t;
@@ -183,15 +193,41 @@ __device__ inline double2 vkeep(const double2 &x, bool keep) { return double2{ke
 template <typename T, bool GUARD, typename Acc>
 __device__ inline void gemm_chunk(T (*As)[GEMM_BM + LDS_PAD], T (*Bs)[GT<T>::BN + LDS_PAD], Acc (&E)[2][GT<T>::WB],
                                   int lane, int wm, int wn, int kmax = GEMM_BK, int nbj = GT<T>::WB) {
+    constexpr int NK = GEMM_BK / 4, WB = GT<T>::WB;
+    const int li = lane & 15, kl = lane >> 4;
+    if constexpr (!GUARD) {
+        // all operands of the chunk first, so the LDS reads overlap each other
+        // and the matrix-core work instead of waiting before every k-step
+        T a[NK][2], b[NK][WB];
+#pragma unroll
+        for (int q = 0; q < NK; q++) {
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) a[q][bi] = As[4 * q + kl][wm + bi * 16 + li];
+#pragma unroll
+            for (int bj = 0; bj < WB; bj++) b[q][bj] = Bs[4 * q + kl][wn + bj * 16 + li];
+        }
+#pragma unroll
+        for (int q = 0; q < NK; q++)
+#pragma unroll
+            for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+                for (int bi = 0; bi < 2; bi++) {
+                    if (Vec<T>::swap)
+                        E[bi][bj] = MF<T>::mma(b[q][bj], a[q][bi], E[bi][bj]);
+                    else
+                        E[bi][bj] = MF<T>::mma(a[q][bi], b[q][bj], E[bi][bj]);
+                }
+        return;
+    }
 #pragma unroll
     for (int ks = 0; ks < GEMM_BK; ks += 4) {
         if (GUARD && ks >= kmax) break;
-        const int kr = ks + (lane >> 4), li = lane & 15;
+        const int kr = ks + kl;
         T a[2];
 #pragma unroll
         for (int bi = 0; bi < 2; bi++) a[bi] = As[kr][wm + bi * 16 + li];
 #pragma unroll
-        for (int bj = 0; bj < GT<T>::WB; bj++) {
+        for (int bj = 0; bj < WB; bj++) {
             if (GUARD && bj >= nbj) break;
             T b = Bs[kr][wn + bj * 16 + li];
 #pragma unroll
@@ -219,9 +255,14 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     constexpr int VW = Vec<T>::n;
     constexpr int BN = GT<T>::BN, WB = GT<T>::WB;
     constexpr int NA = GEMM_BK * GEMM_BM / VW / 256, NB = GEMM_BK * BN / VW / 256;
-    const int tc = blockIdx.x;
-    const int par = blockIdx.y & 1, mt = blockIdx.y >> 1;
-    const int m = mt / ntj, tj = mt - m * ntj;
+    // logical tile ((i * 2 + par) * ntj + tj) * ntc + tc, contiguous per XCD: the
+    // column and latitude tiles of one (m, parity) share its Legendre block and
+    // spectral columns in that XCD's L2. Orders are taken as m = 0, T, 1, T-1, ...
+    // so that every XCD's range holds a like amount of work (K = T + 2 - m).
+    const int ntc = gridDim.x / (2 * (ntrunc + 1) * ntj);
+    const int L = xcd_tile(blockIdx.x, gridDim.x);
+    const int tc = L % ntc, g = L / ntc, tj = g % ntj, ip = g / ntj, par = ip & 1, i = ip >> 1;
+    const int m = (i & 1) ? ntrunc - (i >> 1) : (i >> 1);
     const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1, Kp = par ? Ko : Ke;
     const T *P = plm + poff[m] + (par ? (long)Ke * nhp : 0L);
     const T *S = sx + (long)keoff[m] * ldx + (long)par * ldx;
@@ -236,31 +277,30 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
 #pragma unroll
         for (int b = 0; b < WB; b++) E[a][b] = acc_t{0, 0, 0, 0};
     V ra[NA], rb[NB];
+    // clamped in-bounds loads; the rows past Kp are zeroed by value when stored
+    // (no select between pointers, and no wait on the prefetch before the MFMAs)
     auto load = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW, i = k0 + kk;
-            // clamped in-bounds load, zeroed by value (no select between pointers)
-            V x = *(const V *)(P + (long)min(i, Kp - 1) * nhp + j0 + jj);
-            ra[u] = vkeep(x, i < Kp);
+            ra[u] = *(const V *)(P + (long)min(i, Kp - 1) * nhp + j0 + jj);
         }
 #pragma unroll
         for (int u = 0; u < NB; u++) {
             int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW, i = k0 + kk;
-            V x = *(const V *)(S + (long)(2 * min(i, Kp - 1)) * ldx + c0 + cc);
-            rb[u] = vkeep(x, i < Kp);
+            rb[u] = *(const V *)(S + (long)(2 * min(i, Kp - 1)) * ldx + c0 + cc);
         }
     };
-    auto store = [&]() {
+    auto store = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
             int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW;
-            vstore_lds<T>(&As[kk][jj], ra[u]);
+            vstore_lds<T>(&As[kk][jj], vkeep(ra[u], k0 + kk < Kp));
         }
 #pragma unroll
         for (int u = 0; u < NB; u++) {
             int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
-            vstore_lds<T>(&Bs[kk][cc], rb[u]);
+            vstore_lds<T>(&Bs[kk][cc], vkeep(rb[u], k0 + kk < Kp));
         }
     };
     // FP32 prefetches chunk k+1 during chunk k; FP64 (twice the registers per
@@ -271,7 +311,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     if (PREFETCH && Kp > 0) load(0);
     for (int k0 = 0; k0 < Kp; k0 += GEMM_BK) {
         if (!PREFETCH) load(k0);
-        store();
+        store(k0);
         __syncthreads();
         if (PREFETCH && k0 + GEMM_BK < Kp) load(k0 + GEMM_BK);
         if (nbj == WB && k0 + GEMM_BK <= Kp)
@@ -288,7 +328,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
             if constexpr (Vec<T>::swap) {
                 int j = j0 + wm + bi * 16 + (lane & 15);
                 int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
-                if (j < nhalf) {
+                if (j < nhalf && c < ncol) {  // padding columns are never read
                     acc_t e = E[bi][bj];
                     *(float4 *)(FN + (long)(par ? nlat - 1 - j : j) * ldf + c) = float4{e[0], e[1], e[2], e[3]};
                 }
@@ -297,7 +337,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
                 for (int r = 0; r < 4; r++) {
                     int j = j0 + wm + bi * 16 + MF<T>::row(lane, r);
                     int c = c0 + wn + bj * 16 + (lane & 15);
-                    if (j < nhalf) FN[(long)(par ? nlat - 1 - j : j) * ldf + c] = E[bi][bj][r];
+                    if (j < nhalf && c < ncol) FN[(long)(par ? nlat - 1 - j : j) * ldf + c] = E[bi][bj][r];
                 }
             }
         }
@@ -345,14 +385,17 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, const long *__restrict__ poff,
                                                   const int *__restrict__ keoff, const int *__restrict__ mtile_m,
                                                   const int *__restrict__ mtile_i, const T *__restrict__ fa,
-                                                  int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc, int ncol) {
+                                                  int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc, int ncol, int nmt) {
     typedef typename MF<T>::acc acc_t;
     typedef typename Vec<T>::t V;
     constexpr int VW = Vec<T>::n;
     constexpr int BN = GT<T>::BN, WB = GT<T>::WB;
     constexpr int NA = GEMM_BK * GEMM_BM / VW / 256, NB = GEMM_BK * BN / VW / 256;
-    const int tc = blockIdx.x;
-    const int m = mtile_m[blockIdx.y], i0 = mtile_i[blockIdx.y];
+    // logical tile mt * ntc + tc, contiguous per XCD: the column tiles of one
+    // (m, degree tile) share its Legendre block in that XCD's L2
+    const int ntc = gridDim.x / nmt;
+    const int L = xcd_tile(blockIdx.x, gridDim.x), tc = L % ntc, mt = L / ntc;
+    const int m = mtile_m[mt], i0 = mtile_i[mt];
     const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
     const T *P0 = plm + poff[m], *P1 = P0 + (long)Ke * nhp;
     const T *F0 = fa + (long)(2 * m) * nhp * ldf, *F1 = F0 + (long)nhp * ldf;
@@ -750,14 +793,6 @@ __device__ __forceinline__ void fft_stages(cplx<T> *buf, const cplx<T> *tw, int 
     }
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8).
-// Give each XCD a contiguous range of logical tiles, so that tiles sharing
-// cache lines (the field groups of one latitude pair) run on the same XCD and
-// meet in its L2 instead of fetching the same lines through eight L2s.
-__device__ inline int xcd_tile(int b, int nb) {
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + min(x, r) + k;
-}
 
 template <typename T> struct FftCfg;
 template <> struct FftCfg<float> { static constexpr int NFFT = 4, MAXN = 768; };
@@ -1218,7 +1253,7 @@ void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, in
             return;
         }
     }
-    hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc, 2 * (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff,
+    hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc * 2 * (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff,
                        p.keoff, sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj, ncol);
 }
 
@@ -1232,8 +1267,8 @@ void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int
             return;
         }
     }
-    hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc, p.n_mtiles), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
-                       p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol);
+    hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc * p.n_mtiles), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
+                       p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol, p.n_mtiles);
 }
 
 static Radices radices_of(int n, const int *r) {
