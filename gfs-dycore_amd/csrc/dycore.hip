@@ -1329,60 +1329,83 @@ __global__ __launch_bounds__(256) void k_si_stage(
 // product Y = M X on MFMA 16x16x4 (wave w computes output levels 16w..16w+15).
 // Real and imaginary parts are independent columns because all operators and
 // coefficients are real. Level vectors stay in LDS between the products.
-constexpr int SI_C = 16, SI_W = 2 * SI_C, SI_LD = SI_W + 4;
-typedef float si_f4 __attribute__((ext_vector_type(4)));
+// Coefficients per workgroup: 16 in FP32, 8 in FP64 (twice the bytes per value
+// in the same LDS); 2 C real columns, padded rows of LD values.
+template <typename T> struct SiCfg {
+    static constexpr int C = 16, W = 2 * C, LD = W + 4;
+};
+template <> struct SiCfg<double> {
+    static constexpr int C = 8, W = 2 * C, LD = W + 4;
+};
+// MFMA 16x16x4 in the precision of the stage: A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k; accumulator element r of a lane is row 4 (lane/16)
+// + r (f32) or lane/16 + 4 r (f64), column lane % 16.
+template <typename T> struct SiMma;
+template <> struct SiMma<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    __device__ static acc mma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <> struct SiMma<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    __device__ static acc mma(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
 
 // Y[i][col] = sum_k M[i][k] X[k][col], M given transposed (Mt[k * nl + i] = M[i][k]),
 // zero beyond nl levels
-__device__ inline void si_gemm(const float *__restrict__ Mt, int nl, const float (*X)[SI_LD], float (*Y)[SI_LD],
+template <typename T>
+__device__ inline void si_gemm(const T *__restrict__ Mt, int nl, const T (*X)[SiCfg<T>::LD], T (*Y)[SiCfg<T>::LD],
                                int wave, int lane) {
-    constexpr int NB = SI_W / 16;
-    si_f4 acc[NB];
+    constexpr int NB = SiCfg<T>::W / 16;
+    typename SiMma<T>::acc acc[NB];
 #pragma unroll
-    for (int b = 0; b < NB; b++) acc[b] = si_f4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NB; b++) acc[b] = typename SiMma<T>::acc{T(0), T(0), T(0), T(0)};
     const int i = wave * 16 + (lane & 15), kq = lane >> 4;
 #pragma unroll 4
     for (int k0 = 0; k0 < 64; k0 += 4) {
         const int k = k0 + kq;
-        const float a = (i < nl && k < nl) ? Mt[k * nl + i] : 0.f;
+        const T a = (i < nl && k < nl) ? Mt[k * nl + i] : T(0);
 #pragma unroll
-        for (int b = 0; b < NB; b++)
-            acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, X[k][b * 16 + (lane & 15)], acc[b], 0, 0, 0);
+        for (int b = 0; b < NB; b++) acc[b] = SiMma<T>::mma(a, X[k][b * 16 + (lane & 15)], acc[b]);
     }
 #pragma unroll
     for (int b = 0; b < NB; b++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) Y[wave * 16 + 4 * (lane >> 4) + r][b * 16 + (lane & 15)] = acc[b][r];
+        for (int r = 0; r < 4; r++) Y[wave * 16 + SiMma<T>::row(lane, r)][b * 16 + (lane & 15)] = acc[b][r];
 }
 
-// column sums over levels: out[col] = sum_{lev < nl} w[lev] X[lev][col]
-__device__ inline void si_colsum(const float *__restrict__ w, int nl, const float (*X)[SI_LD], float *out) {
+template <typename T>
+__device__ inline void si_colsum(const T *__restrict__ w, int nl, const T (*X)[SiCfg<T>::LD], T *out) {
     const int col = threadIdx.x;
-    if (col < SI_W) {
-        float s = 0.f;
+    if (col < SiCfg<T>::W) {
+        T s = T(0);
         for (int l = 0; l < nl; l++) s += w[l] * X[l][col];
         out[col] = s;
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_si_mfma(
-    float *__restrict__ spec, const float *__restrict__ spec0, float *__restrict__ ten0, float *__restrict__ ten1,
-    float *__restrict__ ten2, float *__restrict__ lin0, float *__restrict__ lin1, float *__restrict__ lin2,
-    const float *__restrict__ At, const float *__restrict__ Bt, const float *__restrict__ Dt,
-    const float *__restrict__ sv, const float *__restrict__ tor, const float *__restrict__ disspec,
-    const float *__restrict__ diffp, const float *__restrict__ dmpp, const int *__restrict__ kbydeg,
-    const int2 *__restrict__ tiles, int stage, Coeffs3 cf, float cimp, int do_implicit, int damp, int nl, int ntr,
-    int nfs, int nm, int ntrunc, float dt) {
-    __shared__ float S0[64][SI_LD], S1[64][SI_LD], S2[64][SI_LD], S3[64][SI_LD];
-    __shared__ float lpin[SI_W], lpc[SI_W], csum[SI_W];
-    __shared__ long sbase[SI_C], slbase[SI_C];
-    __shared__ int sk[SI_C];
+    T *__restrict__ spec, const T *__restrict__ spec0, T *__restrict__ ten0, T *__restrict__ ten1,
+    T *__restrict__ ten2, T *__restrict__ lin0, T *__restrict__ lin1, T *__restrict__ lin2,
+    const T *__restrict__ At, const T *__restrict__ Bt, const T *__restrict__ Dt,
+    const T *__restrict__ sv, const T *__restrict__ tor, const T *__restrict__ disspec,
+    const T *__restrict__ diffp, const T *__restrict__ dmpp, const int *__restrict__ kbydeg,
+    const int2 *__restrict__ tiles, int stage, Coeffs3 cf, T cimp, int do_implicit, int damp, int nl, int ntr,
+    int nfs, int nm, int ntrunc, T dt) {
+    constexpr int C = SiCfg<T>::C, W = SiCfg<T>::W, LD = SiCfg<T>::LD;
+    typedef cplx<T> V2;
+    __shared__ T S0[64][LD], S1[64][LD], S2[64][LD], S3[64][LD];
+    __shared__ T lpin[W], lpc[W], csum[W];
+    __shared__ long sbase[C], slbase[C];
+    __shared__ int sk[C];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int2 tl = tiles[blockIdx.x];
     const int n = tl.x, i0 = tl.y, per = (n + 1) * nm;
-    const int cnt = min(SI_C, per - i0);
+    const int cnt = min(C, per - i0);
     const long NFS = (long)nfs * nm, nlin = 2L * nl + 1;
-    if (tid < SI_C) {
+    if (tid < C) {
         long b = 0, lb = 0;
         int k = 0;
         if (tid < cnt) {
@@ -1396,38 +1419,38 @@ __global__ __launch_bounds__(256) void k_si_mfma(
         sk[tid] = k;
     }
     __syncthreads();
-    const float lap = -(float)n * (float)(n + 1);
-    float2 *sp = (float2 *)spec;
-    const float2 *sp0 = (const float2 *)spec0;
-    float2 *tn[3] = {(float2 *)ten0, (float2 *)ten1, (float2 *)ten2};
-    float2 *ln[3] = {(float2 *)lin0, (float2 *)lin1, (float2 *)lin2};
+    const T lap = -T(n) * T(n + 1);
+    V2 *sp = (V2 *)spec;
+    const V2 *sp0 = (const V2 *)spec0;
+    V2 *tn[3] = {(V2 *)ten0, (V2 *)ten1, (V2 *)ten2};
+    V2 *ln[3] = {(V2 *)lin0, (V2 *)lin1, (V2 *)lin2};
     // element e = c * nl + lev: one complex value (re, im) per element
     const int nel = cnt * nl;
     // vorticity and tracers: always explicit (plus damping after the last stage)
     for (int e = tid; e < nel * (1 + ntr); e += 256) {
         const int fld = e / nel, r0 = e - fld * nel, c = r0 / nl, lev = r0 - c * nl;
         const long off = sbase[c] + (fld == 0 ? 0L : 3L * nl + (long)(fld - 1) * nl) + lev;
-        float2 acc = make_float2(0.f, 0.f);
+        V2 acc = V2{T(0), T(0)};
         for (int r = 0; r <= stage; r++) {
-            const float2 t = tn[r][off];
+            const V2 t = tn[r][off];
             acc.x += cf.a[r] * t.x;
             acc.y += cf.a[r] * t.y;
         }
-        const float2 o = sp0[off];
-        float2 v = make_float2(o.x + dt * acc.x, o.y + dt * acc.y);
+        const V2 o = sp0[off];
+        V2 v = V2{o.x + dt * acc.x, o.y + dt * acc.y};
         if (damp) {
-            const float ds = disspec[sk[c]];
-            const float f = (fld == 0) ? 1.f - (ds * diffp[lev] - dmpp[lev]) * dt : 1.f - ds * diffp[lev] * dt;
+            const T ds = disspec[sk[c]];
+            const T f = (fld == 0) ? T(1) - (ds * diffp[lev] - dmpp[lev]) * dt : T(1) - ds * diffp[lev] * dt;
             v.x /= f;
             v.y /= f;
         }
         sp[off] = v;
     }
     // stage input: tv -> S0, div -> S1 (zero padded), ln ps -> lpin
-    const float2 *sin_ = (stage == 0) ? sp0 : sp;
-    for (int e = tid; e < 64 * SI_C; e += 256) {
+    const V2 *sin_ = (stage == 0) ? sp0 : sp;
+    for (int e = tid; e < 64 * C; e += 256) {
         const int c = e >> 6, lev = e & 63;
-        float2 tv = make_float2(0.f, 0.f), dv = tv;
+        V2 tv = V2{T(0), T(0)}, dv = tv;
         if (c < cnt && lev < nl) {
             tv = sin_[sbase[c] + 2L * nl + lev];
             dv = sin_[sbase[c] + nl + lev];
@@ -1437,46 +1460,46 @@ __global__ __launch_bounds__(256) void k_si_mfma(
         S1[lev][2 * c] = dv.x;
         S1[lev][2 * c + 1] = dv.y;
     }
-    if (tid < SI_C) {
-        const float2 v = (tid < cnt) ? sin_[sbase[tid] + 3L * nl + (long)ntr * nl] : make_float2(0.f, 0.f);
+    if (tid < C) {
+        const V2 v = (tid < cnt) ? sin_[sbase[tid] + 3L * nl + (long)ntr * nl] : V2{T(0), T(0)};
         lpin[2 * tid] = v.x;
         lpin[2 * tid + 1] = v.y;
     }
     __syncthreads();
     // linear terms of the stage input (run.f90:329-335)
-    si_gemm(At, nl, S0, S2, wave, lane);
-    si_gemm(Bt, nl, S1, S3, wave, lane);
-    si_colsum(sv, nl, S1, csum);
+    si_gemm<T>(At, nl, S0, S2, wave, lane);
+    si_gemm<T>(Bt, nl, S1, S3, wave, lane);
+    si_colsum<T>(sv, nl, S1, csum);
     __syncthreads();
     // store them, remove them from this stage's tendency, explicit combination
-    float2 *L = ln[stage], *Tn = tn[stage];
+    V2 *L = ln[stage], *Tn = tn[stage];
     for (int e = tid; e < nel; e += 256) {
         const int c = e / nl, lev = e - c * nl;
         const long lb = slbase[c] + lev, b = sbase[c] + lev;
-        float2 ddl, dtl;
+        V2 ddl, dtl;
         ddl.x = -lap * (S2[lev][2 * c] + tor[lev] * lpin[2 * c]);
         ddl.y = -lap * (S2[lev][2 * c + 1] + tor[lev] * lpin[2 * c + 1]);
         dtl.x = -S3[lev][2 * c];
         dtl.y = -S3[lev][2 * c + 1];
         L[lb] = ddl;
         L[lb + nl] = dtl;
-        float2 dd = Tn[b + nl], dT = Tn[b + 2L * nl];
+        V2 dd = Tn[b + nl], dT = Tn[b + 2L * nl];
         dd.x -= ddl.x;
         dd.y -= ddl.y;
         dT.x -= dtl.x;
         dT.y -= dtl.y;
         Tn[b + nl] = dd;
         Tn[b + 2L * nl] = dT;
-        float2 a1 = make_float2(cf.a[stage] * dd.x + cf.aa[stage] * ddl.x, cf.a[stage] * dd.y + cf.aa[stage] * ddl.y);
-        float2 a2 = make_float2(cf.a[stage] * dT.x + cf.aa[stage] * dtl.x, cf.a[stage] * dT.y + cf.aa[stage] * dtl.y);
+        V2 a1 = V2{(T)(cf.a[stage] * dd.x + cf.aa[stage] * ddl.x), (T)(cf.a[stage] * dd.y + cf.aa[stage] * ddl.y)};
+        V2 a2 = V2{(T)(cf.a[stage] * dT.x + cf.aa[stage] * dtl.x), (T)(cf.a[stage] * dT.y + cf.aa[stage] * dtl.y)};
         for (int r = 0; r < stage; r++) {
-            const float2 td = tn[r][b + nl], tt = tn[r][b + 2L * nl], ld = ln[r][lb], lt = ln[r][lb + nl];
+            const V2 td = tn[r][b + nl], tt = tn[r][b + 2L * nl], ld = ln[r][lb], lt = ln[r][lb + nl];
             a1.x += cf.a[r] * td.x + cf.aa[r] * ld.x;
             a1.y += cf.a[r] * td.y + cf.aa[r] * ld.y;
             a2.x += cf.a[r] * tt.x + cf.aa[r] * lt.x;
             a2.y += cf.a[r] * tt.y + cf.aa[r] * lt.y;
         }
-        const float2 d0 = sp0[b + nl], t0 = sp0[b + 2L * nl];
+        const V2 d0 = sp0[b + nl], t0 = sp0[b + 2L * nl];
         S1[lev][2 * c] = d0.x + dt * a1.x;  // dvc
         S1[lev][2 * c + 1] = d0.y + dt * a1.y;
         S0[lev][2 * c] = t0.x + dt * a2.x;  // tvc
@@ -1484,47 +1507,47 @@ __global__ __launch_bounds__(256) void k_si_mfma(
     }
     if (tid < 2 * cnt) {
         const int c = tid >> 1, p = tid & 1;
-        float *Lf = (float *)L, *Tf = (float *)Tn;
-        const float *s0f = (const float *)sp0;
+        T *Lf = (T *)L, *Tf = (T *)Tn;
+        const T *s0f = (const T *)sp0;
         const long lb = 2 * (slbase[c] + 2L * nl) + p, b = 2 * (sbase[c] + 3L * nl + (long)ntr * nl) + p;
-        const float dpl = -csum[tid];
+        const T dpl = -csum[tid];
         Lf[lb] = dpl;
-        const float dp = Tf[b] - dpl;
+        const T dp = Tf[b] - dpl;
         Tf[b] = dp;
-        float a3 = cf.a[stage] * dp + cf.aa[stage] * dpl;
-        for (int r = 0; r < stage; r++) a3 += cf.a[r] * ((const float *)tn[r])[b] + cf.aa[r] * ((const float *)ln[r])[lb];
+        T a3 = cf.a[stage] * dp + cf.aa[stage] * dpl;
+        for (int r = 0; r < stage; r++) a3 += cf.a[r] * ((const T *)tn[r])[b] + cf.aa[r] * ((const T *)ln[r])[lb];
         lpc[tid] = s0f[b] + dt * a3;
     }
     __syncthreads();
     if (do_implicit) {
         // rhs, solve with d_hyb_m(:,:,n+1,stage) and back-substitution (run.f90:344-349)
-        si_gemm(At, nl, S0, S2, wave, lane);
+        si_gemm<T>(At, nl, S0, S2, wave, lane);
         __syncthreads();
-        for (int e = tid; e < 64 * SI_W; e += 256) {
-            const int lev = e / SI_W, col = e - lev * SI_W;
-            S3[lev][col] = (lev < nl) ? S1[lev][col] - cimp * lap * dt * (S2[lev][col] + tor[lev] * lpc[col]) : 0.f;
+        for (int e = tid; e < 64 * W; e += 256) {
+            const int lev = e / W, col = e - lev * W;
+            S3[lev][col] = (lev < nl) ? S1[lev][col] - cimp * lap * dt * (S2[lev][col] + tor[lev] * lpc[col]) : T(0);
         }
         __syncthreads();
-        si_gemm(Dt + ((long)stage * (ntrunc + 1) + n) * nl * nl, nl, S3, S2, wave, lane);
+        si_gemm<T>(Dt + ((long)stage * (ntrunc + 1) + n) * nl * nl, nl, S3, S2, wave, lane);
         __syncthreads();
-        si_gemm(Bt, nl, S2, S3, wave, lane);
-        si_colsum(sv, nl, S2, csum);
+        si_gemm<T>(Bt, nl, S2, S3, wave, lane);
+        si_colsum<T>(sv, nl, S2, csum);
         __syncthreads();
-        for (int e = tid; e < 64 * SI_W; e += 256) {
-            const int lev = e / SI_W, col = e - lev * SI_W;
+        for (int e = tid; e < 64 * W; e += 256) {
+            const int lev = e / W, col = e - lev * W;
             S0[lev][col] -= cimp * dt * S3[lev][col];
             S1[lev][col] = S2[lev][col];
         }
-        if (tid < SI_W) lpc[tid] -= cimp * dt * csum[tid];
+        if (tid < W) lpc[tid] -= cimp * dt * csum[tid];
         __syncthreads();
     }
     for (int e = tid; e < nel; e += 256) {
         const int c = e / nl, lev = e - c * nl;
         const long b = sbase[c] + lev;
-        float2 dv = make_float2(S1[lev][2 * c], S1[lev][2 * c + 1]), tv = make_float2(S0[lev][2 * c], S0[lev][2 * c + 1]);
+        V2 dv = V2{S1[lev][2 * c], S1[lev][2 * c + 1]}, tv = V2{S0[lev][2 * c], S0[lev][2 * c + 1]};
         if (damp) {
-            const float ds = disspec[sk[c]];
-            const float fv = 1.f - (ds * diffp[lev] - dmpp[lev]) * dt, ft = 1.f - ds * diffp[lev] * dt;
+            const T ds = disspec[sk[c]];
+            const T fv = T(1) - (ds * diffp[lev] - dmpp[lev]) * dt, ft = T(1) - ds * diffp[lev] * dt;
             dv.x /= fv;
             dv.y /= fv;
             tv.x /= ft;
@@ -1533,7 +1556,7 @@ __global__ __launch_bounds__(256) void k_si_mfma(
         sp[b + nl] = dv;
         sp[b + 2L * nl] = tv;
     }
-    if (tid < cnt) sp[sbase[tid] + 3L * nl + (long)ntr * nl] = make_float2(lpc[2 * tid], lpc[2 * tid + 1]);
+    if (tid < cnt) sp[sbase[tid] + 3L * nl + (long)ntr * nl] = V2{lpc[2 * tid], lpc[2 * tid + 1]};
 }
 
 // time-split physics update (run.f90:526-538)
@@ -1835,7 +1858,7 @@ template <typename T> int Engine<T>::dyn_init() {
         B->kbydeg = (int *)p;
         std::vector<int2> tiles;
         for (int n = 0; n <= T_; n++)
-            for (int i0 = 0; i0 < (n + 1) * nm; i0 += SI_C) tiles.push_back(make_int2(n, i0));
+            for (int i0 = 0; i0 < (n + 1) * nm; i0 += SiCfg<T>::C) tiles.push_back(make_int2(n, i0));
         if ((rc = dupload<int2>(&p, tiles, stream))) return rc;
         B->si_tiles = (int2 *)p;
         B->n_si_tiles = (int)tiles.size();
@@ -2113,19 +2136,16 @@ template <typename T> struct Step {
             cf.aa[r] = TAB_AA[stage][r];
         }
         int do_imp = (stage < 2 || std::fabs(TAB_IMP[stage]) > 1.e-5) ? 1 : 0;
-        if constexpr (sizeof(T) == 4) {
-            if (!c.explicit_ && B.nl <= 64) {
-                hipLaunchKernelGGL(k_si_mfma, dim3((unsigned)B.n_si_tiles), dim3(256), 0, s, (float *)B.spec,
-                                   (const float *)B.spec0, (float *)B.ten[0], (float *)B.ten[1], (float *)B.ten[2],
-                                   (float *)B.lin[0], (float *)B.lin[1], (float *)B.lin[2], (const float *)B.at,
-                                   (const float *)B.bt, (const float *)B.dt3, (const float *)B.sv,
-                                   (const float *)B.tor, (const float *)B.disspec, (const float *)B.diffp,
-                                   (const float *)B.dmpp, B.kbydeg, (const int2 *)B.si_tiles, stage, cf,
-                                   (float)TAB_IMP[stage], do_imp, damp ? 1 : 0, B.nl, B.ntr, B.nfs, B.nm,
-                                   E.plan.ntrunc, (float)c.dt);
-                mark("si_stage");
-                return;
-            }
+        if (!c.explicit_ && B.nl <= 64) {
+            hipLaunchKernelGGL(k_si_mfma<T>, dim3((unsigned)B.n_si_tiles), dim3(256), 0, s, (T *)B.spec,
+                               (const T *)B.spec0, (T *)B.ten[0], (T *)B.ten[1], (T *)B.ten[2], (T *)B.lin[0],
+                               (T *)B.lin[1], (T *)B.lin[2], (const T *)B.at, (const T *)B.bt, (const T *)B.dt3,
+                               (const T *)B.sv, (const T *)B.tor, (const T *)B.disspec, (const T *)B.diffp,
+                               (const T *)B.dmpp, B.kbydeg, (const int2 *)B.si_tiles, stage, cf,
+                               (T)TAB_IMP[stage], do_imp, damp ? 1 : 0, B.nl, B.ntr, B.nfs, B.nm, E.plan.ntrunc,
+                               (T)c.dt);
+            mark("si_stage");
+            return;
         }
         long items = (long)E.plan.nlm * B.nm;
         hipLaunchKernelGGL(k_si_stage<T>, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, (cplx<T> *)B.spec,
