@@ -427,17 +427,20 @@ static int splits(const MlpNet *net, int Mb) {
 // forward of Mb (multiple of TM) rows already in L[0].a / at; output in net->y.
 // Training keeps the pre-activations and the transposed activations for the
 // backward pass; inference writes only what the next layer reads.
-static void forward_batch(MlpNet *net, int Mb, bool train, hipStream_t s) {
+// out / ldout / nrow: where the last layer writes (default net->y, all Mb rows);
+// inference can hand the caller's rows straight to the output GEMM.
+static void forward_batch(MlpNet *net, int Mb, bool train, hipStream_t s, float *out = nullptr, long ldout = 0,
+                          int nrow = 0) {
     const int nL = (int)net->L.size(), Bp = net->Bp;
     for (int l = 0; l < nL; l++) {
         MlpLayer &Ly = net->L[l];
         dim3 g((unsigned)((Ly.N + TN - 1) / TN), (unsigned)(Mb / TM), 1);
         Epi e{};
         if (l == nL - 1) {
-            e.c = net->y;
-            e.ldc = Ly.N;  // = Np
-            hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt, (long)Ly.Kp, Mb,
-                               Ly.N, Ly.Kp, e);
+            e.c = out ? out : net->y;
+            e.ldc = out ? ldout : Ly.N;  // = Np
+            hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt, (long)Ly.Kp,
+                               out ? nrow : Mb, Ly.N, Ly.Kp, e);
         } else {
             MlpLayer &Nx = net->L[l + 1];
             e.z = train ? Ly.z : nullptr;
@@ -465,9 +468,14 @@ int mlp_forward(MlpNet *net, const float *x, int ldx, int n, float *y, int ldy, 
     for (int r0 = 0; r0 < n; r0 += net->Bp) {
         const int nb = std::min(net->Bp, n - r0);
         const int Mb = input_batch(net, x + (long)r0 * ldx, ldx, nb, false, s);
-        forward_batch(net, Mb, false, s);
-        GFSDC_CHECK(hipMemcpy2DAsync(y + (long)r0 * ldy, sizeof(float) * ldy, net->y, sizeof(float) * net->Np,
-                                     sizeof(float) * net->nout, nb, hipMemcpyDeviceToDevice, s));
+        if (net->Np == net->nout && ldy >= net->Np) {
+            // no padded outputs: the last GEMM writes the nb rows of y itself
+            forward_batch(net, Mb, false, s, y + (long)r0 * ldy, ldy, nb);
+        } else {
+            forward_batch(net, Mb, false, s);
+            GFSDC_CHECK(hipMemcpy2DAsync(y + (long)r0 * ldy, sizeof(float) * ldy, net->y, sizeof(float) * net->Np,
+                                         sizeof(float) * net->nout, nb, hipMemcpyDeviceToDevice, s));
+        }
     }
     GFSDC_CHECK(hipGetLastError());
     return 0;
