@@ -91,6 +91,10 @@ def algorithmic_work(name, ctx, esz):
     nlat, nlon, T, nlm = ctx.nlat, ctx.nlon, ctx.ntrunc, ctx.nlm
     nl, nm = ctx.nlev, ctx.nmem
     nsyn = (7 + 3 * ctx.ntrac) * nl + 3
+    if ctx.cfg.physics != gfsdc.PHYS_NONE:
+        # three dynamics passes of nsyn fields and the physics pass, which skips
+        # the temperature and tracer gradients: the mean fields per launch
+        nsyn = (3 * nsyn + (5 + ctx.ntrac) * nl + 3) / 4
     ng = nlat * nlon
     four = (T + 1) * nlat * 2 * esz  # Fourier coefficients of one field, both hemispheres
     if name == "leg_synth":

@@ -109,6 +109,12 @@ struct DycoreBuffers {
     unsigned long long bctr = 0;                 // training batch counter
     double *diag = nullptr;                      // FP64 4 per member
     SynthDesc *sdesc = nullptr;                  // nm*nsyn
+    // the physics pass synthesises every field but the temperature and tracer
+    // gradients (nph per member): descriptors, 1/cos flags and grid slots
+    int nph = 0;
+    SynthDesc *sdesc_ph = nullptr;
+    unsigned char *rflag_ph = nullptr;
+    int *slot_ph = nullptr;
     AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr, *adesc_sp = nullptr;
     unsigned char *rflag_syn = nullptr, *rflag_an = nullptr, *rflag_hs = nullptr, *rflag_sp = nullptr;
     // host copies of the FP64 operators
@@ -2020,6 +2026,31 @@ template <typename T> int Engine<T>::dyn_init() {
             pp[3 * nl + k] = k < nl ? AnalDesc{AK_COPY, p0 + 3 * nl + k, -1, rdt, 0, 0, -1, 0.0} : zero;
         pp[3 * nl + ntr * nl] = cfg.massfix ? AnalDesc{AK_COPY, p0 + 4 * nl, -1, 1.0, 0, 0, -1, 0.0} : zero;
     }
+    {
+        // grid fields the physics pass needs: u, v, vrt, div, tv (0..5nl), the
+        // tracers (7nl..7nl+ntr nl) and ln ps with its gradient (last 3)
+        std::vector<int> keep;
+        for (int f = 0; f < 5 * nl; f++) keep.push_back(f);
+        for (int f = 7 * nl; f < 7 * nl + ntr * nl; f++) keep.push_back(f);
+        for (int f = 7 * nl + 3 * ntr * nl; f < B->nsyn; f++) keep.push_back(f);
+        B->nph = (int)keep.size();
+        std::vector<SynthDesc> dph;
+        std::vector<unsigned char> rph;
+        std::vector<int> sph;
+        for (int mem = 0; mem < nm; mem++)
+            for (int f : keep) {
+                dph.push_back(sd[(size_t)mem * B->nsyn + f]);
+                rph.push_back(rs[(size_t)mem * B->nsyn + f]);
+                sph.push_back(mem * B->nsyn + f);
+            }
+        void *q;
+        if ((rc = dupload<SynthDesc>(&q, dph, stream))) return rc;
+        B->sdesc_ph = (SynthDesc *)q;
+        if ((rc = dupload<unsigned char>(&q, rph, stream))) return rc;
+        B->rflag_ph = (unsigned char *)q;
+        if ((rc = dupload<int>(&q, sph, stream))) return rc;
+        B->slot_ph = (int *)q;
+    }
     void *p;
     if ((rc = dupload<SynthDesc>(&p, sd, stream))) return rc;
     B->sdesc = (SynthDesc *)p;
@@ -2055,7 +2086,7 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
-                    B->jflag, B->jctr, B->gmean, B->dak};
+                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph};
     if (B->mlp) mlp_destroy(B->mlp);
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -2091,6 +2122,19 @@ template <typename T> struct Step {
         launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
         mark("leg_synth");
         launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_syn, s, (const T *)B.gmean);
+        mark("fft_synth");
+    }
+    // the same chain for the physics pass, without the temperature and tracer
+    // gradients (written to their grid slots through slot_ph)
+    void synth_phys() {
+        const Plan<T> &p = E.plan;
+        int nf = B.nm * B.nph;
+        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc_ph, nf, W.sx, W.ldc, 1.0, s,
+                            (T *)B.gmean);
+        mark("synth_gather");
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+        mark("leg_synth");
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_ph, s, (const T *)B.gmean, B.slot_ph);
         mark("fft_synth");
     }
     void columns(int inverse_only, bool colq = false) {
@@ -2159,7 +2203,7 @@ template <typename T> struct Step {
     }
     void physics() {
         const gfsdc_cfg &c = E.cfg;
-        synth_all();
+        synth_phys();
         columns(1);
         if (c.physics == GFSDC_PHYS_SIMPLE) {
             if (B.mlp_on)

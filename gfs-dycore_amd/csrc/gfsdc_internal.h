@@ -97,7 +97,8 @@ template <typename T>
 void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s);
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                      const unsigned char *rcos_flag, hipStream_t s, const T *mean = nullptr);
+                      const unsigned char *rcos_flag, hipStream_t s, const T *mean = nullptr,
+                      const int *slot = nullptr);
 template <typename T>
 void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rcos_flag,
                      T *fa, int ldf, hipStream_t s);

@@ -808,9 +808,9 @@ static_assert(fft_pitch(FftCfg<double>::MAXN) * 8 * 16 + FftCfg<double>::MAXN * 
 template <typename T, int NFFT, int MAXN, int NC>
 __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int ldf, T *__restrict__ grid,
                                                    long gstride, int nf, const unsigned char *__restrict__ rflag,
-                                                   const T *__restrict__ mean, const T *__restrict__ rcos,
-                                                   const cplx<T> *__restrict__ tw, int nlon, int nlat, int ntrunc,
-                                                   Radices rd) {
+                                                   const T *__restrict__ mean, const int *__restrict__ slot,
+                                                   const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
+                                                   int nlon, int nlat, int ntrunc, Radices rd) {
     constexpr int LN = NC ? NC : MAXN;
     __shared__ cplx<T> buf[NFFT * fft_pitch(LN)];
     __shared__ cplx<T> tws[LN];
@@ -867,7 +867,8 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
         if (f >= nf) break;
         const T s = rflag[f] ? rc : T(1);
         const T c = mean ? mean[f] : T(0);
-        T *gn = grid + f * gstride + (long)j * nlon, *gs = grid + f * gstride + (long)j2 * nlon;
+        const long gf = slot ? slot[f] : f;  // grid field the transform is written to
+        T *gn = grid + gf * gstride + (long)j * nlon, *gs = grid + gf * gstride + (long)j2 * nlon;
         const cplx<T> *bq = buf + q * NP;
         for (int i = tid; i < N; i += 256) {
             const cplx<T> z = bq[fft_pad(i)];
@@ -1280,11 +1281,11 @@ static Radices radices_of(int n, const int *r) {
 
 template <typename T, int NF>
 static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                         const unsigned char *rflag, const T *mean, hipStream_t s) {
+                         const unsigned char *rflag, const T *mean, const int *slot, hipStream_t s) {
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_SYN(NC)                                                                                             \
-    hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, mean,   \
+    hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, mean, slot, \
                        p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad))
     switch (p.nlon) {
     case 768: FFT_SYN(768); break;
@@ -1317,11 +1318,11 @@ static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, c
 // more workgroups per CU (LDS = fields * nlon * sizeof(complex))
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                      const unsigned char *rflag, hipStream_t s, const T *mean) {
+                      const unsigned char *rflag, hipStream_t s, const T *mean, const int *slot) {
     switch (p.fft_nf) {
-    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, s); break;
-    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, s); break;
-    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, s); break;
+    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, s); break;
+    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, s); break;
+    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, s); break;
     }
 }
 
@@ -1353,7 +1354,7 @@ void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc 
     template void launch_leg_synth<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);            \
     template void launch_leg_anal<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);             \
     template void launch_fft_synth<T>(const Plan<T> &, const T *, int, T *, long, int, const unsigned char *,  \
-                                      hipStream_t, const T *);                                                 \
+                                      hipStream_t, const T *, const int *);                                    \
     template void launch_fft_anal<T>(const Plan<T> &, const T *, long, int, const unsigned char *, T *, int,   \
                                      hipStream_t);                                                             \
     template void launch_anal_scatter<T>(const Plan<T> &, const T *, int, const AnalDesc *, int, cplx<T> *,    \
