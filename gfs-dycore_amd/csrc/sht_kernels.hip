@@ -783,12 +783,54 @@ constexpr RadPlan rad_plan(int n) {
     return p;
 }
 
+// One Stockham stage with one field per wavefront (4 fields per workgroup):
+// the field is the wave index and every divisor is a compile-time constant, so
+// a butterfly's LDS addresses cost a few integer operations instead of the
+// divisions and generic index arithmetic of fft_stage.
+template <typename T, int R, int N, int Ns>
+__device__ __forceinline__ void fft_stage_w(cplx<T> *buf, const cplx<T> *__restrict__ tw, int sign) {
+    constexpr int nbf = N / R, NB = (nbf + 63) / 64, NP = fft_pitch(N);
+    const int lane = threadIdx.x & 63;
+    cplx<T> *bq = buf + (threadIdx.x >> 6) * NP;
+    cplx<T> v[NB][R];
+#pragma unroll
+    for (int bb = 0; bb < NB; bb++) {
+        const int j = lane + 64 * bb;
+        if (nbf % 64 == 0 || j < nbf) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[bb][r] = bq[fft_pad(j + r * nbf)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bb = 0; bb < NB; bb++) {
+        const int j = lane + 64 * bb;
+        if (nbf % 64 == 0 || j < nbf) {
+            const int jm = j % Ns;
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                cplx<T> t = tw[Ns - 1 + (r - 1) * Ns + jm];
+                if (sign > 0) t.y = -t.y;
+                v[bb][r] = cmul(v[bb][r], t);
+            }
+            small_dft<T, R>(v[bb], sign);
+            const int idx = (j / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; r++) bq[fft_pad(idx + r * Ns)] = v[bb][r];
+        }
+    }
+    __syncthreads();
+}
+
 template <typename T, int NFFT, int N, int S, int Ns>
 __device__ __forceinline__ void fft_stages(cplx<T> *buf, const cplx<T> *tw, int sign) {
     constexpr RadPlan P = rad_plan(N);
     if constexpr (S < P.n) {
         constexpr int R = P.r[S];
-        fft_stage<T, R, (NFFT * N / R + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
+        if constexpr (NFFT == 4)
+            fft_stage_w<T, R, N, Ns>(buf, tw, sign);
+        else
+            fft_stage<T, R, (NFFT * N / R + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
         fft_stages<T, NFFT, N, S + 1, Ns * R>(buf, tw, sign);
     }
 }
