@@ -438,6 +438,8 @@ template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa) {
 
 // Held-Suarez forcing on the post-dynamics grid fields (phy_run_hs.f90:42-77)
 // and the dry-mass-fixer field log(ps * pcorr) (dyn_run.f90:626-631).
+constexpr int HS_LCH = 8;  // level chunks of the Held-Suarez kernel (levels are independent)
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, const T *__restrict__ psg,
                                                     T *__restrict__ gan, const T *__restrict__ ak_,
@@ -451,8 +453,10 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
         bk[i] = bk_[i];
     }
     __syncthreads();
+    // the levels are independent: blockIdx.y = member * HS_LCH + level chunk
     const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int mem = blockIdx.y;
+    const int mem = blockIdx.y / HS_LCH, ch = blockIdx.y - mem * HS_LCH;
+    const int CS = (nl + HS_LCH - 1) / HS_LCH, L0 = ch * CS + 1, L1 = min(nl, (ch + 1) * CS);
     if (col >= ng) return;
     const T *G = gsyn + (long)mem * nsyn * ng + col;
     const T *VRT = G + 2L * nl * ng, *DIV = G + 3L * nl * ng, *TV = G + 4L * nl * ng;
@@ -464,7 +468,7 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
     const T kdrag = T(SGLF(1.f / 86400.f)), krada = T(SGLF(1.f / (40.f * 86400.f)));
     const T kradb = T(SGLF(1.f / (4.f * 86400.f))), p0 = T(1.e5), deltmp = T(60);
     const T expo = T(SGLF(2.f / 7.f));
-    for (int L = 1; L <= nl; L++) {
+    for (int L = L0; L <= L1; L++) {
         int K = nl + 1 - L;
         T p1 = ak[K - 1] + bk[K - 1] * ps, p2 = ak[K] + bk[K] * ps;
         T prs = phillips_prs(p1, p2, kappa);
@@ -478,7 +482,7 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
         A[(long)nl * ng + o] = -(blprof * kdrag) * VRT[o];
         A[2L * nl * ng + o] = -(blprof * kdrag) * DIV[o];
     }
-    if (massfix) {
+    if (massfix && ch == 0) {
         // log(ps * pcorr) - c0 = ln ps' + log(pcorr)
         const T *LNPS = G + (7L + 3L * ntr) * nl * ng;
         A[3L * nl * ng] = LNPS[0] + (T)log(pdryini[mem] / pmean[mem]);
@@ -2223,7 +2227,7 @@ template <typename T> struct Step {
         }
         if (c.physics == GFSDC_PHYS_HS) {
             area_sums(nullptr, c.massfix ? B.pmean : (double *)nullptr, nullptr, false);
-            dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+            dim3 g((unsigned)((B.ng + 255) / 256), B.nm * HS_LCH);
             hipLaunchKernelGGL(k_hs_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg, (T *)B.gan,
                                (const T *)B.ak, (const T *)B.bk, (const T *)B.sinlat, B.pdryini, B.pmean, B.nl,
                                B.ntr, B.nsyn, B.nhs, B.ng, E.plan.nlon, (T)(c.rd / c.cp), c.massfix);
@@ -2873,7 +2877,7 @@ template <typename T> int Engine<T>::hs_phytend(void *dvrt, void *ddiv, void *dt
         return GFSDC_ESTATE;
     }
     Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
-    dim3 g((unsigned)((B->ng + 255) / 256), B->nm);
+    dim3 g((unsigned)((B->ng + 255) / 256), B->nm * HS_LCH);
     hipLaunchKernelGGL(k_hs_columns<T>, g, dim3(256), 0, stream, (const T *)B->gsyn, (const T *)B->psg,
                        (T *)B->gan, (const T *)B->ak, (const T *)B->bk, (const T *)B->sinlat, B->pdryini, B->pmean,
                        B->nl, B->ntr, B->nsyn, B->nhs, B->ng, plan.nlon, (T)(cfg.rd / cfg.cp), 0);
