@@ -316,7 +316,7 @@ def run_moist(args):
     ctx.autotune()
     mlp = None
     if args.moist == "mlp":
-        ctx.mlp_init(512, 3, 32768, 1 << 17, seed=1)
+        ctx.mlp_init(512, 3, ctx.nlat * ctx.nlon, 1 << 17, seed=1)  # one forward batch = the whole grid
         ctx.mlp_collect(1, 8)
         ctx.mlp_normalize()
         ctx.mlp_enable(True, judge_stride=50, threshold=1e30)

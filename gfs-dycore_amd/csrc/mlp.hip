@@ -34,6 +34,15 @@ __host__ __device__ inline uint16_t f2bf(float f) {
 }
 
 __device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// forward GELU with erf from Abramowitz & Stegun (1964) 7.1.26 (|error| <= 1.5e-7,
+// far below the bf16 rounding of the activations) on the hardware exp2 / rcp
+__device__ inline float gelu_fast(float x) {
+    const float z = x * 0.70710678118654752f, a = fabsf(z);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * a);
+    const float p = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float y = 1.0f - p * __builtin_amdgcn_exp2f(-a * a * 1.4426950408889634f);
+    return 0.5f * x * (1.0f + copysignf(y, z));
+}
 __device__ inline float dgelu(float x) {
     return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
@@ -53,12 +62,18 @@ struct Epi {
 
 // 128 x 128 output tile per workgroup (4 waves of 64 x 64 = 4 x 4 MFMA blocks),
 // k in chunks of 32 staged through LDS; blockIdx.z selects a k slice of length Kc.
-template <int EPI>
+// SWAP multiplies B A^T instead of A B^T, so a lane's four accumulators are four
+// consecutive columns of one output row: vector stores of the forward outputs
+// (the training forward, which also writes the transposed activations, keeps
+// four consecutive rows per lane instead).
+template <int EPI, bool SWAP = false>
 __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A, long lda,
                                                  const uint16_t *__restrict__ B, long ldb, int M, int N, int Kc,
                                                  Epi e) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[TM][TK + LPAD];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][TK + LPAD];
+    // two LDS stages: chunk k+1 is stored into the other stage while chunk k is
+    // multiplied, so one barrier per chunk suffices
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][TM][TK + LPAD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][TN][TK + LPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const long k0 = (long)blockIdx.z * Kc;
@@ -78,27 +93,64 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
             rb[u] = (n0 + r < N) ? *(const us8 *)(B + (long)(n0 + r) * ldb + kk + q) : zero;
         }
     };
-    load(k0);
-    for (long kk = k0; kk < k0 + Kc; kk += TK) {
+    auto stage = [&](int b) {
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             const int c = tid + 256 * u, r = c >> 2, q = (c & 3) * 8;
-            *(us8 *)&As[r][q] = ra[u];
-            *(us8 *)&Bs[r][q] = rb[u];
+            *(us8 *)&As[b][r][q] = ra[u];
+            *(us8 *)&Bs[b][r][q] = rb[u];
         }
-        __syncthreads();
-        if (kk + TK < k0 + Kc) load(kk + TK);
+    };
+    load(k0);
+    stage(0);
+    __syncthreads();
+    int b = 0;
+    for (long kk = k0; kk < k0 + Kc; kk += TK) {
+        const bool next = kk + TK < k0 + Kc;
+        if (next) load(kk + TK);
         bf16x8 fa[4], fb[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            fa[i] = __builtin_bit_cast(bf16x8, *(const us8 *)&As[wm + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-            fb[i] = __builtin_bit_cast(bf16x8, *(const us8 *)&Bs[wn + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            fa[i] = __builtin_bit_cast(bf16x8, *(const us8 *)&As[b][wm + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            fb[i] = __builtin_bit_cast(bf16x8, *(const us8 *)&Bs[b][wn + i * 16 + (lane & 15)][8 * (lane >> 4)]);
         }
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 4; j++) {
+                if constexpr (SWAP)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+                else
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            }
+        // the other stage was last read before the previous barrier: free to fill
+        if (next) stage(b ^ 1);
         __syncthreads();
+        b ^= 1;
+    }
+    if constexpr (SWAP) {
+        // lane holds row lane % 16, columns 4 (lane / 16) + r of each 16 x 16 block
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int m = m0 + wm + i * 16 + (lane & 15);
+            if (m >= M) continue;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int n = n0 + wn + j * 16 + 4 * (lane >> 4);  // N is a multiple of 32: n + 3 < N
+                if (n >= N) continue;
+                const f4 v = acc[i][j];
+                if constexpr (EPI == EPI_F32) {
+                    *(float4 *)(e.c + (long)m * e.ldc + n) = float4{v[0], v[1], v[2], v[3]};
+                } else {
+                    if (e.z) *(float4 *)(e.z + (long)m * e.ldz + n) = float4{v[0], v[1], v[2], v[3]};
+                    uint2 pk;
+                    pk.x = (uint32_t)f2bf(gelu_fast(v[0])) | ((uint32_t)f2bf(gelu_fast(v[1])) << 16);
+                    pk.y = (uint32_t)f2bf(gelu_fast(v[2])) | ((uint32_t)f2bf(gelu_fast(v[3])) << 16);
+                    *(uint2 *)(e.o + (long)m * e.ldo + n) = pk;
+                }
+            }
+        }
+        return;
     }
     // lane holds rows 4 (lane / 16) + r, column lane % 16 of each 16 x 16 block
 #pragma unroll
@@ -121,7 +173,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
                     if (m < M) {
                         if (EPI == EPI_GELU) {
                             if (e.z) e.z[(long)m * e.ldz + n] = v;  // kept for the backward pass only
-                            v = gelu(v);
+                            v = gelu_fast(v);
                         } else {
                             v *= dgelu(e.z[(long)m * e.ldz + n]);
                         }
@@ -439,7 +491,7 @@ static void forward_batch(MlpNet *net, int Mb, bool train, hipStream_t s, float 
         if (l == nL - 1) {
             e.c = out ? out : net->y;
             e.ldc = out ? ldout : Ly.N;  // = Np
-            hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt, (long)Ly.Kp,
+            hipLaunchKernelGGL((k_gemm_nt<EPI_F32, true>), g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt, (long)Ly.Kp,
                                out ? nrow : Mb, Ly.N, Ly.Kp, e);
         } else {
             MlpLayer &Nx = net->L[l + 1];
@@ -449,8 +501,12 @@ static void forward_batch(MlpNet *net, int Mb, bool train, hipStream_t s, float 
             e.ldo = Nx.Kp;
             e.ot = train ? Nx.at : nullptr;
             e.ldot = Bp;
-            hipLaunchKernelGGL(k_gemm_nt<EPI_GELU>, g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt, (long)Ly.Kp, Mb,
-                               Ly.N, Ly.Kp, e);
+            if (train)
+                hipLaunchKernelGGL(k_gemm_nt<EPI_GELU>, g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt, (long)Ly.Kp,
+                                   Mb, Ly.N, Ly.Kp, e);
+            else
+                hipLaunchKernelGGL((k_gemm_nt<EPI_GELU, true>), g, dim3(256), 0, s, Ly.a, (long)Ly.Kp, Ly.wt,
+                                   (long)Ly.Kp, Mb, Ly.N, Ly.Kp, e);
         }
     }
 }
@@ -468,7 +524,7 @@ int mlp_forward(MlpNet *net, const float *x, int ldx, int n, float *y, int ldy, 
     for (int r0 = 0; r0 < n; r0 += net->Bp) {
         const int nb = std::min(net->Bp, n - r0);
         const int Mb = input_batch(net, x + (long)r0 * ldx, ldx, nb, false, s);
-        if (net->Np == net->nout && ldy >= net->Np) {
+        if (net->Np == net->nout && ldy >= net->Np && ldy % 4 == 0 && ((uintptr_t)y & 15) == 0) {
             // no padded outputs: the last GEMM writes the nb rows of y itself
             forward_batch(net, Mb, false, s, y + (long)r0 * ldy, ldy, nb);
         } else {
