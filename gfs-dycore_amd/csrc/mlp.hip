@@ -34,14 +34,18 @@ __host__ __device__ inline uint16_t f2bf(float f) {
 }
 
 __device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-// forward GELU with erf from Abramowitz & Stegun (1964) 7.1.26 (|error| <= 1.5e-7,
-// far below the bf16 rounding of the activations) on the hardware exp2 / rcp
+// forward GELU with erf from Abramowitz & Stegun (1964) 7.1.25 (|error| <= 2.5e-5,
+// below the bf16 rounding of the activations, 2^-9) on the hardware exp2 / rcp
 __device__ inline float gelu_fast(float x) {
     const float z = x * 0.70710678118654752f, a = fabsf(z);
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * a);
-    const float p = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.47047f * a);
+    const float p = t * (0.3480242f + t * (-0.0958798f + t * 0.7478556f));
     const float y = 1.0f - p * __builtin_amdgcn_exp2f(-a * a * 1.4426950408889634f);
     return 0.5f * x * (1.0f + copysignf(y, z));
+}
+// two floats to packed bf16 (round to nearest even, the hardware conversion)
+__device__ inline uint32_t pack_bf16(float a, float b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)a) | ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)b) << 16);
 }
 __device__ inline float dgelu(float x) {
     return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
@@ -144,8 +148,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t *__restrict__ A,
                 } else {
                     if (e.z) *(float4 *)(e.z + (long)m * e.ldz + n) = float4{v[0], v[1], v[2], v[3]};
                     uint2 pk;
-                    pk.x = (uint32_t)f2bf(gelu_fast(v[0])) | ((uint32_t)f2bf(gelu_fast(v[1])) << 16);
-                    pk.y = (uint32_t)f2bf(gelu_fast(v[2])) | ((uint32_t)f2bf(gelu_fast(v[3])) << 16);
+                    pk.x = pack_bf16(gelu_fast(v[0]), gelu_fast(v[1]));
+                    pk.y = pack_bf16(gelu_fast(v[2]), gelu_fast(v[3]));
                     *(uint2 *)(e.o + (long)m * e.ldo + n) = pk;
                 }
             }
