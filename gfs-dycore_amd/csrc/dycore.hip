@@ -2253,8 +2253,11 @@ template <typename T> struct Step {
                         void *precip = nullptr, const char *name = "simple_columns") {
         const gfsdc_cfg &c = E.cfg;
         long nthreads = cs.stride > 0 ? B.ng / cs.stride : B.ng;
-        dim3 g((unsigned)((nthreads + 255) / 256), B.nm);
-        hipLaunchKernelGGL(k_simple_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg,
+        // a column is a long serial computation: the few judged columns are spread
+        // over as many CUs as possible (one wave per workgroup)
+        const int bs = cs.stride > 0 ? 64 : 256;
+        dim3 g((unsigned)((nthreads + bs - 1) / bs), B.nm);
+        hipLaunchKernelGGL(k_simple_columns<T>, g, dim3(bs), 0, s, (const T *)B.gsyn, (const T *)B.psg,
                            (T *)(gan ? gan : B.gan), (T *)B.pblscr, (T *)(pwat ? pwat : B.pwat),
                            (T *)(precip ? precip : B.precip), (const T *)B.ak, (const T *)B.bk,
                            (const double *)B.latd, B.nl, B.ntr, B.nsyn, B.nsp, B.ng, E.plan.nlon, (T)(c.rd / c.cp),
