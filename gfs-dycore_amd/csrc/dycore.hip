@@ -903,22 +903,22 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_judge_partial(const T *__restrict__ gan, const T *__restrict__ ganj,
                                                        const float *__restrict__ nrm, int nl, int nsp, long ng,
                                                        ColSel cs, double *__restrict__ part) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long col = sel_col(cs, i, ng);
+    // one (judged column, slot, level) value per thread and step of a grid-stride
+    // loop, consecutive threads on consecutive judged columns: every CU reads
     const int mem = blockIdx.y;
+    const long nsel = cs.stride > 0 ? ng / cs.stride : ng, total = nsel * 4L * nl;
+    const int nin = 4 * nl + 3, nout = 4 * nl;
+    const float *sy = nrm + 2 * nin + nout;
     double s = 0.0;
-    if (col >= 0) {
-        const int nin = 4 * nl + 3, nout = 4 * nl;
-        const float *sy = nrm + 2 * nin + nout;
-        const long base = (long)mem * nsp * ng + col;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const long i = t % nsel;
+        const int e = (int)(t / nsel), sl = e / nl, c = e - sl * nl + 1;
+        const long col = sel_col(cs, i, ng);
         // slot order in gan: Tv, u, v, q; target order: Tv, q, u, v
-        const int tslot[4] = {0, 2, 3, 1};
-        for (int sl = 0; sl < 4; sl++)
-            for (int c = 1; c <= nl; c++) {
-                const long o = base + ((long)sl * nl + (nl - c)) * ng;
-                const double d = ((double)gan[o] - (double)ganj[o]) / sy[tslot[sl] * nl + c - 1];
-                s += d * d;
-            }
+        const int ts = sl == 0 ? 0 : (sl == 1 ? 2 : (sl == 2 ? 3 : 1));
+        const long o = (long)mem * nsp * ng + col + ((long)sl * nl + (nl - c)) * ng;
+        const double d = ((double)gan[o] - (double)ganj[o]) / sy[ts * nl + c - 1];
+        s += d * d;
     }
     __shared__ double red[256];
     red[threadIdx.x] = s;
@@ -2286,7 +2286,8 @@ template <typename T> struct Step {
         const ColSel js{B.judge_stride, B.jctr, nullptr};
         simple_columns(js, B.ganj, B.pwatj, B.precipj, "judge_columns");
         const long nsel = B.ng / B.judge_stride;
-        dim3 gj((unsigned)((nsel + 255) / 256), B.nm);
+        // partial sums fit the (ng / 256 + 2) per member of jpart
+        dim3 gj((unsigned)std::min<long>(1024, B.ng / 256 + 2), B.nm);
         hipLaunchKernelGGL(k_judge_partial<T>, gj, dim3(256), 0, s, (const T *)B.gan, (const T *)B.ganj,
                            (const float *)B.nrm, B.nl, B.nsp, B.ng, js, B.jpart);
         hipLaunchKernelGGL(k_judge_final, dim3(1), dim3(256), 0, s, (const double *)B.jpart, (int)(gj.x * gj.y),
