@@ -1037,55 +1037,60 @@ __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk
     }
 }
 
+// SCATTER_KE coefficients per workgroup (one descriptor load, SCATTER_KE rows),
+// threads over the output fields.
+constexpr int SCATTER_KE = 8;
+
 template <typename T>
 __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc *__restrict__ desc, int nfo,
                                cplx<T> *__restrict__ dst, long df, long dk, const int2 *__restrict__ kmn,
                                const int *__restrict__ keoff, const T *__restrict__ eps,
-                               const int *__restrict__ eoff, T rrs) {
-    const int k = blockIdx.y;
+                               const int *__restrict__ eoff, T rrs, int nlm) {
     const int fo = blockIdx.x * blockDim.x + threadIdx.x;
     if (fo >= nfo) return;
-    const int2 mn = kmn[k];
-    const int m = mn.x, n = mn.y;
     const AnalDesc d = desc[fo];
-    const T *X = sx + (long)(keoff[m] + n - m) * ldx;
-    cplx<T> r;
-    if (d.kind == AK_ZERO) {
+    const int kend = min(nlm, (int)(blockIdx.y + 1) * SCATTER_KE);
+    for (int k = blockIdx.y * SCATTER_KE; k < kend; k++) {
         cplx<T> *o = dst + fo * df + (long)k * dk;
-        if (!d.accum) *o = cplx<T>{T(0), T(0)};
-        return;
-    }
-    if (d.kind == AK_COPY) {
-        r = cplx<T>{X[2 * d.a], X[2 * d.a + 1]};
-    } else {
-        const T *ep = eps + eoff[m];
-        T en = ep[n - m], en1 = ep[n + 1 - m];
-        T cp = T(n) * en1, cm = T(n + 1) * en, fm = T(m);
-        int ca = (d.kind == AK_VRT) ? d.b : d.a;  // i m (V for vrt, U for div)
-        int cb = (d.kind == AK_VRT) ? d.a : d.b;  // recurrence partner (U for vrt, V for div)
-        T sgn = (d.kind == AK_VRT) ? T(-1) : T(1);
-        cplx<T> z0{X[2 * ca], X[2 * ca + 1]};
-        cplx<T> zp{X[ldx + 2 * cb], X[ldx + 2 * cb + 1]};
-        cplx<T> zm{T(0), T(0)};
-        if (n > m) zm = cplx<T>{X[-ldx + 2 * cb], X[-ldx + 2 * cb + 1]};
-        // vrt = (i m V_n - n eps_{n+1} U_{n+1} + (n+1) eps_n U_{n-1}) / a
-        // div = (i m U_n + n eps_{n+1} V_{n+1} - (n+1) eps_n V_{n-1}) / a
-        r.x = (-fm * z0.y + sgn * (cp * zp.x - cm * zm.x)) * rrs;
-        r.y = (fm * z0.x + sgn * (cp * zp.y - cm * zm.y)) * rrs;
-    }
-    if (d.c >= 0) {
-        T q = T(d.sc) * T(n) * T(n + 1);
-        r.x += q * X[2 * d.c];
-        r.y += q * X[2 * d.c + 1];
-    }
-    T s = T(d.scale);
-    if (d.inv) s *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
-    cplx<T> *o = dst + fo * df + (long)k * dk;
-    if (d.accum) {
-        o->x += s * r.x;
-        o->y += s * r.y;
-    } else {
-        *o = cplx<T>{s * r.x, s * r.y};
+        if (d.kind == AK_ZERO) {
+            if (!d.accum) *o = cplx<T>{T(0), T(0)};
+            continue;
+        }
+        const int2 mn = kmn[k];
+        const int m = mn.x, n = mn.y;
+        const T *X = sx + (long)(keoff[m] + n - m) * ldx;
+        cplx<T> r;
+        if (d.kind == AK_COPY) {
+            r = cplx<T>{X[2 * d.a], X[2 * d.a + 1]};
+        } else {
+            const T *ep = eps + eoff[m];
+            T en = ep[n - m], en1 = ep[n + 1 - m];
+            T cp = T(n) * en1, cm = T(n + 1) * en, fm = T(m);
+            int ca = (d.kind == AK_VRT) ? d.b : d.a;  // i m (V for vrt, U for div)
+            int cb = (d.kind == AK_VRT) ? d.a : d.b;  // recurrence partner (U for vrt, V for div)
+            T sgn = (d.kind == AK_VRT) ? T(-1) : T(1);
+            cplx<T> z0{X[2 * ca], X[2 * ca + 1]};
+            cplx<T> zp{X[ldx + 2 * cb], X[ldx + 2 * cb + 1]};
+            cplx<T> zm{T(0), T(0)};
+            if (n > m) zm = cplx<T>{X[-ldx + 2 * cb], X[-ldx + 2 * cb + 1]};
+            // vrt = (i m V_n - n eps_{n+1} U_{n+1} + (n+1) eps_n U_{n-1}) / a
+            // div = (i m U_n + n eps_{n+1} V_{n+1} - (n+1) eps_n V_{n-1}) / a
+            r.x = (-fm * z0.y + sgn * (cp * zp.x - cm * zm.x)) * rrs;
+            r.y = (fm * z0.x + sgn * (cp * zp.y - cm * zm.y)) * rrs;
+        }
+        if (d.c >= 0) {
+            T q = T(d.sc) * T(n) * T(n + 1);
+            r.x += q * X[2 * d.c];
+            r.y += q * X[2 * d.c + 1];
+        }
+        T sc = T(d.scale);
+        if (d.inv) sc *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
+        if (d.accum) {
+            o->x += sc * r.x;
+            o->y += sc * r.y;
+        } else {
+            *o = cplx<T>{sc * r.x, sc * r.y};
+        }
     }
 }
 
@@ -1381,9 +1386,10 @@ void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, cons
 template <typename T>
 void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc *desc, int nfo, cplx<T> *dst,
                          long df, long dk, double rsphere, hipStream_t s) {
-    dim3 grid((nfo + 63) / 64, p.nlm);
-    hipLaunchKernelGGL(k_anal_scatter<T>, grid, dim3(64), 0, s, sx, ldx, desc, nfo, dst, df, dk, p.kmn, p.keoff,
-                       p.eps, p.eoff, (T)(1.0 / rsphere));
+    const int bs = nfo >= 256 ? 256 : 64 * ((nfo + 63) / 64);
+    dim3 grid((nfo + bs - 1) / bs, (p.nlm + SCATTER_KE - 1) / SCATTER_KE);
+    hipLaunchKernelGGL(k_anal_scatter<T>, grid, dim3(bs), 0, s, sx, ldx, desc, nfo, dst, df, dk, p.kmn, p.keoff,
+                       p.eps, p.eoff, (T)(1.0 / rsphere), p.nlm);
 }
 
 #define INST(T)                                                                                                \
