@@ -361,7 +361,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the step is timed on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+              "(the path would shard by member, or by zonal wavenumber and latitude pair; that is not run here)",
+              file=sys.stderr)
         sys.exit(2)
     if args.micro:
         line = run_micro(args)

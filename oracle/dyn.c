@@ -15,9 +15,11 @@
  * operators of init_semimpdata (linearisation test); init_semimpdata by
  * (I + c^2 dt^2 n(n+1) ym) d = I and the tableau identities; advance by
  * the IMEX order and a resting-state invariant; HS by closed-form columns.
- * The exact vertical differencing of getvadv / getvadv_tracers has no closed
- * form and no reference-run vectors here: parity unpinned beyond the
- * constant-field and positivity properties.
+ * The exact vertical differencing of getvadv has no closed form and no
+ * reference-run vectors here: it is pinned only by the constant-field and
+ * isothermal properties. getvadv_tracers is pinned by its invariants: the flux
+ * part telescopes, a uniform field has no tendency, a linear profile gives the
+ * centred getvadv difference, and a forward step keeps a tracer non-negative.
  */
 #include "oracle.h"
 #include <math.h>

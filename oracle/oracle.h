@@ -21,8 +21,8 @@
  *   src/dyn_init.f90         (setdampspec)
  *   src/phy_run_hs.f90       (Held-Suarez getphytend)
  *
- * Pinning status of each function is listed in DESIGN.md and in the header
- * comment of each source file.
+ * The pinning status of each function is given in the header comment of each
+ * source file.
  *
  * Layouts follow the reference (Fortran column-major):
  *   spectral  (ndimspec, nlev)  -> spec[lev*ndimspec + k], complex double,

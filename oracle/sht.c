@@ -15,7 +15,9 @@
  *    Vp = (1/sin th) dS/dphi - dT/dth, and spat_to_sphtor its quadrature
  *    inverse. These are evaluated here through U = u cos(lat), V = v cos(lat)
  *    and the recurrence (1-mu^2) dP_n/dmu = -n eps_{n+1} P_{n+1} + (n+1) eps_n P_{n-1},
- *    which gives the same quadrature sums node by node (derivation in DESIGN.md).
+ *    which gives the same quadrature sums node by node (multiplying the
+ *    colatitude derivatives by sin th = cos(lat) turns dP/dth into these
+ *    neighbouring-degree terms).
  *  - getuv / getvrtdivspec / getgrad scalings and role swaps
  *    (shtns.f90:259-261, 310, 339-351).
  *

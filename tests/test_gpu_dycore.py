@@ -108,7 +108,7 @@ def test_advance_fp32_one_day_t42():
     errs = {k: np.linalg.norm(sg[k][0] - so[k]) / np.linalg.norm(so[k])
             for k in ("vrt", "div", "tv", "lnps")}
     print("FP32 one-day spectral relative error (T42L20):", errs)
-    # <= 1e-5 per field after one model day
+    # the accuracy bound of the FP32 path: <= 1e-5 per field after one model day
     assert all(e <= 1e-5 for e in errs.values()), errs
 
 
