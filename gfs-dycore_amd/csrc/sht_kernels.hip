@@ -783,43 +783,80 @@ constexpr RadPlan rad_plan(int n) {
     return p;
 }
 
+// Orders the LDS accesses of one wavefront: the compiler moves no LDS access
+// across it, and the LDS unit serves one wavefront's instructions in issue
+// order, so writes before it are seen by reads after it in every lane.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One Stockham stage with one field per wavefront (4 fields per workgroup):
-// the field is the wave index and every divisor is a compile-time constant, so
-// a butterfly's LDS addresses cost a few integer operations instead of the
-// divisions and generic index arithmetic of fft_stage.
-template <typename T, int R, int N, int Ns>
-__device__ __forceinline__ void fft_stage_w(cplx<T> *buf, const cplx<T> *__restrict__ tw, int sign) {
-    constexpr int nbf = N / R, NB = (nbf + 63) / 64, NP = fft_pitch(N);
+// every divisor is a compile-time constant, so a butterfly's addresses cost a
+// few integer operations. The stage reads its inputs through ld(x) and hands
+// its outputs to st(x, v) (element x of the transform). Only this wavefront
+// touches its field, so the stages are ordered by wave_lds_sync instead of
+// workgroup barriers. The first stage (Ns = 1) has unit twiddles.
+template <typename T, int R, int N, int Ns, typename Ld, typename St>
+__device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int sign, Ld ld, St st) {
+    constexpr int nbf = N / R, NB = (nbf + 63) / 64;
     const int lane = threadIdx.x & 63;
-    cplx<T> *bq = buf + (threadIdx.x >> 6) * NP;
     cplx<T> v[NB][R];
 #pragma unroll
     for (int bb = 0; bb < NB; bb++) {
         const int j = lane + 64 * bb;
         if (nbf % 64 == 0 || j < nbf) {
 #pragma unroll
-            for (int r = 0; r < R; r++) v[bb][r] = bq[fft_pad(j + r * nbf)];
+            for (int r = 0; r < R; r++) v[bb][r] = ld(j + r * nbf);
         }
     }
-    __syncthreads();
+    wave_lds_sync();  // in place: every read of the stage before any write
 #pragma unroll
     for (int bb = 0; bb < NB; bb++) {
         const int j = lane + 64 * bb;
         if (nbf % 64 == 0 || j < nbf) {
             const int jm = j % Ns;
+            if constexpr (Ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; r++) {
-                cplx<T> t = tw[Ns - 1 + (r - 1) * Ns + jm];
-                if (sign > 0) t.y = -t.y;
-                v[bb][r] = cmul(v[bb][r], t);
+                for (int r = 1; r < R; r++) {
+                    cplx<T> t = tw[Ns - 1 + (r - 1) * Ns + jm];
+                    if (sign > 0) t.y = -t.y;
+                    v[bb][r] = cmul(v[bb][r], t);
+                }
             }
             small_dft<T, R>(v[bb], sign);
             const int idx = (j / Ns) * Ns * R + jm;
 #pragma unroll
-            for (int r = 0; r < R; r++) bq[fft_pad(idx + r * Ns)] = v[bb][r];
+            for (int r = 0; r < R; r++) st(idx + r * Ns, v[bb][r]);
         }
     }
-    __syncthreads();
+    wave_lds_sync();
+}
+
+// Stages S.. of the length-N transform of field bq (one wavefront). FIRST reads
+// the first stage's inputs through ld instead of LDS; LAST hands the last
+// stage's outputs to st instead of LDS (the last stage writes x = j + r Ns with
+// j < Ns, consecutive across lanes, so a global store from it is coalesced).
+template <typename T, int N, int S, int Ns, bool FIRST, bool LAST, typename Ld, typename St>
+__device__ __forceinline__ void fft_stages_w(cplx<T> *bq, const cplx<T> *tw, int sign, Ld ld, St st) {
+    constexpr RadPlan P = rad_plan(N);
+    if constexpr (S < P.n) {
+        constexpr int R = P.r[S];
+        auto lds_ld = [&](int x) { return bq[fft_pad(x)]; };
+        auto lds_st = [&](int x, const cplx<T> &v) { bq[fft_pad(x)] = v; };
+        if constexpr (FIRST && S == 0) {
+            if constexpr (LAST && S == P.n - 1)
+                fft_stage_w<T, R, N, Ns>(tw, sign, ld, st);
+            else
+                fft_stage_w<T, R, N, Ns>(tw, sign, ld, lds_st);
+        } else if constexpr (LAST && S == P.n - 1) {
+            fft_stage_w<T, R, N, Ns>(tw, sign, lds_ld, st);
+        } else {
+            fft_stage_w<T, R, N, Ns>(tw, sign, lds_ld, lds_st);
+        }
+        fft_stages_w<T, N, S + 1, Ns * R, FIRST, LAST>(bq, tw, sign, ld, st);
+    }
 }
 
 template <typename T, int NFFT, int N, int S, int Ns>
@@ -827,10 +864,7 @@ __device__ __forceinline__ void fft_stages(cplx<T> *buf, const cplx<T> *tw, int 
     constexpr RadPlan P = rad_plan(N);
     if constexpr (S < P.n) {
         constexpr int R = P.r[S];
-        if constexpr (NFFT == 4)
-            fft_stage_w<T, R, N, Ns>(buf, tw, sign);
-        else
-            fft_stage<T, R, (NFFT * N / R + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
+        fft_stage<T, R, (NFFT * N / R + 255) / 256>(buf, N, NFFT, Ns, tw, sign);
         fft_stages<T, NFFT, N, S + 1, Ns * R>(buf, tw, sign);
     }
 }
@@ -898,6 +932,22 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
         }
     }
     __syncthreads();
+    if constexpr (NC > 0 && NFFT == 4) {
+        // wavefront q transforms field f0 + q and stores the grid rows from its last stage
+        const int q = tid >> 6, f = f0 + q;
+        if (f >= nf) return;
+        const T s = rflag[f] ? rcos[j] : T(1);
+        const T c = mean ? mean[f] : T(0);
+        const long gf = slot ? slot[f] : f;
+        T *gn = grid + gf * gstride + (long)j * nlon, *gs = grid + gf * gstride + (long)j2 * nlon;
+        auto none = [](int) { return cplx<T>{T(0), T(0)}; };
+        auto out = [&](int x, const cplx<T> &z) {
+            gn[x] = z.x * s + c;
+            gs[x] = z.y * s + c;
+        };
+        fft_stages_w<T, NC, 0, 1, false, true>(buf + q * NP, tws, +1, none, out);
+        return;
+    }
     if constexpr (NC > 0)
         fft_stages<T, NFFT, NC, 0, 1>(buf, tws, +1);
     else
@@ -934,35 +984,51 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
     const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
-    constexpr int MAXG = (NFFT * LN + 255) / 256;
-    T xv[MAXG], yv[MAXG];
+    if constexpr (NC > 0 && NFFT == 4) {
+        // wavefront q transforms field f0 + q, its first stage loading the grid rows;
+        // a field past nf repeats field nf - 1 and is not stored
+        constexpr RadPlan P = rad_plan(NC);
+        const int q = tid >> 6, f = min(f0 + q, nf - 1);
+        const T *gn = grid + f * gstride + (long)j * nlon, *gs = grid + f * gstride + (long)j2 * nlon;
+        cplx<T> *bq = buf + q * NP;
+        for (int t = tid; t < N; t += 256) tws[t] = tw[t];
+        auto in = [&](int x) { return cplx<T>{gn[x], gs[x]}; };
+        auto lds_st = [&](int x, const cplx<T> &v) { bq[fft_pad(x)] = v; };
+        fft_stage_w<T, P.r[0], NC, 1>(tws, -1, in, lds_st);
+        __syncthreads();  // the twiddle table
+        fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, tws, -1, in, lds_st);
+        __syncthreads();  // the coefficients below are read across fields
+    } else {
+        constexpr int MAXG = (NFFT * LN + 255) / 256;
+        T xv[MAXG], yv[MAXG];
 #pragma unroll
-    for (int c = 0; c < MAXG; c++) {
-        int t = tid + c * 256;
-        xv[c] = T(0);
-        yv[c] = T(0);
-        if (t < NFFT * N) {
-            int q = t / N, i = t - q * N, f = f0 + q;
-            if (f < nf) {
-                xv[c] = grid[f * gstride + (long)j * nlon + i];
-                yv[c] = grid[f * gstride + (long)j2 * nlon + i];
+        for (int c = 0; c < MAXG; c++) {
+            int t = tid + c * 256;
+            xv[c] = T(0);
+            yv[c] = T(0);
+            if (t < NFFT * N) {
+                int q = t / N, i = t - q * N, f = f0 + q;
+                if (f < nf) {
+                    xv[c] = grid[f * gstride + (long)j * nlon + i];
+                    yv[c] = grid[f * gstride + (long)j2 * nlon + i];
+                }
             }
         }
-    }
-    for (int t = tid; t < N; t += 256) tws[t] = tw[t];
+        for (int t = tid; t < N; t += 256) tws[t] = tw[t];
 #pragma unroll
-    for (int c = 0; c < MAXG; c++) {
-        int t = tid + c * 256;
-        if (t < NFFT * N) {
-            const int q = t / N;
-            buf[q * NP + fft_pad(t - q * N)] = cplx<T>{xv[c], yv[c]};
+        for (int c = 0; c < MAXG; c++) {
+            int t = tid + c * 256;
+            if (t < NFFT * N) {
+                const int q = t / N;
+                buf[q * NP + fft_pad(t - q * N)] = cplx<T>{xv[c], yv[c]};
+            }
         }
+        __syncthreads();
+        if constexpr (NC > 0)
+            fft_stages<T, NFFT, NC, 0, 1>(buf, tws, -1);
+        else
+            fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, -1);
     }
-    __syncthreads();
-    if constexpr (NC > 0)
-        fft_stages<T, NFFT, NC, 0, 1>(buf, tws, -1);
-    else
-        fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, -1);
     const T wj = wfac[j], rc = rcos[j];
     for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
         int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
