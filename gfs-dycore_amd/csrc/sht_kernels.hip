@@ -693,14 +693,56 @@ template <typename T, int R> __device__ __forceinline__ void small_dft(cplx<T> *
 
 // LDS layout of the transforms: field q's element x sits at q * fft_pitch(N) +
 // swz(x), where swz permutes x within its block of 16 complex values by XOR
-// with 5 * (x / 16) mod 16. The early Stockham stages write with stride R
-// (Ns = 1) and in runs of Ns = 4; under this permutation both patterns, and the
-// contiguous reads, put 16 consecutive lanes on 16 different bank pairs. The
-// field pitch is 8 complex values past a multiple of 16, so the lanes that
-// alternate between two fields (loading and storing the Fourier coefficients)
-// fall on different banks.
+// with key(x / 16), key(v) = (v ^ 4 v) mod 16 (5 v multiplied without carries).
+// The early Stockham stages write with stride R (Ns = 1) and in runs of Ns = 4;
+// under this permutation both patterns, and the contiguous reads, put 16
+// consecutive lanes on 16 different bank pairs. The field pitch is 8 complex
+// values past a multiple of 16, so the lanes that alternate between two fields
+// (loading and storing the Fourier coefficients) fall on different banks.
+// key is linear under XOR: for x = g + C, with C a multiple of 16 whose bits
+// above the fourth miss those of g / 16, swz(x) = (g ^ key(g / 16) ^ key(C / 16)) + C,
+// a per-lane value XOR a constant, plus a constant the LDS instruction carries.
 __host__ __device__ constexpr int fft_pitch(int N) { return ((N + 15) & ~15) + 8; }
-__device__ inline int fft_pad(int x) { return x ^ (((x >> 4) * 5) & 15); }
+__host__ __device__ constexpr int fft_key(int v) { return (v ^ (v << 2)) & 15; }
+__device__ inline int fft_pad(int x) { return x ^ fft_key(x >> 4); }
+
+// Split of the element indices one wavefront's Stockham stage reads
+// (x = j + r nbf) and writes (x = (j / Ns) Ns R + j % Ns + r Ns), j = lane + 64 bb,
+// into a lane part g and a constant part c, x = g + c.
+struct XSplit {
+    int g, c;
+};
+constexpr XSplit split_rd(int lane, int bb, int r, int nbf) { return XSplit{lane, 64 * bb + r * nbf}; }
+constexpr XSplit split_wr(int lane, int bb, int r, int R, int Ns) {
+    if (64 % Ns == 0) {  // j / Ns = lane / Ns + 64 bb / Ns, j % Ns = lane % Ns
+        int g = (lane / Ns) * Ns * R + lane % Ns, c = 64 * R * bb;
+        if (Ns % 16 == 0)
+            c += r * Ns;
+        else
+            g += r * Ns;
+        return XSplit{g, c};
+    }
+    return XSplit{lane, 64 * bb + r * Ns};  // valid when every j < Ns (a last stage)
+}
+constexpr bool split_ok(XSplit s, int x) { return s.g + s.c == x && s.c % 16 == 0 && ((s.g >> 4) & (s.c >> 4)) == 0; }
+// whether every read (WR = false) or write (WR = true) of the stage splits so
+constexpr bool split_all(int N, int R, int Ns, bool WR) {
+    const int nbf = N / R, NB = (nbf + 63) / 64;
+    for (int bb = 0; bb < NB; bb++)
+        for (int r = 0; r < R; r++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int j = lane + 64 * bb;
+                if (j >= nbf) continue;
+                const int x = WR ? (j / Ns) * Ns * R + j % Ns + r * Ns : j + r * nbf;
+                if (!split_ok(WR ? split_wr(lane, bb, r, R, Ns) : split_rd(lane, bb, r, nbf), x)) return false;
+            }
+    return true;
+}
+__device__ __forceinline__ int swz_split(XSplit s) { return ((s.g ^ fft_key(s.g >> 4)) ^ fft_key(s.c >> 4)) + s.c; }
+static_assert(split_all(768, 4, 1, true) && split_all(768, 4, 4, true) && split_all(768, 4, 16, true) &&
+                  split_all(768, 4, 64, true) && split_all(768, 3, 256, true) && split_all(768, 4, 1, false) &&
+                  split_all(768, 3, 256, false),
+              "the 768-point stages take the split LDS indices");
 
 template <typename T, int R, int MAXB>
 __device__ __forceinline__ void fft_stage(cplx<T> *buf, int N, int nfft, int Ns, const cplx<T> *__restrict__ tw,
@@ -793,14 +835,16 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // One Stockham stage with one field per wavefront (4 fields per workgroup):
-// every divisor is a compile-time constant, so a butterfly's addresses cost a
-// few integer operations. The stage reads its inputs through ld(x) and hands
-// its outputs to st(x, v) (element x of the transform). Only this wavefront
-// touches its field, so the stages are ordered by wave_lds_sync instead of
-// workgroup barriers. The first stage (Ns = 1) has unit twiddles.
+// every divisor is a compile-time constant, and where the stage's indices split
+// (split_all) an LDS address is a per-lane register plus an immediate offset.
+// The stage reads element x through ld(x, p) and hands its outputs to
+// st(x, p, v), p being x's permuted LDS index. Only this wavefront touches its
+// field, so the stages are ordered by wave_lds_sync instead of workgroup
+// barriers. The first stage (Ns = 1) has unit twiddles.
 template <typename T, int R, int N, int Ns, typename Ld, typename St>
 __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int sign, Ld ld, St st) {
     constexpr int nbf = N / R, NB = (nbf + 63) / 64;
+    constexpr bool SR = split_all(N, R, Ns, false), SW = split_all(N, R, Ns, true);
     const int lane = threadIdx.x & 63;
     cplx<T> v[NB][R];
 #pragma unroll
@@ -808,7 +852,10 @@ __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int 
         const int j = lane + 64 * bb;
         if (nbf % 64 == 0 || j < nbf) {
 #pragma unroll
-            for (int r = 0; r < R; r++) v[bb][r] = ld(j + r * nbf);
+            for (int r = 0; r < R; r++) {
+                const int x = j + r * nbf;
+                v[bb][r] = ld(x, SR ? swz_split(split_rd(lane, bb, r, nbf)) : fft_pad(x));
+            }
         }
     }
     wave_lds_sync();  // in place: every read of the stage before any write
@@ -828,7 +875,10 @@ __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int 
             small_dft<T, R>(v[bb], sign);
             const int idx = (j / Ns) * Ns * R + jm;
 #pragma unroll
-            for (int r = 0; r < R; r++) st(idx + r * Ns, v[bb][r]);
+            for (int r = 0; r < R; r++) {
+                const int x = idx + r * Ns;
+                st(x, SW ? swz_split(split_wr(lane, bb, r, R, Ns)) : fft_pad(x), v[bb][r]);
+            }
         }
     }
     wave_lds_sync();
@@ -843,8 +893,8 @@ __device__ __forceinline__ void fft_stages_w(cplx<T> *bq, const cplx<T> *tw, int
     constexpr RadPlan P = rad_plan(N);
     if constexpr (S < P.n) {
         constexpr int R = P.r[S];
-        auto lds_ld = [&](int x) { return bq[fft_pad(x)]; };
-        auto lds_st = [&](int x, const cplx<T> &v) { bq[fft_pad(x)] = v; };
+        auto lds_ld = [&](int, int p) { return bq[p]; };
+        auto lds_st = [&](int, int p, const cplx<T> &v) { bq[p] = v; };
         if constexpr (FIRST && S == 0) {
             if constexpr (LAST && S == P.n - 1)
                 fft_stage_w<T, R, N, Ns>(tw, sign, ld, st);
@@ -940,8 +990,8 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
         const T c = mean ? mean[f] : T(0);
         const long gf = slot ? slot[f] : f;
         T *gn = grid + gf * gstride + (long)j * nlon, *gs = grid + gf * gstride + (long)j2 * nlon;
-        auto none = [](int) { return cplx<T>{T(0), T(0)}; };
-        auto out = [&](int x, const cplx<T> &z) {
+        auto none = [](int, int) { return cplx<T>{T(0), T(0)}; };
+        auto out = [&](int x, int, const cplx<T> &z) {
             gn[x] = z.x * s + c;
             gs[x] = z.y * s + c;
         };
@@ -992,8 +1042,8 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
         const T *gn = grid + f * gstride + (long)j * nlon, *gs = grid + f * gstride + (long)j2 * nlon;
         cplx<T> *bq = buf + q * NP;
         for (int t = tid; t < N; t += 256) tws[t] = tw[t];
-        auto in = [&](int x) { return cplx<T>{gn[x], gs[x]}; };
-        auto lds_st = [&](int x, const cplx<T> &v) { bq[fft_pad(x)] = v; };
+        auto in = [&](int x, int) { return cplx<T>{gn[x], gs[x]}; };
+        auto lds_st = [&](int, int p, const cplx<T> &v) { bq[p] = v; };
         fft_stage_w<T, P.r[0], NC, 1>(tws, -1, in, lds_st);
         __syncthreads();  // the twiddle table
         fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, tws, -1, in, lds_st);
