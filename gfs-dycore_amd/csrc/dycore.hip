@@ -151,6 +151,10 @@ struct DycoreBuffers {
 template <typename T> struct LayerCoef {
     T rlnp, alfa, cofa, dbp;  // dbp = bk_K / p2 - bk_{K-1} / p1
 };
+// 1 / x: the hardware reciprocal (1 ulp) in FP32, where it feeds products and
+// never a cancellation; a division in FP64
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 template <typename T>
 __device__ inline LayerCoef<T> layer_coef(T p1, T p2, T dpk, T bk1, T bk2, T ck, T dbk) {
     LayerCoef<T> c;
@@ -182,10 +186,11 @@ __device__ inline LayerCoef<T> layer_coef(T p1, T p2, T dpk, T bk1, T bk2, T ck,
         c.cofa = ck * rdp * (c.alfa - dpk * rp2) * rp1;
         c.dbp = -ck * rp1 * rp2;
     } else {
-        c.rlnp = log(p2 / p1);
-        c.alfa = T(1) - (p1 / dpk) * c.rlnp;
-        c.cofa = -(T(1) / dpk) * (bk2 * p1 / p2 - bk1 + c.rlnp * (bk1 - p1 * dbk / dpk));
-        c.dbp = bk2 / p2 - bk1 / p1;
+        const T rp1 = T(1) / p1, rp2 = T(1) / p2, rdp = T(1) / dpk;
+        c.rlnp = log(p2 * rp1);
+        c.alfa = T(1) - (p1 * rdp) * c.rlnp;
+        c.cofa = -rdp * (bk2 * p1 * rp2 - bk1 + c.rlnp * (bk1 - p1 * dbk * rdp));
+        c.dbp = bk2 * rp2 - bk1 * rp1;
     }
     return c;
 }
@@ -237,29 +242,43 @@ __global__ __launch_bounds__(256) void k_columns(
     T db = 0, cb = 0;
     ET[0] = T(0);
     ET[(long)nl * ng] = T(0);
-    for (int K = 1; K <= nl; K++) {
-        int L = nl + 1 - K;
-        T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K);
-        T rlnp = rlnp1, alfa = alfa1;
-        if (K > 1) {
-            const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
-            rlnp = lc.rlnp;
-            alfa = lc.alfa;
+    constexpr int P1G = 4;  // levels whose loads are issued together
+    for (int K0 = 1; K0 <= nl; K0 += P1G) {
+        T gu[P1G], gv[P1G], gd[P1G];
+#pragma unroll
+        for (int i = 0; i < P1G; i++) {
+            const int Lc = max(nl + 1 - (K0 + i), 1);  // clamped: past the bottom level nothing is used
+            gu[i] = LV(U, Lc);
+            gv[i] = LV(V, Lc);
+            gd[i] = LV(DIV, Lc);
         }
-        T cg = LV(U, L) * dpx + LV(V, L) * dpy;
-        T dv = LV(DIV, L);
-        T workb, workc;
-        if (K == 1) {
-            workb = alfa * (dv * dpk + ps * (cg * dbk[0]) * dbk[0]);
-            workc = ps * cg * dbk[0];
-        } else {
-            workb = rlnp * (db + ps * cb) + alfa * (dv * dpk + ps * cg * dbk[K - 1]);
-            workc = ps * cg * (dbk[K - 1] + ck[K - 1] * rlnp / dpk);
+#pragma unroll
+        for (int i = 0; i < P1G; i++) {
+            const int K = K0 + i;
+            if (K > nl) break;
+            int L = nl + 1 - K;
+            T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K), rdp = recip(dpk);
+            T rlnp = rlnp1, alfa = alfa1;
+            if (K > 1) {
+                const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
+                rlnp = lc.rlnp;
+                alfa = lc.alfa;
+            }
+            T cg = gu[i] * dpx + gv[i] * dpy;
+            T dv = gd[i];
+            T workb, workc;
+            if (K == 1) {
+                workb = alfa * (dv * dpk + ps * (cg * dbk[0]) * dbk[0]);
+                workc = ps * cg * dbk[0];
+            } else {
+                workb = rlnp * (db + ps * cb) + alfa * (dv * dpk + ps * cg * dbk[K - 1]);
+                workc = ps * cg * (dbk[K - 1] + ck[K - 1] * rlnp * rdp);
+            }
+            LV(WD, L) = (workc - workb) * rdp;
+            db += dv * dpk;
+            cb += cg * dbk[K - 1];
+            if (K <= nl - 1) ET[(long)K * ng] = -(ps * cb) - db;
         }
-        LV(WD, L) = (workc - workb) / dpk;
-        db += dv * dpk;
-        cb += cg * dbk[K - 1];
-        if (K <= nl - 1) ET[(long)K * ng] = -(ps * cb) - db;
     }
     const T dlnpsdt = -db / ps - cb;
     auto et_fix = [&](int K) { return -(ps * bk[K]) * dlnpsdt; };  // etadot - stored value, 0 < K < nl
@@ -291,76 +310,96 @@ __global__ __launch_bounds__(256) void k_columns(
     T um = 0, vm = 0, tm = 0;                          // level L - 1
     T u = LV(U, 1), v = LV(V, 1), tv = LV(TV, 1);      // level L
     T eb = T(0);                                       // etadot at the layer's bottom interface
-    for (int K = nl; K >= 1; K--) {
-        int L = nl + 1 - K;
-        T up = 0, vp = 0, tp = 0;                      // level L + 1
-        if (L < nl) {
-            up = LV(U, L + 1);
-            vp = LV(V, L + 1);
-            tp = LV(TV, L + 1);
+    constexpr int P3G = 2;  // levels whose loads are issued together
+    for (int K0 = nl; K0 >= 1; K0 -= P3G) {
+        T gup[P3G], gvp[P3G], gtp[P3G], gtx[P3G], gty[P3G], gwd[P3G], gvr[P3G];
+#pragma unroll
+        for (int i = 0; i < P3G; i++) {
+            const int Lc = min(nl + 1 - (K0 - i), nl), Lp = min(Lc + 1, nl);  // clamped to the top level
+            gup[i] = LV(U, Lp);
+            gvp[i] = LV(V, Lp);
+            gtp[i] = LV(TV, Lp);
+            gtx[i] = LV(TX, Lc);
+            gty[i] = LV(TY, Lc);
+            gwd[i] = LV(WD, Lc);
+            gvr[i] = LV(VRT, Lc);
         }
-        T ea = T(0);                                   // etadot at the top interface K - 1
-        if (K > 1) {
-            ea = ET[(long)(K - 1) * ng] + et_fix(K - 1);
-            ET[(long)(K - 1) * ng] = ea;
+#pragma unroll
+        for (int i = 0; i < P3G; i++) {
+            const int K = K0 - i;
+            if (K < 1) break;
+            int L = nl + 1 - K;
+            T up = 0, vp = 0, tp = 0;                      // level L + 1
+            if (L < nl) {
+                up = gup[i];
+                vp = gvp[i];
+                tp = gtp[i];
+            }
+            T ea = T(0);                                   // etadot at the top interface K - 1
+            if (K > 1) {
+                ea = ET[(long)(K - 1) * ng] + et_fix(K - 1);
+                ET[(long)(K - 1) * ng] = ea;
+            }
+            T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K), rdp = recip(dpk);
+            T rlnp, alfa, cofa, dbp;
+            if (K == 1) {
+                const T rp1 = recip(p1), rp2 = recip(p2);
+                rlnp = rlnp1;
+                alfa = alfa1;
+                cofa = -rdp * (bk[K] * p1 * rp2 - bk[K - 1] + rlnp * (bk[K - 1] - p1 * dbk[K - 1] * rdp));
+                dbp = bk[K] * rp2 - bk[K - 1] * rp1;
+            } else {
+                const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
+                rlnp = lc.rlnp;
+                alfa = lc.alfa;
+                cofa = lc.cofa;
+                dbp = lc.dbp;
+            }
+            T cofb0 = (K == 1) ? -rdp * (alfa * dbk[0]) : -rdp * (bk[K - 1] * rlnp + alfa * dbk[K - 1]);
+            T tx = gtx[i], ty = gty[i];
+            T prsgx = cofb0 * rd * tv * ps * dpx - dphx + cofb_acc * ps * dpx + px3u - rd * alfa * tx -
+                      cofa * rd * tv * ps * dpx;
+            T prsgy = cofb0 * rd * tv * ps * dpy - dphy + cofb_acc * ps * dpy + px3v - rd * alfa * ty -
+                      cofa * rd * tv * ps * dpy;
+            // getvadv for u, v, tv at level L
+            T vu, vv, vt;
+            T h = T(0.5) * rdp;
+            if (L == nl) {  // top layer: etadot at interface 1 (its bottom)
+                vu = h * eb * (um - u);
+                vv = h * eb * (vm - v);
+                vt = h * eb * (tm - tv);
+            } else if (L == 1) {  // bottom layer: etadot at interface nl - 1 (its top)
+                vu = h * ea * (u - up);
+                vv = h * ea * (v - vp);
+                vt = h * ea * (tv - tp);
+            } else {
+                vu = h * (eb * (um - u) + ea * (u - up));
+                vv = h * (eb * (vm - v) + ea * (v - vp));
+                vt = h * (eb * (tm - tv) + ea * (tv - tp));
+            }
+            T vq = kappa * gwd[i] * tv;
+            if (ntr > 0) vq = vq / (T(1) + cvcp * LV(Q, L));
+            vv -= prsgy;
+            vu -= prsgx;
+            T z = gvr[i] + f;
+            LV(TT, L) = -u * tx - v * ty - vt + vq;
+            LV(FX, L) = u * z + vv;
+            LV(FY, L) = v * z - vu;
+            LV(KE, L) = T(0.5) * (u * u + v * v);
+            // accumulations for the layer above (getpresgrad cofb and px3 recursions)
+            cofb_acc -= rd * dbp * tv;
+            px3u -= rd * rlnp * tx;
+            px3v -= rd * rlnp * ty;
+            um = u;
+            vm = v;
+            tm = tv;
+            u = up;
+            v = vp;
+            tv = tp;
+            eb = ea;
         }
-        T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K);
-        T rlnp, alfa, cofa, dbp;
-        if (K == 1) {
-            rlnp = rlnp1;
-            alfa = alfa1;
-            cofa = -(T(1) / dpk) * (bk[K] * p1 / p2 - bk[K - 1] + rlnp * (bk[K - 1] - p1 * dbk[K - 1] / dpk));
-            dbp = bk[K] / p2 - bk[K - 1] / p1;
-        } else {
-            const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
-            rlnp = lc.rlnp;
-            alfa = lc.alfa;
-            cofa = lc.cofa;
-            dbp = lc.dbp;
-        }
-        T cofb0 = (K == 1) ? -(T(1) / dpk) * (alfa * dbk[0]) : -(T(1) / dpk) * (bk[K - 1] * rlnp + alfa * dbk[K - 1]);
-        T tx = LV(TX, L), ty = LV(TY, L);
-        T prsgx = cofb0 * rd * tv * ps * dpx - dphx + cofb_acc * ps * dpx + px3u - rd * alfa * tx -
-                  cofa * rd * tv * ps * dpx;
-        T prsgy = cofb0 * rd * tv * ps * dpy - dphy + cofb_acc * ps * dpy + px3v - rd * alfa * ty -
-                  cofa * rd * tv * ps * dpy;
-        // getvadv for u, v, tv at level L
-        T vu, vv, vt;
-        T h = T(0.5) / dpk;
-        if (L == nl) {  // top layer: etadot at interface 1 (its bottom)
-            vu = h * eb * (um - u);
-            vv = h * eb * (vm - v);
-            vt = h * eb * (tm - tv);
-        } else if (L == 1) {  // bottom layer: etadot at interface nl - 1 (its top)
-            vu = h * ea * (u - up);
-            vv = h * ea * (v - vp);
-            vt = h * ea * (tv - tp);
-        } else {
-            vu = h * (eb * (um - u) + ea * (u - up));
-            vv = h * (eb * (vm - v) + ea * (v - vp));
-            vt = h * (eb * (tm - tv) + ea * (tv - tp));
-        }
-        T vq = kappa * LV(WD, L) * tv;
-        if (ntr > 0) vq = vq / (T(1) + cvcp * LV(Q, L));
-        vv -= prsgy;
-        vu -= prsgx;
-        T z = LV(VRT, L) + f;
-        LV(TT, L) = -u * tx - v * ty - vt + vq;
-        LV(FX, L) = u * z + vv;
-        LV(FY, L) = v * z - vu;
-        LV(KE, L) = T(0.5) * (u * u + v * v);
-        // accumulations for the layer above (getpresgrad cofb and px3 recursions)
-        cofb_acc -= rd * dbp * tv;
-        px3u -= rd * rlnp * tx;
-        px3v -= rd * rlnp * ty;
-        um = u;
-        vm = v;
-        tm = tv;
-        u = up;
-        v = vp;
-        tv = tp;
-        eb = ea;
     }
+
     // pass 4: tracers (dyn_run.f90:279-291): horizontal advection with the
     // synthesised gradient and the Thuburn (1993) flux-limited vertical
     // advection of getvadv_tracers (dyn_run.f90:478-603), top -> bottom with a
@@ -398,7 +437,7 @@ __global__ __launch_bounds__(256) void k_columns(
             } else {
                 hk = Dk;  // half_nl = lowest layer value
             }
-            T vadv = (T(1) / dpk) * ((hk * ebot - hprev * etop) + (Dk * (etop - ebot)));
+            T vadv = recip(dpk) * ((hk * ebot - hprev * etop) + (Dk * (etop - ebot)));
             const int L = nl + 1 - K;
             LV(DQ, L) = -LV(U, L) * LV(QXn, L) - LV(V, L) * LV(QYn, L) - vadv;
             hprev = hk;

@@ -28,6 +28,10 @@ constexpr int GEMM_BM = 64;
 constexpr int GEMM_BN = 128;
 constexpr int GEMM_BK = 16;
 constexpr int COL_PAD = GEMM_BN;  // column padding of transform work buffers
+// columns past the padded width: keeps the work buffers' row pitch off a power
+// of two (at 1024 columns the rows that one Fourier workgroup reads or writes,
+// 1.5 MiB apart, crowd onto few memory channels)
+constexpr int COL_SKEW = 32;
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
@@ -111,7 +115,7 @@ void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc 
 // Workspace for one batched transform pipeline.
 template <typename T> struct Work {
     int ncap = 0;          // field capacity
-    int ldc = 0;           // padded column count (2*ncap rounded to COL_PAD)
+    int ldc = 0;           // row pitch: 2*ncap rounded to COL_PAD, plus COL_SKEW
     T *sx = nullptr;       // nlme x ldc
     T *fs = nullptr;       // (T+1) x nlat x ldc (synthesis Fourier)
     T *fa = nullptr;       // (T+1) x 2 x nhp x ldc (analysis Fourier, zero padded)

@@ -1374,7 +1374,7 @@ template <typename T> int work_alloc(Work<T> &w, const Plan<T> &p, int nfield) {
     if (nfield <= w.ncap) return 0;
     work_free(w);
     w.ncap = nfield;
-    w.ldc = round_up(2 * nfield, COL_PAD);
+    w.ldc = round_up(2 * nfield, COL_PAD) + COL_SKEW;
     size_t nsx = (size_t)p.nlme * w.ldc, nfs = (size_t)(p.ntrunc + 1) * p.nlat * w.ldc;
     size_t nfa = (size_t)(p.ntrunc + 1) * 2 * p.nhp * w.ldc, ng = (size_t)nfield * p.nlat * p.nlon;
     GFSDC_CHECK(hipMalloc((void **)&w.sx, sizeof(T) * nsx));
