@@ -1371,19 +1371,19 @@ __global__ __launch_bounds__(256) void k_si_stage(
 }
 
 // The same IMEX stage (run.f90:313-350, 374-417, 440-489, damping 493-502) as
-// batched matrix products on the FP32 matrix cores: a workgroup takes up to
-// SI_C spectral coefficients of one total wavenumber n (each member's m = 0..n),
-// i.e. 2 SI_C real columns of level vectors, and every level operator
+// batched matrix products on the matrix cores: a workgroup takes up to
+// C = SiCfg<T>::C spectral coefficients of one total wavenumber n (each member's m = 0..n),
+// i.e. 2 C real columns of level vectors, and every level operator
 // (amhyb, bmhyb and d_hyb_m of degree n, all real) becomes one 64 x 64 x 64
 // product Y = M X on MFMA 16x16x4 (wave w computes output levels 16w..16w+15).
 // Real and imaginary parts are independent columns because all operators and
 // coefficients are real. Level vectors stay in LDS between the products.
-// Coefficients per workgroup: 16 in FP32, 8 in FP64 (twice the bytes per value
-// in the same LDS); 2 C real columns, padded rows of LD values.
+// Coefficients per workgroup: 8, the fewest that fill a 16-column MFMA block
+// (2 C real columns, padded rows of LD values). The stage is bound by the
+// latency of its load/product phases, not by the matrix cores, so the small
+// tile's LDS (about 20 KB in FP32) and twice the workgroups per CU outrun 16
+// coefficients per workgroup.
 template <typename T> struct SiCfg {
-    static constexpr int C = 16, W = 2 * C, LD = W + 4;
-};
-template <> struct SiCfg<double> {
     static constexpr int C = 8, W = 2 * C, LD = W + 4;
 };
 // MFMA 16x16x4 in the precision of the stage: A[i][k] from lane i + 16k,
@@ -1411,12 +1411,18 @@ __device__ inline void si_gemm(const T *__restrict__ Mt, int nl, const T (*X)[Si
 #pragma unroll
     for (int b = 0; b < NB; b++) acc[b] = typename SiMma<T>::acc{T(0), T(0), T(0), T(0)};
     const int i = wave * 16 + (lane & 15), kq = lane >> 4;
-#pragma unroll 4
-    for (int k0 = 0; k0 < 64; k0 += 4) {
-        const int k = k0 + kq;
-        const T a = (i < nl && k < nl) ? Mt[k * nl + i] : T(0);
+    // every operator value of this lane first: one wait for the (cached) operator, not one per k-step
+    T a[16];
 #pragma unroll
-        for (int b = 0; b < NB; b++) acc[b] = SiMma<T>::mma(a, X[k][b * 16 + (lane & 15)], acc[b]);
+    for (int q = 0; q < 16; q++) {
+        const int k = 4 * q + kq;
+        a[q] = (i < nl && k < nl) ? Mt[min(k, nl - 1) * nl + min(i, nl - 1)] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int k = 4 * q + kq;
+#pragma unroll
+        for (int b = 0; b < NB; b++) acc[b] = SiMma<T>::mma(a[q], X[k][b * 16 + (lane & 15)], acc[b]);
     }
 #pragma unroll
     for (int b = 0; b < NB; b++)
