@@ -1165,47 +1165,65 @@ __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc
     const int fo = blockIdx.x * blockDim.x + threadIdx.x;
     if (fo >= nfo) return;
     const AnalDesc d = desc[fo];
-    const int kend = min(nlm, (int)(blockIdx.y + 1) * SCATTER_KE);
-    for (int k = blockIdx.y * SCATTER_KE; k < kend; k++) {
-        cplx<T> *o = dst + fo * df + (long)k * dk;
-        if (d.kind == AK_ZERO) {
-            if (!d.accum) *o = cplx<T>{T(0), T(0)};
-            continue;
+    const int k0 = blockIdx.y * SCATTER_KE, kend = min(nlm, k0 + SCATTER_KE);
+    if (d.kind == AK_ZERO) {
+        if (!d.accum)
+            for (int k = k0; k < kend; k++) dst[fo * df + (long)k * dk] = cplx<T>{T(0), T(0)};
+        return;
+    }
+    // the inputs of QB rows first (loads all in flight together), then those rows;
+    // FP64 takes half the rows per batch to stay within the registers
+    constexpr int QB = sizeof(T) == 4 ? SCATTER_KE : SCATTER_KE / 2;
+    const int ca = (d.kind == AK_VRT) ? d.b : d.a;  // i m (V for vrt, U for div; the field for a copy)
+    const int cb = (d.kind == AK_VRT) ? d.a : d.b;  // recurrence partner (U for vrt, V for div)
+    for (int kb = k0; kb < kend; kb += QB) {
+        cplx<T> z0[QB], zp[QB], zm[QB], zc[QB], za[QB];
+        int2 mnq[QB];
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int k = min(kb + q, kend - 1);  // clamped: rows past kend are not stored
+            const int2 mn = kmn[k];
+            const int m = mn.x, n = mn.y;
+            mnq[q] = mn;
+            const T *X = sx + (long)(keoff[m] + n - m) * ldx;
+            const T *C = (d.kind == AK_COPY) ? X + 2 * d.a : X + 2 * ca;
+            z0[q] = cplx<T>{C[0], C[1]};
+            zp[q] = zm[q] = zc[q] = za[q] = cplx<T>{T(0), T(0)};
+            if (d.kind != AK_COPY) {
+                zp[q] = cplx<T>{X[ldx + 2 * cb], X[ldx + 2 * cb + 1]};
+                if (n > m) zm[q] = cplx<T>{X[-ldx + 2 * cb], X[-ldx + 2 * cb + 1]};
+            }
+            if (d.c >= 0) zc[q] = cplx<T>{X[2 * d.c], X[2 * d.c + 1]};
+            if (d.accum) za[q] = dst[fo * df + (long)k * dk];
         }
-        const int2 mn = kmn[k];
-        const int m = mn.x, n = mn.y;
-        const T *X = sx + (long)(keoff[m] + n - m) * ldx;
-        cplx<T> r;
-        if (d.kind == AK_COPY) {
-            r = cplx<T>{X[2 * d.a], X[2 * d.a + 1]};
-        } else {
-            const T *ep = eps + eoff[m];
-            T en = ep[n - m], en1 = ep[n + 1 - m];
-            T cp = T(n) * en1, cm = T(n + 1) * en, fm = T(m);
-            int ca = (d.kind == AK_VRT) ? d.b : d.a;  // i m (V for vrt, U for div)
-            int cb = (d.kind == AK_VRT) ? d.a : d.b;  // recurrence partner (U for vrt, V for div)
-            T sgn = (d.kind == AK_VRT) ? T(-1) : T(1);
-            cplx<T> z0{X[2 * ca], X[2 * ca + 1]};
-            cplx<T> zp{X[ldx + 2 * cb], X[ldx + 2 * cb + 1]};
-            cplx<T> zm{T(0), T(0)};
-            if (n > m) zm = cplx<T>{X[-ldx + 2 * cb], X[-ldx + 2 * cb + 1]};
-            // vrt = (i m V_n - n eps_{n+1} U_{n+1} + (n+1) eps_n U_{n-1}) / a
-            // div = (i m U_n + n eps_{n+1} V_{n+1} - (n+1) eps_n V_{n-1}) / a
-            r.x = (-fm * z0.y + sgn * (cp * zp.x - cm * zm.x)) * rrs;
-            r.y = (fm * z0.x + sgn * (cp * zp.y - cm * zm.y)) * rrs;
-        }
-        if (d.c >= 0) {
-            T q = T(d.sc) * T(n) * T(n + 1);
-            r.x += q * X[2 * d.c];
-            r.y += q * X[2 * d.c + 1];
-        }
-        T sc = T(d.scale);
-        if (d.inv) sc *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
-        if (d.accum) {
-            o->x += sc * r.x;
-            o->y += sc * r.y;
-        } else {
-            *o = cplx<T>{sc * r.x, sc * r.y};
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int k = kb + q;
+            if (k >= kend) break;
+            const int m = mnq[q].x, n = mnq[q].y;
+            cplx<T> r = z0[q];
+            if (d.kind != AK_COPY) {
+                const T *ep = eps + eoff[m];
+                T en = ep[n - m], en1 = ep[n + 1 - m];
+                T cp = T(n) * en1, cm = T(n + 1) * en, fm = T(m);
+                T sgn = (d.kind == AK_VRT) ? T(-1) : T(1);
+                // vrt = (i m V_n - n eps_{n+1} U_{n+1} + (n+1) eps_n U_{n-1}) / a
+                // div = (i m U_n + n eps_{n+1} V_{n+1} - (n+1) eps_n V_{n-1}) / a
+                r.x = (-fm * z0[q].y + sgn * (cp * zp[q].x - cm * zm[q].x)) * rrs;
+                r.y = (fm * z0[q].x + sgn * (cp * zp[q].y - cm * zm[q].y)) * rrs;
+            }
+            if (d.c >= 0) {
+                T qq = T(d.sc) * T(n) * T(n + 1);
+                r.x += qq * zc[q].x;
+                r.y += qq * zc[q].y;
+            }
+            T sc = T(d.scale);
+            if (d.inv) sc *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
+            cplx<T> *o = dst + fo * df + (long)k * dk;
+            if (d.accum)
+                *o = cplx<T>{za[q].x + sc * r.x, za[q].y + sc * r.y};
+            else
+                *o = cplx<T>{sc * r.x, sc * r.y};
         }
     }
 }
