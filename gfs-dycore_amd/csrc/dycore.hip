@@ -2408,8 +2408,10 @@ template <typename T> int Engine<T>::get_pattern(void *out, int dtype, int loc) 
 // ---- tile autotuner ----------------------------------------------------------
 // Times the step's Fourier transforms (the synthesis of every grid field and
 // the analysis of the dynamics tendencies) with 2, 4 and 8 fields per FFT
-// workgroup on this context's own sizes and keeps the fastest. Leaves the
-// model state untouched; the next graph advance captures the new choice.
+// workgroup on this context's own sizes and keeps the fastest, for the
+// synthesis and the analysis separately. ms[c] is the synthesis plus analysis
+// time of candidate c; best is the synthesis choice. Leaves the model state
+// untouched; the next graph advance captures the new choice.
 template <typename T> int Engine<T>::autotune(double *ms, int *best) {
     DycoreBuffers *B = dyn;
     if (!B) {
@@ -2418,34 +2420,42 @@ template <typename T> int Engine<T>::autotune(double *ms, int *best) {
     }
     Work<T> &W = *(Work<T> *)B->Work;
     const int cand[3] = {2, 4, 8};
-    const int keep = plan.fft_nf;
-    hipEvent_t e0, e1;
+    hipEvent_t e0, e1, e2;
     GFSDC_CHECK(hipEventCreate(&e0));
     GFSDC_CHECK(hipEventCreate(&e1));
-    double bestms = 1e300;
-    int bestnf = keep;
+    GFSDC_CHECK(hipEventCreate(&e2));
+    double best_syn = 1e300, best_an = 1e300;
+    int nf_syn = plan.fft_nf, nf_an = plan.fft_nf_an;
     for (int c = 0; c < 3; c++) {
-        plan.fft_nf = cand[c];
-        float t = 0.f;
+        plan.fft_nf = plan.fft_nf_an = cand[c];
+        float ts = 0.f, ta = 0.f;
         for (int rep = 0; rep < 2; rep++) {  // first pass warms up
             GFSDC_CHECK(hipEventRecord(e0, stream));
             launch_fft_synth(plan, W.fs, W.ldc, (T *)B->gsyn, B->ng, B->nm * B->nsyn, B->rflag_syn, stream,
                              (const T *)B->gmean);
-            launch_fft_anal(plan, (const T *)B->gan, B->ng, B->nm * B->nan, B->rflag_an, W.fa, W.ldc, stream);
             GFSDC_CHECK(hipEventRecord(e1, stream));
-            GFSDC_CHECK(hipEventSynchronize(e1));
-            GFSDC_CHECK(hipEventElapsedTime(&t, e0, e1));
+            launch_fft_anal(plan, (const T *)B->gan, B->ng, B->nm * B->nan, B->rflag_an, W.fa, W.ldc, stream);
+            GFSDC_CHECK(hipEventRecord(e2, stream));
+            GFSDC_CHECK(hipEventSynchronize(e2));
+            GFSDC_CHECK(hipEventElapsedTime(&ts, e0, e1));
+            GFSDC_CHECK(hipEventElapsedTime(&ta, e1, e2));
         }
-        if (ms) ms[c] = t;
-        if (t < bestms) {
-            bestms = t;
-            bestnf = cand[c];
+        if (ms) ms[c] = (double)ts + (double)ta;
+        if (ts < best_syn) {
+            best_syn = ts;
+            nf_syn = cand[c];
+        }
+        if (ta < best_an) {
+            best_an = ta;
+            nf_an = cand[c];
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    plan.fft_nf = bestnf;
-    if (best) *best = bestnf;
+    (void)hipEventDestroy(e2);
+    plan.fft_nf = nf_syn;
+    plan.fft_nf_an = nf_an;
+    if (best) *best = nf_syn;
     if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
     B->gexec = nullptr;

@@ -40,7 +40,8 @@ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 template <typename T> struct Plan {
     int nlon = 0, nlat = 0, nhalf = 0, nhp = 0, ntrunc = 0, nlm = 0, nlme = 0;
     int nrad = 0, rad[16] = {0};
-    int fft_nf = 8;          // fields per FFT workgroup (2, 4 or 8)
+    int fft_nf = 8;          // fields per synthesis FFT workgroup (2, 4 or 8)
+    int fft_nf_an = 8;       // fields per analysis FFT workgroup (2, 4 or 8)
     int leg_bf16x3 = 0;      // FP32 plan only: Legendre GEMMs as three bf16 products (hi.hi + hi.lo + lo.hi)
     T *plm = nullptr;        // per m: [even i'][nhp] then [odd i'][nhp], n = m + 2i' (+1)
     long *poff = nullptr;    // device, per m (ntrunc+2 entries)

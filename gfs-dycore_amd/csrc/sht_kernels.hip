@@ -931,24 +931,33 @@ static_assert(fft_pitch(FftCfg<double>::MAXN) * 8 * 16 + FftCfg<double>::MAXN * 
 // only the spectral gap T < k < N-T is zero-filled.
 // NC > 0 specialises the kernel to transform length NC (compile-time radix
 // stages and divisors, LDS sized to NC); NC = 0 handles any length <= MAXN.
+// Workgroup size of the FFT kernels: one wavefront per field on the wavefront
+// path (a transform length known at compile time, 4 or 8 fields), else 256.
+__host__ __device__ constexpr bool fft_wave(int nfft, int nc) { return nc > 0 && (nfft == 4 || nfft == 8); }
+__host__ __device__ constexpr int fft_threads(int nfft, int nc) { return fft_wave(nfft, nc) ? 64 * nfft : 256; }
+
 template <typename T, int NFFT, int MAXN, int NC>
-__global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int ldf, T *__restrict__ grid,
+__global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__restrict__ fs, int ldf, T *__restrict__ grid,
                                                    long gstride, int nf, const unsigned char *__restrict__ rflag,
                                                    const T *__restrict__ mean, const int *__restrict__ slot,
                                                    const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
                                                    int nlon, int nlat, int ntrunc, Radices rd) {
     constexpr int LN = NC ? NC : MAXN;
+    // wavefront path; with 8 fields the twiddles are read from the (cached)
+    // global table so that three workgroups fit a CU's LDS
+    constexpr bool WAVE = fft_wave(NFFT, NC), GTW = WAVE && NFFT == 8;
+    constexpr int NT = fft_threads(NFFT, NC);
     __shared__ cplx<T> buf[NFFT * fft_pitch(LN)];
-    __shared__ cplx<T> tws[LN];
+    __shared__ cplx<T> tws[GTW ? 1 : LN];
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
     const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
-    constexpr int MAXI = (NFFT * (LN / 2) + 255) / 256;
+    constexpr int MAXI = (NFFT * (LN / 2) + NT - 1) / NT;
     const int nitems = (ntrunc + 1) * NFFT;
     cplx<T> xn[MAXI], xs[MAXI];
 #pragma unroll
     for (int c = 0; c < MAXI; c++) {
-        int idx = tid + c * 256;
+        int idx = tid + c * NT;
         xn[c] = cplx<T>{T(0), T(0)};
         xs[c] = xn[c];
         if (idx < nitems) {
@@ -962,14 +971,15 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
             }
         }
     }
-    for (int t = tid; t < N; t += 256) tws[t] = tw[t];
+    if constexpr (!GTW)
+        for (int t = tid; t < N; t += NT) tws[t] = tw[t];
     const int gap = N - 2 * ntrunc - 1;
 #pragma unroll
     for (int q = 0; q < NFFT; q++)
-        for (int i = tid; i < gap; i += 256) buf[q * NP + fft_pad(ntrunc + 1 + i)] = cplx<T>{T(0), T(0)};
+        for (int i = tid; i < gap; i += NT) buf[q * NP + fft_pad(ntrunc + 1 + i)] = cplx<T>{T(0), T(0)};
 #pragma unroll
     for (int c = 0; c < MAXI; c++) {
-        int idx = tid + c * 256;
+        int idx = tid + c * NT;
         if (idx < nitems) {
             int m = idx / NFFT, q = idx - m * NFFT;
             T xr = xn[c].x, xi = xn[c].y, yr = xs[c].x, yi = xs[c].y;
@@ -982,7 +992,7 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
         }
     }
     __syncthreads();
-    if constexpr (NC > 0 && NFFT == 4) {
+    if constexpr (WAVE) {
         // wavefront q transforms field f0 + q and stores the grid rows from its last stage
         const int q = tid >> 6, f = f0 + q;
         if (f >= nf) return;
@@ -995,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
             gn[x] = z.x * s + c;
             gs[x] = z.y * s + c;
         };
-        fft_stages_w<T, NC, 0, 1, false, true>(buf + q * NP, tws, +1, none, out);
+        fft_stages_w<T, NC, 0, 1, false, true>(buf + q * NP, GTW ? tw : tws, +1, none, out);
         return;
     }
     if constexpr (NC > 0)
@@ -1023,30 +1033,33 @@ __global__ __launch_bounds__(256) void k_fft_synth(const T *__restrict__ fs, int
 // Fourier analysis: rows of a latitude pair -> weighted symmetric / antisymmetric
 // Fourier coefficients, m = 0..T. Grid rows are loaded in one batch per thread.
 template <typename T, int NFFT, int MAXN, int NC>
-__global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, long gstride, int nf,
+__global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__restrict__ grid, long gstride, int nf,
                                                   const unsigned char *__restrict__ rflag,
                                                   const T *__restrict__ rcos, const T *__restrict__ wfac,
                                                   const cplx<T> *__restrict__ tw, T *__restrict__ fa, int ldf,
                                                   int nlon, int nlat, int nhp, int ntrunc, Radices rd) {
     constexpr int LN = NC ? NC : MAXN;
+    constexpr bool WAVE = fft_wave(NFFT, NC), GTW = WAVE && NFFT == 8;
+    constexpr int NT = fft_threads(NFFT, NC);
     __shared__ cplx<T> buf[NFFT * fft_pitch(LN)];
-    __shared__ cplx<T> tws[LN];
+    __shared__ cplx<T> tws[GTW ? 1 : LN];
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
     const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
-    if constexpr (NC > 0 && NFFT == 4) {
+    if constexpr (WAVE) {
         // wavefront q transforms field f0 + q, its first stage loading the grid rows;
         // a field past nf repeats field nf - 1 and is not stored
         constexpr RadPlan P = rad_plan(NC);
         const int q = tid >> 6, f = min(f0 + q, nf - 1);
         const T *gn = grid + f * gstride + (long)j * nlon, *gs = grid + f * gstride + (long)j2 * nlon;
         cplx<T> *bq = buf + q * NP;
-        for (int t = tid; t < N; t += 256) tws[t] = tw[t];
+        if constexpr (!GTW)
+            for (int t = tid; t < N; t += NT) tws[t] = tw[t];
         auto in = [&](int x, int) { return cplx<T>{gn[x], gs[x]}; };
         auto lds_st = [&](int, int p, const cplx<T> &v) { bq[p] = v; };
-        fft_stage_w<T, P.r[0], NC, 1>(tws, -1, in, lds_st);
+        fft_stage_w<T, P.r[0], NC, 1>(GTW ? tw : tws, -1, in, lds_st);
         __syncthreads();  // the twiddle table
-        fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, tws, -1, in, lds_st);
+        fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, GTW ? tw : tws, -1, in, lds_st);
         __syncthreads();  // the coefficients below are read across fields
     } else {
         constexpr int MAXG = (NFFT * LN + 255) / 256;
@@ -1080,7 +1093,7 @@ __global__ __launch_bounds__(256) void k_fft_anal(const T *__restrict__ grid, lo
             fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, -1);
     }
     const T wj = wfac[j], rc = rcos[j];
-    for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += 256) {
+    for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += NT) {
         int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
         if (f >= nf) continue;
         cplx<T> zk = buf[q * NP + fft_pad(m)], zn = buf[q * NP + fft_pad(m == 0 ? 0 : N - m)];
@@ -1284,6 +1297,7 @@ template <typename T> int plan_create(Plan<T> &p, int nlon, int nlat, int ntrunc
         int v = atoi(e);
         if (v == 2 || v == 4 || v == 8) p.fft_nf = v;
     }
+    p.fft_nf_an = p.fft_nf;
     const int T_ = ntrunc;
     // offsets
     p.poff_h.resize(T_ + 2);
@@ -1466,7 +1480,8 @@ static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long g
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_SYN(NC)                                                                                             \
-    hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(256), 0, s, fs, ldf, grid, gstride, nf, rflag, mean, slot, \
+    hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(fft_threads(NF, NC)), 0, s, fs, ldf, grid, gstride, nf,   \
+                       rflag, mean, slot,                                                                     \
                        p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad))
     switch (p.nlon) {
     case 768: FFT_SYN(768); break;
@@ -1483,7 +1498,8 @@ static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, c
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_ANA(NC)                                                                                             \
-    hipLaunchKernelGGL((k_fft_anal<T, NF, MN, NC>), g, dim3(256), 0, s, grid, gstride, nf, rflag, p.rcos, p.wfac,   \
+    hipLaunchKernelGGL((k_fft_anal<T, NF, MN, NC>), g, dim3(fft_threads(NF, NC)), 0, s, grid, gstride, nf, rflag,    \
+                       p.rcos, p.wfac,                                                                        \
                        (const cplx<T> *)p.tw, fa, ldf, p.nlon, p.nlat, p.nhp, p.ntrunc, radices_of(p.nrad, p.rad))
     switch (p.nlon) {
     case 768: FFT_ANA(768); break;
@@ -1494,9 +1510,10 @@ static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, c
 #undef FFT_ANA
 }
 
-// fields per workgroup of the FFT kernels (p.fft_nf, chosen by the tile autotuner or
-// GFSDC_FFT_NF): more fields give longer contiguous Fourier segments, fewer give
-// more workgroups per CU (LDS = fields * nlon * sizeof(complex))
+// fields per workgroup of the FFT kernels (p.fft_nf for the synthesis and
+// p.fft_nf_an for the analysis, chosen by the tile autotuner or GFSDC_FFT_NF):
+// more fields give longer contiguous Fourier segments, fewer give more
+// workgroups per CU (LDS = fields * nlon * sizeof(complex))
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
                       const unsigned char *rflag, hipStream_t s, const T *mean, const int *slot) {
@@ -1510,7 +1527,7 @@ void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstr
 template <typename T>
 void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
                      int ldf, hipStream_t s) {
-    switch (p.fft_nf) {
+    switch (p.fft_nf_an) {
     case 2: fft_anal_nf<T, 2>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
     case 4: fft_anal_nf<T, 4>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
     default: fft_anal_nf<T, 8>(p, grid, gstride, nf, rflag, fa, ldf, s); break;

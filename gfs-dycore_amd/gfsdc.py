@@ -303,7 +303,8 @@ class Context:
         return out
 
     def autotune(self):
-        """pick the FFT fields-per-workgroup variant by timing; returns (choice, ms per variant 2/4/8)"""
+        """pick the FFT fields-per-workgroup variants by timing (synthesis and analysis
+        separately); returns (synthesis choice, synthesis + analysis ms per variant 2/4/8)"""
         ms = np.zeros(3)
         best = C.c_int(0)
         _check(load().gfsdc_autotune(self.h, _p(ms), C.byref(best)))

@@ -175,8 +175,9 @@ int gfsdc_simple_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, void
                          int loc);
 /* tile autotuner (no counterpart in the reference): times the step's Fourier
  * transforms with 2, 4 and 8 fields per FFT workgroup on this context's sizes
- * and keeps the fastest; ms (3, may be NULL) receives the timings, best the
- * choice. The model state is unchanged. */
+ * and keeps the fastest, for the synthesis and the analysis separately; ms (3,
+ * may be NULL) receives each candidate's synthesis + analysis time, best the
+ * synthesis choice. The model state is unchanged. */
 int gfsdc_autotune(gfsdc_ctx *ctx, double *ms, int *best);
 /* transform microbenchmark (no counterpart in the reference): milliseconds per
  * iteration of a spectogrd + grdtospec round trip of nscalar fields plus a

@@ -66,6 +66,25 @@ def test_getdyntend(kind, nlev, prec):
         assert rel(a, b) < (1e-9 if prec == "fp64" else 2e-4), (name, rel(a, b))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_getdyntend_after_autotune(prec):
+    """the tile autotuner times the FFT variants on the context's own buffers and
+    leaves the model state alone: getdyntend still matches the oracle after it"""
+    import gfsdc
+    p = gfsdc.FP64 if prec == "fp64" else gfsdc.FP32
+    o, g = make_pair("hybrid", 64, p)
+    vrt, div, tv, lnps = cases.perturbed_state(o.sht.degree, o.sht.order, 64, seed=9)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.set_state(vrt, div, tv, lnps)
+    best, ms = g.autotune()
+    assert best in (2, 4, 8) and np.all(ms > 0)
+    to = o.getdyntend(1)
+    tg = g.getdyntend(1)
+    tol = 1e-9 if prec == "fp64" else 2e-4
+    for k in ("dvrt", "ddiv", "dtv", "dlnps"):
+        assert rel(tg[k][0], to[k]) < tol, (k, rel(tg[k][0], to[k]))
+
+
 def test_rest_state_matches_numpy():
     import gfsdc
     o, g = make_pair("sigma", 20, gfsdc.FP64)

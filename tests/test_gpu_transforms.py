@@ -61,6 +61,27 @@ def test_spectogrd_grdtospec(pair, nf):
     assert rel(g32.grdtospec(ref.astype(np.float32), dtype=1), o.grdtospec(ref)) < 2e-6
 
 
+@pytest.mark.parametrize("nfw", [2, 4, 8])
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: f"T{g[2]}")
+def test_fft_fields_per_workgroup(grid, nfw, monkeypatch):
+    """Every FFT workgroup variant (2, 4 or 8 fields per workgroup: the tile
+    autotuner's candidates, also set by GFSDC_FFT_NF) gives the oracle's
+    transforms; 13 fields leave a partial last field group for each."""
+    import gfsdc
+    monkeypatch.setenv("GFSDC_FFT_NF", str(nfw))
+    nlon, nlat, T = grid
+    o = O.Sht(nlon, nlat, T)
+    q = rand_spec(o, 13, 17)
+    ref = o.spectogrd(q)
+    back = o.grdtospec(ref)
+    g64 = gfsdc.Context(nlon, nlat, T, gfsdc.FP64)
+    assert rel(g64.spectogrd(q), ref) < 1e-11
+    assert rel(g64.grdtospec(ref), back) < 1e-11
+    g32 = gfsdc.Context(nlon, nlat, T, gfsdc.FP32)
+    assert rel(g32.spectogrd(q.astype(np.complex64), dtype=1), ref) < 2e-6
+    assert rel(g32.grdtospec(ref.astype(np.float32), dtype=1), back) < 2e-6
+
+
 @pytest.mark.parametrize("nf", [1, 9])
 def test_vector_transforms(pair, nf):
     o, g64, g32 = pair
