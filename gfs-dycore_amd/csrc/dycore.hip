@@ -457,22 +457,31 @@ __global__ __launch_bounds__(256) void k_columns(
 // ln((e^z - 1) / z) = z/2 + sum_j B_2j z^2j / (2j (2j)!) taken at z = (k+1) y and
 // z = y (B_2j Bernoulli numbers; five terms, truncation 3e-9 at y = 1), which
 // keeps it to a few ulp. The top layer (p1 = 0) has no cancellation.
-template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa) {
+// coefficients d1..d5 of the series S(y) above; they depend on kappa only
+template <typename T> struct PrsSeries {
+    T d1, d2, d3, d4, d5;
+};
+template <typename T> __device__ inline PrsSeries<T> prs_series(T kappa) {
+    const T a = kappa + T(1), a2 = a * a, a4 = a2 * a2, a6 = a4 * a2, a8 = a4 * a4, a10 = a8 * a2;
+    return PrsSeries<T>{(a2 - T(1)) / (T(24) * kappa), -(a4 - T(1)) / (T(2880) * kappa),
+                        (a6 - T(1)) / (T(181440) * kappa), -(a8 - T(1)) / (T(9676800) * kappa),
+                        (a10 - T(1)) / (T(479001600) * kappa)};
+}
+template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa, const PrsSeries<T> &c) {
     if constexpr (sizeof(T) == 4) {
         if (p1 > T(0)) {
-            const T y = log1p((p2 - p1) / p1);
+            const T y = log1p((p2 - p1) * recip(p1));
             if (y <= T(1)) {
-                const T a = kappa + T(1), a2 = a * a, a4 = a2 * a2, a6 = a4 * a2, a8 = a4 * a4, a10 = a8 * a2;
                 const T y2 = y * y;
-                const T d1 = (a2 - T(1)) / (T(24) * kappa), d2 = -(a4 - T(1)) / (T(2880) * kappa);
-                const T d3 = (a6 - T(1)) / (T(181440) * kappa), d4 = -(a8 - T(1)) / (T(9676800) * kappa);
-                const T d5 = (a10 - T(1)) / (T(479001600) * kappa);
-                const T S = y2 * (d1 + y2 * (d2 + y2 * (d3 + y2 * (d4 + y2 * d5))));
+                const T S = y2 * (c.d1 + y2 * (c.d2 + y2 * (c.d3 + y2 * (c.d4 + y2 * c.d5))));
                 return sqrt(p1 * p2) * exp(S);
             }
         }
     }
     return pow((pow(p2, kappa) * p2 - pow(p1, kappa) * p1) / ((kappa + T(1)) * (p2 - p1)), T(1) / kappa);
+}
+template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa) {
+    return phillips_prs(p1, p2, kappa, prs_series(kappa));
 }
 
 // Held-Suarez forcing on the post-dynamics grid fields (phy_run_hs.f90:42-77)
@@ -507,13 +516,19 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
     const T kdrag = T(SGLF(1.f / 86400.f)), krada = T(SGLF(1.f / (40.f * 86400.f)));
     const T kradb = T(SGLF(1.f / (4.f * 86400.f))), p0 = T(1.e5), deltmp = T(60);
     const T expo = T(SGLF(2.f / 7.f));
+    // per column: the layer-pressure series, 1 / ps, 1 / (1 - sigbot)
+    const PrsSeries<T> prc = prs_series(kappa);
+    const T rps = recip(ps), rsb = T(1) / (T(1) - sigbot);
     for (int L = L0; L <= L1; L++) {
         int K = nl + 1 - L;
         T p1 = ak[K - 1] + bk[K - 1] * ps, p2 = ak[K] + bk[K] * ps;
-        T prs = phillips_prs(p1, p2, kappa);
-        T blprof = prs / ps;
-        T teq = pow(prs / p0, expo) * (T(315) - deltmp * sl * sl - delthz * log(prs / p0) * cl2);
-        blprof = (blprof - sigbot) / (T(1) - sigbot);
+        T prs = phillips_prs(p1, p2, kappa, prc);
+        T blprof = prs * rps;
+        // (prs / p0)^(2/7) as exp(2/7 ln(prs / p0)), sharing the logarithm of the
+        // temperature profile
+        const T lr = log(prs / p0);
+        T teq = exp(expo * lr) * (T(315) - deltmp * sl * sl - delthz * lr * cl2);
+        blprof = (blprof - sigbot) * rsb;
         if (blprof < T(0)) blprof = T(0);
         if (teq < tempstrat) teq = tempstrat;
         long o = (long)(L - 1) * ng;
