@@ -2846,8 +2846,9 @@ template <typename T> int Engine<T>::set_topo(const void *topo, int dtype, int l
     // dphisdx then dphisdy are contiguous (nm fields each)
     launch_fft_synth(plan, W.fs, W.ldc, (T *)B->dphisdx, B->ng, nm, d_rflag, stream);
     GFSDC_CHECK(hipGetLastError());
-    // second half of the fields (dphisdy): shift the Fourier columns by nm fields
-    launch_fft_synth(plan, W.fs + 2 * nm, W.ldc, (T *)B->dphisdy, B->ng, nm, d_rflag + nm, stream);
+    // second half of the fields (dphisdy): Fourier fields nm..2 nm - 1
+    launch_fft_synth(plan, W.fs, W.ldc, (T *)B->dphisdy, B->ng, nm, d_rflag + nm, stream, (const T *)nullptr,
+                     (const int *)nullptr, nm);
     GFSDC_CHECK(hipGetLastError());
     GFSDC_CHECK(hipStreamSynchronize(stream));
     return 0;

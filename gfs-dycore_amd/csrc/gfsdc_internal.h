@@ -33,6 +33,19 @@ constexpr int COL_PAD = GEMM_BN;  // column padding of transform work buffers
 // 1.5 MiB apart, crowd onto few memory channels)
 constexpr int COL_SKEW = 32;
 
+// Synthesis Fourier buffer: for each row (E sums at row j, O sums at row
+// nlat-1-j) and each group of 16 fields (32 values, one 128-byte line), the
+// lines of all orders m = 0..T are contiguous, so that a Fourier workgroup's
+// rows are one contiguous run per group instead of 255 scattered pieces, while
+// each line is still written whole by one wavefront of the Legendre synthesis.
+// Value c (2 field + re/im) of order m in row r sits at fs_index(r, m, c, ldf,
+// T + 1); ldf (the work buffers' column pitch) is a multiple of 32, and the
+// buffer holds nlat ldf (T+1) values.
+__host__ __device__ inline long fs_index(int r, int m, int c, int ldf, int nm1) {
+    return (((long)r * (ldf >> 5) + (c >> 5)) * nm1 + m) * 32 + (c & 31);
+}
+static_assert(COL_PAD % 32 == 0 && COL_SKEW % 32 == 0, "the Fourier buffer groups 32 columns");
+
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
 // Transform plan: Gauss grid, Legendre tables and FFT twiddles for one
@@ -103,7 +116,7 @@ void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, in
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
                       const unsigned char *rcos_flag, hipStream_t s, const T *mean = nullptr,
-                      const int *slot = nullptr);
+                      const int *slot = nullptr, int fofs = 0);
 template <typename T>
 void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rcos_flag,
                      T *fa, int ldf, hipStream_t s);
@@ -118,7 +131,7 @@ template <typename T> struct Work {
     int ncap = 0;          // field capacity
     int ldc = 0;           // row pitch: 2*ncap rounded to COL_PAD, plus COL_SKEW
     T *sx = nullptr;       // nlme x ldc
-    T *fs = nullptr;       // (T+1) x nlat x ldc (synthesis Fourier)
+    T *fs = nullptr;       // nlat x ldc x (T+1) (synthesis Fourier, laid out by fs_index)
     T *fa = nullptr;       // (T+1) x 2 x nhp x ldc (analysis Fourier, zero padded)
     T *grid = nullptr;     // ncap x nlat x nlon
 };

@@ -320,24 +320,24 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
             gemm_chunk<T, true>(As, Bs, E, lane, wm, wn, Kp - k0, nbj);
         __syncthreads();
     }
-    T *FN = fs + (long)m * nlat * ldf;
+    const int nm1 = ntrunc + 1;
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
         for (int bj = 0; bj < WB; bj++) {
             if constexpr (Vec<T>::swap) {
                 int j = j0 + wm + bi * 16 + (lane & 15);
-                int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
+                int c = c0 + wn + bj * 16 + 4 * (lane >> 4);  // a multiple of 4: 16-byte aligned
                 if (j < nhalf && c < ncol) {  // padding columns are never read
                     acc_t e = E[bi][bj];
-                    *(float4 *)(FN + (long)(par ? nlat - 1 - j : j) * ldf + c) = float4{e[0], e[1], e[2], e[3]};
+                    *(float4 *)(fs + fs_index(par ? nlat - 1 - j : j, m, c, ldf, nm1)) = float4{e[0], e[1], e[2], e[3]};
                 }
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     int j = j0 + wm + bi * 16 + MF<T>::row(lane, r);
                     int c = c0 + wn + bj * 16 + (lane & 15);
-                    if (j < nhalf && c < ncol) FN[(long)(par ? nlat - 1 - j : j) * ldf + c] = E[bi][bj][r];
+                    if (j < nhalf && c < ncol) fs[fs_index(par ? nlat - 1 - j : j, m, c, ldf, nm1)] = E[bi][bj][r];
                 }
             }
         }
@@ -576,7 +576,6 @@ __global__ __launch_bounds__(256) void k_leg_synth_b3(const float *__restrict__ 
             }
         __syncthreads();
     }
-    float *FN = fs + (long)m * nlat * ldf;
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -584,8 +583,8 @@ __global__ __launch_bounds__(256) void k_leg_synth_b3(const float *__restrict__ 
             const int j = j0 + wm + bi * 16 + (lane & 15), c = c0 + wn + bj * 16 + 4 * (lane >> 4);
             if (j < nhalf) {  // even sum to row j, odd sum to row nlat-1-j (as k_leg_synth)
                 const b3f4 e = E[bi][bj], o = O[bi][bj];
-                *(float4 *)(FN + (long)j * ldf + c) = float4{e[0], e[1], e[2], e[3]};
-                *(float4 *)(FN + (long)(nlat - 1 - j) * ldf + c) = float4{o[0], o[1], o[2], o[3]};
+                *(float4 *)(fs + fs_index(j, m, c, ldf, ntrunc + 1)) = float4{e[0], e[1], e[2], e[3]};
+                *(float4 *)(fs + fs_index(nlat - 1 - j, m, c, ldf, ntrunc + 1)) = float4{o[0], o[1], o[2], o[3]};
             }
         }
 }
@@ -941,7 +940,7 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
                                                    long gstride, int nf, const unsigned char *__restrict__ rflag,
                                                    const T *__restrict__ mean, const int *__restrict__ slot,
                                                    const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
-                                                   int nlon, int nlat, int ntrunc, Radices rd) {
+                                                   int nlon, int nlat, int ntrunc, Radices rd, int fofs) {
     constexpr int LN = NC ? NC : MAXN;
     // wavefront path; with 8 fields the twiddles are read from the (cached)
     // global table so that three workgroups fit a CU's LDS
@@ -963,9 +962,11 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
         if (idx < nitems) {
             int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
             if (f < nf) {
-                // rows j / j2 hold the even / odd Legendre sums: north E + O, south E - O
-                const cplx<T> e = *(const cplx<T> *)(fs + ((long)m * nlat + j) * ldf + 2 * f);
-                const cplx<T> o = *(const cplx<T> *)(fs + ((long)m * nlat + j2) * ldf + 2 * f);
+                // rows j / j2 hold the even / odd Legendre sums: north E + O, south E - O;
+                // field f is Fourier field f + fofs
+                const int cf = 2 * (f + fofs);
+                const cplx<T> e = *(const cplx<T> *)(fs + fs_index(j, m, cf, ldf, ntrunc + 1));
+                const cplx<T> o = *(const cplx<T> *)(fs + fs_index(j2, m, cf, ldf, ntrunc + 1));
                 xn[c] = cplx<T>{e.x + o.x, e.y + o.y};
                 xs[c] = cplx<T>{e.x - o.x, e.y - o.y};
             }
@@ -1476,13 +1477,13 @@ static Radices radices_of(int n, const int *r) {
 
 template <typename T, int NF>
 static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                         const unsigned char *rflag, const T *mean, const int *slot, hipStream_t s) {
+                         const unsigned char *rflag, const T *mean, const int *slot, int fofs, hipStream_t s) {
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_SYN(NC)                                                                                             \
     hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(fft_threads(NF, NC)), 0, s, fs, ldf, grid, gstride, nf,   \
                        rflag, mean, slot,                                                                     \
-                       p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad))
+                       p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad), fofs)
     switch (p.nlon) {
     case 768: FFT_SYN(768); break;
     case 384: FFT_SYN(384); break;
@@ -1516,11 +1517,11 @@ static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, c
 // workgroups per CU (LDS = fields * nlon * sizeof(complex))
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                      const unsigned char *rflag, hipStream_t s, const T *mean, const int *slot) {
+                      const unsigned char *rflag, hipStream_t s, const T *mean, const int *slot, int fofs) {
     switch (p.fft_nf) {
-    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, s); break;
-    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, s); break;
-    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, s); break;
+    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s); break;
+    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s); break;
+    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s); break;
     }
 }
 
@@ -1553,7 +1554,7 @@ void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc 
     template void launch_leg_synth<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);            \
     template void launch_leg_anal<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);             \
     template void launch_fft_synth<T>(const Plan<T> &, const T *, int, T *, long, int, const unsigned char *,  \
-                                      hipStream_t, const T *, const int *);                                    \
+                                      hipStream_t, const T *, const int *, int);                               \
     template void launch_fft_anal<T>(const Plan<T> &, const T *, long, int, const unsigned char *, T *, int,   \
                                      hipStream_t);                                                             \
     template void launch_anal_scatter<T>(const Plan<T> &, const T *, int, const AnalDesc *, int, cplx<T> *,    \
