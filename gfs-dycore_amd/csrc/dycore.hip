@@ -64,7 +64,7 @@ struct DycoreBuffers {
     void *tphys = nullptr;                       // physics tendencies
     void *gsyn = nullptr, *gan = nullptr;        // grid fields (T)
     void *gmean = nullptr;                       // global means added back by the Fourier synthesis (nm * nsyn)
-    void *etadot = nullptr, *dlnpdt = nullptr;   // per member (nl+1)*ng, nl*ng
+    void *etadot = nullptr, *dlnpdt = nullptr;   // per member (nl+1)*ng (etadot less -ps bk dlnpsdt), nl*ng
     void *psg = nullptr, *dlnpsdt = nullptr;     // per member ng
     void *dphisdx = nullptr, *dphisdy = nullptr; // per member ng
     void *topo = nullptr;                        // spectral orography per member (complex)
@@ -237,8 +237,9 @@ __global__ __launch_bounds__(256) void k_columns(
     auto dp = [&](int K) { return dak[K - 1] + dbk[K - 1] * ps; };  // pk(K + 1) - pk(K)
     // pass 1 (top -> bottom): getomega's running sums db, cb, omega/p (dlnpdtg)
     // on layers, and etadot at interfaces less its term -ps bk_K dlnpsdt, which
-    // needs the column totals and is added where etadot is read (pass 3, or
-    // the loop below when only the inverse is wanted)
+    // needs the column totals. The etadot buffer keeps that partial value; its
+    // readers add the term (passes 3 and 4 here, and k_etadot_complete for the
+    // etadot grid handed to callers)
     T db = 0, cb = 0;
     ET[0] = T(0);
     ET[(long)nl * ng] = T(0);
@@ -294,10 +295,7 @@ __global__ __launch_bounds__(256) void k_columns(
         }
         colq[(long)mem * ng + col] = s;
     }
-    if (inverse_only) {
-        for (int K = 1; K <= nl - 1; K++) ET[(long)K * ng] += et_fix(K);
-        return;
-    }
+    if (inverse_only) return;
     T *A = gan + (long)mem * nan * ng + col;
     T *TT = A, *FX = A + (long)nl * ng, *FY = A + 2L * nl * ng, *KE = A + 3L * nl * ng;
     A[4L * nl * ng] = dlnpsdt;
@@ -338,7 +336,6 @@ __global__ __launch_bounds__(256) void k_columns(
             T ea = T(0);                                   // etadot at the top interface K - 1
             if (K > 1) {
                 ea = ET[(long)(K - 1) * ng] + et_fix(K - 1);
-                ET[(long)(K - 1) * ng] = ea;
             }
             T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K), rdp = recip(dpk);
             T rlnp, alfa, cofa, dbp;
@@ -415,7 +412,9 @@ __global__ __launch_bounds__(256) void k_columns(
         T ddk = guard(Dk1 - Dk);                                                    // dd_1
         T hprev = Dk;                                                                // half_0
         for (int K = 1; K <= nl; K++) {
-            const T etop = ET[(long)(K - 1) * ng], ebot = ET[(long)K * ng];
+            // the interfaces 0 and nl are exactly zero; the others are completed here
+            const T etop = (K > 1) ? ET[(long)(K - 1) * ng] + et_fix(K - 1) : T(0);
+            const T ebot = (K < nl) ? ET[(long)K * ng] + et_fix(K) : T(0);
             const T dpk = dp(K);
             T hk, Dk2 = T(0), ddn = T(0);
             if (K <= nl - 1) {
@@ -1673,6 +1672,23 @@ __global__ void k_pack(cplx<T> *__restrict__ spec, const cplx<T> *__restrict__ r
 }
 
 // grid fields of a member-major field set -> caller layout
+// etadot handed to callers: the column kernel stores interface K (0 < K < nl)
+// without its term -ps bk_K dlnpsdt (added by its readers); this adds it to a
+// gathered copy, [member][nl + 1 interfaces][ng]
+template <typename T>
+__global__ void k_etadot_complete(T *__restrict__ et, const T *__restrict__ psg, const T *__restrict__ dlnpsdt,
+                                  const T *__restrict__ bk, int nl, long ng, int nm) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)(nl + 1) * ng;
+    if (i >= per * nm) return;
+    const int mem = (int)(i / per);
+    const long r = i - mem * per;
+    const int K = (int)(r / ng);
+    const long col = r - K * ng;
+    if (K == 0 || K == nl) return;
+    et[i] += -(psg[(long)mem * ng + col] * bk[K]) * dlnpsdt[(long)mem * ng + col];
+}
+
 template <typename T>
 __global__ void k_grid_gather(const T *__restrict__ src, long mstride, T *__restrict__ dst, long per_mem, int nm) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3016,6 +3032,11 @@ template <typename T> int Engine<T>::get_grid(int which, void *out, int dtype, i
     hipLaunchKernelGGL(k_grid_gather<T>, dim3((unsigned)((per * nm + 255) / 256)), dim3(256), 0, stream, src,
                        mstride, (T *)stage[3], per, nm);
     GFSDC_CHECK(hipGetLastError());
+    if (which == GF_ETADOT) {  // the column kernel keeps etadot without its -ps bk dlnpsdt term
+        hipLaunchKernelGGL(k_etadot_complete<T>, dim3((unsigned)((per * nm + 255) / 256)), dim3(256), 0, stream,
+                           (T *)stage[3], (const T *)B->psg, (const T *)B->dlnpsdt, (const T *)B->bk, nl, ng, nm);
+        GFSDC_CHECK(hipGetLastError());
+    }
     return output((const T *)stage[3], out, (size_t)per * nm, dtype, loc, 1);
 }
 
