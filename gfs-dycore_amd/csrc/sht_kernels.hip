@@ -343,44 +343,39 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
         }
 }
 
-// Both parities of one LDS chunk (analysis). GUARD skips, wave-uniformly, the
-// 16-row blocks of E (O) at or beyond ne (no) and the 16-column blocks at or
-// beyond nbj: the rows past the last degree of a parity and the columns past
-// the last field of a partial tile.
+// One LDS chunk of one parity (analysis). GUARD skips, wave-uniformly, the
+// 16-row blocks at or beyond ne (rows past the parity's last degree) and the
+// 16-column blocks at or beyond nbj (columns past the last field).
 template <typename T, bool GUARD, typename Acc>
-__device__ inline void gemm_chunk2(T (*As)[GEMM_BK][GEMM_BM + LDS_PAD], T (*Bs)[GEMM_BK][GT<T>::BN + LDS_PAD],
-                                   Acc (&E)[2][GT<T>::WB], Acc (&O)[2][GT<T>::WB], int lane, int wm, int wn,
-                                   int ne = 2, int no = 2, int nbj = GT<T>::WB) {
+__device__ inline void gemm_chunk1(T (*As)[GEMM_BM + LDS_PAD], T (*Bs)[GT<T>::BN + LDS_PAD],
+                                   Acc (&E)[2][GT<T>::WB], int lane, int wm, int wn, int ne = 2,
+                                   int nbj = GT<T>::WB) {
 #pragma unroll
     for (int ks = 0; ks < GEMM_BK; ks += 4) {
         const int kr = ks + (lane >> 4), li = lane & 15;
-        T ae[2], ao[2];
+        T a[2];
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++) {
-            ae[bi] = As[0][kr][wm + bi * 16 + li];
-            ao[bi] = As[1][kr][wm + bi * 16 + li];
-        }
+        for (int bi = 0; bi < 2; bi++) a[bi] = As[kr][wm + bi * 16 + li];
 #pragma unroll
         for (int bj = 0; bj < GT<T>::WB; bj++) {
             if (GUARD && bj >= nbj) break;
-            T be = Bs[0][kr][wn + bj * 16 + li];
-            T bo = Bs[1][kr][wn + bj * 16 + li];
+            T b = Bs[kr][wn + bj * 16 + li];
 #pragma unroll
             for (int bi = 0; bi < 2; bi++) {
                 if (Vec<T>::swap) {
-                    if (!GUARD || bi < ne) E[bi][bj] = MF<T>::mma(be, ae[bi], E[bi][bj]);
-                    if (!GUARD || bi < no) O[bi][bj] = MF<T>::mma(bo, ao[bi], O[bi][bj]);
+                    if (!GUARD || bi < ne) E[bi][bj] = MF<T>::mma(b, a[bi], E[bi][bj]);
                 } else {
-                    if (!GUARD || bi < ne) E[bi][bj] = MF<T>::mma(ae[bi], be, E[bi][bj]);
-                    if (!GUARD || bi < no) O[bi][bj] = MF<T>::mma(ao[bi], bo, O[bi][bj]);
+                    if (!GUARD || bi < ne) E[bi][bj] = MF<T>::mma(a[bi], b, E[bi][bj]);
                 }
             }
         }
     }
 }
 
-// Analysis: output degrees i' of each parity, K over the (zero-padded) latitude
-// pairs. fa[m][0][j][c] = w_j (F_N + F_S), fa[m][1][j][c] = w_j (F_N - F_S).
+// Analysis, one parity per workgroup: output degrees i' of that parity, K over
+// the (zero-padded) latitude pairs. fa[m][0][j][c] = w_j (F_N + F_S),
+// fa[m][1][j][c] = w_j (F_N - F_S). Each workgroup holds one parity's
+// accumulators only (half the registers and LDS of a two-parity tile).
 template <typename T>
 __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, const long *__restrict__ poff,
                                                   const int *__restrict__ keoff, const int *__restrict__ mtile_m,
@@ -391,67 +386,59 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
     constexpr int VW = Vec<T>::n;
     constexpr int BN = GT<T>::BN, WB = GT<T>::WB;
     constexpr int NA = GEMM_BK * GEMM_BM / VW / 256, NB = GEMM_BK * BN / VW / 256;
-    // logical tile mt * ntc + tc, contiguous per XCD: the column tiles of one
-    // (m, degree tile) share its Legendre block in that XCD's L2
-    const int ntc = gridDim.x / nmt;
-    const int L = xcd_tile(blockIdx.x, gridDim.x), tc = L % ntc, mt = L / ntc;
+    // logical tile (mt * 2 + parity) * ntc + tc, contiguous per XCD: the column
+    // tiles of one (m, degree tile, parity) share its Legendre block in that XCD's L2
+    const int ntc = gridDim.x / (2 * nmt);
+    const int L = xcd_tile(blockIdx.x, gridDim.x), tc = L % ntc, g = L / ntc, par = g & 1, mt = g >> 1;
     const int m = mtile_m[mt], i0 = mtile_i[mt];
-    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1;
-    const T *P0 = plm + poff[m], *P1 = P0 + (long)Ke * nhp;
-    const T *F0 = fa + (long)(2 * m) * nhp * ldf, *F1 = F0 + (long)nhp * ldf;
+    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1, Kp = par ? Ko : Ke;
+    if (i0 >= Kp) return;  // an odd-parity tile past the last odd degree (the tiles follow the even block)
+    const T *P = plm + poff[m] + (par ? (long)Ke * nhp : 0L);
+    const T *F = fa + (long)(2 * m + par) * nhp * ldf;
     const int c0 = tc * BN;
-    __shared__ T As[2][GEMM_BK][GEMM_BM + LDS_PAD];
-    __shared__ T Bs[2][GEMM_BK][BN + LDS_PAD];
+    __shared__ T As[GEMM_BK][GEMM_BM + LDS_PAD];
+    __shared__ T Bs[GEMM_BK][BN + LDS_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * (BN / 2);
-    acc_t E[2][WB], O[2][WB];
+    acc_t E[2][WB];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < WB; b++) {
-            E[a][b] = acc_t{0, 0, 0, 0};
-            O[a][b] = acc_t{0, 0, 0, 0};
-        }
-    V ra[2][NA], rb[2][NB];
+        for (int b = 0; b < WB; b++) E[a][b] = acc_t{0, 0, 0, 0};
+    V ra[NA], rb[NB];
     auto load = [&](int j0) {
 #pragma unroll
-        for (int par = 0; par < 2; par++) {
+        for (int u = 0; u < NA; u++) {
+            int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW, i = i0 + ii;
+            V x = *(const V *)(P + (long)min(i, Kp - 1) * nhp + j0 + kk);
+            ra[u] = vkeep(x, i < Kp);
+        }
 #pragma unroll
-            for (int u = 0; u < NA; u++) {
-                int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW, i = i0 + ii;
-                V x = *(const V *)((par ? P1 : P0) + (long)min(i, (par ? Ko : Ke) - 1) * nhp + j0 + kk);
-                ra[par][u] = vkeep(x, i < (par ? Ko : Ke));
-            }
-#pragma unroll
-            for (int u = 0; u < NB; u++) {
-                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
-                rb[par][u] = vkeep(*(const V *)((par ? F1 : F0) + (long)(j0 + kk) * ldf + c0 + cc), true);
-            }
+        for (int u = 0; u < NB; u++) {
+            int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
+            rb[u] = vkeep(*(const V *)(F + (long)(j0 + kk) * ldf + c0 + cc), true);
         }
     };
     auto store = [&]() {
 #pragma unroll
-        for (int par = 0; par < 2; par++) {
+        for (int u = 0; u < NA; u++) {
+            int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW;
 #pragma unroll
-            for (int u = 0; u < NA; u++) {
-                int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW;
+            for (int q = 0; q < VW; q++) As[kk + q][ii] = vget(ra[u], q);
+        }
 #pragma unroll
-                for (int q = 0; q < VW; q++) As[par][kk + q][ii] = vget(ra[par][u], q);
-            }
-#pragma unroll
-            for (int u = 0; u < NB; u++) {
-                int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
-                vstore_lds<T>(&Bs[par][kk][cc], rb[par][u]);
-            }
+        for (int u = 0; u < NB; u++) {
+            int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW;
+            vstore_lds<T>(&Bs[kk][cc], rb[u]);
         }
     };
     // FP32 prefetches chunk k+1 during chunk k; FP64 (twice the registers per
     // element) loads each chunk just before storing it to keep occupancy
     constexpr bool PREFETCH = sizeof(T) == 4;
-    // valid 16-row blocks per parity and 16-column blocks of this wave
-    const int ne = min(2, max(0, (Ke - i0 - wm + 15) / 16)), no = min(2, max(0, (Ko - i0 - wm + 15) / 16));
+    // valid 16-row blocks and 16-column blocks of this wave
+    const int ne = min(2, max(0, (Kp - i0 - wm + 15) / 16));
     const int nbj = min(WB, max(0, (ncol - c0 - wn + 15) / 16));
-    const bool full = ne == 2 && no == 2 && nbj == WB;
+    const bool full = ne == 2 && nbj == WB;
     if (PREFETCH) load(0);
     for (int j0 = 0; j0 < nhp; j0 += GEMM_BK) {
         if (!PREFETCH) load(j0);
@@ -459,9 +446,9 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
         __syncthreads();
         if (PREFETCH && j0 + GEMM_BK < nhp) load(j0 + GEMM_BK);
         if (full)
-            gemm_chunk2<T, false>(As, Bs, E, O, lane, wm, wn);
+            gemm_chunk1<T, false>(As, Bs, E, lane, wm, wn);
         else
-            gemm_chunk2<T, true>(As, Bs, E, O, lane, wm, wn, ne, no, nbj);
+            gemm_chunk1<T, true>(As, Bs, E, lane, wm, wn, ne, nbj);
         __syncthreads();
     }
     T *X = sx + (long)keoff[m] * ldx;
@@ -472,16 +459,14 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
             if constexpr (Vec<T>::swap) {
                 int i = i0 + wm + bi * 16 + (lane & 15);
                 int c = c0 + wn + bj * 16 + 4 * (lane >> 4);
-                acc_t e = E[bi][bj], o = O[bi][bj];
-                if (i < Ke) *(float4 *)(X + (long)(2 * i) * ldx + c) = float4{e[0], e[1], e[2], e[3]};
-                if (i < Ko) *(float4 *)(X + (long)(2 * i + 1) * ldx + c) = float4{o[0], o[1], o[2], o[3]};
+                acc_t e = E[bi][bj];
+                if (i < Kp) *(float4 *)(X + (long)(2 * i + par) * ldx + c) = float4{e[0], e[1], e[2], e[3]};
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     int i = i0 + wm + bi * 16 + MF<T>::row(lane, r);
                     int c = c0 + wn + bj * 16 + (lane & 15);
-                    if (i < Ke) X[(long)(2 * i) * ldx + c] = E[bi][bj][r];
-                    if (i < Ko) X[(long)(2 * i + 1) * ldx + c] = O[bi][bj][r];
+                    if (i < Kp) X[(long)(2 * i + par) * ldx + c] = E[bi][bj][r];
                 }
             }
         }
@@ -1464,7 +1449,7 @@ void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int
             return;
         }
     }
-    hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc * p.n_mtiles), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
+    hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc * p.n_mtiles * 2), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
                        p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol, p.n_mtiles);
 }
 
