@@ -168,6 +168,41 @@ def test_graph_replay_matches_eager(prec):
         assert np.array_equal(sa[k], sb[k]), k
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_topography_enters_as_the_geopotential_gradient(prec):
+    """set_topo (dyn_init.f90:84: dphisdx, dphisdy = getgrad(grav topo)) adds
+    -grad(phi_s) to the pressure-gradient force and nothing else. With the same
+    state, the tendencies with and without topography differ by exactly the
+    divergence of that force, -lap(phi_s) = n(n+1)/a^2 phi_s, in ddiv; the
+    vorticity, temperature and ln ps tendencies do not change (the curl of a
+    gradient is zero). Ten members with distinct topographies put the y
+    gradients of the last members into the next 16-field group of the
+    Fourier buffer."""
+    import gfsdc
+    p = gfsdc.FP64 if prec == "fp64" else gfsdc.FP32
+    nm = 10
+    ak, bk = coords("hybrid", 20)
+    ctx = [gfsdc.Context(NLON, NLAT, T, p, nlev=20, ak=ak, bk=bk, dt=1200.0, nmem=nm) for _ in range(2)]
+    for c in ctx:
+        c.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=40)
+    a = ctx[0]
+    rng = np.random.default_rng(41)
+    n = a.degree.astype(float)
+    topo = 1.0e3 * (np.maximum(n, 1) ** -1.5)[None, :] * np.exp(2j * np.pi * rng.random((nm, a.nlm)))
+    topo[:, a.order == 0] = topo[:, a.order == 0].real
+    ctx[1].set_topo(topo)
+    t0, t1 = ctx[0].getdyntend(1), ctx[1].getdyntend(1)
+    want = (n * (n + 1) / a.cfg.rerth ** 2)[None, :] * (a.cfg.grav * topo)  # (nm, nlm)
+    tol = 1e-9 if prec == "fp64" else 1e-4
+    for mem in range(nm):
+        scale = np.abs(want[mem]).max()
+        dd = t1["ddiv"][mem] - t0["ddiv"][mem]
+        assert np.abs(dd - want[mem][None, :]).max() < tol * scale, (mem, np.abs(dd - want[mem]).max() / scale)
+        assert np.abs(t1["dvrt"][mem] - t0["dvrt"][mem]).max() < tol * scale, mem
+        assert np.abs(t1["dtv"][mem] - t0["dtv"][mem]).max() == 0, mem
+        assert np.abs(t1["dlnps"][mem] - t0["dlnps"][mem]).max() == 0, mem
+
+
 def test_ensemble_members_independent():
     import gfsdc
     ak, bk = coords("sigma", 20)
