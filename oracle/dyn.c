@@ -15,11 +15,13 @@
  * operators of init_semimpdata (linearisation test); init_semimpdata by
  * (I + c^2 dt^2 n(n+1) ym) d = I and the tableau identities; advance by
  * the IMEX order and a resting-state invariant; HS by closed-form columns.
- * The exact vertical differencing of getvadv has no closed form and no
- * reference-run vectors here: it is pinned only by the constant-field and
- * isothermal properties. getvadv_tracers is pinned by its invariants: the flux
- * part telescopes, a uniform field has no tendency, a linear profile gives the
- * centred getvadv difference, and a forward step keeps a tracer non-negative.
+ * The vertical differencing of getvadv has no closed form and no
+ * reference-run vectors here: it is pinned by the constant-field and
+ * isothermal properties and by its identity with the flux-form scheme of
+ * getvadv_tracers on linear profiles (limiter inactive), in every layer.
+ * getvadv_tracers is pinned by its invariants: the flux part telescopes, a
+ * uniform field has no tendency, a linear profile gives the centred getvadv
+ * difference, and a forward step keeps a tracer non-negative.
  */
 #include "oracle.h"
 #include <math.h>
@@ -530,20 +532,26 @@ static void getpresgrad(orc_model *m, const double *dtvdx, const double *dtvdy, 
     }
 }
 
-/* ---- dyn_run.f90:453-476 getvadv ----------------------------------- */
-static void getvadv(orc_model *m, const double *datag, double *vadv) {
-    int nl = m->nlev;
-    long ng = m->ng;
-    const double *et = m->etadot, *dpk = m->dpk;
+/* ---- dyn_run.f90:453-476 getvadv -----------------------------------
+ * Columns-fastest arrays as orc_vadv_tracers_cols: datag (nl levels bottom ->
+ * top), et (nl+1 interfaces top -> bottom), dpk (nl layers top -> bottom) ->
+ * vadv (bottom -> top). */
+void orc_vadv_cols(int nl, long ng, const double *datag, const double *et, const double *dpk, double *vadv) {
+#define LV(X, k) ((X) + (long)((k) - 1) * ng)
 #pragma omp parallel for
     for (long i = 0; i < ng; i++) {
-        L(vadv, nl)[i] = (0.5 / L(dpk, 1)[i]) * L(et, 2)[i] * (L(datag, nl - 1)[i] - L(datag, nl)[i]);
-        L(vadv, 1)[i] = (0.5 / L(dpk, nl)[i]) * L(et, nl)[i] * (L(datag, 1)[i] - L(datag, 2)[i]);
+        LV(vadv, nl)[i] = (0.5 / LV(dpk, 1)[i]) * LV(et, 2)[i] * (LV(datag, nl - 1)[i] - LV(datag, nl)[i]);
+        LV(vadv, 1)[i] = (0.5 / LV(dpk, nl)[i]) * LV(et, nl)[i] * (LV(datag, 1)[i] - LV(datag, 2)[i]);
         for (int k = 2; k <= nl - 1; k++)
-            L(vadv, nl + 1 - k)[i] =
-                (0.5 / L(dpk, k)[i]) * (L(et, k + 1)[i] * (L(datag, nl - k)[i] - L(datag, nl + 1 - k)[i]) +
-                                        L(et, k)[i] * (L(datag, nl + 1 - k)[i] - L(datag, nl + 2 - k)[i]));
+            LV(vadv, nl + 1 - k)[i] =
+                (0.5 / LV(dpk, k)[i]) * (LV(et, k + 1)[i] * (LV(datag, nl - k)[i] - LV(datag, nl + 1 - k)[i]) +
+                                         LV(et, k)[i] * (LV(datag, nl + 1 - k)[i] - LV(datag, nl + 2 - k)[i]));
     }
+#undef LV
+}
+
+static void getvadv(orc_model *m, const double *datag, double *vadv) {
+    orc_vadv_cols(m->nlev, m->ng, datag, m->etadot, m->dpk, vadv);
 }
 
 /* ---- dyn_run.f90:478-603 getvadv_tracers (Thuburn 1993 limiter) -----

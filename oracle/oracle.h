@@ -145,6 +145,9 @@ void orc_calc_pressdata(int nlev, int ncol, const double *ak, const double *bk, 
 
 /* dyn_run.f90:478 getvadv_tracers on standalone columns (columns fastest):
  * datag (nlev, bottom->top), etadot (nlev+1, top->bottom), dpk (nlev, top->bottom) */
+/* getvadv (dyn_run.f90:453-476) on columns; arrays as orc_vadv_tracers_cols */
+void orc_vadv_cols(int nlev, long ncol, const double *datag, const double *etadot, const double *dpk,
+                   double *vadv);
 void orc_vadv_tracers_cols(int nlev, long ncol, const double *datag, const double *etadot,
                            const double *dpk, double *vadv);
 

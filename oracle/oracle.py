@@ -96,6 +96,7 @@ def lib():
         L.orc_mlp_adamw.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double]
         L.orc_pattern_advance.argtypes = [vp]
         L.orc_vadv_tracers_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
+        L.orc_vadv_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
         L.orc_rng_u64.restype = C.c_uint64
         L.orc_rng_u64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_rng_uniform.restype = C.c_double
@@ -324,6 +325,16 @@ def calc_pressdata(ak, bk, rk, lnpsg):
     rlnp = np.zeros((nlev, ncol)); prs = np.zeros((nlev, ncol))
     lib().orc_calc_pressdata(nlev, ncol, ak, bk, rk, lnpsg, pk, dpk, alfa, rlnp, prs)
     return dict(pk=pk, dpk=dpk, alfa=alfa, rlnp=rlnp, prs=prs)
+
+
+def vadv(datag, etadot, dpk):
+    """getvadv on columns: datag (nlev, ncol) bottom -> top, etadot (nlev+1, ncol)
+    and dpk (nlev, ncol) top -> bottom."""
+    datag, etadot, dpk = (np.ascontiguousarray(x, np.float64) for x in (datag, etadot, dpk))
+    nl, n = datag.shape
+    out = np.zeros_like(datag)
+    lib().orc_vadv_cols(nl, n, datag, etadot, dpk, out)
+    return out
 
 
 def vadv_tracers(datag, etadot, dpk):

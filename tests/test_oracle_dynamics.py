@@ -19,8 +19,10 @@ not come from the oracle:
 * setdampspec against its closed form at GFS default efold;
 * Held-Suarez forcing against the Held & Suarez (1994) formulas;
 * a resting isothermal atmosphere stays at rest through advance.
-The centred vertical-advection stencils (getvadv) have no closed form: beyond
-the constant-field and linear-profile checks below their parity is unpinned.
+The centred vertical-advection stencil (getvadv) has no closed form. It is
+pinned by the constant-field checks and by its identity with the independently
+coded flux-form scheme of getvadv_tracers on linear profiles, where the limiter
+is inactive (every layer, top and bottom included).
 """
 import numpy as np
 import pytest
@@ -281,6 +283,29 @@ def test_vadv_tracers_invariants():
     Dl = lin[::-1]
     centred = 0.5 / dpk[1:-1] * (et[2:-1] * (Dl[2:] - Dl[1:-1]) + et[1:-2] * (Dl[1:-1] - Dl[:-2]))
     np.testing.assert_allclose(v[1:-1], centred, rtol=1e-9, atol=1e-18)
+
+
+def test_vadv_equals_the_flux_scheme_on_linear_profiles():
+    """getvadv (dyn_run.f90:453-476, the centred advective difference) against
+    the independently coded flux-form scheme of getvadv_tracers (dyn_run.f90:
+    478-603): on a profile linear in the level index whose values and end
+    extrapolations keep one sign, every limiter ratio is 1, the limited
+    interface values are the midpoints, and the flux form with its continuity
+    term reduces to the centred difference in every layer, the top and bottom
+    ones included, for either sign of etadot."""
+    nl = 30
+    dpk, et = _vadv_columns(5, nl)
+    ncol = dpk.shape[1]
+    rng = np.random.default_rng(6)
+    a = rng.uniform(1.0, 2.0, ncol)
+    b = rng.uniform(-0.02, 0.02, ncol)
+    lin = a[None, :] + b[None, :] * np.arange(nl)[:, None]       # bottom -> top, positive
+    np.testing.assert_allclose(O.vadv(lin, et, dpk), O.vadv_tracers(lin, et, dpk), rtol=1e-12, atol=1e-15)
+    # a uniform field is not advected, whatever etadot is
+    assert np.abs(O.vadv(np.full((nl, ncol), 3.0), et, dpk)).max() == 0
+    # and a non-linear profile is where the two schemes part (the limiter acts)
+    wavy = 1.5 + 0.4 * np.sin(0.9 * np.arange(nl))[:, None] * np.ones((1, ncol))
+    assert np.abs(O.vadv(wavy, et, dpk) - O.vadv_tracers(wavy, et, dpk)).max() > 1e-6
 
 
 def test_vadv_tracers_keeps_tracer_positive():
