@@ -100,7 +100,14 @@ def algorithmic_work(name, ctx, esz):
     if name == "leg_synth":
         return "mfma", nm * nsyn * 2.0 * nlat * nlm
     if name == "leg_anal":
-        return "mfma", None  # batch size differs between dynamics and physics passes
+        # dynamics passes: T, the two flux components and KE per level, ln ps and
+        # the tracers; the Held-Suarez forcing: T, vorticity, divergence, ln ps
+        nan = (4 + ctx.ntrac) * nl + 1
+        if ctx.cfg.physics == gfsdc.PHYS_HS:
+            nan = (3 * nan + 3 * nl + 1) / 4
+        elif ctx.cfg.physics != gfsdc.PHYS_NONE:
+            return "mfma", None
+        return "mfma", nm * nan * 2.0 * nlat * nlm
     if name == "fft_synth":
         return "hbm", nm * nsyn * (four + ng * esz)
     if name == "columns":
@@ -205,12 +212,14 @@ def run_headline(args):
         del c64
     tab = kernel_table(c32, args.prof_steps)
     roof, per_step = roofline(tab, c32, 4)
-    leg = tab.get("leg_synth")
-    legendre = None
-    if leg:
-        _, f = algorithmic_work("leg_synth", c32, 4)
-        legendre = {"kernel": "leg_synth", "avg_launch_ms": leg[0], "achieved_tflops": f / (leg[0] * 1e-3) / 1e12,
-                    "peak_tflops": PEAK["fp32_mfma"] / 1e12, "frac": f / (leg[0] * 1e-3) / PEAK["fp32_mfma"]}
+    def mfma_frac(name):
+        leg = tab.get(name)
+        _, f = algorithmic_work(name, c32, 4)
+        if not leg or f is None:
+            return None
+        return {"kernel": name, "avg_launch_ms": leg[0], "achieved_tflops": f / (leg[0] * 1e-3) / 1e12,
+                "peak_tflops": PEAK["fp32_mfma"] / 1e12, "frac": f / (leg[0] * 1e-3) / PEAK["fp32_mfma"]}
+    legendre, legendre_anal = mfma_frac("leg_synth"), mfma_frac("leg_anal")
     err = None
     if args.error_steps > 0 and not args.no_fp64:
         a, b = make(args.config, gfsdc.FP32), make(args.config, gfsdc.FP64)
@@ -227,6 +236,7 @@ def run_headline(args):
          "members": 1, "hip_graph": True},
         ms_per_step=t32 * 1e3, **out, error_vs_fp64_after_steps=args.error_steps if err else 0,
         spectral_rel_error_vs_fp64=err, roofline=roof, legendre_gemm=legendre,
+        legendre_gemm_analysis=legendre_anal,
         kernels_ms_per_step={k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
         cpu_baseline=cpu)
     return line
