@@ -10,11 +10,25 @@
  * pressure_data.f90:92-93). They are rounded to float here with SGL() so the
  * restatement carries the same values as that build.
  *
- * Pinning: calc_pressdata against closed forms; getomega/getpresgrad/
- * energy conversion/continuity against the independently built linear
- * operators of init_semimpdata (linearisation test); init_semimpdata by
- * (I + c^2 dt^2 n(n+1) ym) d = I and the tableau identities; advance by
- * the IMEX order and a resting-state invariant; HS by closed-form columns.
+ * Pinning (tests/test_oracle_*.py):
+ *  - calc_pressdata against closed forms;
+ *  - getomega on general columns (non-isothermal, level-dependent winds,
+ *    V.grad ln ps != 0, sigma and hybrid levels) by the column continuity
+ *    equation, each layer's discrete mass budget, omega/p in the Simmons &
+ *    Burridge (1981) form and their energy-conversion identity, all built in
+ *    numpy with complex-step gradients;
+ *  - getpresgrad on the same columns against -(grad phi + R T (grad ln p))
+ *    with the hydrostatic geopotential, the sigma-coordinate textbook form,
+ *    exact isothermal balance and the column angular-momentum identity;
+ *  - energy conversion and continuity in getdyntend by the linearisation
+ *    about a resting isothermal state against the independently built
+ *    init_semimpdata operators, and by an exact balanced zonal flow;
+ *  - init_semimpdata by (I + c^2 dt^2 n(n+1) ym) d = I and the tableau
+ *    identities; setdampspec (sponge included) by its closed form;
+ *  - advance by a generic IMEX-RK step (dense implicit solve per
+ *    coefficient, backward-Euler damping) from the oracle's own tendencies,
+ *    and by second-order convergence to expm(J t) on linear gravity waves;
+ *  - HS by closed-form columns.
  * The vertical differencing of getvadv has no closed form and no
  * reference-run vectors here: it is pinned by the constant-field and
  * isothermal properties and by its identity with the flux-form scheme of
@@ -449,87 +463,124 @@ static double area_sum(const orc_model *m, const double *x) {
     return s;
 }
 
-/* ---- dyn_run.f90:307-373 getomega ---------------------------------- */
-static void getomega(orc_model *m) {
-    int nl = m->nlev;
-    long ng = m->ng;
-    double *cg = m->cg, *cb = m->cb, *db = m->db, *wb = m->w[8], *wc = m->w[9];
-    double *dpk = m->dpk, *dbk = m->dbk, *bk = m->bk, *ck = m->ck, *psg = m->psg;
-#pragma omp parallel for
-    for (long i = 0; i < ng; i++) {
-        for (int k = 1; k <= nl; k++)
-            L(cg, k)[i] = L(m->ug, nl + 1 - k)[i] * m->dlnpsdx[i] + L(m->vg, nl + 1 - k)[i] * m->dlnpsdy[i];
-        L(db, 1)[i] = L(m->divg, nl)[i] * L(dpk, 1)[i];
-        L(cb, 1)[i] = L(cg, 1)[i] * dbk[0];
-        for (int k = 1; k <= nl - 1; k++) {
-            L(db, k + 1)[i] = L(db, k)[i] + L(m->divg, nl - k)[i] * L(dpk, k + 1)[i];
-            L(cb, k + 1)[i] = L(cb, k)[i] + L(cg, k + 1)[i] * dbk[k];
+/* ---- dyn_run.f90:307-373 getomega ----------------------------------
+ * On ng columns, arrays columns-fastest: ug, vg, divg (nl levels bottom ->
+ * top); bk (nl+1), dbk, ck (nl) and pk (nl+1), dpk, alfa, rlnp (nl) top ->
+ * bottom; psg, dlnpsdx, dlnpsdy per column -> dlnpsdt (per column), etadot
+ * (nl+1 interfaces top -> bottom) and dlnpdtg = omega/p (bottom -> top).
+ * cg, cb, db, workb and workc are per-column scratch (reference work arrays). */
+void orc_omega_cols(int nl, long ng, const double *bk, const double *dbk, const double *ck,
+                    const double *psg, const double *dpk, const double *alfa, const double *rlnp,
+                    const double *ug, const double *vg, const double *divg, const double *dlnpsdx,
+                    const double *dlnpsdy, double *dlnpsdt, double *etadot, double *dlnpdtg) {
+#define LV(X, k) ((X) + (long)((k) - 1) * ng)
+#pragma omp parallel
+    {
+        double *cg = malloc(sizeof(double) * (nl + 1)), *cb = malloc(sizeof(double) * (nl + 1));
+        double *db = malloc(sizeof(double) * (nl + 1)), *wb = malloc(sizeof(double) * (nl + 1));
+        double *wc = malloc(sizeof(double) * (nl + 1));
+#pragma omp for
+        for (long i = 0; i < ng; i++) {
+            double ps = psg[i];
+            /* 1-based level index k on the scratch arrays */
+            for (int k = 1; k <= nl; k++)
+                cg[k] = LV(ug, nl + 1 - k)[i] * dlnpsdx[i] + LV(vg, nl + 1 - k)[i] * dlnpsdy[i];
+            db[1] = LV(divg, nl)[i] * LV(dpk, 1)[i];
+            cb[1] = cg[1] * dbk[0];
+            for (int k = 1; k <= nl - 1; k++) {
+                db[k + 1] = db[k] + LV(divg, nl - k)[i] * LV(dpk, k + 1)[i];
+                cb[k + 1] = cb[k] + cg[k + 1] * dbk[k];
+            }
+            double dpsdt = -db[nl] / ps - cb[nl];
+            dlnpsdt[i] = dpsdt;
+            LV(etadot, 1)[i] = 0.0;
+            LV(etadot, nl + 1)[i] = 0.0;
+            for (int k = 1; k <= nl - 1; k++)
+                LV(etadot, k + 1)[i] = -ps * (bk[k] * dpsdt + cb[k]) - db[k];
+            wb[1] = LV(alfa, 1)[i] * (LV(divg, nl)[i] * LV(dpk, 1)[i] + ps * cb[1] * dbk[0]);
+            for (int k = 2; k <= nl; k++)
+                wb[k] = LV(rlnp, k)[i] * (db[k - 1] + ps * cb[k - 1]) +
+                        LV(alfa, k)[i] * (LV(divg, nl + 1 - k)[i] * LV(dpk, k)[i] + ps * cg[k] * dbk[k - 1]);
+            wc[1] = ps * cg[1] * dbk[0];
+            for (int k = 2; k <= nl; k++)
+                wc[k] = ps * cg[k] * (dbk[k - 1] + ck[k - 1] * LV(rlnp, k)[i] / LV(dpk, k)[i]);
+            for (int k = 1; k <= nl; k++)
+                LV(dlnpdtg, nl + 1 - k)[i] = (wc[k] - wb[k]) / LV(dpk, k)[i];
         }
-        m->dlnpsdt[i] = -L(db, nl)[i] / psg[i] - L(cb, nl)[i];
-        L(m->etadot, 1)[i] = 0.0;
-        L(m->etadot, nl + 1)[i] = 0.0;
-        for (int k = 1; k <= nl - 1; k++)
-            L(m->etadot, k + 1)[i] = -psg[i] * (bk[k] * m->dlnpsdt[i] + L(cb, k)[i]) - L(db, k)[i];
-        L(wb, 1)[i] = L(m->alfa, 1)[i] * (L(m->divg, nl)[i] * L(dpk, 1)[i] + psg[i] * L(cb, 1)[i] * dbk[0]);
-        for (int k = 2; k <= nl; k++)
-            L(wb, k)[i] = L(m->rlnp, k)[i] * (L(db, k - 1)[i] + psg[i] * L(cb, k - 1)[i]) +
-                          L(m->alfa, k)[i] * (L(m->divg, nl + 1 - k)[i] * L(dpk, k)[i] +
-                                              psg[i] * L(cg, k)[i] * dbk[k - 1]);
-        L(wc, 1)[i] = psg[i] * L(cg, 1)[i] * dbk[0];
-        for (int k = 2; k <= nl; k++)
-            L(wc, k)[i] = psg[i] * L(cg, k)[i] * (dbk[k - 1] + ck[k - 1] * L(m->rlnp, k)[i] / L(dpk, k)[i]);
-        for (int k = 1; k <= nl; k++)
-            L(m->dlnpdtg, nl + 1 - k)[i] = (L(wc, k)[i] - L(wb, k)[i]) / L(dpk, k)[i];
+        free(cg); free(cb); free(db); free(wb); free(wc);
     }
+#undef LV
 }
 
-/* ---- dyn_run.f90:375-451 getpresgrad ------------------------------- */
+static void getomega(orc_model *m) {
+    orc_omega_cols(m->nlev, m->ng, m->bk, m->dbk, m->ck, m->psg, m->dpk, m->alfa, m->rlnp, m->ug, m->vg,
+                   m->divg, m->dlnpsdx, m->dlnpsdy, m->dlnpsdt, m->etadot, m->dlnpdtg);
+}
+
+/* ---- dyn_run.f90:375-451 getpresgrad -------------------------------
+ * On ng columns, arrays columns-fastest: tvg, dtvdx, dtvdy (nl levels bottom
+ * -> top); bk (nl+1), dbk (nl) and pk (nl+1), dpk, alfa, rlnp (nl) top ->
+ * bottom; psg, dphisdx/y, dlnpsdx/y per column -> prsgx, prsgy (bottom ->
+ * top). cofa, cofb, px3u and px3v are per-column scratch (the reference's
+ * work arrays, 1-based level index). */
+void orc_presgrad_cols(int nl, long ng, double rd, const double *bk, const double *dbk,
+                       const double *psg, const double *pk, const double *dpk, const double *alfa,
+                       const double *rlnp, const double *tvg, const double *dtvdx, const double *dtvdy,
+                       const double *dphisdx, const double *dphisdy, const double *dlnpsdx,
+                       const double *dlnpsdy, double *prsgx, double *prsgy) {
+#define LV(X, k) ((X) + (long)((k) - 1) * ng)
+#pragma omp parallel
+    {
+        double *cofa = malloc(sizeof(double) * (nl + 1)), *cofb = malloc(sizeof(double) * (nl + 1));
+        double *px3u = malloc(sizeof(double) * (nl + 1)), *px3v = malloc(sizeof(double) * (nl + 1));
+#pragma omp for
+        for (long i = 0; i < ng; i++) {
+            double ps = psg[i];
+#define P(k) LV(pk, k)[i]
+#define TV(k) LV(tvg, k)[i]
+            cofb[1] = -(1. / LV(dpk, 1)[i]) * (LV(alfa, 1)[i] * dbk[0]);
+            for (int k = 2; k <= nl; k++)
+                cofb[k] = -(1. / LV(dpk, k)[i]) * (bk[k - 1] * LV(rlnp, k)[i] + LV(alfa, k)[i] * dbk[k - 1]);
+            for (int k = 1; k <= nl; k++) {
+                LV(prsgx, nl - k + 1)[i] = cofb[k] * rd * TV(nl + 1 - k) * ps * dlnpsdx[i];
+                LV(prsgy, nl - k + 1)[i] = cofb[k] * rd * TV(nl + 1 - k) * ps * dlnpsdy[i];
+            }
+            for (int k = 1; k <= nl; k++)
+                cofa[k] = -(1. / LV(dpk, k)[i]) *
+                          (bk[k] * P(k) / P(k + 1) - bk[k - 1] +
+                           LV(rlnp, k)[i] * (bk[k - 1] - P(k) * dbk[k - 1] / LV(dpk, k)[i]));
+            cofb[nl] = 0.0;
+            cofb[nl - 1] = -rd * (bk[nl] / P(nl + 1) - bk[nl - 1] / P(nl)) * TV(1);
+            for (int k = 2; k <= nl - 1; k++)
+                cofb[nl - k] = cofb[nl + 1 - k] - rd * (bk[nl + 1 - k] / P(nl + 2 - k) - bk[nl - k] / P(nl + 1 - k)) * TV(k);
+            px3u[nl] = 0.0;
+            px3v[nl] = 0.0;
+            px3u[nl - 1] = -rd * LV(rlnp, nl)[i] * LV(dtvdx, 1)[i];
+            px3v[nl - 1] = -rd * LV(rlnp, nl)[i] * LV(dtvdy, 1)[i];
+            for (int k = 2; k <= nl - 1; k++) {
+                px3u[nl - k] = px3u[nl + 1 - k] - rd * LV(rlnp, nl + 1 - k)[i] * LV(dtvdx, k)[i];
+                px3v[nl - k] = px3v[nl + 1 - k] - rd * LV(rlnp, nl + 1 - k)[i] * LV(dtvdy, k)[i];
+            }
+            for (int k = 1; k <= nl; k++) {
+                LV(prsgx, nl - k + 1)[i] = LV(prsgx, nl - k + 1)[i] - dphisdx[i] + cofb[k] * ps * dlnpsdx[i] +
+                                           px3u[k] - rd * LV(alfa, k)[i] * LV(dtvdx, nl + 1 - k)[i] -
+                                           cofa[k] * rd * TV(nl + 1 - k) * ps * dlnpsdx[i];
+                LV(prsgy, nl - k + 1)[i] = LV(prsgy, nl - k + 1)[i] - dphisdy[i] + cofb[k] * ps * dlnpsdy[i] +
+                                           px3v[k] - rd * LV(alfa, k)[i] * LV(dtvdy, nl + 1 - k)[i] -
+                                           cofa[k] * rd * TV(nl + 1 - k) * ps * dlnpsdy[i];
+            }
+#undef P
+#undef TV
+        }
+        free(cofa); free(cofb); free(px3u); free(px3v);
+    }
+#undef LV
+}
+
 static void getpresgrad(orc_model *m, const double *dtvdx, const double *dtvdy, double *prsgx,
                         double *prsgy) {
-    int nl = m->nlev;
-    long ng = m->ng;
-    double rd = m->c.rd;
-    double *cofa = m->w[8], *cofb = m->w[9], *px3u = m->cb, *px3v = m->db;
-    double *dpk = m->dpk, *dbk = m->dbk, *bk = m->bk, *pk = m->pk, *psg = m->psg;
-    const double *tvg = m->tvg;
-#pragma omp parallel for
-    for (long i = 0; i < ng; i++) {
-        L(cofb, 1)[i] = -(1. / L(dpk, 1)[i]) * (L(m->alfa, 1)[i] * dbk[0]);
-        for (int k = 2; k <= nl; k++)
-            L(cofb, k)[i] = -(1. / L(dpk, k)[i]) * (bk[k - 1] * L(m->rlnp, k)[i] + L(m->alfa, k)[i] * dbk[k - 1]);
-        for (int k = 1; k <= nl; k++) {
-            L(prsgx, nl - k + 1)[i] = L(cofb, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdx[i];
-            L(prsgy, nl - k + 1)[i] = L(cofb, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdy[i];
-        }
-        for (int k = 1; k <= nl; k++)
-            L(cofa, k)[i] = -(1. / L(dpk, k)[i]) *
-                            (bk[k] * L(pk, k)[i] / L(pk, k + 1)[i] - bk[k - 1] +
-                             L(m->rlnp, k)[i] * (bk[k - 1] - L(pk, k)[i] * dbk[k - 1] / L(dpk, k)[i]));
-        L(cofb, nl)[i] = 0.0;
-        L(cofb, nl - 1)[i] = -rd * (bk[nl] / L(pk, nl + 1)[i] - bk[nl - 1] / L(pk, nl)[i]) * L(tvg, 1)[i];
-        for (int k = 2; k <= nl - 1; k++)
-            L(cofb, nl - k)[i] = L(cofb, nl + 1 - k)[i] -
-                                 rd * (bk[nl + 1 - k] / L(pk, nl + 2 - k)[i] - bk[nl - k] / L(pk, nl + 1 - k)[i]) *
-                                     L(tvg, k)[i];
-        L(px3u, nl)[i] = 0.0;
-        L(px3v, nl)[i] = 0.0;
-        L(px3u, nl - 1)[i] = -rd * L(m->rlnp, nl)[i] * L(dtvdx, 1)[i];
-        L(px3v, nl - 1)[i] = -rd * L(m->rlnp, nl)[i] * L(dtvdy, 1)[i];
-        for (int k = 2; k <= nl - 1; k++) {
-            L(px3u, nl - k)[i] = L(px3u, nl + 1 - k)[i] - rd * L(m->rlnp, nl + 1 - k)[i] * L(dtvdx, k)[i];
-            L(px3v, nl - k)[i] = L(px3v, nl + 1 - k)[i] - rd * L(m->rlnp, nl + 1 - k)[i] * L(dtvdy, k)[i];
-        }
-        for (int k = 1; k <= nl; k++) {
-            L(prsgx, nl - k + 1)[i] = L(prsgx, nl - k + 1)[i] - m->dphisdx[i] +
-                                      L(cofb, k)[i] * psg[i] * m->dlnpsdx[i] + L(px3u, k)[i] -
-                                      rd * L(m->alfa, k)[i] * L(dtvdx, nl + 1 - k)[i] -
-                                      L(cofa, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdx[i];
-            L(prsgy, nl - k + 1)[i] = L(prsgy, nl - k + 1)[i] - m->dphisdy[i] +
-                                      L(cofb, k)[i] * psg[i] * m->dlnpsdy[i] + L(px3v, k)[i] -
-                                      rd * L(m->alfa, k)[i] * L(dtvdy, nl + 1 - k)[i] -
-                                      L(cofa, k)[i] * rd * L(tvg, nl + 1 - k)[i] * psg[i] * m->dlnpsdy[i];
-        }
-    }
+    orc_presgrad_cols(m->nlev, m->ng, m->c.rd, m->bk, m->dbk, m->psg, m->pk, m->dpk, m->alfa, m->rlnp,
+                      m->tvg, dtvdx, dtvdy, m->dphisdx, m->dphisdy, m->dlnpsdx, m->dlnpsdy, prsgx, prsgy);
 }
 
 /* ---- dyn_run.f90:453-476 getvadv -----------------------------------

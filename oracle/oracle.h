@@ -150,6 +150,22 @@ void orc_vadv_cols(int nlev, long ncol, const double *datag, const double *etado
                    double *vadv);
 void orc_vadv_tracers_cols(int nlev, long ncol, const double *datag, const double *etadot,
                            const double *dpk, double *vadv);
+/* getomega (dyn_run.f90:307-373) on standalone columns (columns fastest):
+ * ug, vg, divg (nlev, bottom->top); bk (nlev+1), dbk, ck (nlev) top->bottom;
+ * psg, dpk, alfa, rlnp as calc_pressdata; dlnpsdx/y per column ->
+ * dlnpsdt (ncol), etadot (nlev+1, top->bottom), dlnpdtg = omega/p (bottom->top) */
+void orc_omega_cols(int nlev, long ncol, const double *bk, const double *dbk, const double *ck,
+                    const double *psg, const double *dpk, const double *alfa, const double *rlnp,
+                    const double *ug, const double *vg, const double *divg, const double *dlnpsdx,
+                    const double *dlnpsdy, double *dlnpsdt, double *etadot, double *dlnpdtg);
+/* getpresgrad (dyn_run.f90:375-451) on standalone columns (columns fastest):
+ * tvg, dtvdx, dtvdy (nlev, bottom->top); pk (nlev+1), dpk, alfa, rlnp top->bottom;
+ * psg, dphisdx/y, dlnpsdx/y per column -> prsgx, prsgy (bottom->top) */
+void orc_presgrad_cols(int nlev, long ncol, double rd, const double *bk, const double *dbk,
+                       const double *psg, const double *pk, const double *dpk, const double *alfa,
+                       const double *rlnp, const double *tvg, const double *dtvdx, const double *dtvdy,
+                       const double *dphisdx, const double *dphisdy, const double *dlnpsdx,
+                       const double *dlnpsdy, double *prsgx, double *prsgy);
 
 /* AR(1) pattern of the model (patterngenerator.f90): current spectral pattern
  * (ndimspec complex), the normalised variance spectrum (ndimspec), phi and the

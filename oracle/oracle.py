@@ -97,6 +97,8 @@ def lib():
         L.orc_pattern_advance.argtypes = [vp]
         L.orc_vadv_tracers_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
         L.orc_vadv_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
+        L.orc_omega_cols.argtypes = [C.c_int, C.c_long] + [_dp] * 12 + [_dp] * 3
+        L.orc_presgrad_cols.argtypes = [C.c_int, C.c_long, C.c_double] + [_dp] * 14 + [_dp] * 2
         L.orc_rng_u64.restype = C.c_uint64
         L.orc_rng_u64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_rng_uniform.restype = C.c_double
@@ -345,6 +347,42 @@ def vadv_tracers(datag, etadot, dpk):
     out = np.zeros_like(datag)
     lib().orc_vadv_tracers_cols(nl, n, datag, etadot, dpk, out)
     return out
+
+
+def _press_cols(ak, bk, lnpsg, rk):
+    ak = np.ascontiguousarray(ak, np.float64)
+    bk = np.ascontiguousarray(bk, np.float64)
+    p = calc_pressdata(ak, bk, rk, lnpsg)
+    dbk = np.diff(bk)
+    ck = ak[1:] * bk[:-1] - ak[:-1] * bk[1:]
+    return ak, bk, dbk, ck, np.exp(np.ascontiguousarray(lnpsg, np.float64).ravel()), p
+
+
+def omega_cols(ak, bk, lnpsg, ug, vg, divg, dlnpsdx, dlnpsdy, rk=287.05 / 1004.6):
+    """getomega on columns: ug, vg, divg (nlev, ncol) bottom -> top; lnpsg,
+    dlnpsdx, dlnpsdy (ncol). Pressure fields come from calc_pressdata on the
+    same columns. Returns dlnpsdt (ncol), etadot (nlev+1, ncol) top -> bottom
+    and dlnpdtg = omega/p (nlev, ncol) bottom -> top."""
+    ak, bk, dbk, ck, ps, p = _press_cols(ak, bk, lnpsg, rk)
+    ug, vg, divg, dx, dy = (np.ascontiguousarray(x, np.float64) for x in (ug, vg, divg, dlnpsdx, dlnpsdy))
+    nl, n = ug.shape
+    dlnpsdt = np.zeros(n); etadot = np.zeros((nl + 1, n)); omg = np.zeros((nl, n))
+    lib().orc_omega_cols(nl, n, bk, dbk, ck, ps, p["dpk"], p["alfa"], p["rlnp"], ug, vg, divg, dx, dy,
+                         dlnpsdt, etadot, omg)
+    return dlnpsdt, etadot, omg
+
+
+def presgrad_cols(ak, bk, lnpsg, tvg, dtvdx, dtvdy, dphisdx, dphisdy, dlnpsdx, dlnpsdy, rd=287.05,
+                  rk=287.05 / 1004.6):
+    """getpresgrad on columns: tvg, dtvdx, dtvdy (nlev, ncol) bottom -> top;
+    lnpsg, dphisdx/y, dlnpsdx/y (ncol). Returns prsgx, prsgy (nlev, ncol)
+    bottom -> top."""
+    ak, bk, dbk, ck, ps, p = _press_cols(ak, bk, lnpsg, rk)
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (tvg, dtvdx, dtvdy, dphisdx, dphisdy, dlnpsdx, dlnpsdy)]
+    nl, n = arrs[0].shape
+    px = np.zeros((nl, n)); py = np.zeros((nl, n))
+    lib().orc_presgrad_cols(nl, n, rd, bk, dbk, ps, p["pk"], p["dpk"], p["alfa"], p["rlnp"], *arrs, px, py)
+    return px, py
 
 
 class Mlp:
