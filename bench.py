@@ -13,7 +13,9 @@ sums) simulated days per wall-day; the FP64 GPU path is timed beside it and
 the spectral error of the mixed path after --error-steps steps (default one
 model day) against the FP64 path is reported. roofline = the dominant kernel
 of the step, timed with HIP events on the context stream; cpu_baseline = the
-FP64 CPU oracle timed on one step of the same workload on the host cores.
+FP64 CPU oracle timed on the same workload on the host cores (median of 5
+steps after a warm-up step on every physical core of the process's CPU set,
+plus one single-thread step).
 
 Other workloads (each prints its own JSON line instead of the headline):
   --members N [--pattern]  ensemble of N members folded into every launch
@@ -115,12 +117,13 @@ def algorithmic_work(name, ctx, esz):
     return "hbm", None
 
 
-def roofline(tab, ctx, esz):
+def roofline(tab, ctx, esz, workload):
     per_step = {k: v[0] * v[1] for k, v in tab.items()}
     dom = max(per_step, key=per_step.get)
     kind, work = algorithmic_work(dom, ctx, esz)
     avg_s = tab[dom][0] * 1e-3
-    roof = {"kernel": dom, "bound": kind, "avg_launch_ms": tab[dom][0], "traffic": measured_traffic(dom)}
+    roof = {"kernel": dom, "bound": kind, "avg_launch_ms": tab[dom][0],
+            "traffic": measured_traffic(dom, workload)}
     if work is not None:
         if kind == "mfma":
             roof.update(achieved=work / avg_s / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s")
@@ -130,16 +133,15 @@ def roofline(tab, ctx, esz):
     return roof, per_step
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_T254L64.md")
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch of a step kernel from the rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the counter's
-    reading for a copy of known size): the row '| kernel | ... | bytes |' of
-    the summary's traffic table, or None when the file or row is absent."""
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of a step kernel, from the rocprofv3 counter summary
+    of this same workload (profiles/pmc_traffic_<workload>.md: FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by the counter's reading for a copy of
+    known size) -- the row '| kernel | ... | bytes |' of its traffic table; None
+    when no summary was recorded for this workload or it has no such row."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.md")
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
                 if len(cells) >= 6 and cells[0] == kernel:
@@ -147,6 +149,29 @@ def measured_traffic(kernel):
     except (OSError, ValueError):
         return None
     return None
+
+
+def physical_cores():
+    """physical cores among the CPUs this process may run on: (core, socket)
+    pairs from `lscpu -p` restricted to the affinity mask; the logical CPU
+    count if lscpu is unavailable"""
+    import subprocess
+    allowed = os.sched_getaffinity(0)
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=20).stdout
+        cores = set()
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(x) if x else 0 for x in line.split(",")[:3])
+            if cpu in allowed:
+                cores.add((core, sock))
+        if cores:
+            return len(cores)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return len(allowed)
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -166,23 +191,54 @@ def spectral_error(a, b):
             for k in ("vrt", "div", "tv", "lnps")}
 
 
-def cpu_baseline(cfgname):
+def cpu_baseline(cfgname, nsteps=5):
+    """The FP64 C oracle on the host cores: median step time of nsteps steps
+    after one warm-up step, with OpenMP on every physical core the process may
+    use, and one step on a single thread (after the same warm-up)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cases
     import oracle as O
     c = levels.CONFIGS[cfgname]
     ak, bk = levels.levels_for(cfgname)
+    cores = physical_cores()
+    prev = O.set_threads(cores)
     m = O.Model(c["nlon"], c["nlat"], c["ntrunc"], c["nlev"], ak, bk, c["dt"])
     vrt, div, tv, lnps = cases.rest_state(m.sht.degree, m.sht.order, c["nlev"], seed=0)
     m.set_state(vrt[0], div[0], tv[0], lnps[0])
+    times = []
+    for k in range(nsteps + 1):
+        t0 = time.perf_counter()
+        m.advance(1)
+        times.append(time.perf_counter() - t0)
+        print(f"cpu_baseline: oracle step {k} ({'warm-up' if k == 0 else 'timed'}) on {cores} threads: "
+              f"{times[-1]:.2f} s", file=sys.stderr, flush=True)
+    med = float(np.median(times[1:]))
+    O.set_threads(1)
     t0 = time.perf_counter()
     m.advance(1)
-    t = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": c["dt"] / t, "unit": "simulated days/wall-day", "cores": cores, "kind": "port",
-            "sample": f"1 step (advance: 3 getdyntend + HS physics + mass fixer) of the FP64 C "
-                      f"oracle at {cfgname}, OpenMP over fields, {t:.1f} s"}
+    t1 = time.perf_counter() - t0
+    print(f"cpu_baseline: oracle step on 1 thread: {t1:.2f} s", file=sys.stderr, flush=True)
+    O.set_threads(prev)
+    return {"value": c["dt"] / med, "unit": "simulated days/wall-day", "cores": cores, "kind": "port",
+            "sample": f"median of {nsteps} steps after 1 warm-up step (advance: 3 getdyntend + HS physics + "
+                      f"mass fixer) of the FP64 C oracle at {cfgname}, OpenMP over fields on the {cores} "
+                      f"physical cores of the process's CPU set (lscpu): {med:.2f} s per step",
+            "ms_per_step": med * 1e3, "step_times_s": [round(t, 3) for t in times],
+            "one_thread": {"value": c["dt"] / t1, "ms_per_step": t1 * 1e3, "cores": 1,
+                           "sample": "1 step after the warm-up, one OpenMP thread"}}
+
+
+def zonal_mean_drift(a, b):
+    """largest difference of the zonal-mean temperature (K) and zonal wind
+    (m/s) between two contexts' current states, over levels and latitudes"""
+    out = {}
+    for ctx in (a, b):
+        ctx.getdyntend(1, inverse_only=True)
+    for name, key in (("virtempg", "T"), ("ug", "u")):
+        ga, gb = a.grid(name)[0], b.grid(name)[0]
+        out[key] = float(np.abs(ga.mean(-1) - gb.mean(-1)).max())
+    return out
 
 
 def base_line(args, value, unit, metric, config, dtype="fp32", **extra):
@@ -211,7 +267,7 @@ def run_headline(args):
         out["speedup_mixed_vs_fp64"] = t64 / t32
         del c64
     tab = kernel_table(c32, args.prof_steps)
-    roof, per_step = roofline(tab, c32, 4)
+    roof, per_step = roofline(tab, c32, 4, args.config)
     def mfma_frac(name):
         leg = tab.get(name)
         _, f = algorithmic_work(name, c32, 4)
@@ -220,22 +276,23 @@ def run_headline(args):
         return {"kernel": name, "avg_launch_ms": leg[0], "achieved_tflops": f / (leg[0] * 1e-3) / 1e12,
                 "peak_tflops": PEAK["fp32_mfma"] / 1e12, "frac": f / (leg[0] * 1e-3) / PEAK["fp32_mfma"]}
     legendre, legendre_anal = mfma_frac("leg_synth"), mfma_frac("leg_anal")
-    err = None
+    err = drift = None
     if args.error_steps > 0 and not args.no_fp64:
         a, b = make(args.config, gfsdc.FP32), make(args.config, gfsdc.FP64)
         a.advance(args.error_steps, use_graph=True)
         b.advance(args.error_steps, use_graph=True)
         err = spectral_error(a, b)
+        drift = zonal_mean_drift(a, b)
         del a, b
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.config)
     line = base_line(
         args, dt / t32, "simulated days/wall-day",
-        "simulated days/wall-day at T254L64, mixed vs FP64 GPU, at stated error",
+        f"simulated days/wall-day at {args.config}, mixed vs FP64 GPU, at stated error",
         {"workload": f"{args.config} dry Held-Suarez dycore step (IMEX RK3, dt={dt:g} s)",
          "grid": f"{cfg['nlon']}x{cfg['nlat']}", "ntrunc": cfg["ntrunc"], "nlev": cfg["nlev"],
          "members": 1, "hip_graph": True},
         ms_per_step=t32 * 1e3, **out, error_vs_fp64_after_steps=args.error_steps if err else 0,
-        spectral_rel_error_vs_fp64=err, roofline=roof, legendre_gemm=legendre,
+        spectral_rel_error_vs_fp64=err, zonal_mean_drift_vs_fp64=drift, roofline=roof, legendre_gemm=legendre,
         legendre_gemm_analysis=legendre_anal,
         kernels_ms_per_step={k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
         cpu_baseline=cpu)
@@ -255,7 +312,7 @@ def run_ensemble(args):
     one.autotune()
     t1 = time_steps(one, args.steps, args.warmup)
     tab = kernel_table(ctx, 1)
-    roof, per_step = roofline(tab, ctx, 4)
+    roof, per_step = roofline(tab, ctx, 4, f"{args.config}_m{nm}" + ("_pattern" if args.pattern else ""))
     return base_line(
         args, dt * nm / t, "member-days/wall-day",
         f"ensemble throughput, {nm} members of {args.config}" + (" with AR(1) temperature pattern" if args.pattern else ""),
@@ -332,7 +389,7 @@ def run_moist(args):
         ctx.mlp_enable(True, judge_stride=50, threshold=1e30)
     t = time_steps(ctx, args.steps, args.warmup)
     tab = kernel_table(ctx, 1)
-    roof, per_step = roofline(tab, ctx, 4)
+    roof, per_step = roofline(tab, ctx, 4, f"{args.config}_moist_{args.moist}")
     if args.moist == "mlp":
         ng = ctx.nlat * ctx.nlon
         nin, h, nout = 4 * ctx.nlev + 3, 512, 4 * ctx.nlev

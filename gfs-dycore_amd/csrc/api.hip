@@ -51,6 +51,8 @@ void gfsdc_cfg_defaults(gfsdc_cfg *c) {
     c->tpat_seed = 1;
 }
 
+size_t gfsdc_cfg_size(void) { return sizeof(gfsdc_cfg); }
+
 int gfsdc_create(const gfsdc_cfg *cfg, gfsdc_ctx **out) {
     if (!cfg || !out) {
         set_error("null argument");
@@ -245,9 +247,43 @@ static gfsdc_ctx *g_shtns_ctx = nullptr;
 static int g_lmax = -1, g_mmax = -1, g_mres = 1, g_norm = 0, g_nthreads = 1;
 static int g_nlat = 0, g_nphi = 0;
 
+// Errors: the SHTns symbols return nothing, so an error cannot travel back
+// through them. In the default mode (0) the layer prints the error and ends
+// the process with exit status 1 -- what SHTns' own error path and the
+// reference wrapper's `stop` do. In mode 1 it records the error and returns;
+// gfsdc_shtns_status() then reports (and clears) it, and the outputs of the
+// failed call are unchanged.
+static int g_shtns_mode = 0;
+static int g_shtns_status = 0;
+
 static void shtns_fail(const char *what) {
-    fprintf(stderr, "gfsdc shtns layer: %s: %s\n", what, gfsdc::g_last_error.c_str());
-    abort();
+    std::string msg = std::string(what) + ": " + gfsdc::g_last_error;
+    g_shtns_status = GFSDC_ESTATE;
+    if (g_shtns_mode == 0) {
+        fprintf(stderr, "gfsdc shtns layer: %s\n", msg.c_str());
+        fflush(stderr);
+        exit(1);
+    }
+    gfsdc::g_last_error = msg;
+}
+
+static bool shtns_ready(const char *what) {
+    if (g_shtns_ctx) return true;
+    gfsdc::g_last_error = "no SHTns configuration (shtns_precompute_ not called)";
+    shtns_fail(what);
+    return false;
+}
+
+void gfsdc_shtns_error_mode(int mode) {
+    std::lock_guard<std::mutex> lk(g_shtns_mu);
+    g_shtns_mode = mode ? 1 : 0;
+}
+
+int gfsdc_shtns_status(void) {
+    std::lock_guard<std::mutex> lk(g_shtns_mu);
+    int s = g_shtns_status;
+    g_shtns_status = 0;
+    return s;
 }
 
 void shtns_use_threads_(int *nthreads) { g_nthreads = nthreads ? *nthreads : 1; }
@@ -259,15 +295,26 @@ void shtns_set_size_(int *lmax, int *mmax, int *mres, int *norm) {
     g_mres = *mres;
     g_norm = *norm;
     // SHT_ORTHONORMAL (0) + SHT_NO_CS_PHASE (1024), triangular truncation, mres 1
-    if (g_mmax != g_lmax || g_mres != 1 || (g_norm & 0xFF) != 0 || !(g_norm & 1024))
-        shtns_fail("only lmax == mmax, mres 1, orthonormal without CS phase is supported");
+    if (g_mmax != g_lmax || g_mres != 1 || (g_norm & 0xFF) != 0 || !(g_norm & 1024)) {
+        gfsdc::g_last_error = "only lmax == mmax, mres 1, orthonormal without CS phase is supported";
+        shtns_fail("shtns_set_size");
+    }
 }
 
 void shtns_precompute_(int *type, int *layout, double *eps, int *nlat, int *nphi) {
     (void)eps;
     std::lock_guard<std::mutex> lk(g_shtns_mu);
     // SHT_GAUSS (0) or SHT_GAUSS_FLY (6); SHT_PHI_CONTIGUOUS (512)
-    if ((*type != 0 && *type != 6) || (*layout & 512) == 0) shtns_fail("only Gauss grids, phi-contiguous layout");
+    if ((*type != 0 && *type != 6) || (*layout & 512) == 0) {
+        gfsdc::g_last_error = "only Gauss grids (SHT_GAUSS, SHT_GAUSS_FLY) in the phi-contiguous layout";
+        shtns_fail("shtns_precompute");
+        return;
+    }
+    if (g_lmax < 0) {
+        gfsdc::g_last_error = "shtns_set_size_ must come first";
+        shtns_fail("shtns_precompute");
+        return;
+    }
     if (g_shtns_ctx) gfsdc_destroy(g_shtns_ctx);
     g_shtns_ctx = nullptr;
     gfsdc_cfg c;
@@ -277,7 +324,11 @@ void shtns_precompute_(int *type, int *layout, double *eps, int *nlat, int *nphi
     c.ntrunc = g_lmax;
     c.nlev = 0;
     c.precision = GFSDC_FP64;
-    if (gfsdc_create(&c, &g_shtns_ctx) != 0) shtns_fail("precompute");
+    if (gfsdc_create(&c, &g_shtns_ctx) != 0) {
+        g_shtns_ctx = nullptr;
+        shtns_fail("shtns_precompute");
+        return;
+    }
     g_nlat = *nlat;
     g_nphi = *nphi;
 }
@@ -289,38 +340,42 @@ void shtns_calc_nlm_(int *nlm, int *lmax, int *mmax, int *mres) {
 
 void shtns_cos_array_(double *costh) {
     std::lock_guard<std::mutex> lk(g_shtns_mu);
-    if (!g_shtns_ctx) shtns_fail("cos_array before precompute");
+    if (!shtns_ready("shtns_cos_array")) return;
     const std::vector<double> &mu = g_shtns_ctx->ed->plan.mu;
     for (int j = 0; j < g_nlat; j++) costh[j] = mu[j];
 }
 
 void shtns_gauss_wts_(double *wts) {
     std::lock_guard<std::mutex> lk(g_shtns_mu);
-    if (!g_shtns_ctx) shtns_fail("gauss_wts before precompute");
+    if (!shtns_ready("shtns_gauss_wts")) return;
     const std::vector<double> &w = g_shtns_ctx->ed->plan.w;
     for (int j = 0; j < g_nlat / 2; j++) wts[j] = w[j];
 }
 
 void shtns_spat_to_sh_(double *vr, double *qlm) {
     std::lock_guard<std::mutex> lk(g_shtns_mu);
-    if (gfsdc_grdtospec(g_shtns_ctx, vr, qlm, 1, GFSDC_DT_F64, GFSDC_HOST)) shtns_fail("spat_to_sh");
+    if (!shtns_ready("shtns_spat_to_sh")) return;
+    if (gfsdc_grdtospec(g_shtns_ctx, vr, qlm, 1, GFSDC_DT_F64, GFSDC_HOST)) shtns_fail("shtns_spat_to_sh");
 }
 
 void shtns_sh_to_spat_(double *qlm, double *vr) {
     std::lock_guard<std::mutex> lk(g_shtns_mu);
-    if (gfsdc_spectogrd(g_shtns_ctx, qlm, vr, 1, GFSDC_DT_F64, GFSDC_HOST)) shtns_fail("sh_to_spat");
+    if (!shtns_ready("shtns_sh_to_spat")) return;
+    if (gfsdc_spectogrd(g_shtns_ctx, qlm, vr, 1, GFSDC_DT_F64, GFSDC_HOST)) shtns_fail("shtns_sh_to_spat");
 }
 
 void shtns_sphtor_to_spat_(double *slm, double *tlm, double *vt, double *vp) {
     std::lock_guard<std::mutex> lk(g_shtns_mu);
+    if (!shtns_ready("shtns_sphtor_to_spat")) return;
     if (g_shtns_ctx->ed->sphtor_to_spat(slm, tlm, vt, vp, 1, 1.0, 0, 1.0, 0, GFSDC_DT_F64, GFSDC_HOST))
-        shtns_fail("sphtor_to_spat");
+        shtns_fail("shtns_sphtor_to_spat");
 }
 
 void shtns_spat_to_sphtor_(double *vt, double *vp, double *slm, double *tlm) {
     std::lock_guard<std::mutex> lk(g_shtns_mu);
+    if (!shtns_ready("shtns_spat_to_sphtor")) return;
     if (g_shtns_ctx->ed->spat_to_sphtor(vt, vp, slm, tlm, 1, 1.0, 1, GFSDC_DT_F64, GFSDC_HOST))
-        shtns_fail("spat_to_sphtor");
+        shtns_fail("shtns_spat_to_sphtor");
 }
 
 void shtns_reset_(void) {

@@ -7,7 +7,7 @@ module gfsdc_dycore
   use iso_c_binding
   implicit none
   private
-  public :: gfsdc_cfg, dyc_defaults, dyc_create, dyc_destroy, dyc_set_state, dyc_get_state, &
+  public :: gfsdc_cfg, dyc_defaults, dyc_cfg_size, dyc_check_abi, dyc_create, dyc_destroy, dyc_set_state, dyc_get_state, &
             dyc_getdyntend, dyc_advance, dyc_step_diag, dyc_init_rest_state, dyc_sync, &
             GFSDC_FP64, GFSDC_FP32, GFSDC_HOST, DT_F64, DT_F32, PHYS_NONE, PHYS_HS
 
@@ -28,6 +28,10 @@ module gfsdc_dycore
   end type
 
   interface
+    ! sizeof(gfsdc_cfg) in the library: the layout check of dyc_check_abi
+    integer(c_size_t) function dyc_cfg_size() bind(c, name="gfsdc_cfg_size")
+      import :: c_size_t
+    end function
     subroutine dyc_defaults(cfg) bind(c, name="gfsdc_cfg_defaults")
       import :: gfsdc_cfg
       type(gfsdc_cfg), intent(inout) :: cfg
@@ -121,4 +125,17 @@ module gfsdc_dycore
       type(c_ptr), value :: h, out3
     end function
   end interface
+
+contains
+
+  ! stop unless this bind(c) gfsdc_cfg has the library's size: a type that
+  ! misses fields would let gfsdc_cfg_defaults write past it
+  subroutine dyc_check_abi()
+    type(gfsdc_cfg) :: probe
+    if (c_sizeof(probe) /= dyc_cfg_size()) then
+      print *, 'gfsdc: gfsdc_cfg is', c_sizeof(probe), 'bytes here and', dyc_cfg_size(), &
+               'in libgfsdc (include/gfsdc.h); rebuild the Fortran layer against the header'
+      stop 1
+    end if
+  end subroutine
 end module gfsdc_dycore

@@ -1,12 +1,14 @@
 ! Held-Suarez driver with the Fortran host in charge: sets up the T42L20
 ! (or T254L64) configuration, runs run.f90's time loop on the GPU through
 ! libgfsdc and prints run.f90's per-step diagnostic line (run.f90:159-163).
-! Also exercises module shtns (the transform wrapper) with a round trip.
+! Also exercises module shtns (the transform wrapper) with a round trip and a
+! solid-body rotation through getuv.
 ! Usage: gfsdc_hs_driver [nsteps] [T42L20|T254L64] [fp64|fp32]
+!        gfsdc_hs_driver abi   (layout check of gfsdc_cfg only; no GPU call)
 program hs_driver
   use iso_c_binding
   use kinds, only: r_kind
-  use shtns, only: shtns_init, spectogrd, grdtospec, nlm
+  use shtns, only: shtns_init, spectogrd, grdtospec, getuv, nlm, lats
   use gfsdc_dycore
   implicit none
   type(gfsdc_cfg) :: cfg
@@ -14,15 +16,20 @@ program hs_driver
   integer :: nsteps, nt, k, nlev
   character(len=16) :: arg, cname, prec
   real(c_double), allocatable, target :: ak(:), bk(:), diag(:)
-  complex(r_kind), allocatable :: spec(:), spec2(:)
-  real(r_kind), allocatable :: grid(:,:)
-  real(c_double) :: t, eta, b, err
+  complex(r_kind), allocatable :: spec(:), spec2(:), zero(:)
+  real(r_kind), allocatable :: grid(:,:), ug(:,:), vg(:,:)
+  real(c_double) :: t, eta, b, err, u0, pi
   real :: tc0, tc1
   nsteps = 72
   cname = 'T42L20'
   prec = 'fp64'
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
+    if (arg == 'abi') then
+      call dyc_check_abi()
+      print '(a,i0,a,i0)', 'gfsdc_cfg bytes: fortran ', c_sizeof(cfg), ' library ', dyc_cfg_size()
+      stop
+    end if
     read (arg, *) nsteps
   end if
   if (command_argument_count() >= 2) call get_command_argument(2, cname)
@@ -63,6 +70,17 @@ program hs_driver
   call grdtospec(grid, spec2)
   err = maxval(abs(spec2 - spec))
   print '(a,es10.3)', 'shtns round trip max error = ', err
+  ! solid-body rotation: vorticity 2 u0 sin(lat) / a, i.e. q_10 = 2 u0 / (a sqrt(3/4pi)),
+  ! gives u = u0 cos(lat), v = 0
+  pi = 4.d0 * atan(1.d0)
+  u0 = 20.d0
+  allocate(zero(nlm), ug(cfg%nlon, cfg%nlat), vg(cfg%nlon, cfg%nlat))
+  zero = 0
+  spec = 0
+  spec(2) = cmplx(2.d0 * u0 / (cfg%rerth * sqrt(3.d0 / (4.d0 * pi))), 0.d0, r_kind)
+  call getuv(spec, zero, ug, vg, real(cfg%rerth, r_kind))
+  err = max(maxval(abs(ug - u0 * cos(lats))), maxval(abs(vg))) / u0
+  print '(a,es10.3)', 'shtns getuv solid-body max error = ', err
   t = 0.d0
   do nt = 1, nsteps
     call cpu_time(tc0)

@@ -20,6 +20,8 @@
 module shtns
   use iso_c_binding
   use kinds, only: r_kind, r_double
+  ! the one bind(c) definition of gfsdc_cfg, checked against the library's size
+  use gfsdc_dycore, only: gfsdc_cfg, dyc_defaults, dyc_create, dyc_destroy, dyc_check_abi
   implicit none
   private
   public :: shtns_init, grdtospec, spectogrd, getuv, getvrtdivspec, getgrad, shtns_destroy
@@ -36,31 +38,7 @@ module shtns
   integer(c_int), parameter :: GFSDC_HOST = 0, DT_F64 = 0, DT_F32 = 1
   integer(c_int), parameter :: GFSDC_FP64 = 0, GFSDC_FP32 = 1
 
-  ! layout of gfsdc_cfg (include/gfsdc.h)
-  type, bind(c) :: gfsdc_cfg
-    integer(c_int) :: nlon, nlat, ntrunc, nlev, ntrac, nmem, precision
-    real(c_double) :: dt
-    type(c_ptr) :: ak, bk
-    integer(c_int) :: explicit_, physics, massfix, ndiss
-    real(c_double) :: efold, hdif_fac, hdif_fac2, fshk, taustratdamp
-    real(c_double) :: rerth, grav, omega, rd, rv, cp, cvap
-    integer(c_int) :: device
-  end type
-
   interface
-    subroutine c_cfg_defaults(cfg) bind(c, name="gfsdc_cfg_defaults")
-      import :: gfsdc_cfg
-      type(gfsdc_cfg), intent(inout) :: cfg
-    end subroutine
-    integer(c_int) function c_create(cfg, out) bind(c, name="gfsdc_create")
-      import :: gfsdc_cfg, c_ptr, c_int
-      type(gfsdc_cfg), intent(in) :: cfg
-      type(c_ptr), intent(out) :: out
-    end function
-    integer(c_int) function c_destroy(h) bind(c, name="gfsdc_destroy")
-      import :: c_ptr, c_int
-      type(c_ptr), value :: h
-    end function
     integer(c_int) function c_griddata(h, lats, gauwts, areawts, degree, order, lap, invlap) &
         bind(c, name="gfsdc_griddata")
       import :: c_ptr, c_int
@@ -162,7 +140,8 @@ contains
     integer(c_int), allocatable, target :: dg(:), od(:)
     real(r_double) :: pi
     call shtns_destroy()
-    call c_cfg_defaults(cfg)
+    call dyc_check_abi()
+    call dyc_defaults(cfg)
     cfg%nlon = nlon
     cfg%nlat = nlat
     cfg%ntrunc = ntrunc
@@ -170,7 +149,7 @@ contains
     cfg%precision = GFSDC_FP64
     call get_environment_variable("GFSDC_PRECISION", prec, status=stat)
     if (stat == 0 .and. (prec == "fp32" .or. prec == "FP32")) cfg%precision = GFSDC_FP32
-    call check(c_create(cfg, ctx), "shtns_init")
+    call check(dyc_create(cfg, ctx), "shtns_init")
     nlm = c_nlm(ctx)
     if (nlm /= (ntrunc + 1) * (ntrunc + 2) / 2) then
       print *, 'error: nlm not what expected', nlm, ntrunc
@@ -201,7 +180,7 @@ contains
 
   subroutine shtns_destroy()
     integer(c_int) :: rc
-    if (c_associated(ctx)) rc = c_destroy(ctx)
+    if (c_associated(ctx)) rc = dyc_destroy(ctx)
     ctx = c_null_ptr
     current_nlon = -1
     current_nlat = -1

@@ -62,6 +62,14 @@ void shtns_sphtor_to_spat_(double *slm, double *tlm, double *vt, double *vp);
 void shtns_spat_to_sphtor_(double *vt, double *vp, double *slm, double *tlm);
 /* replaces SHTns shtns_reset (shtns.f90:160) */
 void shtns_reset_(void);
+/* Errors of the symbols above (no counterpart in SHTns, whose symbols return
+ * nothing): in mode 0 (default) an error is printed and the process exits
+ * with status 1, as SHTns' own error path and the wrapper's `stop` do; in
+ * mode 1 it is recorded and the call returns with its outputs unchanged, and
+ * gfsdc_shtns_status() returns that error's code (0 if none) and clears it.
+ * gfsdc_last_error() holds its text. For callers that must not lose the process. */
+void gfsdc_shtns_error_mode(int mode);
+int gfsdc_shtns_status(void);
 
 /* ------------------------------------------------------------------ */
 /* Tier 2: gfsdc handle API                                            */
@@ -116,6 +124,10 @@ typedef struct gfsdc_ctx gfsdc_ctx;
 
 /* fill defaults of params.f90 / physcons.f90 (sizes left 0) */
 void gfsdc_cfg_defaults(gfsdc_cfg *cfg);
+/* sizeof(gfsdc_cfg) as this library was built (no counterpart in the
+ * reference): a binding in another language (the Fortran bind(c) type, a
+ * ctypes Structure) checks its own layout against it before calling */
+size_t gfsdc_cfg_size(void);
 /* replaces init_dyn's shtns_init (dyn_init.f90:43), setdampspec (:86) and
  * init_semimpdata (:88): builds Gauss grid, Legendre tables, damping and
  * semi-implicit operators on the device */

@@ -39,6 +39,7 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -119,6 +120,10 @@ void orc_cfg_defaults(orc_cfg *c) {
     c->simple_test = 0;
     c->turbflux = 1;
 }
+
+/* OpenMP threads of the oracle's parallel loops (CPU-baseline timing) */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 /* counter-based RNG: splitmix64 finaliser over a mixed (seed, stream, index) key */
 uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index) {

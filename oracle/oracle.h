@@ -179,8 +179,17 @@ void orc_pattern_advance(orc_model *m);
 long orc_mlp_nparams(int nin, int nout, int hidden, int nlayers);
 double orc_mlp_loss_grad(int nin, int nout, int hidden, int nlayers, const double *params, int n,
                          const double *x, const double *t, double *grad, double *y);
+/* the same with the bfloat16 storage rounding of a matrix-core network:
+ * inputs, weights, biases, hidden activations and back-propagated gradients
+ * rounded to bf16, sums in FP64, output layer unrounded */
+double orc_mlp_loss_grad_bf16(int nin, int nout, int hidden, int nlayers, const double *params, int n,
+                              const double *x, const double *t, double *grad, double *y);
 void orc_mlp_adamw(int nin, int nout, int hidden, int nlayers, double *params, double *m, double *v,
                    const double *grad, int step, double lr, double wd);
+
+/* OpenMP thread count of the oracle's loops (for timing the CPU baseline) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 
 /* counter-based RNG shared with the product: splitmix64 of (seed, stream, index) */
 uint64_t orc_rng_u64(uint64_t seed, uint64_t stream, uint64_t index);

@@ -93,12 +93,16 @@ def lib():
         L.orc_mlp_nparams.argtypes = [C.c_int] * 4
         L.orc_mlp_loss_grad.restype = C.c_double
         L.orc_mlp_loss_grad.argtypes = [C.c_int] * 4 + [_dp, C.c_int, _dp, _dp, vp, vp]
+        L.orc_mlp_loss_grad_bf16.restype = C.c_double
+        L.orc_mlp_loss_grad_bf16.argtypes = [C.c_int] * 4 + [_dp, C.c_int, _dp, _dp, vp, vp]
         L.orc_mlp_adamw.argtypes = [C.c_int] * 4 + [_dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double]
         L.orc_pattern_advance.argtypes = [vp]
         L.orc_vadv_tracers_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
         L.orc_vadv_cols.argtypes = [C.c_int, C.c_long, _dp, _dp, _dp, _dp]
         L.orc_omega_cols.argtypes = [C.c_int, C.c_long] + [_dp] * 12 + [_dp] * 3
         L.orc_presgrad_cols.argtypes = [C.c_int, C.c_long, C.c_double] + [_dp] * 14 + [_dp] * 2
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         L.orc_rng_u64.restype = C.c_uint64
         L.orc_rng_u64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_rng_uniform.restype = C.c_double
@@ -393,13 +397,17 @@ class Mlp:
         self.dims = (nin, nout, hidden, nlayers)
         self.nparams = lib().orc_mlp_nparams(*self.dims)
 
-    def loss_grad(self, params, x, t, grad=True):
+    def loss_grad(self, params, x, t, grad=True, bf16=False):
+        """loss, parameter gradient and outputs; bf16=True rounds inputs,
+        weights, hidden activations and back-propagated gradients to bfloat16
+        as a bf16 matrix-core network stores them (sums in FP64)"""
         params = np.ascontiguousarray(params, np.float64)
         x = np.ascontiguousarray(x, np.float64)
         t = np.ascontiguousarray(t, np.float64)
         g = np.zeros(self.nparams) if grad else None
         y = np.zeros((x.shape[0], self.dims[1]))
-        loss = lib().orc_mlp_loss_grad(*self.dims, params, x.shape[0], x, t, _ptr(g), _ptr(y))
+        f = lib().orc_mlp_loss_grad_bf16 if bf16 else lib().orc_mlp_loss_grad
+        loss = f(*self.dims, params, x.shape[0], x, t, _ptr(g), _ptr(y))
         return loss, g, y
 
     def adamw(self, params, m, v, grad, step, lr, wd):
@@ -421,6 +429,13 @@ def simple_column(dtime, lat, t, q, u, v, pmid, pint, pdel, rpdel, ps, sst_mode=
 
 def simple_sst(sst_mode, lat):
     return lib().orc_simple_sst(int(sst_mode), float(lat))
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops; returns the previous count"""
+    prev = lib().orc_get_threads()
+    lib().orc_set_threads(int(n))
+    return prev
 
 
 def rng_uniform(seed, stream, index):

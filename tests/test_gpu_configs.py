@@ -1,0 +1,182 @@
+"""The large configurations on the GPU against the FP64 oracle.
+
+* T126L64 and T254L64 (hybrid levels, Held-Suarez): FP64 getdyntend and one
+  advance step equal the oracle to 1e-9; the FP32 tendencies to 2e-4.
+* C3: 32 members of T126L64 with the AR(1) temperature pattern folded into
+  every launch. Each member equals its own single-member run (1e-12, FP64),
+  and each member's pattern equals the oracle's for its seed.
+* C4: T254L64 moist run with the 259 -> 3 x 512 -> 256 column network. Its
+  forward and gradient equal the oracle network evaluated with the same bf16
+  storage rounding (a row sample); with a 2 % judge (every 50th column) and a
+  zero threshold every step falls back, and the step is then bit for bit the
+  plain simple-physics step.
+"""
+import numpy as np
+import pytest
+
+import cases
+import levels
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PAT = dict(tpat_amp=0.5, tpat_tau=6 * 3600.0, tpat_lscale=5.0e5)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def rel2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def pair(name, precision, **kw):
+    import gfsdc
+    c = levels.CONFIGS[name]
+    ak, bk = levels.levels_for(name)
+    o = O.Model(c["nlon"], c["nlat"], c["ntrunc"], c["nlev"], ak, bk, c["dt"], **kw)
+    g = gfsdc.Context(c["nlon"], c["nlat"], c["ntrunc"], precision, nlev=c["nlev"], ak=ak, bk=bk, dt=c["dt"],
+                      **kw)
+    return o, g
+
+
+@pytest.mark.parametrize("name", ["T126L64", "T254L64"])
+def test_large_grid_fp64_getdyntend_and_advance(name):
+    import gfsdc
+    print(f"{name}: FP64 getdyntend + advance vs oracle", flush=True)
+    o, g = pair(name, gfsdc.FP64)
+    nl = levels.CONFIGS[name]["nlev"]
+    vrt, div, tv, lnps = cases.perturbed_state(o.sht.degree, o.sht.order, nl, seed=21, amp=0.3, nmax=60)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.set_state(vrt, div, tv, lnps)
+    to, tg = o.getdyntend(1), g.getdyntend(1)
+    for k in ("dvrt", "ddiv", "dtv", "dlnps"):
+        assert rel(tg[k][0], to[k]) < 1e-9, (k, rel(tg[k][0], to[k]))
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.set_state(vrt, div, tv, lnps)
+    o.pdryini = 0.0
+    g.pdryini = 0.0
+    o.advance(1)
+    g.advance(1)
+    so, sg = o.get_state(), g.get_state()
+    for k in ("vrt", "div", "tv", "lnps"):
+        assert rel(sg[k][0], so[k]) < 1e-9, (k, rel(sg[k][0], so[k]))
+
+
+def test_t254l64_fp32_tendencies():
+    import gfsdc
+    o, g = pair("T254L64", gfsdc.FP32)
+    vrt, div, tv, lnps = cases.perturbed_state(o.sht.degree, o.sht.order, 64, seed=22, amp=0.3, nmax=60)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.set_state(vrt, div, tv, lnps)
+    to, tg = o.getdyntend(1), g.getdyntend(1)
+    for k in ("dvrt", "ddiv", "dtv", "dlnps"):
+        assert rel(tg[k][0], to[k]) < 2e-4, (k, rel(tg[k][0], to[k]))
+
+
+def test_c3_ensemble_32_members_with_pattern():
+    import gfsdc
+    c = levels.CONFIGS["T126L64"]
+    ak, bk = levels.levels_for("T126L64")
+    nm, seed = 32, 100
+    mk = lambda nmem, s: gfsdc.Context(c["nlon"], c["nlat"], c["ntrunc"], gfsdc.FP64, nlev=64, ak=ak, bk=bk,
+                                       dt=c["dt"], nmem=nmem, tpat_seed=s, **PAT)
+    ens = mk(nm, seed)
+    ens.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=seed)
+    print("C3: 32-member T126L64 ensemble, 2 steps", flush=True)
+    ens.advance(2, use_graph=True)
+    se, pe = ens.get_state(), ens.pattern()
+    del ens
+    for mem in range(nm):
+        one = mk(1, seed + mem)
+        one.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=seed + mem)
+        one.advance(2, use_graph=True)
+        s1 = one.get_state()
+        for k in ("vrt", "div", "tv", "lnps"):
+            assert rel(se[k][mem], s1[k][0]) < 1e-12, (mem, k, rel(se[k][mem], s1[k][0]))
+        assert rel2(pe[mem], one.pattern()[0]) < 1e-13, mem
+        del one
+    # the oracle's pattern for a few members' seeds after the same two advances
+    for mem in (0, 13, 31):
+        o = O.Model(c["nlon"], c["nlat"], c["ntrunc"], 64, ak, bk, c["dt"], tpat_seed=seed + mem, **PAT)
+        o.pattern_advance()
+        o.pattern_advance()
+        assert rel2(pe[mem], o.pattern()["spec"]) < 1e-12, mem
+    assert rel2(pe[0], pe[1]) > 0.5
+
+
+def _moist_t254(g):
+    ak, bk = levels.levels_for("T254L64")
+    g.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=0)
+    st = g.get_state()
+    q = g.grdtospec(levels.moist_grid(g.lats, g.nlon, ak, bk))
+    g.set_state(st["vrt"], st["div"], st["tv"], st["lnps"], trac=q[None, None])
+
+
+def moist_ctx():
+    import gfsdc
+    c = levels.CONFIGS["T254L64"]
+    ak, bk = levels.levels_for("T254L64")
+    g = gfsdc.Context(c["nlon"], c["nlat"], c["ntrunc"], gfsdc.FP32, nlev=64, ak=ak, bk=bk, dt=c["dt"], ntrac=1,
+                      physics=gfsdc.PHYS_SIMPLE, simple_test=2, turbflux=1)
+    _moist_t254(g)
+    return g
+
+
+def test_c4_network_forward_and_gradient_full_size():
+    g = moist_ctx()
+    g.mlp_init(512, 3, 1024, 1 << 14, seed=9)
+    nin, nout = g.mlp_nin, g.mlp_nout
+    assert (nin, nout) == (259, 256)
+    net = O.Mlp(nin, nout, 512, 3)
+    p = g.mlp_params().astype(np.float64)
+    rng = np.random.default_rng(3)
+    x = rng.normal(0, 1, (1500, nin)).astype(np.float32)     # more rows than one batch
+    y = g.mlp_forward(x)
+    rows = rng.choice(1500, 192, replace=False)
+    _, _, yo = net.loss_grad(p, x[rows], np.zeros((rows.size, nout)), grad=False, bf16=True)
+    e = rel2(y[rows], yo)
+    print(f"C4 forward rel L2 vs bf16 oracle: {e:.2e}")
+    assert e < 4e-3, e
+    xb, tb = x[:256], rng.normal(0, 1, (256, nout)).astype(np.float32)
+    grad, loss = g.mlp_grad(xb, tb)
+    lo, go, _ = net.loss_grad(p, xb, tb, bf16=True)
+    assert abs(loss - lo) < 1e-3 * lo, (loss, lo)
+    o = 0
+    for K, N in [(nin, 512), (512, 512), (512, 512), (512, nout)]:
+        for sl in (slice(o, o + K * N), slice(o + K * N, o + K * N + N)):
+            e = rel2(grad[sl], go[sl])
+            print(f"C4 gradient block {K}x{N} rel L2 vs bf16 oracle: {e:.2e}")
+            assert e < 6e-3, (K, N, e)
+        o += K * N + N
+
+
+def test_c4_judge_and_fallback_full_size():
+    """2 % of the columns judged (every 50th); threshold 0: every step falls
+    back and equals the plain simple-physics step bit for bit; a large
+    threshold keeps the network's step, which then differs and stays finite"""
+    a, b = moist_ctx(), moist_ctx()
+    a.mlp_init(512, 3, 768 * 384, 1 << 16, seed=4)
+    a.mlp_collect(1, 50)
+    a.mlp_normalize()
+    st = a.get_state()
+    for c in (a, b):
+        c.set_state(st["vrt"], st["div"], st["tv"], st["lnps"], trac=st["trac"])
+        c.pdryini = 0.0
+    a.mlp_enable(True, judge_stride=50, threshold=0.0)
+    a.advance(1)
+    b.advance(1)
+    sa, sb = a.get_state(), b.get_state()
+    js = a.mlp_judge_stats()
+    assert js["fallbacks"] == 1 and js["judged"] == 1
+    for k in ("vrt", "div", "tv", "lnps", "trac"):
+        assert np.array_equal(sa[k], sb[k]), k
+    a.mlp_enable(True, judge_stride=50, threshold=1e30)
+    a.advance(1)
+    b.advance(1)
+    sa, sb = a.get_state(), b.get_state()
+    js = a.mlp_judge_stats()
+    assert js["fallbacks"] == 0 and js["judged"] == 1 and np.isfinite(js["error"])
+    assert all(np.isfinite(sa[k]).all() for k in ("vrt", "div", "tv", "lnps", "trac"))
+    assert not np.array_equal(sa["tv"], sb["tv"])

@@ -3,11 +3,13 @@ AdamW kernels against the FP64 oracle network, training on columns the plain
 simple physics generates, the gain over the method without the network, and
 the judge / fallback inside advance().
 
-Tolerances: the GPU network keeps weights, activations and gradients in bf16
-(8-bit mantissa) with FP32 accumulation, so outputs and gradients are compared
-against the oracle run on the same bf16-rounded weights and inputs at 3e-2
-relative L2 (forward) and 6e-2 (gradients); AdamW runs in FP32 and is
-compared with its closed form at 1e-5.
+Tolerances: the GPU network stores inputs, weights, hidden activations and
+back-propagated gradients in bf16 (8-bit mantissa) and sums in FP32. The
+oracle network is evaluated with the same bf16 storage rounding (sums in
+FP64), so what remains is FP32 summation and the rare activation that the
+GPU's fast GELU rounds to the neighbouring bf16 value: 4e-3 relative L2
+(forward) and 6e-3 (gradients). AdamW runs in FP32 and is compared with its
+closed form at 1e-5.
 """
 import numpy as np
 import pytest
@@ -61,9 +63,12 @@ def test_forward_matches_oracle():
     x = rng.normal(0, 1, (300, nin)).astype(np.float32)     # more rows than one batch
     y = g.mlp_forward(x)
     net = O.Mlp(nin, nout, HID, NLAY)
-    p = bf16(g.mlp_params())
-    _, _, yo = net.loss_grad(p, bf16(x), np.zeros((300, nout)), grad=False)
-    assert rel2(y, yo) < 3e-2, rel2(y, yo)
+    _, _, yo = net.loss_grad(g.mlp_params(), x, np.zeros((300, nout)), grad=False, bf16=True)
+    print(f"forward rel L2 vs bf16 oracle: {rel2(y, yo):.2e}")
+    assert rel2(y, yo) < 4e-3, rel2(y, yo)
+    # a network that skipped the rounding, or mis-scaled a layer, is far off
+    _, _, y64 = net.loss_grad(g.mlp_params(), x, np.zeros((300, nout)), grad=False)
+    assert rel2(y64, yo) > 2 * rel2(y, yo)
 
 
 def test_gradient_matches_oracle():
@@ -75,10 +80,11 @@ def test_gradient_matches_oracle():
     t = rng.normal(0, 1, (400, nout)).astype(np.float32)
     grad, loss = g.mlp_grad(x, t)
     net = O.Mlp(nin, nout, HID, NLAY)
-    lo, go, _ = net.loss_grad(bf16(g.mlp_params()), bf16(x), t.astype(np.float64))
-    assert abs(loss - lo) < 2e-2 * lo
+    lo, go, _ = net.loss_grad(g.mlp_params(), x, t, bf16=True)
+    assert abs(loss - lo) < 1e-3 * lo, (loss, lo)
     for a, b in zip(layers(grad, nin, nout), layers(go, nin, nout)):
-        assert rel2(a, b) < 6e-2, rel2(a, b)
+        print(f"gradient block rel L2 vs bf16 oracle: {rel2(a, b):.2e}")
+        assert rel2(a, b) < 6e-3, rel2(a, b)
 
 
 def test_adamw_matches_closed_form():

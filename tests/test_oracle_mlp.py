@@ -70,3 +70,50 @@ def test_adamw_closed_form():
         q = np.where(isw, q * (1 - lr * wd), q)
         q = q - lr * (mm / (1 - 0.9 ** s)) / (np.sqrt(vv / (1 - 0.999 ** s)) + 1e-8)
     np.testing.assert_allclose(p, q, rtol=1e-14, atol=1e-15)
+
+
+def _bf16(a):
+    """round to bfloat16 (nearest even) through float32"""
+    u = np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32)).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_bf16_storage_mode():
+    """the oracle's bf16 mode rounds exactly where a bf16 matrix-core network
+    stores values: inputs, weights, biases, hidden activations after GELU, the
+    output gradient and each back-propagated pre-activation gradient (written
+    out here in numpy over two hidden layers, forward and backward)"""
+    from scipy.special import erf
+    rng = np.random.default_rng(4)
+    nin, nout, hid, nlay, n = 9, 5, 8, 2, 6
+    net = O.Mlp(nin, nout, hid, nlay)
+    p = rng.normal(0, 0.4, net.nparams)
+    x = rng.normal(0, 1, (n, nin))
+    t = rng.normal(0, 1, (n, nout))
+    loss, g, y = net.loss_grad(p, x, t, bf16=True)
+    dims = [(nin, hid), (hid, hid), (hid, nout)]
+    Ws, bs, o = [], [], 0
+    for K, N in dims:
+        Ws.append(_bf16(p[o:o + K * N].reshape(K, N)))
+        bs.append(_bf16(p[o + K * N:o + K * N + N]))
+        o += K * N + N
+    gelu = lambda z: 0.5 * z * (1 + erf(z / np.sqrt(2)))
+    dgelu = lambda z: 0.5 * (1 + erf(z / np.sqrt(2))) + z * np.exp(-0.5 * z * z) / np.sqrt(2 * np.pi)
+    acts, pres = [_bf16(x)], []
+    for l, (W, b) in enumerate(zip(Ws, bs)):
+        z = acts[-1] @ W + b
+        pres.append(z)
+        acts.append(_bf16(gelu(z)) if l < len(Ws) - 1 else z)
+    np.testing.assert_allclose(y, acts[-1], rtol=1e-12, atol=1e-14)
+    assert abs(loss - ((acts[-1] - t) ** 2).mean()) < 1e-12
+    d = _bf16(2 * (acts[-1] - t) / (n * nout))
+    gr = []
+    for l in range(len(Ws) - 1, -1, -1):
+        gr = [(acts[l].T @ d).ravel(), d.sum(0)] + gr
+        if l > 0:
+            d = _bf16((d @ Ws[l].T) * dgelu(pres[l - 1]))
+    np.testing.assert_allclose(g, np.concatenate(gr), rtol=1e-11, atol=1e-15)
+    # and the rounding is real: the plain FP64 network differs at the bf16 level
+    _, _, y64 = net.loss_grad(p, x, t, grad=False)
+    assert 1e-4 < np.abs(y - y64).max() / np.abs(y64).max() < 5e-2
