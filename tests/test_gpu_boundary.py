@@ -152,8 +152,10 @@ def test_shtns_symbols_as_the_reference_wrapper_calls_them(nlon, nlat, T):
         assert np.abs(w.lats - o.lats).max() < 1e-14                   # N -> S
         assert np.abs(w.areawts - o.areawts.reshape(nlat, nlon)).max() < 1e-17
         assert np.array_equal(w.degree, o.degree) and np.array_equal(w.order, o.order)
-        q, d = rand_spec(o.degree, o.order, 1, 2)
-        q[0], d[0] = q[0] * 3e-5, d[0] * 1e-6                          # vorticity-, divergence-sized
+        qd = rand_spec(o.degree, o.order, 1, 3)
+        q = qd[:2].copy()
+        q[0] *= 3e-5                                                   # vorticity-sized
+        d = qd[2:3] * 1e-6                                             # divergence-sized
         g = o.spectogrd(q[:1])[0]
         assert rel(w.spectogrd(q[0]), g) < 1e-11
         assert rel(w.grdtospec(g), o.grdtospec(g[None])[0]) < 1e-11
