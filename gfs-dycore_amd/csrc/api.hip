@@ -135,6 +135,9 @@ int gfsdc_spectogrd(gfsdc_ctx *c, const void *spec, void *grid, int nf, int dtyp
 int gfsdc_grdtospec(gfsdc_ctx *c, const void *grid, void *spec, int nf, int dtype, int loc) {
     DISPATCH(c, grdtospec(grid, spec, nf, dtype, loc));
 }
+int gfsdc_spectogrd_noalloc(gfsdc_ctx *c, const void *spec, void *grid, int nf) {
+    DISPATCH(c, spectogrd_noalloc(spec, grid, nf));
+}
 // shtns.f90:259-261: S = invlap*rsphere*vrt, T = invlap*rsphere*div; u = Vt, v = Vp
 int gfsdc_getuv(gfsdc_ctx *c, const void *vrt, const void *div, void *u, void *v, int nf, double rsphere, int dtype,
                 int loc) {

@@ -44,6 +44,7 @@ template <typename T> struct Engine {
     int output(const T *d, void *p, size_t n, int dtype, int loc, int slot);
 
     int spectogrd(const void *spec, void *grid, int nf, int dtype, int loc);
+    int spectogrd_noalloc(const void *spec, void *grid, int nf);
     int bench_transforms(int ns, int nv, int reps, double *ms);
     int bench_copy(long nbytes, int reps, double *ms);
     int grdtospec(const void *grid, void *spec, int nf, int dtype, int loc);

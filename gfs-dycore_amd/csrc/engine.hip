@@ -2,10 +2,55 @@
 // transforms of the shtns.f90 API (spectogrd, grdtospec, getuv,
 // getvrtdivspec, getgrad) on the device.
 #include <cstring>
+#include <mutex>
 
 #include "engine.h"
 
 namespace gfsdc {
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation synthesis (gfsdc_spectogrd_noalloc): one
+// file-scope device array per GPU, held until the process ends, holding the
+// synthesis descriptors, the extended spectral columns and the Fourier buffer
+// that gfsdc_spectogrd keeps in the context's lazily allocated work buffers.
+// Its device address is looked up once per device at context creation, with
+// that device current; host code uses only that address, never the symbol.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MAX_DEVICES] = {};
+
+static int scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) {
+        set_error("device ordinal out of range for the no-allocation scratch");
+        return GFSDC_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return GFSDC_EHIP;
+    }
+    g_scratch_ptr[device] = ptr;
+    return 0;
+}
+
+static void *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
+// descriptors of a plain batched synthesis: field f copied to column pair f
+__global__ void k_copy_descs(SynthDesc *__restrict__ d, unsigned char *__restrict__ rflag, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    d[f] = SynthDesc{SK_COPY, f, -1, 1.0, 0.0, 0, 0};
+    rflag[f] = 0;
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 template <typename Tin, typename Tout>
 __global__ void k_convert(const Tin *__restrict__ a, Tout *__restrict__ b, size_t n) {
@@ -42,7 +87,9 @@ template <typename T> int Engine<T>::init(const gfsdc_cfg &c) {
         return GFSDC_ENODEV;
     }
     GFSDC_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    int rc = plan_create(plan, c.nlon, c.nlat, c.ntrunc, stream);
+    int rc = scratch_setup(c.device);
+    if (rc) return rc;
+    rc = plan_create(plan, c.nlon, c.nlat, c.ntrunc, stream);
     plan.leg_bf16x3 = (c.precision == GFSDC_BF16X3) ? 1 : 0;
     if (rc) return rc;
     if (c.nlev > 0) {
@@ -187,6 +234,43 @@ template <typename T> int Engine<T>::spectogrd(const void *spec, void *grid, int
     launch_fft_synth(plan, work.fs, work.ldc, work.grid, (long)plan.nlat * plan.nlon, nf, d_rflag, stream);
     GFSDC_CHECK(hipGetLastError());
     return output(work.grid, grid, (size_t)nf * plan.nlat * plan.nlon, dtype, loc, 1);
+}
+
+// spectogrd on device buffers of the context's element type with every
+// intermediate in the per-device scratch: no allocation, no host copy, no
+// synchronisation (work is queued on the context stream).
+template <typename T> int Engine<T>::spectogrd_noalloc(const void *spec, void *grid, int nf) {
+    if (nf == 0) return 0;
+    if (nf < 0 || !spec || !grid) {
+        set_error("spectogrd_noalloc: need nfield >= 0 and device buffers");
+        return GFSDC_EINVAL;
+    }
+    char *base = (char *)scratch_ptr(cfg.device);
+    if (!base) {
+        set_error("spectogrd_noalloc: no scratch address for this device");
+        return GFSDC_ESTATE;
+    }
+    const int ldc = round_up(2 * nf, COL_PAD) + COL_SKEW;
+    const size_t o_rf = align256(sizeof(SynthDesc) * nf);
+    const size_t o_sx = o_rf + align256((size_t)nf);
+    const size_t o_fs = o_sx + align256(sizeof(T) * (size_t)plan.nlme * ldc);
+    const size_t need = o_fs + sizeof(T) * (size_t)(plan.ntrunc + 1) * plan.nlat * ldc;
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        set_error("spectogrd_noalloc: " + std::to_string(nf) + " fields at T" + std::to_string(plan.ntrunc) + " (" +
+                  std::to_string(plan.nlon) + "x" + std::to_string(plan.nlat) + ") need " + std::to_string(need >> 20) +
+                  " MiB of scratch; the maximum is " + std::to_string(NOALLOC_SCRATCH_BYTES >> 20) +
+                  " MiB (use gfsdc_spectogrd, or fewer fields per call)");
+        return GFSDC_EINVAL;
+    }
+    SynthDesc *desc = (SynthDesc *)base;
+    unsigned char *rflag = (unsigned char *)(base + o_rf);
+    T *sx = (T *)(base + o_sx), *fs = (T *)(base + o_fs);
+    hipLaunchKernelGGL(k_copy_descs, dim3((nf + 255) / 256), dim3(256), 0, stream, desc, rflag, nf);
+    launch_synth_gather(plan, (const cplx<T> *)spec, plan.nlm, 1, desc, nf, sx, ldc, 1.0, stream);
+    launch_leg_synth(plan, sx, ldc, fs, ldc, 2 * nf, stream);
+    launch_fft_synth(plan, fs, ldc, (T *)grid, (long)plan.nlat * plan.nlon, nf, rflag, stream);
+    GFSDC_CHECK(hipGetLastError());
+    return 0;
 }
 
 template <typename T> int Engine<T>::grdtospec(const void *grid, void *spec, int nf, int dtype, int loc) {

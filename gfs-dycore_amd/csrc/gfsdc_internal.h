@@ -27,6 +27,8 @@ template <typename T> struct cplx { T x, y; };
 constexpr int GEMM_BM = 64;
 constexpr int GEMM_BN = 128;
 constexpr int GEMM_BK = 16;
+// bytes of the per-device scratch of gfsdc_spectogrd_noalloc
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)128 << 20;
 constexpr int COL_PAD = GEMM_BN;  // column padding of transform work buffers
 // columns past the padded width: keeps the work buffers' row pitch off a power
 // of two (at 1024 columns the rows that one Fourier workgroup reads or writes,

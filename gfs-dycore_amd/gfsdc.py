@@ -49,7 +49,7 @@ SYMBOLS = [
     "shtns_cos_array_", "shtns_gauss_wts_", "shtns_spat_to_sh_", "shtns_sh_to_spat_",
     "shtns_sphtor_to_spat_", "shtns_spat_to_sphtor_", "shtns_reset_",
     "gfsdc_cfg_defaults", "gfsdc_create", "gfsdc_destroy", "gfsdc_last_error", "gfsdc_nlm",
-    "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_grdtospec", "gfsdc_getuv",
+    "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_spectogrd_noalloc", "gfsdc_grdtospec", "gfsdc_getuv",
     "gfsdc_getvrtdivspec", "gfsdc_getgrad", "gfsdc_set_state", "gfsdc_get_state",
     "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
     "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_autotune", "gfsdc_bench_transforms", "gfsdc_bench_copy", "gfsdc_mlp_init", "gfsdc_mlp_collect",

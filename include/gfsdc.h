@@ -144,6 +144,18 @@ int gfsdc_griddata(const gfsdc_ctx *ctx, double *lats, double *gauwts, double *a
  * dtype; grid: (nlon, nlat, nfield) of dtype; loc GFSDC_HOST / GFSDC_DEVICE. */
 /* replaces shtns.f90:203 spectogrd */
 int gfsdc_spectogrd(gfsdc_ctx *ctx, const void *spec, void *grid, int nfield, int dtype, int loc);
+/* No counterpart in the reference: spectogrd for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). spec and grid are device buffers of the context's
+ * element type (double for GFSDC_FP64, float otherwise), layouts as
+ * gfsdc_spectogrd. Every intermediate lives in one per-GPU static scratch of
+ * 128 MiB (enough for 64 fields at T254 in FP32, or at T126 in FP64); a call
+ * that needs more returns GFSDC_EINVAL with the sizes in gfsdc_last_error().
+ * The call allocates nothing, copies nothing to or from the host and does not
+ * synchronise: the work is queued on the context stream (gfsdc_sync waits).
+ * All calls on one GPU share its scratch, so calls that overlap in time on one
+ * GPU (two contexts, two streams) race; serialise them. */
+int gfsdc_spectogrd_noalloc(gfsdc_ctx *ctx, const void *spec, void *grid, int nfield);
 /* replaces shtns.f90:171 grdtospec */
 int gfsdc_grdtospec(gfsdc_ctx *ctx, const void *grid, void *spec, int nfield, int dtype, int loc);
 /* replaces shtns.f90:233 getuv */
