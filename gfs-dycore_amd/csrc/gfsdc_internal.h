@@ -59,6 +59,8 @@ template <typename T> struct Plan {
     int fft_nf_an = 8;       // fields per analysis FFT workgroup (2, 4 or 8)
     int leg_bf16x3 = 0;      // FP32 plan only: Legendre GEMMs as three bf16 products (hi.hi + hi.lo + lo.hi)
     T *plm = nullptr;        // per m: [even i'][nhp] then [odd i'][nhp], n = m + 2i' (+1)
+    T *plmt = nullptr;       // per m: [parity][nhp][Kw] latitude-major copy (analysis), Kw = roundup(Ke, GEMM_BM)
+    long *tpoff = nullptr;   // device, offset of order m in plmt
     long *poff = nullptr;    // device, per m (ntrunc+2 entries)
     int *keoff = nullptr;    // device, extended spectral offset of order m
     int *koff = nullptr;     // device, spectral offset of order m

@@ -241,10 +241,13 @@ __device__ inline void gemm_chunk(T (*As)[GEMM_BM + LDS_PAD], T (*Bs)[GT<T>::BN 
     }
 }
 
-// Synthesis, one parity per workgroup (blockIdx.y = 2 (m ntj + tj) + parity):
-// the even-degree sum E_j goes to row j of fs[m], the odd-degree sum O_j to row
-// nlat-1-j; the Fourier synthesis forms the hemispheres E + O (north) and E - O
-// (south) as it loads them. Each workgroup holds one parity's accumulators only.
+// Synthesis, one parity per workgroup: the even-degree sum E_j goes to row j of
+// fs[m], the odd-degree sum O_j to row nlat-1-j; the Fourier synthesis forms
+// the hemispheres E + O (north) and E - O (south) as it loads them. Each
+// workgroup holds one parity's accumulators only. In FP32 the output tile is
+// restaged through LDS after the k-loop so that each store instruction writes
+// whole 128-byte lines of the Fourier buffer (8 lines per instruction instead
+// of 16 half lines): the stores dominated the memory pipeline of this kernel.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, const long *__restrict__ poff,
                                                    const int *__restrict__ keoff, const T *__restrict__ sx,
@@ -255,6 +258,7 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     constexpr int VW = Vec<T>::n;
     constexpr int BN = GT<T>::BN, WB = GT<T>::WB;
     constexpr int NA = GEMM_BK * GEMM_BM / VW / 256, NB = GEMM_BK * BN / VW / 256;
+    constexpr bool STG = Vec<T>::swap;
     // logical tile ((i * 2 + par) * ntj + tj) * ntc + tc, contiguous per XCD: the
     // column and latitude tiles of one (m, parity) share its Legendre block and
     // spectral columns in that XCD's L2. Orders are taken as m = 0, T, 1, T-1, ...
@@ -267,8 +271,13 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     const T *P = plm + poff[m] + (par ? (long)Ke * nhp : 0L);
     const T *S = sx + (long)keoff[m] * ldx + (long)par * ldx;
     const int j0 = tj * GEMM_BM, c0 = tc * BN;
-    __shared__ T As[GEMM_BK][GEMM_BM + LDS_PAD];
-    __shared__ T Bs[GEMM_BK][BN + LDS_PAD];
+    // operand chunks; after the k-loop (STG) each wave's 16 x 64 output slab
+    constexpr int STG_P = BN / 2 + 4;
+    constexpr int LDS_OPS = GEMM_BK * (GEMM_BM + LDS_PAD) + GEMM_BK * (BN + LDS_PAD);
+    constexpr int LDS_STG = STG ? 4 * 16 * STG_P : 0;
+    __shared__ __attribute__((aligned(16))) T lds[LDS_OPS > LDS_STG ? LDS_OPS : LDS_STG];
+    T (*As)[GEMM_BM + LDS_PAD] = (T (*)[GEMM_BM + LDS_PAD])lds;
+    T (*Bs)[BN + LDS_PAD] = (T (*)[BN + LDS_PAD])(lds + GEMM_BK * (GEMM_BM + LDS_PAD));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * (BN / 2);
     acc_t E[2][WB];
@@ -282,13 +291,13 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     auto load = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
-            int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW, i = k0 + kk;
-            ra[u] = *(const V *)(P + (long)min(i, Kp - 1) * nhp + j0 + jj);
+            int v = tid + 256 * u, kk = v / (GEMM_BM / VW), jj = (v % (GEMM_BM / VW)) * VW, r = k0 + kk;
+            ra[u] = *(const V *)(P + (long)min(r, Kp - 1) * nhp + j0 + jj);
         }
 #pragma unroll
         for (int u = 0; u < NB; u++) {
-            int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW, i = k0 + kk;
-            rb[u] = *(const V *)(S + (long)(2 * min(i, Kp - 1)) * ldx + c0 + cc);
+            int v = tid + 256 * u, kk = v / (BN / VW), cc = (v % (BN / VW)) * VW, r = k0 + kk;
+            rb[u] = *(const V *)(S + (long)(2 * min(r, Kp - 1)) * ldx + c0 + cc);
         }
     };
     auto store = [&](int k0) {
@@ -321,45 +330,58 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
         __syncthreads();
     }
     const int nm1 = ntrunc + 1;
+    if constexpr (STG) {
+        // per 16-row slab bi: lanes hold row lane % 16, columns 4 (lane / 16) + r of each
+        // 16 x 16 block; restaged, lane holds columns 4 (lane % 16) + r of row 4 q + lane / 16,
+        // so 16 lanes write 64 consecutive columns (two whole lines) of one row. The slab
+        // area of each wave is its own; LDS operations of one wave complete in order.
+        T *st = lds + wave * 16 * STG_P;
+        const int c = c0 + wn + 4 * (lane & 15);
 #pragma unroll
-    for (int bi = 0; bi < 2; bi++)
+        for (int bi = 0; bi < 2; bi++) {
 #pragma unroll
-        for (int bj = 0; bj < WB; bj++) {
-            if constexpr (Vec<T>::swap) {
-                int j = j0 + wm + bi * 16 + (lane & 15);
-                int c = c0 + wn + bj * 16 + 4 * (lane >> 4);  // a multiple of 4: 16-byte aligned
-                if (j < nhalf && c < ncol) {  // padding columns are never read
-                    acc_t e = E[bi][bj];
-                    *(float4 *)(fs + fs_index(par ? nlat - 1 - j : j, m, c, ldf, nm1)) = float4{e[0], e[1], e[2], e[3]};
-                }
-            } else {
+            for (int bj = 0; bj < WB; bj++) {
+                acc_t e = E[bi][bj];
+                *(float4 *)(st + (lane & 15) * STG_P + bj * 16 + 4 * (lane >> 4)) = float4{e[0], e[1], e[2], e[3]};
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = 4 * q + (lane >> 4), j = j0 + wm + bi * 16 + r;
+                const float4 v = *(const float4 *)(st + r * STG_P + 4 * (lane & 15));
+                if (j < nhalf && c < ncol) *(float4 *)(fs + fs_index(par ? nlat - 1 - j : j, m, c, ldf, nm1)) = v;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bj = 0; bj < WB; bj++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     int j = j0 + wm + bi * 16 + MF<T>::row(lane, r);
                     int c = c0 + wn + bj * 16 + (lane & 15);
                     if (j < nhalf && c < ncol) fs[fs_index(par ? nlat - 1 - j : j, m, c, ldf, nm1)] = E[bi][bj][r];
                 }
-            }
-        }
+    }
 }
 
 // One LDS chunk of one parity (analysis). GUARD skips, wave-uniformly, the
 // 16-row blocks at or beyond ne (rows past the parity's last degree) and the
-// 16-column blocks at or beyond nbj (columns past the last field).
-template <typename T, bool GUARD, typename Acc>
-__device__ inline void gemm_chunk1(T (*As)[GEMM_BM + LDS_PAD], T (*Bs)[GT<T>::BN + LDS_PAD],
-                                   Acc (&E)[2][GT<T>::WB], int lane, int wm, int wn, int ne = 2,
-                                   int nbj = GT<T>::WB) {
+// 16-column blocks at or beyond nbj (columns past the last field). A is
+// k-major, As[k * (GEMM_BM + LDS_PAD) + i]; B is Bs[k * bp + c].
+template <typename T, bool GUARD, int WB, typename Acc>
+__device__ inline void gemm_chunk1(const T *As, const T *Bs, int bp, Acc (&E)[2][WB], int lane, int wm, int wn,
+                                   int ne = 2, int nbj = WB) {
 #pragma unroll
     for (int ks = 0; ks < GEMM_BK; ks += 4) {
         const int kr = ks + (lane >> 4), li = lane & 15;
         T a[2];
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++) a[bi] = As[kr][wm + bi * 16 + li];
+        for (int bi = 0; bi < 2; bi++) a[bi] = As[kr * (GEMM_BM + LDS_PAD) + wm + bi * 16 + li];
 #pragma unroll
-        for (int bj = 0; bj < GT<T>::WB; bj++) {
+        for (int bj = 0; bj < WB; bj++) {
             if (GUARD && bj >= nbj) break;
-            T b = Bs[kr][wn + bj * 16 + li];
+            T b = Bs[kr * bp + wn + bj * 16 + li];
 #pragma unroll
             for (int bi = 0; bi < 2; bi++) {
                 if (Vec<T>::swap) {
@@ -372,15 +394,34 @@ __device__ inline void gemm_chunk1(T (*As)[GEMM_BM + LDS_PAD], T (*Bs)[GT<T>::BN
     }
 }
 
+__host__ __device__ inline int round_up_d(int a, int b) { return (a + b - 1) / b * b; }
+
+// plmt[tpoff[m] + (par nhp + j) Kw + i'] = P(m, n = m + 2 i' + par)(lat j), with
+// Kw = roundup(Ke, GEMM_BM); zero for i' past the parity's last degree.
+// Grid (j, par, m), GEMM_BM threads striding over i'.
+template <typename T>
+__global__ void k_transpose_plm(const T *__restrict__ plm, const long *__restrict__ poff, T *__restrict__ plmt,
+                                const long *__restrict__ tpoff, int nhp, int ntrunc) {
+    const int j = blockIdx.x, par = blockIdx.y, m = blockIdx.z;
+    const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Kp = par ? (K >> 1) : Ke, Kw = round_up_d(Ke, GEMM_BM);
+    const T *P = plm + poff[m] + (par ? (long)Ke * nhp : 0L);
+    T *D = plmt + tpoff[m] + ((long)par * nhp + j) * Kw;
+    for (int i = threadIdx.x; i < Kw; i += blockDim.x) D[i] = i < Kp ? P[(long)i * nhp + j] : T(0);
+}
+
 // Analysis, one parity per workgroup: output degrees i' of that parity, K over
 // the (zero-padded) latitude pairs. fa[m][0][j][c] = w_j (F_N + F_S),
 // fa[m][1][j][c] = w_j (F_N - F_S). Each workgroup holds one parity's
-// accumulators only (half the registers and LDS of a two-parity tile).
+// accumulators only (half the registers and LDS of a two-parity tile). The
+// Legendre operand comes from the latitude-major copy of the table (plmt), so
+// one chunk is GEMM_BK rows of GEMM_BM consecutive degrees: whole-line loads
+// and 16-byte LDS stores (the degree-major table gave 64-byte pieces).
 template <typename T>
-__global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, const long *__restrict__ poff,
+__global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plmt, const long *__restrict__ tpoff,
                                                   const int *__restrict__ keoff, const int *__restrict__ mtile_m,
                                                   const int *__restrict__ mtile_i, const T *__restrict__ fa,
-                                                  int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc, int ncol, int nmt) {
+                                                  int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc, int ncol,
+                                                  int nmt) {
     typedef typename MF<T>::acc acc_t;
     typedef typename Vec<T>::t V;
     constexpr int VW = Vec<T>::n;
@@ -393,11 +434,14 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
     const int m = mtile_m[mt], i0 = mtile_i[mt];
     const int K = ntrunc + 2 - m, Ke = (K + 1) >> 1, Ko = K >> 1, Kp = par ? Ko : Ke;
     if (i0 >= Kp) return;  // an odd-parity tile past the last odd degree (the tiles follow the even block)
-    const T *P = plm + poff[m] + (par ? (long)Ke * nhp : 0L);
     const T *F = fa + (long)(2 * m + par) * nhp * ldf;
     const int c0 = tc * BN;
-    __shared__ T As[GEMM_BK][GEMM_BM + LDS_PAD];
-    __shared__ T Bs[GEMM_BK][BN + LDS_PAD];
+    // [parity][j][i'] rows of Kw = roundup(Ke, GEMM_BM) degrees, zero past the
+    // parity's last degree
+    const int Kw = round_up_d(Ke, GEMM_BM);
+    const T *PT = plmt + tpoff[m] + (long)par * nhp * Kw + i0;
+    __shared__ __attribute__((aligned(16))) T As[GEMM_BK][GEMM_BM + LDS_PAD];
+    __shared__ __attribute__((aligned(16))) T Bs[GEMM_BK][BN + LDS_PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * (BN / 2);
     acc_t E[2][WB];
@@ -405,13 +449,14 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < WB; b++) E[a][b] = acc_t{0, 0, 0, 0};
+    // vectors are assembled component by component (vkeep): a plain copy of the
+    // vector struct from memory puts these arrays in scratch
     V ra[NA], rb[NB];
     auto load = [&](int j0) {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
-            int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW, i = i0 + ii;
-            V x = *(const V *)(P + (long)min(i, Kp - 1) * nhp + j0 + kk);
-            ra[u] = vkeep(x, i < Kp);
+            int v = tid + 256 * u, kk = v / (GEMM_BM / VW), ii = (v % (GEMM_BM / VW)) * VW;
+            ra[u] = vkeep(*(const V *)(PT + (long)(j0 + kk) * Kw + ii), true);
         }
 #pragma unroll
         for (int u = 0; u < NB; u++) {
@@ -422,9 +467,8 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
     auto store = [&]() {
 #pragma unroll
         for (int u = 0; u < NA; u++) {
-            int v = tid + 256 * u, ii = v / (GEMM_BK / VW), kk = (v % (GEMM_BK / VW)) * VW;
-#pragma unroll
-            for (int q = 0; q < VW; q++) As[kk + q][ii] = vget(ra[u], q);
+            int v = tid + 256 * u, kk = v / (GEMM_BM / VW), ii = (v % (GEMM_BM / VW)) * VW;
+            vstore_lds<T>(&As[kk][ii], ra[u]);
         }
 #pragma unroll
         for (int u = 0; u < NB; u++) {
@@ -446,9 +490,9 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plm, con
         __syncthreads();
         if (PREFETCH && j0 + GEMM_BK < nhp) load(j0 + GEMM_BK);
         if (full)
-            gemm_chunk1<T, false>(As, Bs, E, lane, wm, wn);
+            gemm_chunk1<T, false, WB>(&As[0][0], Bs[0], BN + LDS_PAD, E, lane, wm, wn);
         else
-            gemm_chunk1<T, true>(As, Bs, E, lane, wm, wn, ne, nbj);
+            gemm_chunk1<T, true, WB>(&As[0][0], Bs[0], BN + LDS_PAD, E, lane, wm, wn, ne, nbj);
         __syncthreads();
     }
     T *X = sx + (long)keoff[m] * ldx;
@@ -1347,6 +1391,19 @@ template <typename T> int plan_create(Plan<T> &p, int nlon, int nlat, int ntrunc
     hipLaunchKernelGGL(k_legendre_table<T>, dim3(p.nhp / 64, T_ + 1), dim3(64), 0, s, p.plm, p.poff, p.mu_d,
                        p.nhalf, p.nhp, T_);
     GFSDC_CHECK(hipGetLastError());
+    // latitude-major copy for the analysis: per order, [parity][j][i'], rows of
+    // Kw = roundup(Ke, GEMM_BM) degrees, zero past each parity's last degree
+    std::vector<long> tpoff_h(T_ + 2);
+    long to = 0;
+    for (int m = 0; m <= T_ + 1; m++) {
+        tpoff_h[m] = to;
+        if (m <= T_) to += 2L * p.nhp * round_up((T_ + 3 - m) / 2, GEMM_BM);
+    }
+    if ((rc = upload(&p.tpoff, tpoff_h.data(), sizeof(long) * (T_ + 2), s))) return rc;
+    GFSDC_CHECK(hipMalloc((void **)&p.plmt, sizeof(T) * (size_t)to));
+    hipLaunchKernelGGL(k_transpose_plm<T>, dim3(p.nhp, 2, T_ + 1), dim3(GEMM_BM), 0, s, p.plm, p.poff, p.plmt, p.tpoff,
+                       p.nhp, T_);
+    GFSDC_CHECK(hipGetLastError());
     p.mu.resize(nlat);
     p.w.resize(nlat);
     GFSDC_CHECK(hipMemcpyAsync(p.mu.data(), p.mu_d, sizeof(double) * nlat, hipMemcpyDeviceToHost, s));
@@ -1381,7 +1438,7 @@ template <typename T> int plan_create(Plan<T> &p, int nlon, int nlat, int ntrunc
 }
 
 template <typename T> void plan_destroy(Plan<T> &p) {
-    void *ptrs[] = {p.plm, p.poff, p.keoff, p.koff, p.eps, p.eoff, p.rcos, p.wfac,
+    void *ptrs[] = {p.plm, p.poff, p.plmt, p.tpoff, p.keoff, p.koff, p.eps, p.eoff, p.rcos, p.wfac,
                     p.tw, p.mtile_m, p.mtile_i, p.mu_d, p.w_d, p.kemn, p.kmn};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -1449,8 +1506,8 @@ void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int
             return;
         }
     }
-    hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc * p.n_mtiles * 2), dim3(256), 0, s, p.plm, p.poff, p.keoff, p.mtile_m,
-                       p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol, p.n_mtiles);
+    hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc * p.n_mtiles * 2), dim3(256), 0, s, p.plmt, p.tpoff, p.keoff,
+                       p.mtile_m, p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol, p.n_mtiles);
 }
 
 static Radices radices_of(int n, const int *r) {
