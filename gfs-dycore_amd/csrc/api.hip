@@ -226,6 +226,31 @@ int gfsdc_init_rest_state(gfsdc_ctx *c, double t0, double ps0, double vort_rms, 
 int gfsdc_profile_step(gfsdc_ctx *c, char names[][48], double *ms, int nmax) {
     DISPATCH(c, profile_step(names, ms, nmax));
 }
+int gfsdc_set_time(gfsdc_ctx *c, double t) { DISPATCH(c, set_time(t)); }
+int gfsdc_get_time(gfsdc_ctx *c, double *t) { DISPATCH(c, get_time(t)); }
+int gfsdc_iau_set(gfsdc_ctx *c, int nfiles, const double *fhrs, double delthrs, const void *vrt, const void *div,
+                  const void *tv, const void *trac, const void *lnps, int dtype, int loc) {
+    DISPATCH(c, iau_set(nfiles, fhrs, delthrs, vrt, div, tv, trac, lnps, dtype, loc));
+}
+int gfsdc_dfi(gfsdc_ctx *c, int ntdfi, int use_graph) { DISPATCH(c, dfi(ntdfi, use_graph)); }
+int gfsdc_checkpoint_size(gfsdc_ctx *c, size_t *nbytes) {
+    if (!c || !nbytes) {
+        set_error("null argument");
+        return GFSDC_EINVAL;
+    }
+    *nbytes = c->ef ? c->ef->checkpoint_size() : c->ed->checkpoint_size();
+    if (!*nbytes) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    return 0;
+}
+int gfsdc_checkpoint_save(gfsdc_ctx *c, void *buf, size_t nbytes, int loc) {
+    DISPATCH(c, checkpoint_save(buf, nbytes, loc));
+}
+int gfsdc_checkpoint_load(gfsdc_ctx *c, const void *buf, size_t nbytes, int loc) {
+    DISPATCH(c, checkpoint_load(buf, nbytes, loc));
+}
 int gfsdc_sync(gfsdc_ctx *c) {
     if (!c) {
         set_error("null context");

@@ -108,6 +108,12 @@ struct DycoreBuffers {
     unsigned long long *jctr = nullptr;          // judge / sampling step counter
     unsigned long long bctr = 0;                 // training batch counter
     double *diag = nullptr;                      // FP64 4 per member
+    double *clock = nullptr;                     // model time (s) at the start of the next step
+    // incremental analysis update (iau_module.f90): nfiles increments of the
+    // whole state in the state layout, their times (h) and the forcing of a step
+    int iau_n = 0;
+    double *iau_hrs = nullptr, iau_dt = 0.0;
+    void *iau_inc = nullptr, *iau_F = nullptr;
     SynthDesc *sdesc = nullptr;                  // nm*nsyn
     // the physics pass synthesises every field but the temperature and tracer
     // gradients (nph per member): descriptors, 1/cos flags and grid slots
@@ -119,6 +125,7 @@ struct DycoreBuffers {
     unsigned char *rflag_syn = nullptr, *rflag_an = nullptr, *rflag_hs = nullptr, *rflag_sp = nullptr;
     // host copies of the FP64 operators
     std::vector<double> amhyb, bmhyb, svhyb, tor_hyb, d_hyb_m, disspec_h, diff_h, dmp_h, sl;
+    std::vector<double> ym;                       // tor (x) sv + amhyb bmhyb (semimp_data.f90:225-234)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // profiling
@@ -1136,6 +1143,51 @@ __global__ void k_counter_inc(unsigned long long *ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] += 1ULL;
 }
 
+// model clock: t = t + dt at the end of every step (run.f90:153)
+__global__ void k_clock_add(double *clk, double dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) clk[0] += dt;
+}
+
+// getiauforcing (iau_module.f90:84-115) at the clock's time t: zero outside
+// [hrs[0], hrs[nf-1]] h, the first or last increment / dti at those end
+// points, else ((1 - delt) inc[n] + delt inc[n-1]) / dti with n the first
+// file later than t and delt = (hrs[n] - t/3600) / (hrs[n] - hrs[n-1]).
+// Formed in FP64 (the reference's FP64 delt promotes it) and rounded to T.
+// Every thread finds the weights itself from the clock: the kernel sits in
+// the captured step and reads the time of the replay that runs it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_iau_forcing(T *__restrict__ F, const T *__restrict__ inc,
+                                                     const double *__restrict__ hrs, int nf, double dti,
+                                                     const double *__restrict__ clk, long len) {
+    const double t = clk[0];
+    double whi = 0.0, wlo = 0.0;
+    int n = 0;
+    if (!(t < hrs[0] * 3600. || t > hrs[nf - 1] * 3600.)) {
+        if (t == 3600. * hrs[nf - 1]) {
+            n = nf - 1;
+            whi = 1.0;
+        } else if (t == 3600. * hrs[0]) {
+            whi = 1.0;
+        } else {
+            for (n = 1; n < nf - 1; n++)
+                if (hrs[n] * 3600. > t) break;
+            const double delt = (hrs[n] - t / 3600.) / (hrs[n] - hrs[n - 1]);
+            whi = 1. - delt;
+            wlo = delt;
+        }
+    }
+    const T *hi = inc + (long)n * len, *lo = n > 0 ? hi - len : hi;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (long)gridDim.x * blockDim.x)
+        F[i] = (T)((whi * (double)hi[i] + wlo * (double)lo[i]) / dti);
+}
+
+// y = a x + b y (b == 0: y = a x, whatever y held)
+template <typename T>
+__global__ __launch_bounds__(256) void k_axpby(T *__restrict__ y, const T *__restrict__ x, T a, T b, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = b == (T)0 ? a * x[i] : a * x[i] + b * y[i];
+}
+
 // temperature forcing pattern / tau on every level (vertical taper vf)
 template <typename T>
 __global__ void k_pattern_apply(cplx<T> *__restrict__ spec, const cplx<T> *__restrict__ pat, const T *__restrict__ vf,
@@ -1697,40 +1749,97 @@ __global__ void k_grid_gather(const T *__restrict__ src, long mstride, T *__rest
     dst[i] = src[mem * mstride + r];
 }
 
+// init_semimpdata's implicit operators on the device, in FP64
+// (semimp_data.f90:238-251): for stage s and degree n, d_hyb_m(:,:,n+1,s) =
+// inv(I + (c_s dt)^2 n (n+1) ym), c = (aa22, aa33, bb4). One workgroup per
+// (n, s) runs Gauss-Jordan elimination with partial pivoting (the first row of
+// largest magnitude, as LAPACK's dgetrf pivots) on the matrix and an identity
+// in LDS. ym, d: column-major nl x nl; d holds (T+1) x 3 matrices, stage-major.
+constexpr int SI_LD = MAXLEV + 1;  // LDS row pitch (odd, against bank conflicts)
+__global__ __launch_bounds__(256) void k_semimp_inverse(const double *__restrict__ ym, double c0, double c1,
+                                                        double c2, int nl, int ntrunc, double *__restrict__ d,
+                                                        int *__restrict__ fail) {
+    __shared__ double a[MAXLEV][SI_LD], b[MAXLEV][SI_LD];
+    __shared__ double fr[MAXLEV];
+    __shared__ int piv;
+    const int n = blockIdx.x, st = blockIdx.y, tid = threadIdx.x;
+    const double c = st == 0 ? c0 : st == 1 ? c1 : c2, f = c * c * (double)n * (n + 1);
+    for (int e = tid; e < nl * nl; e += blockDim.x) {
+        const int r = e % nl, j = e / nl;
+        a[r][j] = (r == j ? 1.0 : 0.0) + f * ym[r + (long)j * nl];
+        b[r][j] = r == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int col = 0; col < nl; col++) {
+        if (tid < 64) {
+            // first row of largest |a(r, col)|, r >= col: wavefront argmax, ties to the lower row
+            double v = -1.0;
+            int r = nl;
+            if (col + tid < nl) {
+                v = fabs(a[col + tid][col]);
+                r = col + tid;
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v2 = __shfl_down(v, o);
+                const int r2 = __shfl_down(r, o);
+                if (v2 > v || (v2 == v && r2 < r)) {
+                    v = v2;
+                    r = r2;
+                }
+            }
+            if (tid == 0) {
+                piv = r;
+                if (!(v > 0.0)) *fail = 1;
+            }
+        }
+        __syncthreads();
+        const int p = piv;
+        if (p != col)
+            for (int j = tid; j < nl; j += blockDim.x) {
+                double t = a[col][j];
+                a[col][j] = a[p][j];
+                a[p][j] = t;
+                t = b[col][j];
+                b[col][j] = b[p][j];
+                b[p][j] = t;
+            }
+        __syncthreads();
+        const double inv = 1.0 / a[col][col];
+        __syncthreads();
+        for (int j = tid; j < nl; j += blockDim.x) {
+            a[col][j] *= inv;
+            b[col][j] *= inv;
+        }
+        for (int r = tid; r < nl; r += blockDim.x) fr[r] = a[r][col];
+        __syncthreads();
+        for (int e = tid; e < nl * nl; e += blockDim.x) {
+            const int r = e / nl, j = e % nl;
+            if (r == col) continue;
+            const double fq = fr[r];
+            a[r][j] -= fq * a[col][j];
+            b[r][j] -= fq * b[col][j];
+        }
+        __syncthreads();
+    }
+    double *D = d + ((long)st * (ntrunc + 1) + n) * nl * nl;
+    for (int e = tid; e < nl * nl; e += blockDim.x) {
+        const int r = e % nl, j = e / nl;
+        D[r + (long)j * nl] = b[r][j];
+    }
+}
+
+// solver operator layout: per (stage, degree) matrix, transposed and at precision T
+template <typename T>
+__global__ void k_semimp_pack(const double *__restrict__ d, T *__restrict__ dt3, int nl, long nmat) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nmat * nl * nl) return;
+    const long b = e / (nl * nl);
+    const int q = (int)(e % (nl * nl)), i = q % nl, j = q / nl;
+    dt3[b * nl * nl + (long)j * nl + i] = (T)d[b * nl * nl + i + (long)j * nl];
+}
+
 // ---------------------------------------------------------------------------
 // host side
-
-static int invert_fp64(int n, std::vector<double> &a) {
-    std::vector<double> b(n * n, 0.0);
-    for (int i = 0; i < n; i++) b[i + i * n] = 1.0;
-    for (int c = 0; c < n; c++) {
-        int piv = c;
-        for (int r = c + 1; r < n; r++)
-            if (std::fabs(a[r + c * n]) > std::fabs(a[piv + c * n])) piv = r;
-        if (a[piv + c * n] == 0.0) return -1;
-        if (piv != c)
-            for (int j = 0; j < n; j++) {
-                std::swap(a[c + j * n], a[piv + j * n]);
-                std::swap(b[c + j * n], b[piv + j * n]);
-            }
-        double d = 1.0 / a[c + c * n];
-        for (int j = 0; j < n; j++) {
-            a[c + j * n] *= d;
-            b[c + j * n] *= d;
-        }
-        for (int r = 0; r < n; r++) {
-            if (r == c) continue;
-            double f = a[r + c * n];
-            if (f == 0.0) continue;
-            for (int j = 0; j < n; j++) {
-                a[r + j * n] -= f * a[c + j * n];
-                b[r + j * n] -= f * b[c + j * n];
-            }
-        }
-    }
-    a = b;
-    return 0;
-}
 
 template <typename T> static T *dalloc(size_t n) {
     T *p = nullptr;
@@ -1789,7 +1898,7 @@ template <typename T> int Engine<T>::dyn_init() {
     B->ng = (long)plan.nlat * plan.nlon;
     const long ng = B->ng, NFS = (long)B->nfs * nm;
     // ---- one-off operators in FP64 on the host (semimp_data.f90:127-256, dyn_init.f90:450-494)
-    const double rd = cfg.rd, kappa = cfg.rd / cfg.cp, rerth = cfg.rerth, dt = cfg.dt;
+    const double rd = cfg.rd, kappa = cfg.rd / cfg.cp, rerth = cfg.rerth;
     std::vector<double> dbk(nl), ck(nl), si(nl + 1);
     B->sl.resize(nl);
     for (int k = 1; k <= nl; k++) {
@@ -1832,19 +1941,7 @@ template <typename T> int Engine<T>::dyn_init() {
                 for (int k = 1; k <= nl; k++) s += CM(B->amhyb, i, k) * CM(B->bmhyb, k, j);
                 CM(ym, i, j) = s;
             }
-        B->d_hyb_m.assign((size_t)nl * nl * (T_ + 1) * 3, 0.0);
-        for (int s = 0; s < 3; s++)
-            for (int n = 0; n <= T_; n++) {
-                std::vector<double> m(nl * nl);
-                double c = TAB_IMP[s] * dt, rnn1 = (double)n * (n + 1);
-                for (int i = 1; i <= nl; i++)
-                    for (int j = 1; j <= nl; j++) CM(m, i, j) = (i == j ? 1.0 : 0.0) + c * c * rnn1 * CM(ym, i, j);
-                if (invert_fp64(nl, m)) {
-                    set_error("singular semi-implicit matrix");
-                    return GFSDC_EINVAL;
-                }
-                std::copy(m.begin(), m.end(), B->d_hyb_m.begin() + ((size_t)s * (T_ + 1) + n) * nl * nl);
-            }
+        B->ym = ym;
 #undef CM
     }
     {  // setdampspec
@@ -1893,12 +1990,39 @@ template <typename T> int Engine<T>::dyn_init() {
     if ((rc = dupload<T>(&B->at, transp(B->amhyb, 0), stream))) return rc;
     if ((rc = dupload<T>(&B->bt, transp(B->bmhyb, 0), stream))) return rc;
     {
-        std::vector<T> d3((size_t)nl * nl * (T_ + 1) * 3);
-        for (size_t b = 0; b < (size_t)(T_ + 1) * 3; b++) {
-            std::vector<T> t = transp(B->d_hyb_m, b * nl * nl);
-            std::copy(t.begin(), t.end(), d3.begin() + b * nl * nl);
+        // the implicit operators, inverted in FP64 on the device (k_semimp_inverse),
+        // then packed for the solver at precision T; the FP64 matrices are kept
+        // on the host for gfsdc_get_semimp
+        const size_t nmat = (size_t)(T_ + 1) * 3, nd = nmat * nl * nl;
+        double *dym = nullptr, *dd = nullptr;
+        int *dfail = nullptr;
+        if ((rc = dupload<double>((void **)&dym, B->ym, stream))) return rc;
+        dd = dalloc<double>(nd);
+        dfail = dalloc<int>(1);
+        B->dt3 = dalloc<T>(nd);
+        if (!dd || !dfail || !B->dt3) {
+            set_error("device allocation failed");
+            return GFSDC_ENOMEM;
         }
-        if ((rc = dupload<T>(&B->dt3, d3, stream))) return rc;
+        GFSDC_CHECK(hipMemsetAsync(dfail, 0, sizeof(int), stream));
+        const double dt = cfg.dt;
+        hipLaunchKernelGGL(k_semimp_inverse, dim3(T_ + 1, 3), dim3(256), 0, stream, dym, TAB_IMP[0] * dt,
+                           TAB_IMP[1] * dt, TAB_IMP[2] * dt, nl, T_, dd, dfail);
+        hipLaunchKernelGGL(k_semimp_pack<T>, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, stream,
+                           (const double *)dd, (T *)B->dt3, nl, (long)nmat);
+        GFSDC_CHECK(hipGetLastError());
+        int fail = 0;
+        B->d_hyb_m.resize(nd);
+        GFSDC_CHECK(hipMemcpyAsync(B->d_hyb_m.data(), dd, sizeof(double) * nd, hipMemcpyDeviceToHost, stream));
+        GFSDC_CHECK(hipMemcpyAsync(&fail, dfail, sizeof(int), hipMemcpyDeviceToHost, stream));
+        GFSDC_CHECK(hipStreamSynchronize(stream));
+        (void)hipFree(dym);
+        (void)hipFree(dd);
+        (void)hipFree(dfail);
+        if (fail) {
+            set_error("singular semi-implicit matrix");
+            return GFSDC_EINVAL;
+        }
     }
     if ((rc = dupload<T>(&B->sv, castv<T>(B->svhyb), stream))) return rc;
     if ((rc = dupload<T>(&B->tor, castv<T>(B->tor_hyb), stream))) return rc;
@@ -1973,6 +2097,7 @@ template <typename T> int Engine<T>::dyn_init() {
     B->pspart = dalloc<double>((size_t)nm * PS_BLOCKS * 2);
     B->pwsum = dalloc<double>(nm);
     B->diag = dalloc<double>(4 * nm);
+    B->clock = dalloc<double>(1);
     B->colq = dalloc<T>((size_t)nm * ng);
     B->pwat = dalloc<T>((size_t)nm * ng);
     B->precip = dalloc<T>((size_t)nm * ng);
@@ -1980,7 +2105,7 @@ template <typename T> int Engine<T>::dyn_init() {
     void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
                     B->tphys, B->topo, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
                     B->dphisdy, B->pdryini, B->pmean, B->diag, B->pspart, B->pwsum, B->colq, B->pwat,
-                    B->precip, B->pblscr, B->gmean};
+                    B->precip, B->pblscr, B->gmean, B->clock};
     for (void *p : ptrs)
         if (!p) {
             set_error("device allocation failed for the dycore state");
@@ -1991,6 +2116,7 @@ template <typename T> int Engine<T>::dyn_init() {
     GFSDC_CHECK(hipMemsetAsync(B->dphisdx, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->dphisdy, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->pdryini, 0, sizeof(double) * nm, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->clock, 0, sizeof(double), stream));
     GFSDC_CHECK(hipMemsetAsync(B->pwat, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->precip, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->pwsum, 0, sizeof(double) * nm, stream));
@@ -2166,7 +2292,8 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
-                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph};
+                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph,
+                    B->clock, B->iau_hrs, B->iau_inc, B->iau_F};
     if (B->mlp) mlp_destroy(B->mlp);
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -2251,6 +2378,21 @@ template <typename T> struct Step {
         // dyn_run.f90:135-153 on this stage's psg (and column tracer mass)
         if (stage == 0) area_sums(B.ntr > 0 ? B.colq : nullptr, nullptr, nullptr, true);
         analyse(B.adesc_dyn, B.rflag_an, B.nm * B.nan, (cplx<T> *)B.ten[stage]);
+        // run.f90:302-309,363-370,429-436: the IAU forcing joins every stage's
+        // tendencies before the linear terms come off
+        if (B.iau_n) {
+            const long n = 2L * E.plan.nlm * B.nfs * B.nm;
+            hipLaunchKernelGGL(k_axpby<T>, dim3(1024), dim3(256), 0, s, (T *)B.ten[stage], (const T *)B.iau_F,
+                               (T)1, (T)1, n);
+            mark("iau_add");
+        }
+    }
+    // run.f90:250-258: the IAU forcing of this step, at the step's start time
+    void iau_forcing() {
+        const long len = 2L * E.plan.nlm * B.nfs * B.nm;
+        hipLaunchKernelGGL(k_iau_forcing<T>, dim3(1024), dim3(256), 0, s, (T *)B.iau_F, (const T *)B.iau_inc,
+                           (const double *)B.iau_hrs, B.iau_n, B.iau_dt, (const double *)B.clock, len);
+        mark("iau_forcing");
     }
     void si_stage(int stage, bool damp) {
         const gfsdc_cfg &c = E.cfg;
@@ -2399,6 +2541,7 @@ template <typename T> struct Step {
     }
     void step() {
         const bool pat = B.tpat != nullptr;
+        if (B.iau_n) iau_forcing();
         // run.f90:272-292: the random pattern used this step
         if (pat) pattern_advance(B.tpat_phi, E.cfg.tpat_amp * std::sqrt(1.0 - B.tpat_phi * B.tpat_phi));
         long n = 2L * E.plan.nlm * B.nfs * B.nm;
@@ -2411,6 +2554,7 @@ template <typename T> struct Step {
         if (E.cfg.physics != GFSDC_PHYS_NONE) physics();
         // stochastic temperature forcing, where run.f90:539-551 adds its tvforcing
         if (pat) pattern_apply();
+        hipLaunchKernelGGL(k_clock_add, dim3(1), dim3(64), 0, s, B.clock, E.cfg.dt);
     }
 };
 
@@ -2936,6 +3080,293 @@ template <typename T> int Engine<T>::advance(int nsteps, int use_graph) {
     }
     for (int i = 0; i < nsteps; i++) GFSDC_CHECK(hipGraphLaunch(B->gexec, stream));
     return 0;
+}
+
+// drop the captured step: the next graph advance captures it again
+static void drop_graph(DycoreBuffers *B) {
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->gexec = nullptr;
+    B->graph = nullptr;
+}
+
+// run.f90:53 t = tstart (params.f90: tstart = fhour of the initial file, in s)
+template <typename T> int Engine<T>::set_time(double t) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(B->clock, &t, sizeof(double), hipMemcpyHostToDevice, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> int Engine<T>::get_time(double *t) {
+    DycoreBuffers *B = dyn;
+    if (!B || !t) {
+        set_error(!B ? "context has no dycore" : "null argument");
+        return !B ? GFSDC_ESTATE : GFSDC_EINVAL;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(t, B->clock, sizeof(double), hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// init_iau (iau_module.f90:20-81) with the increments already formed by the
+// caller (analysis - first guess); each field has a trailing nfiles dimension
+// after the set_state layout. nfiles = 0 switches the forcing off.
+template <typename T>
+int Engine<T>::iau_set(int nfiles, const double *fhrs, double delthrs, const void *vrt, const void *div,
+                       const void *tv, const void *trac, const void *lnps, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    void *old[] = {B->iau_hrs, B->iau_inc, B->iau_F};
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    B->iau_hrs = nullptr;
+    B->iau_inc = B->iau_F = nullptr;
+    B->iau_n = 0;
+    drop_graph(B);
+    if (nfiles == 0) return 0;
+    // iau_module.f90:38-51: two or more files at a constant spacing
+    if (nfiles < 2 || !fhrs) {
+        set_error("iau needs at least two increment times");
+        return GFSDC_EINVAL;
+    }
+    for (int n = 1; n < nfiles; n++)
+        if (!(fhrs[n] > fhrs[n - 1]) || fhrs[n] - fhrs[n - 1] != fhrs[1] - fhrs[0]) {
+            set_error("iau increment times must increase at a constant spacing");
+            return GFSDC_EINVAL;
+        }
+    if (!(delthrs > 0)) {
+        set_error("iau_delthrs must be > 0");
+        return GFSDC_EINVAL;
+    }
+    if (!vrt || !div || !tv || !lnps || (B->ntr > 0 && !trac)) {
+        set_error("iau increments need vrt, div, tv, lnps (and trac when ntrac > 0)");
+        return GFSDC_EINVAL;
+    }
+    const int nl = B->nl, nlm = plan.nlm, nm = B->nm;
+    const long len = (long)nlm * B->nfs * nm;  // complex values per state
+    int rc;
+    if ((rc = dupload<double>((void **)&B->iau_hrs, std::vector<double>(fhrs, fhrs + nfiles), stream))) return rc;
+    B->iau_inc = dalloc<cplx<T>>((size_t)len * nfiles);
+    B->iau_F = dalloc<cplx<T>>((size_t)len);
+    if (!B->iau_inc || !B->iau_F) {
+        set_error("device allocation failed for the iau increments");
+        return GFSDC_ENOMEM;
+    }
+    const size_t esz = dtype == GFSDC_DT_F32 ? 4 : 8;
+    struct Part {
+        const void *p;
+        int nfr, off;
+    } parts[] = {{vrt, nl, 0}, {div, nl, nl}, {tv, nl, 2 * nl}, {trac, B->ntr * nl, 3 * nl},
+                 {lnps, 1, 3 * nl + B->ntr * nl}};
+    for (auto &pt : parts) {
+        if (pt.nfr == 0) continue;
+        const size_t nper = (size_t)pt.nfr * nm * nlm * 2;  // reals per file
+        for (int f = 0; f < nfiles; f++) {
+            const T *in;
+            if ((rc = input((const char *)pt.p + f * nper * esz, nper, dtype, loc, 0, &in))) return rc;
+            long tot = (long)pt.nfr * nm * nlm;
+            hipLaunchKernelGGL(k_pack<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
+                               (cplx<T> *)B->iau_inc + (long)f * len, (const cplx<T> *)in, pt.nfr, pt.off, B->nfs,
+                               nm, nlm, 1);
+            GFSDC_CHECK(hipGetLastError());
+            GFSDC_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+    B->iau_n = nfiles;
+    B->iau_dt = delthrs * 3600.;
+    return 0;
+}
+
+// set_dfi_wts (run.f90:567-587): Lanczos-windowed sinc over the 2 N + 1
+// states of the window, normalised to sum 1 (pi the default-real acos(-1.))
+static std::vector<double> dfi_weights(int N) {
+    const double pi = (double)(float)M_PI;
+    std::vector<double> w(2 * N + 1, 1.0);
+    double tot = 0.0;
+    for (int k = 0; k <= 2 * N; k++) {
+        if (k != N) {
+            const double sx = pi * (k - N) / N, wx = pi * (k - N) / (N + 1);
+            w[k] = std::sin(wx) / wx * std::sin(sx) / sx;
+        }
+        tot += w[k];
+    }
+    for (double &x : w) x /= tot;
+    return w;
+}
+
+// Digital filter initialisation, run.f90:58-145: 2 ntdfi steps with the
+// weighted average of the window's 2 ntdfi + 1 states accumulated on the
+// device, the state replaced by it and the clock set to the window's middle.
+template <typename T> int Engine<T>::dfi(int ntdfi, int use_graph) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    if (ntdfi < 1) {
+        set_error("ntdfi must be >= 1");
+        return GFSDC_EINVAL;
+    }
+    const std::vector<double> w = dfi_weights(ntdfi);
+    const long n = 2L * plan.nlm * B->nfs * B->nm;
+    double t0;
+    int rc;
+    if ((rc = get_time(&t0))) return rc;
+    T *acc = dalloc<T>((size_t)n);
+    if (!acc) {
+        set_error("device allocation failed for the dfi average");
+        return GFSDC_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_axpby<T>, dim3(1024), dim3(256), 0, stream, acc, (const T *)B->spec, (T)w[0], (T)0, n);
+    for (int nt = 1; nt <= 2 * ntdfi && !rc; nt++) {
+        rc = advance(1, use_graph);
+        hipLaunchKernelGGL(k_axpby<T>, dim3(1024), dim3(256), 0, stream, acc, (const T *)B->spec, (T)w[nt], (T)1,
+                           n);
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(k_copy<T>, dim3(2048), dim3(256), 0, stream, (T *)B->spec, (const T *)acc, n);
+        if (hipGetLastError() != hipSuccess) rc = GFSDC_EHIP;
+    }
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(acc);
+    if (rc) {
+        if (rc == GFSDC_EHIP) set_error("dfi kernel launch failed");
+        return rc;
+    }
+    return set_time(t0 + ntdfi * cfg.dt);
+}
+
+// ---- checkpoint / resume ----------------------------------------------------
+// One contiguous image of everything the next steps read that the caller did
+// not configure: the spectral state, orography, pdryini, the clock, the
+// stochastic pattern and its counter, the judge counter. Operators, the iau
+// increments and learned weights are configuration and are not included.
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int32_t version, esz, nlm, nfs, nm, nl, ntr, has_pattern, has_judge, pad0;
+    double time;
+    uint64_t tpat_ctr, jctr, bctr;
+    char pad[128 - 8 - 11 * 4 - 8 - 3 * 8];
+};
+static_assert(sizeof(CkptHeader) == 128, "checkpoint header is 128 bytes");
+const char CKPT_MAGIC[8] = {'G', 'F', 'S', 'D', 'C', 'C', 'K', '1'};
+}  // namespace
+
+template <typename T> size_t Engine<T>::checkpoint_size() {
+    DycoreBuffers *B = dyn;
+    if (!B) return 0;
+    const size_t c = sizeof(cplx<T>), nlm = plan.nlm, nm = B->nm;
+    return sizeof(CkptHeader) + c * nlm * B->nfs * nm + c * nlm * nm + 8 * nm + (B->tpat ? c * nlm * nm : 0);
+}
+
+template <typename T> int Engine<T>::checkpoint_save(void *buf, size_t nbytes, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    if (!buf || nbytes < checkpoint_size()) {
+        set_error("checkpoint buffer is smaller than gfsdc_checkpoint_size()");
+        return GFSDC_EINVAL;
+    }
+    CkptHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, CKPT_MAGIC, 8);
+    h.version = 1;
+    h.esz = (int32_t)sizeof(T);
+    h.nlm = plan.nlm;
+    h.nfs = B->nfs;
+    h.nm = B->nm;
+    h.nl = B->nl;
+    h.ntr = B->ntr;
+    h.has_pattern = B->tpat ? 1 : 0;
+    h.has_judge = B->jctr ? 1 : 0;
+    h.bctr = B->bctr;
+    GFSDC_CHECK(hipMemcpyAsync(&h.time, B->clock, sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (B->tpat_ctr)
+        GFSDC_CHECK(hipMemcpyAsync(&h.tpat_ctr, B->tpat_ctr, 8, hipMemcpyDeviceToHost, stream));
+    if (B->jctr) GFSDC_CHECK(hipMemcpyAsync(&h.jctr, B->jctr, 8, hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    const hipMemcpyKind k = loc == GFSDC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    char *o = (char *)buf;
+    GFSDC_CHECK(hipMemcpyAsync(o, &h, sizeof(h), loc == GFSDC_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost,
+                               stream));
+    o += sizeof(h);
+    const size_t c = sizeof(cplx<T>), nlm = plan.nlm, nm = B->nm;
+    struct Sec {
+        const void *p;
+        size_t n;
+    } secs[] = {{B->spec, c * nlm * B->nfs * nm}, {B->topo, c * nlm * nm}, {B->pdryini, 8 * nm},
+                {B->tpat, B->tpat ? c * nlm * nm : 0}};
+    for (auto &sc : secs) {
+        if (!sc.n) continue;
+        GFSDC_CHECK(hipMemcpyAsync(o, sc.p, sc.n, k, stream));
+        o += sc.n;
+    }
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> int Engine<T>::checkpoint_load(const void *buf, size_t nbytes, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    CkptHeader h;
+    if (!buf || nbytes < sizeof(h)) {
+        set_error("checkpoint buffer too small");
+        return GFSDC_EINVAL;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(&h, buf, sizeof(h), loc == GFSDC_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost,
+                               stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    if (std::memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 1) {
+        set_error("not a gfsdc checkpoint (bad magic or version)");
+        return GFSDC_EINVAL;
+    }
+    if (h.esz != (int32_t)sizeof(T) || h.nlm != plan.nlm || h.nfs != B->nfs || h.nm != B->nm || h.nl != B->nl ||
+        h.ntr != B->ntr || h.has_pattern != (B->tpat ? 1 : 0)) {
+        set_error("checkpoint was written by a context of another precision, grid, level count, tracer "
+                  "count, member count or pattern setting");
+        return GFSDC_EINVAL;
+    }
+    if (nbytes < checkpoint_size()) {
+        set_error("checkpoint buffer is shorter than its header says");
+        return GFSDC_EINVAL;
+    }
+    const hipMemcpyKind k = loc == GFSDC_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const char *o = (const char *)buf + sizeof(h);
+    const size_t c = sizeof(cplx<T>), nlm = plan.nlm, nm = B->nm;
+    int rc = ensure_stage(2, c * nlm * nm);
+    if (rc) return rc;
+    struct Sec {
+        void *p;
+        size_t n;
+    } secs[] = {{B->spec, c * nlm * B->nfs * nm}, {stage[2], c * nlm * nm}, {B->pdryini, 8 * nm},
+                {B->tpat, B->tpat ? c * nlm * nm : 0}};
+    for (auto &sc : secs) {
+        if (!sc.n) continue;
+        GFSDC_CHECK(hipMemcpyAsync(sc.p, o, sc.n, k, stream));
+        o += sc.n;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(B->clock, &h.time, sizeof(double), hipMemcpyHostToDevice, stream));
+    if (B->tpat_ctr) GFSDC_CHECK(hipMemcpyAsync(B->tpat_ctr, &h.tpat_ctr, 8, hipMemcpyHostToDevice, stream));
+    if (B->jctr && h.has_judge) GFSDC_CHECK(hipMemcpyAsync(B->jctr, &h.jctr, 8, hipMemcpyHostToDevice, stream));
+    B->bctr = h.bctr;
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    // orography and its gradient (dyn_init.f90:84)
+    return set_topo(stage[2], sizeof(T) == 8 ? GFSDC_DT_F64 : GFSDC_DT_F32, GFSDC_DEVICE);
 }
 
 // tendencies of a physics package on the current grid fields, without the

@@ -89,6 +89,14 @@ template <typename T> struct Engine {
     int set_pdryini(const double *p);
     int get_pdryini(double *p);
     int init_rest_state(double t0, double ps0, double vort_rms, int nmin, int nmax, unsigned long long seed);
+    int set_time(double t);
+    int get_time(double *t);
+    int iau_set(int nfiles, const double *fhrs, double delthrs, const void *vrt, const void *div, const void *tv,
+                const void *trac, const void *lnps, int dtype, int loc);
+    int dfi(int ntdfi, int use_graph);
+    size_t checkpoint_size();
+    int checkpoint_save(void *buf, size_t nbytes, int loc);
+    int checkpoint_load(const void *buf, size_t nbytes, int loc);
     int profile_step(char names[][48], double *ms, int nmax);
 };
 

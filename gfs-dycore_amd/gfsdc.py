@@ -56,7 +56,9 @@ SYMBOLS = [
     "gfsdc_mlp_normalize", "gfsdc_mlp_train", "gfsdc_mlp_eval", "gfsdc_mlp_enable", "gfsdc_mlp_judge_stats",
     "gfsdc_mlp_nparams", "gfsdc_mlp_params", "gfsdc_mlp_forward", "gfsdc_mlp_grad", "gfsdc_mlp_adamw",
     "gfsdc_get_grid", "gfsdc_get_semimp",
-    "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_sync",
+    "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_set_time",
+    "gfsdc_get_time", "gfsdc_iau_set", "gfsdc_dfi", "gfsdc_checkpoint_size", "gfsdc_checkpoint_save",
+    "gfsdc_checkpoint_load", "gfsdc_sync",
     "gfsdc_stream", "gfsdc_profile_step",
 ]
 
@@ -116,6 +118,13 @@ def load():
     L.gfsdc_get_damp.argtypes = [vp, vp, vp, vp]
     L.gfsdc_step_diag.argtypes = [vp, vp]
     L.gfsdc_init_rest_state.argtypes = [vp, d, d, d, i, i, C.c_ulonglong]
+    L.gfsdc_set_time.argtypes = [vp, d]
+    L.gfsdc_get_time.argtypes = [vp, C.POINTER(d)]
+    L.gfsdc_iau_set.argtypes = [vp, i, vp, d, vp, vp, vp, vp, vp, i, i]
+    L.gfsdc_dfi.argtypes = [vp, i, i]
+    L.gfsdc_checkpoint_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.gfsdc_checkpoint_save.argtypes = [vp, vp, C.c_size_t, i]
+    L.gfsdc_checkpoint_load.argtypes = [vp, vp, C.c_size_t, i]
     L.gfsdc_sync.argtypes = [vp]
     L.gfsdc_stream.argtypes = [vp]
     L.gfsdc_stream.restype = vp
@@ -254,8 +263,14 @@ class Context:
         return dict(vrt=vrt, div=div, tv=tv, trac=trac[:, :self.ntrac], lnps=lnps)
 
     def set_topo(self, topo):
-        t = np.ascontiguousarray(topo, np.complex128)
+        t = np.ascontiguousarray(np.broadcast_to(topo, (self.nmem, self.nlm)), np.complex128)
         _check(load().gfsdc_set_topo(self.h, _p(t), DT_F64, HOST))
+        self._topo = t.copy()
+
+    def topo(self):
+        """spectral orography (nmem, nlm) as last set or restored"""
+        t = getattr(self, "_topo", None)
+        return np.zeros((self.nmem, self.nlm), np.complex128) if t is None else t.copy()
 
     @property
     def pdryini(self):
@@ -407,6 +422,90 @@ class Context:
 
     def init_rest_state(self, t0=300.0, ps0=1.0e5, vort_rms=1.0e-6, nmin=1, nmax=21, seed=0):
         _check(load().gfsdc_init_rest_state(self.h, t0, ps0, vort_rms, nmin, nmax, seed))
+
+    @property
+    def time(self):
+        """model clock (s) at the start of the next step"""
+        t = C.c_double(0.0)
+        _check(load().gfsdc_get_time(self.h, C.byref(t)))
+        return t.value
+
+    @time.setter
+    def time(self, t):
+        _check(load().gfsdc_set_time(self.h, float(t)))
+
+    def iau_set(self, fhrs, delthrs, vrt=None, div=None, tv=None, lnps=None, trac=None):
+        """IAU increments (analysis - first guess) at the hours fhrs: vrt, div,
+        tv (nfiles, nmem, nlev, nlm), trac (nfiles, nmem, ntrac, nlev, nlm),
+        lnps (nfiles, nmem, nlm); fhrs empty switches the forcing off."""
+        nf = len(fhrs)
+        if nf == 0:
+            _check(load().gfsdc_iau_set(self.h, 0, None, 0.0, None, None, None, None, None, DT_F64,
+                                        HOST))
+            return
+        h = np.ascontiguousarray(fhrs, np.float64)
+        a = [None if x is None else np.ascontiguousarray(x, np.complex128)
+             for x in (vrt, div, tv, trac if self.ntrac else None, lnps)]
+        _check(load().gfsdc_iau_set(self.h, nf, _p(h), float(delthrs), *[_p(x) for x in a], DT_F64,
+                                    HOST))
+
+    def dfi(self, ntdfi, use_graph=False):
+        _check(load().gfsdc_dfi(self.h, ntdfi, int(use_graph)))
+
+    def checkpoint(self):
+        """the device state as one byte image (numpy uint8)"""
+        n = C.c_size_t(0)
+        _check(load().gfsdc_checkpoint_size(self.h, C.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        _check(load().gfsdc_checkpoint_save(self.h, _p(buf), n.value, HOST))
+        return buf
+
+    def restore(self, buf):
+        buf = np.ascontiguousarray(buf, np.uint8)
+        _check(load().gfsdc_checkpoint_load(self.h, _p(buf), buf.size, HOST))
+        # the orography section follows the 128-byte header and the spectral state
+        c = np.complex128 if self.precision == FP64 else np.complex64
+        nfs = self.nlev * (3 + self.ntrac) + 1
+        off = 128 + np.dtype(c).itemsize * self.nlm * nfs * self.nmem
+        n = self.nlm * self.nmem
+        self._topo = buf[off:off + n * np.dtype(c).itemsize].view(c).astype(np.complex128).reshape(
+            self.nmem, self.nlm)
+
+    def read_sigma(self, path):
+        """readin_sig and init_dyn (dyn_init.f90:43-57,120-149) with params.f90:
+        241-264: the state, orography, pdryini and clock of every member from a
+        sigma file written for this context's grid, truncation and levels."""
+        import sigio
+        head, data = sigio.read(path)
+        if (head["jcap"], head["levs"]) != (self.ntrunc, self.nlev) or head["ntrac"] < self.ntrac:
+            raise GfsdcError(f"{path}: T{head['jcap']}L{head['levs']} with {head['ntrac']} tracers, "
+                             f"context is T{self.ntrunc}L{self.nlev} with {self.ntrac}")
+        st = sigio.model_state(head, data, self)
+        tol = 1e-6 * max(np.abs(self._ak).max(), 1.0)
+        if np.abs(st["ak"] - self._ak).max() > tol or np.abs(st["bk"] - self._bk).max() > 1e-6:
+            raise GfsdcError(f"{path}: vertical coordinate differs from the context's ak, bk")
+        b = lambda x: np.ascontiguousarray(np.broadcast_to(x, (self.nmem,) + x.shape))
+        self.set_state(b(st["vrt"]), b(st["div"]), b(st["tv"]), b(st["lnps"]),
+                       b(st["trac"][:self.ntrac]) if self.ntrac else None)
+        self.set_topo(st["topo"])
+        self.pdryini = st["pdryini"]
+        self.time = st["tstart"]
+        return head
+
+    def write_sigma(self, path, member=0, head=None):
+        """wrtout_sig (dyn_init.f90:151-191): member's state at the current
+        clock; head (a header as sigio.read returns) supplies the fields the
+        model does not own, e.g. idate, else a modern hybrid header is made."""
+        import sigio
+        st = self.get_state()
+        state = dict(vrt=st["vrt"][member], div=st["div"][member], tv=st["tv"][member],
+                     trac=st["trac"][member], lnps=st["lnps"][member], topo=self.topo()[member])
+        if head is None:
+            head = sigio.header_for(self.nlon, self.nlat, self.ntrunc, self._ak, self._bk, self.ntrac)
+        head = dict(head)
+        head["fhour"] = self.time / 3600.0
+        head["pdryini"] = float(self.pdryini[member]) / 1000.0
+        sigio.write(path, head, sigio.sigma_data(state, self))
 
     def sync(self):
         _check(load().gfsdc_sync(self.h))

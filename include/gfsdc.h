@@ -271,6 +271,32 @@ int gfsdc_step_diag(gfsdc_ctx *ctx, double *out /* 4 * nmem */);
  * vorticity of vort_rms (s^-1), member i seeded with seed + i */
 int gfsdc_init_rest_state(gfsdc_ctx *ctx, double t0, double ps0, double vort_rms, int nmin,
                           int nmax, unsigned long long seed);
+/* model clock (s) at the start of the next step: run.f90:53 t = tstart
+ * (params.f90: the initial file's fhour * 3600), advanced by dt per step
+ * (run.f90:153); it times the IAU forcing */
+int gfsdc_set_time(gfsdc_ctx *ctx, double t);
+int gfsdc_get_time(gfsdc_ctx *ctx, double *t);
+/* replaces iau_module.f90:20 init_iau (the increments analysis - first guess,
+ * formed by the caller from the sigma files) and the getiauforcing calls of
+ * run.f90:250-258: nfiles >= 2 increment times fhrs (hours, evenly spaced),
+ * scaled by 1 / (delthrs h); each field in the set_state layout with a
+ * trailing nfiles dimension. nfiles = 0 switches the forcing off. */
+int gfsdc_iau_set(gfsdc_ctx *ctx, int nfiles, const double *fhrs, double delthrs, const void *vrt,
+                  const void *div, const void *tv, const void *trac, const void *lnps, int dtype,
+                  int loc);
+/* replaces run.f90:58-145 (digital filter loop with set_dfi_wts :567-587):
+ * 2 ntdfi steps, the state replaced by the Lanczos-weighted average of the
+ * window, the clock set to the window's middle (tstart + ntdfi dt) */
+int gfsdc_dfi(gfsdc_ctx *ctx, int ntdfi, int use_graph);
+/* device checkpoint / resume (the restart that wrtout_sig / readin_sig give the
+ * reference, dyn_init.f90:120-191, without the file format's FP32 rounding):
+ * spectral state, orography, pdryini, clock, stochastic pattern and counters
+ * in one image of gfsdc_checkpoint_size() bytes, at loc (host or device).
+ * Loading checks that the image came from a context of the same precision,
+ * sizes and pattern setting; continuing from it is bit-identical. */
+int gfsdc_checkpoint_size(gfsdc_ctx *ctx, size_t *nbytes);
+int gfsdc_checkpoint_save(gfsdc_ctx *ctx, void *buf, size_t nbytes, int loc);
+int gfsdc_checkpoint_load(gfsdc_ctx *ctx, const void *buf, size_t nbytes, int loc);
 /* wait for all work on the context's stream */
 int gfsdc_sync(gfsdc_ctx *ctx);
 /* HIP stream the context launches on (hipStream_t as void*) */

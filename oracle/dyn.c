@@ -94,6 +94,13 @@ struct orc_model {
     double *tpat_var, *tpat_vfact, tpat_phi;
     uint64_t tpat_step;
     int *order;
+    /* model clock (s) at the start of the next step (run.f90:53,153) */
+    double time;
+    /* incremental analysis update (iau_module.f90): nfiles increments of the
+     * whole state, at iaufhrs hours, scaled by 1 / (iau_delthrs h) */
+    int iau_n;
+    double *iau_hrs, iau_dt;
+    ocplx *iau_inc; /* per file: vrt, div, tv (nl nlm), trac (ntrac nl nlm), lnps (nlm) */
 };
 
 #define L(X, k) ((X) + (long)((k) - 1) * m->ng)
@@ -430,7 +437,7 @@ void orc_model_destroy(orc_model *m) {
                     m->wk1, m->lnpsg, m->psg, m->dlnpsdt, m->dphisdx, m->dphisdy, m->dlnpsdx,
                     m->dlnpsdy, m->amhyb, m->bmhyb, m->svhyb, m->tor_hyb, m->d_hyb_m,
                     m->disspec, m->diff_prof, m->dmp_prof, m->precip, m->pwat, m->order,
-                    m->tpat, m->tpat_var, m->tpat_vfact};
+                    m->tpat, m->tpat_var, m->tpat_vfact, m->iau_hrs, m->iau_inc};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int i = 0; i < 10; i++) free(m->w[i]);
     for (int s = 0; s < 3; s++) { free(m->lin_d[s]); free(m->lin_t[s]); free(m->lin_p[s]); }
@@ -461,6 +468,75 @@ void orc_get_state(const orc_model *m, ocplx *vrt, ocplx *div, ocplx *tv, ocplx 
 
 void orc_set_pdryini(orc_model *m, double p) { m->pdryini = p; }
 double orc_get_pdryini(const orc_model *m) { return m->pdryini; }
+void orc_set_time(orc_model *m, double t) { m->time = t; }
+
+/* dyn_init.f90:84: the orography's geopotential gradient,
+ * getgrad(grav * topospec, dphisdx, dphisdy, rerth) */
+void orc_set_topo(orc_model *m, const ocplx *topo) {
+    ocplx *gt = malloc(sizeof(ocplx) * m->nlm);
+    for (int k = 0; k < m->nlm; k++) {
+        m->topo[k] = topo[k];
+        gt[k] = m->c.grav * topo[k];
+    }
+    orc_getgrad(m->sht, gt, m->dphisdx, m->dphisdy, 1, m->c.rerth);
+    free(gt);
+}
+double orc_get_time(const orc_model *m) { return m->time; }
+
+/* ---- iau_module.f90:20-125 ------------------------------------------------
+ * init_iau keeps, per file time iaufhrs(n), the increment analysis - first
+ * guess of every state field (:64-80); the files are 2 or more at a constant
+ * spacing (:38-51). inc holds the nfiles increments one after the other, each
+ * as vrt, div, tv (nl, nlm), trac (ntrac, nl, nlm), lnps (nlm). */
+static long state_len(const orc_model *m) { return (long)m->nlm * (m->nlev * (3 + m->ntrac) + 1); }
+
+int orc_iau_set(orc_model *m, int nfiles, const double *fhrs, double delthrs, const ocplx *inc) {
+    free(m->iau_hrs);
+    free(m->iau_inc);
+    m->iau_hrs = NULL;
+    m->iau_inc = NULL;
+    m->iau_n = 0;
+    if (nfiles == 0) return 0;
+    if (nfiles < 2) return -1;
+    for (int n = 1; n < nfiles; n++)
+        if (fhrs[n] - fhrs[n - 1] != fhrs[1] - fhrs[0] || !(fhrs[1] > fhrs[0])) return -1;
+    long len = state_len(m);
+    m->iau_hrs = malloc(sizeof(double) * nfiles);
+    m->iau_inc = malloc(sizeof(ocplx) * len * nfiles);
+    memcpy(m->iau_hrs, fhrs, sizeof(double) * nfiles);
+    memcpy(m->iau_inc, inc, sizeof(ocplx) * len * nfiles);
+    m->iau_n = nfiles;
+    m->iau_dt = delthrs * 3600.;
+    return 0;
+}
+
+/* getiauforcing (:84-115) at model time t: zero outside [iaufhrs(1),
+ * iaufhrs(nfiles)] h, the first or last increment / dt at those end points,
+ * else linear in time between the two files around t, ((1 - delt) inc(n) +
+ * delt inc(n-1)) / dt with n the first file later than t. Returns 0 when the
+ * forcing is zero. */
+static int iau_forcing(const orc_model *m, double t, ocplx *F) {
+    int nf = m->iau_n;
+    long len = state_len(m);
+    const double *h = m->iau_hrs;
+    if (nf == 0 || t < h[0] * 3600. || t > h[nf - 1] * 3600.) return 0;
+    double whi = 1.0, wlo = 0.0;
+    int n;
+    if (t == 3600. * h[nf - 1]) {
+        n = nf - 1;
+    } else if (t == 3600. * h[0]) {
+        n = 0;
+    } else {
+        for (n = 0; n < nf; n++)
+            if (h[n] * 3600. > t) break;
+        double delt = (h[n] - t / 3600.) / (h[n] - h[n - 1]);
+        whi = 1. - delt;
+        wlo = delt;
+    }
+    const ocplx *hi = m->iau_inc + (long)n * len, *lo = n > 0 ? hi - len : hi;
+    for (long i = 0; i < len; i++) F[i] = (whi * hi[i] + wlo * lo[i]) / m->iau_dt;
+    return 1;
+}
 
 static double area_sum(const orc_model *m, const double *x) {
     double s = 0.0;
@@ -856,7 +932,11 @@ void orc_advance(orc_model *m, int nsteps) {
     const double ca[3][3] = {{A21, 0, 0}, {A31, A32, 0}, {B1, B2, B3}};
     const double caa[3][3] = {{AA21, 0, 0}, {AA31, AA32, 0}, {BB1, BB2, BB3}};
     const double cimp[3] = {AA22, AA33, BB4};
+    long len = state_len(m);
+    ocplx *F = m->iau_n ? malloc(sizeof(ocplx) * len) : NULL;
     for (int step = 0; step < nsteps; step++) {
+        /* run.f90:250-258: the IAU forcing of this step, at its start time */
+        int iau = F ? iau_forcing(m, m->time, F) : 0;
         /* run.f90:272-292: advance the random pattern used this step */
         if (m->c.tpat_amp > 0) orc_pattern_advance(m);
         memcpy(vrt0, m->vrt, s3); memcpy(div0, m->div, s3); memcpy(tv0, m->tv, s3);
@@ -864,6 +944,20 @@ void orc_advance(orc_model *m, int nsteps) {
         memcpy(lp0, m->lnps, s1);
         for (int s = 0; s < 3; s++) {
             orc_getdyntend(m, s + 1, 0, dv[s], dd[s], dT[s], dq[s], dp[s]);
+            /* run.f90:302-309,363-370,429-436: the IAU forcing is added to
+             * every stage's tendencies (before the linear terms come off) */
+            if (iau) {
+                long s3n = (long)nlm * nl;
+                const ocplx *fv = F, *fd = F + s3n, *ft = F + 2 * s3n, *fq = F + 3 * s3n;
+                const ocplx *fp = F + (long)(3 + ntrac) * s3n;
+                for (long i = 0; i < s3n; i++) {
+                    dv[s][i] += fv[i];
+                    dd[s][i] += fd[i];
+                    dT[s][i] += ft[i];
+                }
+                for (long i = 0; i < s3n * ntrac; i++) dq[s][i] += fq[i];
+                for (int k = 0; k < nlm; k++) dp[s][k] += fp[k];
+            }
             /* vorticity and tracers: always explicit */
             for (long i = 0; i < (long)nlm * nl; i++) {
                 ocplx acc = 0.0;
@@ -945,7 +1039,9 @@ void orc_advance(orc_model *m, int nsteps) {
             for (int k = 0; k < nl; k++)
                 for (int n = 0; n < nlm; n++)
                     m->tv[(long)k * nlm + n] += dt * m->tpat_vfact[k] * m->tpat[n] / m->c.tpat_tau;
+        m->time += dt; /* run.f90:153 */
     }
+    free(F);
     free(vrt0); free(div0); free(tv0); free(tr0); free(lp0);
     for (int s = 0; s < 3; s++) {
         free(dv[s]); free(dd[s]); free(dT[s]); free(dq[s]); free(dp[s]);

@@ -105,6 +105,14 @@ void orc_get_state(const orc_model *m, ocplx *vrt, ocplx *div, ocplx *tv, ocplx 
                    ocplx *lnps);
 void orc_set_pdryini(orc_model *m, double pdryini);
 double orc_get_pdryini(const orc_model *m);
+void orc_set_time(orc_model *m, double t);
+/* spectral orography (nlm), dyn_init.f90:84 */
+void orc_set_topo(orc_model *m, const ocplx *topo);
+double orc_get_time(const orc_model *m);
+/* iau_module.f90 init_iau / getiauforcing: nfiles (>= 2, evenly spaced hours
+ * fhrs) increments, each vrt, div, tv (nlev, nlm), trac (ntrac, nlev, nlm),
+ * lnps (nlm) in turn; nfiles = 0 switches it off. Returns -1 on bad times. */
+int orc_iau_set(orc_model *m, int nfiles, const double *fhrs, double delthrs, const ocplx *inc);
 
 /* dyn_run.f90:33 getdyntend; outputs may be NULL when inverse_only */
 void orc_getdyntend(orc_model *m, int rkstage, int inverse_only, ocplx *dvrt, ocplx *ddiv,

@@ -74,6 +74,11 @@ def lib():
         L.orc_set_pdryini.argtypes = [vp, C.c_double]
         L.orc_get_pdryini.argtypes = [vp]
         L.orc_get_pdryini.restype = C.c_double
+        L.orc_set_time.argtypes = [vp, C.c_double]
+        L.orc_get_time.argtypes = [vp]
+        L.orc_get_time.restype = C.c_double
+        L.orc_iau_set.argtypes = [vp, C.c_int, vp, C.c_double, vp]
+        L.orc_set_topo.argtypes = [vp, vp]
         L.orc_getdyntend.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.orc_advance.argtypes = [vp, C.c_int]
         L.orc_hs_phytend.argtypes = [vp, _cp, _cp, _cp]
@@ -253,6 +258,56 @@ class Model:
     def pdryini(self, v):
         lib().orc_set_pdryini(self.h, float(v))
 
+    def set_topo(self, topo):
+        """spectral orography (m), dyn_init.f90:84"""
+        t = np.ascontiguousarray(topo, np.complex128)
+        lib().orc_set_topo(self.h, t.ctypes.data)
+
+    @property
+    def time(self):
+        """model clock (s) at the start of the next step"""
+        return lib().orc_get_time(self.h)
+
+    @time.setter
+    def time(self, t):
+        lib().orc_set_time(self.h, float(t))
+
+    def iau_set(self, fhrs, delthrs, vrt, div, tv, lnps, trac=None):
+        """Increments (analysis - first guess) at the hours fhrs: vrt, div, tv
+        (nfiles, nlev, nlm), lnps (nfiles, nlm), trac (nfiles, ntrac, nlev, nlm);
+        an empty fhrs switches the forcing off."""
+        nf = len(fhrs)
+        if nf == 0:
+            lib().orc_iau_set(self.h, 0, None, 0.0, None)
+            return
+        parts = [np.asarray(vrt).reshape(nf, -1), np.asarray(div).reshape(nf, -1),
+                 np.asarray(tv).reshape(nf, -1)]
+        if self.ntrac:
+            parts.append(np.asarray(trac).reshape(nf, -1))
+        parts.append(np.asarray(lnps).reshape(nf, -1))
+        inc = np.ascontiguousarray(np.concatenate(parts, axis=1), np.complex128)
+        h = np.ascontiguousarray(fhrs, np.float64)
+        if lib().orc_iau_set(self.h, nf, h.ctypes.data, float(delthrs), inc.ctypes.data) != 0:
+            raise ValueError("iau times must be two or more, increasing and evenly spaced")
+
+    def dfi(self, ntdfi):
+        """Digital filter initialisation of run.f90:58-145: 2 ntdfi steps,
+        the state replaced by the dfi_weights(ntdfi) average of the 2 ntdfi + 1
+        states of the window and the clock set to the window's middle."""
+        w = dfi_weights(ntdfi)
+        keys = ("vrt", "div", "tv", "trac", "lnps")
+        t0 = self.time
+        s = self.get_state()
+        acc = {k: w[0] * s[k] for k in keys}
+        for nt in range(1, 2 * ntdfi + 1):
+            self.advance(1)
+            s = self.get_state()
+            for k in keys:
+                acc[k] = acc[k] + w[nt] * s[k]
+        self.set_state(acc["vrt"], acc["div"], acc["tv"], acc["lnps"],
+                       acc["trac"] if self.ntrac else None)
+        self.time = t0 + ntdfi * self.cfg.dt
+
     def getdyntend(self, rkstage=1, inverse_only=False):
         s3 = (self.nlev, self.nlm)
         out = dict(dvrt=np.zeros(s3, np.complex128), ddiv=np.zeros(s3, np.complex128),
@@ -320,6 +375,21 @@ class Model:
         out = np.zeros(4)
         lib().orc_step_diag(self.h, out)
         return dict(spdmax=out[0], psmin=out[1], psmax=out[2], pstend=out[3])
+
+
+def dfi_weights(ntdfi):
+    """set_dfi_wts (run.f90:567-587): Lanczos-windowed sinc over the 2 ntdfi + 1
+    states of the window, sin(wx)/wx sin(sx)/sx with sx = pi (k - N) / N,
+    wx = pi (k - N) / (N + 1), 1 at k = N, normalised to sum 1. pi is the
+    default-real acos(-1.)."""
+    pi = float(np.float32(np.pi))
+    k = np.arange(2 * ntdfi + 1, dtype=np.float64) - ntdfi
+    w = np.ones(2 * ntdfi + 1)
+    nz = k != 0
+    sx = pi * k[nz] / ntdfi
+    wx = pi * k[nz] / (ntdfi + 1)
+    w[nz] = np.sin(wx) / wx * np.sin(sx) / sx
+    return w / w.sum()
 
 
 def calc_pressdata(ak, bk, rk, lnpsg):

@@ -328,3 +328,30 @@ def test_vadv_tracers_keeps_tracer_positive():
     fl[1:-1] = half * et[1:-1]
     vc = ((fl[1:] - fl[:-1]) + D * (et[:-1] - et[1:])) / dpk
     assert (D - dt * vc).min() < -1e-6 * q.max()
+
+
+def test_topography_enters_as_the_geopotential_gradient():
+    """dyn_init.f90:84 (dphisdx, dphisdy = getgrad(grav topo) / a) feeds
+    -grad(phi_s) into the pressure-gradient force of getpresgrad and nothing
+    else: with the same state the divergence tendency changes by exactly
+    -lap(phi_s) = n(n+1)/a^2 g topo, the vorticity tendency by rounding (the
+    curl of a gradient), temperature and ln ps not at all."""
+    ak, bk = levels.hybrid_levels(12)
+    ms = [O.Model(64, 32, 21, 12, ak, bk, 1200.0) for _ in range(2)]
+    s = ms[0].sht
+    vrt, div, tv, lnps = cases.rest_state(s.degree, s.order, 12, seed=3, vort_rms=1e-5)
+    rng = np.random.default_rng(12)
+    n = s.degree.astype(float)
+    topo = 1.0e3 * np.maximum(n, 1) ** -1.5 * np.exp(2j * np.pi * rng.random(s.nlm))
+    topo[s.order == 0] = topo[s.order == 0].real
+    topo[0] = 0.0
+    ms[1].set_topo(topo)
+    for m in ms:
+        m.set_state(vrt[0], div[0], tv[0], lnps[0])
+    t0, t1 = ms[0].getdyntend(1), ms[1].getdyntend(1)
+    c = ms[0].cfg
+    want = n * (n + 1) / c.rerth ** 2 * (c.grav * topo)
+    scale = np.abs(want).max()
+    assert np.abs(t1["ddiv"] - t0["ddiv"] - want[None, :]).max() < 1e-12 * scale
+    assert np.abs(t1["dvrt"] - t0["dvrt"]).max() < 1e-12 * scale
+    assert np.abs(t1["dtv"] - t0["dtv"]).max() == 0 and np.abs(t1["dlnps"] - t0["dlnps"]).max() == 0
