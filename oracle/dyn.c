@@ -38,6 +38,7 @@
  * difference, and a forward step keeps a tracer non-negative.
  */
 #include "oracle.h"
+#include <float.h>
 #include <math.h>
 #include <omp.h>
 #include <stdio.h>
@@ -65,6 +66,15 @@
 #define BB2 0.0
 #define BB3 SGL(3. / 8.)
 #define BB4 SGL(1. / 4.)
+
+/* one AR(1) pattern kind: nf independent 2-D patterns (nf x nlm), noise drawn
+ * from stream sbase + ctr nf + field (ctr counts advances) */
+struct orc_pat {
+    int nf;
+    ocplx *spec;
+    double *var, *vfact, phi, stdev;
+    uint64_t seed, sbase, ctr;
+};
 
 struct orc_model {
     orc_cfg c;
@@ -101,6 +111,14 @@ struct orc_model {
     int iau_n;
     double *iau_hrs, iau_dt;
     ocplx *iau_inc; /* per file: vrt, div, tv (nl nlm), trac (ntrac nl nlm), lnps (nlm) */
+    /* stochastic physics, kinds 0 svc, 1 sppt, 2 shum, 3 addnoise */
+    orc_stoch_cfg sc;
+    int stoch_on[4], vc_on;
+    struct orc_pat pat[4];
+    double *pgrd[3];    /* grids of svc, sppt (logit transform), shum */
+    double *vcx, *vcy;  /* vorticity gradient (nlev grids) */
+    ocplx *psif;        /* additive streamfunction forcing (nlev x nlm) */
+    double *smoothfact; /* nlm */
 };
 
 #define L(X, k) ((X) + (long)((k) - 1) * m->ng)
@@ -296,10 +314,9 @@ static void setdampspec(orc_model *m) {
  * coefficient) in place of the reference's Mersenne twister stream:
  * member seed = tpat_seed (+ member), stream = 2^40 + advance count,
  * index = 2k, 2k+1. */
-static void pattern_noise(const orc_model *m, uint64_t step, ocplx *noise) {
+static void pat_noise(const orc_model *m, uint64_t seed, uint64_t stream, ocplx *noise) {
     const double normfact = 2.0 * M_PI; /* stoch_data.f90 passes 2 pi */
     const double sc = sqrt(normfact / m->T);
-    const uint64_t seed = (uint64_t)(int64_t)m->c.tpat_seed, stream = (1ULL << 40) + step;
     for (int k = 0; k < m->nlm; k++) {
         if (k == 0) { noise[0] = 0.0; continue; } /* no global mean (115-116) */
         double u1 = orc_rng_uniform(seed, stream, 2 * (uint64_t)k);
@@ -309,6 +326,27 @@ static void pattern_noise(const orc_model *m, uint64_t step, ocplx *noise) {
         double n = m->degree[k], w = 1.0 / sqrt(2.0 * n + 1.0);
         noise[k] = (m->order[k] != 0) ? (gr + I * gi) * w * sc : sqrt(2.0) * gr * w * sc;
     }
+}
+
+static void pattern_noise(const orc_model *m, uint64_t step, ocplx *noise) {
+    pat_noise(m, (uint64_t)(int64_t)m->c.tpat_seed, (1ULL << 40) + step, noise);
+}
+
+/* gaussian_spect (145-177): sqrt(T exp(L^2 lap / 4a^2)), normalised so that
+ * the deterministic reference noise has unit grid variance */
+static void pat_varspec(const orc_model *m, double L, double *var) {
+    double a = m->c.rerth, v = 0.0;
+    int T = m->T;
+    for (int k = 0; k < m->nlm; k++) {
+        double n = m->degree[k], lap = -n * (n + 1.0);
+        var[k] = sqrt(T * exp(L * L * lap / (4.0 * a * a)));
+        double nz2 = (k == 0) ? 0.0 : 2.0 / (2.0 * n + 1.0); /* |(1+i)|^2 or |sqrt 2|^2 over 2n+1 */
+        nz2 *= (2.0 * M_PI) / T;
+        double t = var[k] * var[k] * nz2;
+        v += (m->order[k] != 0) ? t : 0.5 * t;
+    }
+    v /= 2.0 * M_PI;
+    for (int k = 0; k < m->nlm; k++) var[k] /= sqrt(v);
 }
 
 void orc_pattern_advance(orc_model *m) {
@@ -321,21 +359,8 @@ void orc_pattern_advance(orc_model *m) {
 }
 
 static void pattern_init(orc_model *m) {
-    int nlm = m->nlm, T = m->T;
-    double L = m->c.tpat_lscale, a = m->c.rerth;
-    /* gaussian_spect (145-177): sqrt(T exp(L^2 lap / 4a^2)), normalised so
-     * that the deterministic reference noise has unit grid variance */
-    double var = 0.0;
-    for (int k = 0; k < nlm; k++) {
-        double n = m->degree[k], lap = -n * (n + 1.0);
-        m->tpat_var[k] = sqrt(T * exp(L * L * lap / (4.0 * a * a)));
-        double nz2 = (k == 0) ? 0.0 : 2.0 / (2.0 * n + 1.0); /* |(1+i)|^2 or |sqrt 2|^2 over 2n+1 */
-        nz2 *= (2.0 * M_PI) / T;
-        double v = m->tpat_var[k] * m->tpat_var[k] * nz2;
-        var += (m->order[k] != 0) ? v : 0.5 * v;
-    }
-    var /= 2.0 * M_PI;
-    for (int k = 0; k < nlm; k++) m->tpat_var[k] /= sqrt(var);
+    int nlm = m->nlm;
+    pat_varspec(m, m->c.tpat_lscale, m->tpat_var);
     m->tpat_phi = exp(-m->c.dt / m->c.tpat_tau);
     /* vertical taper of stoch_data.f90 (sigma 0.05 - 0.10) */
     for (int k = 0; k < m->nlev; k++) {
@@ -350,6 +375,207 @@ static void pattern_init(orc_model *m) {
     for (int k = 0; k < nlm; k++) m->tpat[k] *= m->c.tpat_amp * m->tpat_var[k];
     int nspinup = (int)(5.0 * m->c.tpat_tau / m->c.dt);
     for (int i = 0; i < nspinup; i++) orc_pattern_advance(m);
+}
+
+/* ---- stochastic physics (stoch_data.f90, run.f90:272-292,509-553) ------
+ * Kinds 0 svc, 1 sppt, 2 shum (one 2-D pattern each) and 3 addnoise (one per
+ * level), each an AR(1) process of patterngenerator.f90 with its own seed and
+ * random stream (2^40 (3 + kind) + draw). */
+static void pat_advance(orc_model *m, struct orc_pat *p, double phi, double f) {
+    ocplx *noise = malloc(sizeof(ocplx) * m->nlm);
+    for (int fld = 0; fld < p->nf; fld++) {
+        pat_noise(m, p->seed, p->sbase + p->ctr * p->nf + fld, noise);
+        ocplx *q = p->spec + (long)fld * m->nlm;
+        for (int k = 0; k < m->nlm; k++) q[k] = phi * q[k] + f * p->var[k] * noise[k];
+    }
+    p->ctr++;
+    free(noise);
+}
+
+/* vertical factors of stoch_data.f90: a taper to 0 between sigma 0.10 and
+ * 0.05 (svc :63-71, sppt :85-93, addnoise :128-136), exp((sl - 1) / 0.2) for
+ * shum (:103-106) */
+static void stoch_vfact(const orc_model *m, int kind, double *vf) {
+    for (int k = 0; k < m->nlev; k++) {
+        double sl = m->sl[k];
+        if (kind == 2) {
+            vf[k] = exp((sl - 1.) / SGL(0.2));
+            continue;
+        }
+        vf[k] = 1.0;
+        if (sl < SGL(0.10) && sl > SGL(0.05)) vf[k] = (sl - SGL(0.05)) / (SGL(0.10) - SGL(0.05));
+        else if (sl <= SGL(0.05)) vf[k] = 0.0;
+    }
+}
+
+int orc_stoch_init(orc_model *m, const orc_stoch_cfg *s) {
+    m->sc = *s;
+    const double amp[4] = {s->svc, s->sppt, s->shum, s->addnoise};
+    const double tau[4] = {s->svc_tau, s->sppt_tau, s->shum_tau, s->addnoise_tau};
+    const double lsc[4] = {s->svc_lscale, s->sppt_lscale, s->shum_lscale, s->addnoise_lscale};
+    const int seed[4] = {s->iseed_svc, s->iseed_sppt, s->iseed_shum, s->iseed_addnoise};
+    int nlm = m->nlm, nl = m->nlev;
+    m->vc_on = s->vcamp > FLT_MIN || s->svc > FLT_MIN;
+    if (m->vc_on && !m->vcx) {
+        m->vcx = malloc(sizeof(double) * m->ng * nl);
+        m->vcy = malloc(sizeof(double) * m->ng * nl);
+    }
+    for (int kd = 0; kd < 4; kd++) {
+        struct orc_pat *p = &m->pat[kd];
+        m->stoch_on[kd] = amp[kd] > FLT_MIN;
+        if (!m->stoch_on[kd]) continue;
+        if (!(tau[kd] > 0) || (kd == 3 && !s->addnoise_vrtonly)) return -1;
+        p->nf = kd == 3 ? nl : 1;
+        free(p->spec); free(p->var); free(p->vfact);
+        p->spec = calloc((size_t)p->nf * nlm, sizeof(ocplx));
+        p->var = malloc(sizeof(double) * nlm);
+        p->vfact = malloc(sizeof(double) * nl);
+        pat_varspec(m, lsc[kd], p->var);
+        stoch_vfact(m, kd, p->vfact);
+        p->phi = exp(-m->c.dt / tau[kd]);
+        p->stdev = amp[kd];
+        p->seed = (uint64_t)(int64_t)seed[kd];
+        p->sbase = (uint64_t)(3 + kd) << 40;
+        p->ctr = 0;
+        /* stdev x varspectrum x noise, then 5 e-folding times of spin-up */
+        pat_advance(m, p, 0.0, p->stdev);
+        int nspinup = (int)(5.0 * tau[kd] / m->c.dt);
+        for (int i = 0; i < nspinup; i++) pat_advance(m, p, p->phi, p->stdev * sqrt(1.0 - p->phi * p->phi));
+        if (kd < 3 && !m->pgrd[kd]) m->pgrd[kd] = calloc(m->ng, sizeof(double));
+        /* the grid of the spun-up pattern, for tendencies asked before the first step */
+        if (kd < 3) orc_spectogrd(m->sht, p->spec, m->pgrd[kd], 1);
+        if (kd == 1)
+            for (long i = 0; i < m->ng; i++) m->pgrd[1][i] = 2. / (1. + exp(m->pgrd[1][i])) - 1.;
+    }
+    if (m->stoch_on[3]) {
+        if (!m->psif) m->psif = calloc((size_t)nl * nlm, sizeof(ocplx));
+        /* getstochforcing :269-276: Gaussian smoothing at three times the noise scale */
+        if (!m->smoothfact) m->smoothfact = malloc(sizeof(double) * nlm);
+        double rnn1 = 2. * m->c.rerth / (3. * s->addnoise_lscale), rnn0 = rnn1 * (rnn1 + 1.);
+        for (int k = 0; k < nlm; k++) {
+            double n = m->degree[k];
+            m->smoothfact[k] = exp(-(n * (n + 1.) / rnn0));
+        }
+    }
+    return 0;
+}
+
+/* run.f90:272-292: the patterns used this step and their grids; sppt through
+ * the logit transform 2 / (1 + e^g) - 1 */
+static void stoch_patterns(orc_model *m) {
+    for (int kd = 0; kd < 4; kd++) {
+        struct orc_pat *p = &m->pat[kd];
+        if (!m->stoch_on[kd]) continue;
+        pat_advance(m, p, p->phi, p->stdev * sqrt(1.0 - p->phi * p->phi));
+        if (kd < 3) orc_spectogrd(m->sht, p->spec, m->pgrd[kd], 1);
+        if (kd == 1)
+            for (long i = 0; i < m->ng; i++) m->pgrd[1][i] = 2. / (1. + exp(m->pgrd[1][i])) - 1.;
+    }
+}
+
+/* getstochforcing (stoch_data.f90:254-354) with addnoise_vrtonly: the
+ * streamfunction noise, optionally 1-2-1 filtered in the vertical (one pass
+ * whatever addnoise_vfilt is, as the reference's loop re-filters the
+ * unfiltered patterns), on the grid times vfact and, with addnoise_dissfact,
+ * times the smoothed square root of the kinetic energy dissipation rate
+ * |u . u_diss| (u_diss the wind of -disspec vrt, -disspec div), analysed back.
+ * Without addnoise_dissfact the reference reads an array it never set; here
+ * the forcing is then the filtered noise times vfact. On the grid fields of the
+ * inverse transform before the physics. */
+static void stoch_forcing(orc_model *m) {
+    int nl = m->nlev, nlm = m->nlm;
+    long ng = m->ng;
+    const struct orc_pat *p = &m->pat[3];
+    const double epstiny = FLT_MIN;
+    ocplx *ps = m->psif;
+    if (m->sc.addnoise_vfilt > 0) {
+        const double c = SGL(1. + 1. / 3.);
+        for (int k = 0; k < nl; k++) {
+            const ocplx *q = p->spec + (long)k * nlm;
+            for (int n = 0; n < nlm; n++) {
+                ocplx w;
+                if (k == 0)
+                    w = c * q[nlm + n] + 2. * c * q[n];
+                else if (k == nl - 1)
+                    w = c * q[n - nlm] + 2. * c * q[n];
+                else
+                    w = q[n + nlm] + 2. * q[n] + q[n - nlm];
+                ps[(long)k * nlm + n] = 0.25 * w;
+            }
+        }
+    } else {
+        memcpy(ps, p->spec, sizeof(ocplx) * nl * nlm);
+    }
+    double *ud = malloc(sizeof(double) * ng), *vd = malloc(sizeof(double) * ng);
+    double *wg = malloc(sizeof(double) * ng);
+    ocplx *w1 = malloc(sizeof(ocplx) * nlm), *w2 = malloc(sizeof(ocplx) * nlm);
+    for (int k = 0; k < nl; k++) {
+        const long o = (long)k * ng;
+        orc_spectogrd(m->sht, ps + (long)k * nlm, wg, 1);
+        if (m->sc.addnoise_dissfact) {
+            for (int n = 0; n < nlm; n++) {
+                w1[n] = -m->disspec[n] * m->vrt[(long)k * nlm + n];
+                w2[n] = -m->disspec[n] * m->div[(long)k * nlm + n];
+            }
+            orc_getuv(m->sht, w1, w2, ud, vd, 1, m->c.rerth);
+            for (long i = 0; i < ng; i++) {
+                double d = fabs(m->ug[o + i] * ud[i] + m->vg[o + i] * vd[i]);
+                ud[i] = sqrt(d < epstiny ? epstiny : d);
+            }
+            orc_grdtospec(m->sht, ud, w1, 1);
+            for (int n = 0; n < nlm; n++) w1[n] *= m->smoothfact[n];
+            orc_spectogrd(m->sht, w1, vd, 1);
+            for (long i = 0; i < ng; i++) ud[i] = p->vfact[k] * wg[i] * vd[i];
+        } else {
+            for (long i = 0; i < ng; i++) ud[i] = p->vfact[k] * wg[i];
+        }
+        orc_grdtospec(m->sht, ud, ps + (long)k * nlm, 1);
+    }
+    free(ud); free(vd); free(wg); free(w1); free(w2);
+}
+
+/* run.f90:541-552: the streamfunction forcing into vorticity (lap / a^2, or
+ * -sqrt(-lap) / a with addnoise_kenorm and addnoise_vrtonly); the balanced
+ * temperature forcing is zero with addnoise_vrtonly */
+static void stoch_apply(orc_model *m) {
+    double dt = m->c.dt, a = m->c.rerth;
+    int ke = m->sc.addnoise_kenorm && m->sc.addnoise_vrtonly;
+    for (int k = 0; k < m->nlev; k++)
+        for (int n = 0; n < m->nlm; n++) {
+            long i = (long)k * m->nlm + n;
+            if (ke)
+                m->vrt[i] -= dt * (sqrt(-m->lap[n]) / a) * m->psif[i];
+            else
+                m->vrt[i] += dt * (m->lap[n] / (a * a)) * m->psif[i];
+        }
+}
+
+/* getstochforcing on the current state and grid fields (after an
+ * inverse-only getdyntend); the result in psiforcing */
+void orc_stoch_forcing(orc_model *m) {
+    if (m->stoch_on[3] && m->have_grid) stoch_forcing(m);
+}
+
+/* replace a spectral pattern (which 3..5: svc, sppt, shum; 6: addnoise) */
+void orc_set_stoch(orc_model *m, int which, const void *in) {
+    if (which >= 3 && which < 6 && m->stoch_on[which - 3])
+        memcpy(m->pat[which - 3].spec, in, sizeof(ocplx) * m->nlm);
+    else if (which == 6 && m->stoch_on[3])
+        memcpy(m->pat[3].spec, in, sizeof(ocplx) * m->nlm * m->nlev);
+}
+
+void orc_get_stoch(const orc_model *m, int which, void *out) {
+    if (which >= 0 && which < 3) {
+        if (m->stoch_on[which]) memcpy(out, m->pgrd[which], sizeof(double) * m->ng);
+    } else if (which >= 3 && which < 6) {
+        if (m->stoch_on[which - 3]) memcpy(out, m->pat[which - 3].spec, sizeof(ocplx) * m->nlm);
+    } else if (which == 6) {
+        if (m->stoch_on[3]) memcpy(out, m->pat[3].spec, sizeof(ocplx) * m->nlm * m->nlev);
+    } else if (which == 7) {
+        if (m->stoch_on[3]) memcpy(out, m->psif, sizeof(ocplx) * m->nlm * m->nlev);
+    } else if (which >= 8 && which < 12) {
+        if (m->stoch_on[which - 8]) memcpy(out, m->pat[which - 8].vfact, sizeof(double) * m->nlev);
+    }
 }
 
 void orc_get_pattern(const orc_model *m, ocplx *pattern, double *varspectrum, double *phi, double *vfact) {
@@ -437,7 +663,11 @@ void orc_model_destroy(orc_model *m) {
                     m->wk1, m->lnpsg, m->psg, m->dlnpsdt, m->dphisdx, m->dphisdy, m->dlnpsdx,
                     m->dlnpsdy, m->amhyb, m->bmhyb, m->svhyb, m->tor_hyb, m->d_hyb_m,
                     m->disspec, m->diff_prof, m->dmp_prof, m->precip, m->pwat, m->order,
-                    m->tpat, m->tpat_var, m->tpat_vfact, m->iau_hrs, m->iau_inc};
+                    m->tpat, m->tpat_var, m->tpat_vfact, m->iau_hrs, m->iau_inc, m->pgrd[0],
+                    m->pgrd[1], m->pgrd[2], m->vcx, m->vcy, m->psif, m->smoothfact,
+                    m->pat[0].spec, m->pat[0].var, m->pat[0].vfact, m->pat[1].spec, m->pat[1].var,
+                    m->pat[1].vfact, m->pat[2].spec, m->pat[2].var, m->pat[2].vfact, m->pat[3].spec,
+                    m->pat[3].var, m->pat[3].vfact};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int i = 0; i < 10; i++) free(m->w[i]);
     for (int s = 0; s < 3; s++) { free(m->lin_d[s]); free(m->lin_t[s]); free(m->lin_p[s]); }
@@ -753,6 +983,8 @@ void orc_getdyntend(orc_model *m, int rkstage, int inverse_only, ocplx *dvrt, oc
     orc_spectogrd(h, m->tv, m->tvg, nl);
     orc_getgrad(h, m->tv, dtvdx, dtvdy, nl, rerth);
     if (ntrac > 0) orc_spectogrd(h, m->trac, m->tracg, nl * ntrac);
+    /* 114-116: vorticity gradient for vorticity confinement */
+    if (m->vc_on && !inverse_only) orc_getgrad(h, m->vrt, m->vcx, m->vcy, nl, rerth);
     /* 126-131 */
     orc_spectogrd(h, m->lnps, m->lnpsg, 1);
     calc_press_cols(nl, ng, m->ak, m->bk, m->kappa, m->lnpsg, m->psg, m->pk, m->dpk, m->alfa,
@@ -802,6 +1034,17 @@ void orc_getdyntend(orc_model *m, int rkstage, int inverse_only, ocplx *dvrt, oc
         prsgx[idx] = u * (z + f) + vadvv[idx];
         prsgy[idx] = v * (z + f) - vadvu[idx];
         m->w[9][idx] = 0.5 * (u * u + v * v);
+        if (m->vc_on) {
+            /* 230-248: up-gradient vorticity flux vc grad(vrt)/|grad(vrt)| |vrt| with
+             * vc = vcamp, times (1 + vfact_svc grd_svc) with the svc pattern */
+            double gx = m->vcx[idx], gy = m->vcy[idx], ag = sqrt(gx * gx + gy * gy);
+            double vc = m->sc.vcamp;
+            if (m->stoch_on[0]) vc = (1. + m->pat[0].vfact[idx / ng] * m->pgrd[0][i]) * vc;
+            if (ag > FLT_MIN) {
+                prsgx[idx] += vc * (gx / ag) * fabs(z);
+                prsgy[idx] += vc * (gy / ag) * fabs(z);
+            }
+        }
     }
     orc_grdtospec(h, m->w[8], dtv, nl);
     orc_getvrtdivspec(h, prsgx, prsgy, ddiv, dvrt, nl, rerth); /* swapped outputs */
@@ -937,8 +1180,9 @@ void orc_advance(orc_model *m, int nsteps) {
     for (int step = 0; step < nsteps; step++) {
         /* run.f90:250-258: the IAU forcing of this step, at its start time */
         int iau = F ? iau_forcing(m, m->time, F) : 0;
-        /* run.f90:272-292: advance the random pattern used this step */
+        /* run.f90:272-292: advance the random patterns used this step */
         if (m->c.tpat_amp > 0) orc_pattern_advance(m);
+        stoch_patterns(m);
         memcpy(vrt0, m->vrt, s3); memcpy(div0, m->div, s3); memcpy(tv0, m->tv, s3);
         if (ntrac > 0) memcpy(tr0, m->trac, st);
         memcpy(lp0, m->lnps, s1);
@@ -1000,6 +1244,8 @@ void orc_advance(orc_model *m, int nsteps) {
         /* run.f90:509-538 time-split physics */
         if (m->c.physics != 0) {
             orc_getdyntend(m, 3, 1, NULL, NULL, NULL, NULL, NULL);
+            /* run.f90:517-518 */
+            if (m->stoch_on[3]) stoch_forcing(m);
             if (m->c.physics == 1) {
                 orc_hs_phytend(m, dv[1], dd[1], dT[1]);
                 for (long i = 0; i < (long)nlm * nl; i++) {
@@ -1032,6 +1278,7 @@ void orc_advance(orc_model *m, int nsteps) {
                 free(ws);
             }
             for (int k = 0; k < nlm; k++) m->lnps[k] += dt * dp[1][k];
+            if (m->stoch_on[3]) stoch_apply(m);
         }
         /* stochastic temperature forcing pattern/tau, in the place of the
          * additive-noise tvforcing of run.f90:539-551 */

@@ -93,6 +93,22 @@ typedef struct {
 
 typedef struct orc_model orc_model;
 
+/* stochastic physics (params.f90:140-160, stoch_data.f90 init_stochdata):
+ * an amplitude of 0 switches a scheme off. Patterns draw from the counter RNG
+ * with seed iseed_* (+ member) in place of the reference's Mersenne twister. */
+typedef struct {
+    double vcamp;                          /* vorticity confinement speed (m/s) */
+    double svc, svc_tau, svc_lscale;       /* pattern modulating vcamp */
+    int iseed_svc;
+    double sppt, sppt_tau, sppt_lscale;    /* physics-tendency pattern */
+    int iseed_sppt;
+    double shum, shum_tau, shum_lscale;    /* boundary-layer humidity pattern */
+    int iseed_shum;
+    double addnoise, addnoise_tau, addnoise_lscale;  /* streamfunction noise per level */
+    int iseed_addnoise;
+    int addnoise_vfilt, addnoise_kenorm, addnoise_dissfact, addnoise_vrtonly;
+} orc_stoch_cfg;
+
 void orc_cfg_defaults(orc_cfg *c);
 orc_model *orc_model_create(const orc_cfg *c);
 void orc_model_destroy(orc_model *m);
@@ -106,6 +122,20 @@ void orc_get_state(const orc_model *m, ocplx *vrt, ocplx *div, ocplx *tv, ocplx 
 void orc_set_pdryini(orc_model *m, double pdryini);
 double orc_get_pdryini(const orc_model *m);
 void orc_set_time(orc_model *m, double t);
+/* init_stochdata (stoch_data.f90:37-238): patterns set up and spun up 5
+ * e-folding times; returns -1 on a scheme with amplitude > 0 and tau <= 0,
+ * or on addnoise_vrtonly = 0 (its balance operator is read from an external
+ * GSI file the reference does not include). */
+int orc_stoch_init(orc_model *m, const orc_stoch_cfg *s);
+/* 0..2: grids of svc, sppt (after the logit transform), shum (nlat*nlon);
+ * 3..5: their spectral patterns (nlm); 6: addnoise patterns (nlev*nlm);
+ * 7: the last step's additive streamfunction forcing psiforcing (nlev*nlm);
+ * 8..11: vertical factors of svc, sppt, shum, addnoise (nlev) */
+void orc_get_stoch(const orc_model *m, int which, void *out);
+/* replace a spectral pattern: 3..5 svc, sppt, shum (nlm); 6 addnoise (nlev*nlm) */
+void orc_set_stoch(orc_model *m, int which, const void *in);
+/* getstochforcing on the current state and grids (after getdyntend(3, 1)) */
+void orc_stoch_forcing(orc_model *m);
 /* spectral orography (nlm), dyn_init.f90:84 */
 void orc_set_topo(orc_model *m, const ocplx *topo);
 double orc_get_time(const orc_model *m);

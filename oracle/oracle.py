@@ -37,6 +37,28 @@ class Cfg(C.Structure):
     ]
 
 
+class StochCfg(C.Structure):
+    """orc_stoch_cfg (params.f90:140-160 names)"""
+    _fields_ = [("vcamp", C.c_double),
+                ("svc", C.c_double), ("svc_tau", C.c_double), ("svc_lscale", C.c_double), ("iseed_svc", C.c_int),
+                ("sppt", C.c_double), ("sppt_tau", C.c_double), ("sppt_lscale", C.c_double),
+                ("iseed_sppt", C.c_int),
+                ("shum", C.c_double), ("shum_tau", C.c_double), ("shum_lscale", C.c_double),
+                ("iseed_shum", C.c_int),
+                ("addnoise", C.c_double), ("addnoise_tau", C.c_double), ("addnoise_lscale", C.c_double),
+                ("iseed_addnoise", C.c_int),
+                ("addnoise_vfilt", C.c_int), ("addnoise_kenorm", C.c_int), ("addnoise_dissfact", C.c_int),
+                ("addnoise_vrtonly", C.c_int)]
+
+
+def stoch_cfg(**kw):
+    c = StochCfg()
+    c.addnoise_vrtonly = 1
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
 def build():
     """Compile liboracle.so in place (gcc)."""
     import subprocess
@@ -79,6 +101,10 @@ def lib():
         L.orc_get_time.restype = C.c_double
         L.orc_iau_set.argtypes = [vp, C.c_int, vp, C.c_double, vp]
         L.orc_set_topo.argtypes = [vp, vp]
+        L.orc_stoch_init.argtypes = [vp, C.POINTER(StochCfg)]
+        L.orc_get_stoch.argtypes = [vp, C.c_int, vp]
+        L.orc_set_stoch.argtypes = [vp, C.c_int, vp]
+        L.orc_stoch_forcing.argtypes = [vp]
         L.orc_getdyntend.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.orc_advance.argtypes = [vp, C.c_int]
         L.orc_hs_phytend.argtypes = [vp, _cp, _cp, _cp]
@@ -262,6 +288,40 @@ class Model:
         """spectral orography (m), dyn_init.f90:84"""
         t = np.ascontiguousarray(topo, np.complex128)
         lib().orc_set_topo(self.h, t.ctypes.data)
+
+    def stoch_init(self, **kw):
+        """init_stochdata with the params.f90 stochastic settings given as keywords"""
+        self._stoch = stoch_cfg(**kw)
+        if lib().orc_stoch_init(self.h, C.byref(self._stoch)) != 0:
+            raise ValueError("stochastic scheme with amplitude > 0 needs tau > 0 (and addnoise_vrtonly)")
+
+    def stoch(self, which):
+        """'svc', 'sppt', 'shum' grids; 'svc_spec', 'sppt_spec', 'shum_spec'; 'addnoise'
+        (nlev, nlm) patterns; 'psiforcing' (nlev, nlm), the last step's forcing"""
+        w = self.STOCH.index(which)
+        if w < 3:
+            out = np.zeros((self.nlat, self.nlon))
+        elif w < 6:
+            out = np.zeros(self.nlm, np.complex128)
+        elif w < 8:
+            out = np.zeros((self.nlev, self.nlm), np.complex128)
+        else:
+            out = np.zeros(self.nlev)
+        lib().orc_get_stoch(self.h, w, out.ctypes.data)
+        return out
+
+    STOCH = ["svc", "sppt", "shum", "svc_spec", "sppt_spec", "shum_spec", "addnoise", "psiforcing",
+             "svc_vfact", "sppt_vfact", "shum_vfact", "addnoise_vfact"]
+
+    def set_stoch(self, which, spec):
+        """replace the spectral pattern 'svc_spec', 'sppt_spec', 'shum_spec' or 'addnoise'"""
+        a = np.ascontiguousarray(spec, np.complex128)
+        lib().orc_set_stoch(self.h, self.STOCH.index(which), a.ctypes.data)
+
+    def stoch_forcing(self):
+        """getstochforcing on the current state and grid fields"""
+        lib().orc_stoch_forcing(self.h)
+        return self.stoch("psiforcing")
 
     @property
     def time(self):
