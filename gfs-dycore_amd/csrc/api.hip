@@ -233,6 +233,14 @@ int gfsdc_iau_set(gfsdc_ctx *c, int nfiles, const double *fhrs, double delthrs, 
     DISPATCH(c, iau_set(nfiles, fhrs, delthrs, vrt, div, tv, trac, lnps, dtype, loc));
 }
 int gfsdc_dfi(gfsdc_ctx *c, int ntdfi, int use_graph) { DISPATCH(c, dfi(ntdfi, use_graph)); }
+void gfsdc_stoch_defaults(gfsdc_stoch_cfg *s) {
+    std::memset(s, 0, sizeof(*s));
+    s->addnoise_vrtonly = 1;
+}
+int gfsdc_stoch_init(gfsdc_ctx *c, const gfsdc_stoch_cfg *s) { DISPATCH(c, stoch_init(s)); }
+int gfsdc_get_stoch(gfsdc_ctx *c, int which, void *out, int dtype, int loc) {
+    DISPATCH(c, get_stoch(which, out, dtype, loc));
+}
 int gfsdc_checkpoint_size(gfsdc_ctx *c, size_t *nbytes) {
     if (!c || !nbytes) {
         set_error("null argument");

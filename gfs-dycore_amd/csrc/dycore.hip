@@ -19,6 +19,7 @@
 // ln ps' + ln(pcorr) (= log(ps pcorr) - c0), adding c0 back to q_00. This is the
 // reference's arithmetic (ln(ps) and log(ps pcorr) transformed to spectral space),
 // rearranged so that the FP32 path does not round a ~11.5 grid field to ~1e-6.
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -108,6 +109,30 @@ struct DycoreBuffers {
     unsigned long long *jctr = nullptr;          // judge / sampling step counter
     unsigned long long bctr = 0;                 // training batch counter
     double *diag = nullptr;                      // FP64 4 per member
+    // stochastic physics (stoch_data.f90): kinds 0 svc, 1 sppt, 2 shum (one
+    // pattern per member each, st_pat2 [kind][member][nlm]) and 3 addnoise (one
+    // per level and member, [member][level][nlm])
+    gfsdc_stoch_cfg sc{};
+    int st_on[4] = {0, 0, 0, 0}, st_nf[4] = {1, 1, 1, 1};
+    double st_phi[4] = {0, 0, 0, 0}, st_sd[4] = {0, 0, 0, 0};
+    unsigned long long st_seed[4] = {0, 0, 0, 0};
+    void *st_pat2 = nullptr, *st_pat[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *st_var = nullptr;                    // FP64 [4][nlm]
+    void *st_vf = nullptr;                       // T [4][nl]
+    unsigned long long *st_ctr = nullptr;        // advances per kind [4]
+    void *st_grd = nullptr;                      // T [kind 0..2][member][ng]
+    SynthDesc *sd_pat = nullptr;                 // [kind 0..2][member]
+    unsigned char *rf0 = nullptr;                // zero 1/cos flags
+    bool vc_on = false;
+    void *vcg = nullptr;                         // T [member][2 nl][ng] vorticity gradient
+    SynthDesc *sd_vc = nullptr;                  // [member][2 nl]
+    unsigned char *rf1 = nullptr;                // one 1/cos flags
+    void *an_w = nullptr;                        // cplx<T> [group 3][member][nl][nlm]
+    void *an_g = nullptr;                        // T [group 3][member][nl][ng]
+    void *an_sm = nullptr;                       // T nlm smoothing factors
+    SynthDesc *sd_an = nullptr;                  // [ud, vd, psi | group 1 copy | group 0 copy]
+    AnalDesc *ad_an = nullptr;                   // [member][nl]
+    unsigned char *rf_an = nullptr;              // [group][member][nl]: 1, 1, 0
     double *clock = nullptr;                     // model time (s) at the start of the next step
     // incremental analysis update (iau_module.f90): nfiles increments of the
     // whole state in the state layout, their times (h) and the forcing of a step
@@ -206,14 +231,18 @@ __device__ inline LayerCoef<T> layer_coef(T p1, T p2, T dpk, T bk1, T bk2, T ck,
 // tendency assembly unless inverse_only). One thread per grid column and member.
 // Level indices: K counts interfaces/layers top -> bottom (1-based, as pk, dpk,
 // alfa, rlnp, etadot), L counts model levels bottom -> top (as u, v, tv).
-template <typename T>
+// VC adds the vorticity-confinement flux (dyn_run.f90:230-248) from the
+// vorticity gradient vcg [member][2 nl][ng] (x then y), with speed vcamp
+// times 1 + vfsvc[level] svcg (the svc pattern grid [member][ng]) when svcg is set.
+template <typename T, bool VC = false>
 __global__ __launch_bounds__(256) void k_columns(
     const T *__restrict__ gsyn, T *__restrict__ gan, T *__restrict__ etadot, T *__restrict__ dlnpdt,
     T *__restrict__ psg_out, T *__restrict__ dlnpsdt_out, const T *__restrict__ dphisdx,
     const T *__restrict__ dphisdy, const T *__restrict__ ak_, const T *__restrict__ bk_,
     const T *__restrict__ ck_, const T *__restrict__ dbk_, const T *__restrict__ dak_, const T *__restrict__ sinlat,
     int nl, int ntr, int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only,
-    const cplx<T> *__restrict__ spec, int nfs, T *__restrict__ colq) {
+    const cplx<T> *__restrict__ spec, int nfs, T *__restrict__ colq, const T *__restrict__ vcg,
+    const T *__restrict__ svcg, const T *__restrict__ vfsvc, T vcamp) {
     __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1], ck[MAXLEV], dbk[MAXLEV], dak[MAXLEV];
     for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
         ak[i] = ak_[i];
@@ -387,8 +416,20 @@ __global__ __launch_bounds__(256) void k_columns(
             vu -= prsgx;
             T z = gvr[i] + f;
             LV(TT, L) = -u * tx - v * ty - vt + vq;
-            LV(FX, L) = u * z + vv;
-            LV(FY, L) = v * z - vu;
+            T fx = u * z + vv, fy = v * z - vu;
+            if constexpr (VC) {
+                // up-gradient vorticity flux vc |vrt| grad(vrt) / |grad(vrt)|
+                const T *VCX = vcg + (long)mem * 2 * nl * ng + col, *VCY = VCX + (long)nl * ng;
+                const T gx = LV(VCX, L), gy = LV(VCY, L), ag = sqrt(gx * gx + gy * gy);
+                T vcl = vcamp;
+                if (svcg) vcl = (T(1) + vfsvc[L - 1] * svcg[(long)mem * ng + col]) * vcl;
+                if (ag > T(FLT_MIN)) {
+                    fx += vcl * (gx / ag) * fabs(gvr[i]);
+                    fy += vcl * (gy / ag) * fabs(gvr[i]);
+                }
+            }
+            LV(FX, L) = fx;
+            LV(FY, L) = fy;
             LV(KE, L) = T(0.5) * (u * u + v * v);
             // accumulations for the layer above (getpresgrad cofb and px3 recursions)
             cofb_acc -= rd * dbp * tv;
@@ -1106,21 +1147,23 @@ __global__ __launch_bounds__(256) void k_massfix_field(const T *__restrict__ gsy
 }
 
 // AR(1) pattern advance (patterngenerator.f90:134-143 with getnoise 115-132):
-// pat = phi pat + f var noise, one thread per (coefficient, member). Noise is
-// Box-Muller on the counter RNG: seed + member, stream 2^40 + advance count
-// (read from the device counter so graph replays continue the stream),
-// index 2k / 2k+1. sc = sqrt(normfact / ntrunc) with normfact = 2 pi.
+// pat = phi pat + f var noise for nf independent patterns per member, one
+// thread per (coefficient, member, pattern). Noise is Box-Muller on the counter
+// RNG: seed + member, stream sbase + nf * (advance count) + pattern (the count
+// read from the device counter so graph replays continue the stream), index
+// 2k / 2k+1. sc = sqrt(normfact / ntrunc) with normfact = 2 pi.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pattern_advance(cplx<T> *__restrict__ pat, const double *__restrict__ var,
                                                          const int2 *__restrict__ kmn, int nlm,
                                                          const unsigned long long *__restrict__ ctr,
-                                                         unsigned long long seed, double phi, double f, double sc) {
+                                                         unsigned long long seed, double phi, double f, double sc,
+                                                         int nf, unsigned long long sbase) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int mem = blockIdx.y;
+    const int mem = blockIdx.y / nf, fld = blockIdx.y - mem * nf;
     if (k >= nlm) return;
     double nr = 0.0, ni = 0.0;
     if (k > 0) {
-        const uint64_t stream = (1ULL << 40) + ctr[0], sd = seed + (uint64_t)mem;
+        const uint64_t stream = sbase + ctr[0] * (uint64_t)nf + (uint64_t)fld, sd = seed + (uint64_t)mem;
         const double u1 = (double)(rng_mix(sd, stream, 2ULL * k) >> 11) * 0x1p-53;
         const double u2 = (double)(rng_mix(sd, stream, 2ULL * k + 1) >> 11) * 0x1p-53;
         const double r = sqrt(-2.0 * log(1.0 - u1));
@@ -1134,7 +1177,7 @@ __global__ __launch_bounds__(256) void k_pattern_advance(cplx<T> *__restrict__ p
             nr = sqrt(2.0) * gr * w * sc;
         }
     }
-    cplx<T> *p = pat + (long)mem * nlm + k;
+    cplx<T> *p = pat + ((long)mem * nf + fld) * nlm + k;
     const double v = f * var[k];
     *p = cplx<T>{(T)(phi * (double)p->x + v * nr), (T)(phi * (double)p->y + v * ni)};
 }
@@ -1146,6 +1189,110 @@ __global__ void k_counter_inc(unsigned long long *ctr) {
 // model clock: t = t + dt at the end of every step (run.f90:153)
 __global__ void k_clock_add(double *clk, double dt) {
     if (threadIdx.x == 0 && blockIdx.x == 0) clk[0] += dt;
+}
+
+// sppt grid: the logit transform 2 / (1 + e^g) - 1 of run.f90:281
+template <typename T> __global__ __launch_bounds__(256) void k_logit(T *__restrict__ g, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        g[i] = T(2) / (T(1) + exp(g[i])) - T(1);
+}
+
+// getstochforcing inputs (stoch_data.f90:264-290), work layout w[group][member][level][nlm]:
+// group 0 the streamfunction patterns, 1-2-1 filtered in the vertical when
+// vfilt (one pass: the reference re-filters the unfiltered patterns on every
+// pass; the end levels with the factor 1 + 1/3 of a default-real literal),
+// group 1 -disspec vrt, group 2 -disspec div (the dissipation wind)
+template <typename T>
+__global__ __launch_bounds__(256) void k_an_prep(cplx<T> *__restrict__ w, const cplx<T> *__restrict__ psi,
+                                                 const cplx<T> *__restrict__ spec, const T *__restrict__ disspec,
+                                                 int nl, int nfs, int nm, int nlm, int vfilt, int diss) {
+    const long tot = (long)nm * nl * nlm;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % nlm);
+        const long r = i / nlm;
+        const int lev = (int)(r % nl), mem = (int)(r / nl);
+        const cplx<T> *q = psi + i;
+        cplx<T> o = *q;
+        if (vfilt) {
+            const T c = T(SGLF(1. + 1. / 3.));
+            cplx<T> a;
+            if (lev == 0)
+                a = cplx<T>{c * q[nlm].x + T(2) * c * q[0].x, c * q[nlm].y + T(2) * c * q[0].y};
+            else if (lev == nl - 1)
+                a = cplx<T>{c * q[-nlm].x + T(2) * c * q[0].x, c * q[-nlm].y + T(2) * c * q[0].y};
+            else
+                a = cplx<T>{q[nlm].x + T(2) * q[0].x + q[-nlm].x, q[nlm].y + T(2) * q[0].y + q[-nlm].y};
+            o = cplx<T>{T(0.25) * a.x, T(0.25) * a.y};
+        }
+        w[i] = o;
+        if (diss) {
+            const long s = (long)k * nfs * nm + (long)mem * nfs + lev;
+            const T d = -disspec[k];
+            const cplx<T> v = spec[s], dv = spec[s + nl];
+            w[tot + i] = cplx<T>{d * v.x, d * v.y};
+            w[2 * tot + i] = cplx<T>{d * dv.x, d * dv.y};
+        }
+    }
+}
+
+// smoothed-input field of getstochforcing :277-283 on the grid:
+// sqrt(max(|u ud + v vd|, tiny)) into the ud slot (grid layout g[group][member][level][ng])
+template <typename T>
+__global__ __launch_bounds__(256) void k_an_kediss(T *__restrict__ g, const T *__restrict__ gsyn, int nl, int nsyn,
+                                                   int nm, long ng) {
+    const long tot = (long)nm * nl * ng;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+        const long col = i % ng, r = i / ng;
+        const int lev = (int)(r % nl), mem = (int)(r / nl);
+        const T *G = gsyn + (long)mem * nsyn * ng + col;
+        const T d = fabs(G[(long)lev * ng] * g[i] + G[(long)(nl + lev) * ng] * g[tot + i]);
+        const T tiny = (T)FLT_MIN;
+        g[i] = sqrt(d < tiny ? tiny : d);
+    }
+}
+
+// per-coefficient scaling of nf fields [field][nlm]: x *= f[k]
+template <typename T>
+__global__ __launch_bounds__(256) void k_spec_scale(cplx<T> *__restrict__ x, const T *__restrict__ f, long nf,
+                                                    int nlm) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nf * nlm; i += (long)gridDim.x * blockDim.x) {
+        const T s = f[i % nlm];
+        x[i] = cplx<T>{s * x[i].x, s * x[i].y};
+    }
+}
+
+// modulated streamfunction forcing on the grid (:284-289): vfact psi (x the
+// smoothed dissipation field with diss) into the group-0 slot
+template <typename T>
+__global__ __launch_bounds__(256) void k_an_modulate(T *__restrict__ g, const T *__restrict__ psig,
+                                                     const T *__restrict__ sm, const T *__restrict__ vf, int nl, int nm,
+                                                     long ng) {
+    const long tot = (long)nm * nl * ng;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+        const int lev = (int)((i / ng) % nl);
+        T v = vf[lev] * psig[i];
+        if (sm) v *= sm[i];
+        g[i] = v;
+    }
+}
+
+// run.f90:541-552: vrt += dt lap / a^2 psif, or -dt sqrt(-lap) / a psif with
+// kenorm (the temperature forcing is zero with addnoise_vrtonly)
+template <typename T>
+__global__ __launch_bounds__(256) void k_an_apply(cplx<T> *__restrict__ spec, const cplx<T> *__restrict__ psif,
+                                                  const int *__restrict__ degree, int nl, int nfs, int nm, int nlm,
+                                                  double dt, double rerth, int kenorm) {
+    const long tot = (long)nm * nl * nlm;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % nlm);
+        const long r = i / nlm;
+        const int lev = (int)(r % nl), mem = (int)(r / nl);
+        const double n = degree[k], lap = -n * (n + 1.0);
+        const T c = kenorm ? (T)(-dt * sqrt(-lap) / rerth) : (T)(dt * lap / (rerth * rerth));
+        cplx<T> *o = spec + (long)k * nfs * nm + (long)mem * nfs + lev;
+        const cplx<T> f = psif[i];
+        *o = cplx<T>{o->x + c * f.x, o->y + c * f.y};
+    }
 }
 
 // getiauforcing (iau_module.f90:84-115) at the clock's time t: zero outside
@@ -1841,6 +1988,23 @@ __global__ void k_semimp_pack(const double *__restrict__ d, T *__restrict__ dt3,
 // ---------------------------------------------------------------------------
 // host side
 
+// gaussian_spect (patterngenerator.f90:145-177): sqrt(T exp(L^2 lap / 4a^2)),
+// normalised so that the deterministic reference noise has unit variance
+static std::vector<double> pattern_varspec(int T_, double L, double a) {
+    std::vector<double> var((size_t)(T_ + 1) * (T_ + 2) / 2);
+    const double twopi = 2.0 * M_PI;
+    double v = 0.0;
+    for (int m = 0, k = 0; m <= T_; m++)
+        for (int n = m; n <= T_; n++, k++) {
+            var[k] = std::sqrt(T_ * std::exp(L * L * (-(double)n * (n + 1.0)) / (4.0 * a * a)));
+            double nz2 = (k == 0) ? 0.0 : 2.0 / (2.0 * n + 1.0) * (twopi / T_);
+            v += (m != 0 ? 1.0 : 0.5) * var[k] * var[k] * nz2;
+        }
+    v /= twopi;
+    for (double &x : var) x /= std::sqrt(v);
+    return var;
+}
+
 template <typename T> static T *dalloc(size_t n) {
     T *p = nullptr;
     if (hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
@@ -2130,17 +2294,7 @@ template <typename T> int Engine<T>::dyn_init() {
             set_error("tpat_tau must be > 0 and tpat_lscale >= 0 when tpat_amp > 0");
             return GFSDC_EINVAL;
         }
-        std::vector<double> var(nlm), vf(nl);
-        const double L = cfg.tpat_lscale, a = cfg.rerth, twopi = 2.0 * M_PI;
-        double v = 0.0;
-        for (int m = 0, k = 0; m <= T_; m++)
-            for (int n = m; n <= T_; n++, k++) {
-                var[k] = std::sqrt(T_ * std::exp(L * L * (-(double)n * (n + 1.0)) / (4.0 * a * a)));
-                double nz2 = (k == 0) ? 0.0 : 2.0 / (2.0 * n + 1.0) * (twopi / T_);
-                v += (m != 0 ? 1.0 : 0.5) * var[k] * var[k] * nz2;
-            }
-        v /= twopi;
-        for (double &x : var) x /= std::sqrt(v);
+        std::vector<double> var = pattern_varspec(T_, cfg.tpat_lscale, cfg.rerth), vf(nl);
         for (int k = 0; k < nl; k++) {
             const double sl = B->sl[k];
             vf[k] = 1.0;
@@ -2293,7 +2447,9 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
                     B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph,
-                    B->clock, B->iau_hrs, B->iau_inc, B->iau_F};
+                    B->clock, B->iau_hrs, B->iau_inc, B->iau_F, B->st_pat2, B->st_pat[3], B->st_var, B->st_vf,
+                    B->st_ctr, B->st_grd, B->sd_pat, B->rf0, B->vcg, B->sd_vc, B->rf1, B->an_w, B->an_g,
+                    B->an_sm, B->sd_an, B->ad_an, B->rf_an};
     if (B->mlp) mlp_destroy(B->mlp);
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -2344,15 +2500,29 @@ template <typename T> struct Step {
         launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_ph, s, (const T *)B.gmean, B.slot_ph);
         mark("fft_synth");
     }
+    // dyn_run.f90:114-116: the vorticity gradient of vorticity confinement
+    void synth_vc() {
+        const Plan<T> &p = E.plan;
+        const int nf = B.nm * 2 * B.nl;
+        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sd_vc, nf, W.sx, W.ldc, 1.0, s);
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.vcg, B.ng, nf, B.rf1, s);
+        mark("synth_vc");
+    }
     void columns(int inverse_only, bool colq = false) {
         const gfsdc_cfg &c = E.cfg;
         dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
-        hipLaunchKernelGGL(k_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (T *)B.gan, (T *)B.etadot,
+        const bool vc = B.vc_on && !inverse_only;
+        if (vc) synth_vc();
+        auto kern = vc ? k_columns<T, true> : k_columns<T, false>;
+        hipLaunchKernelGGL(kern, g, dim3(256), 0, s, (const T *)B.gsyn, (T *)B.gan, (T *)B.etadot,
                            (T *)B.dlnpdt, (T *)B.psg, (T *)B.dlnpsdt, (const T *)B.dphisdx, (const T *)B.dphisdy,
                            (const T *)B.ak, (const T *)B.bk, (const T *)B.ck, (const T *)B.dbk,
                            (const T *)B.dak, (const T *)B.sinlat, B.nl, B.ntr, B.nsyn, B.nan, B.ng, E.plan.nlon, (T)c.rd,
                            (T)(c.rd / c.cp), (T)(c.cvap / c.cp - 1.0), (T)(2.0 * c.omega), inverse_only,
-                           (const cplx<T> *)B.spec, B.nfs, colq ? (T *)B.colq : (T *)nullptr);
+                           (const cplx<T> *)B.spec, B.nfs, colq ? (T *)B.colq : (T *)nullptr, (const T *)B.vcg,
+                           B.st_on[0] ? (const T *)B.st_grd : (const T *)nullptr, (const T *)B.st_vf,
+                           (T)B.sc.vcamp);
         mark(inverse_only ? "columns_inverse" : "columns");
     }
     // FP64 area sums: S(psg) -> pmean, S(b) -> bsum; pdryini = S(psg) - S(b) where 0
@@ -2427,6 +2597,8 @@ template <typename T> struct Step {
         const gfsdc_cfg &c = E.cfg;
         synth_phys();
         columns(1);
+        // run.f90:517-518: the additive noise forcing, before the physics
+        if (B.st_on[3]) stoch_forcing();
         if (c.physics == GFSDC_PHYS_SIMPLE) {
             if (B.mlp_on)
                 mlp_columns();
@@ -2453,6 +2625,14 @@ template <typename T> struct Step {
             // HS fields per member: T, vrt, div forcing and the mass-fixer field
             analyse(B.adesc_hs, B.rflag_hs, B.nm * B.nhs, (cplx<T> *)B.tphys);
             phys_update();
+        }
+        // run.f90:541-552: the streamfunction forcing into vorticity, after the physics
+        if (B.st_on[3]) {
+            const gfsdc_stoch_cfg &sc = B.sc;
+            hipLaunchKernelGGL(k_an_apply<T>, dim3(1024), dim3(256), 0, s, (cplx<T> *)B.spec,
+                               (const cplx<T> *)B.an_w, (const int *)B.degree, B.nl, B.nfs, B.nm, E.plan.nlm, c.dt,
+                               c.rerth, (sc.addnoise_kenorm && sc.addnoise_vrtonly) ? 1 : 0);
+            mark("stoch_apply");
         }
     }
     SimpleCfg simple_cfg() const {
@@ -2528,7 +2708,7 @@ template <typename T> struct Step {
                            (cplx<T> *)B.tpat, (const double *)B.tpat_var, (const int2 *)p.kmn, p.nlm,
                            (const unsigned long long *)B.tpat_ctr,
                            (unsigned long long)(long long)E.cfg.tpat_seed, phi, f,
-                           std::sqrt(2.0 * M_PI / p.ntrunc));
+                           std::sqrt(2.0 * M_PI / p.ntrunc), 1, 1ULL << 40);
         hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, s, B.tpat_ctr);
         mark("pattern_advance");
     }
@@ -2539,11 +2719,83 @@ template <typename T> struct Step {
                            (T)(E.cfg.dt / E.cfg.tpat_tau));
         mark("pattern_apply");
     }
+    // one advance of stochastic pattern kind kd (phi = 0 and f = stdev start it)
+    void stoch_advance(int kd, double phi, double f) {
+        const Plan<T> &p = E.plan;
+        hipLaunchKernelGGL(k_pattern_advance<T>, dim3((unsigned)((p.nlm + 255) / 256), B.nm * B.st_nf[kd]),
+                           dim3(256), 0, s, (cplx<T> *)B.st_pat[kd], (const double *)B.st_var + (long)kd * p.nlm,
+                           (const int2 *)p.kmn, p.nlm, (const unsigned long long *)B.st_ctr + kd, B.st_seed[kd], phi,
+                           f, std::sqrt(2.0 * M_PI / p.ntrunc), B.st_nf[kd], (3ULL + kd) << 40);
+        hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, s, B.st_ctr + kd);
+        mark("stoch_pattern");
+    }
+    // grids of the svc, sppt (logit transform) and shum patterns
+    void stoch_grids() {
+        const Plan<T> &p = E.plan;
+        const int nf = 3 * B.nm;
+        launch_synth_gather(p, (const cplx<T> *)B.st_pat2, p.nlm, 1, B.sd_pat, nf, W.sx, W.ldc, 1.0, s);
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.st_grd, B.ng, nf, B.rf0, s);
+        if (B.st_on[1])
+            hipLaunchKernelGGL(k_logit<T>, dim3(512), dim3(256), 0, s, (T *)B.st_grd + (long)B.nm * B.ng,
+                               (long)B.nm * B.ng);
+        mark("stoch_grids");
+    }
+    // run.f90:272-292: the patterns used this step
+    void stoch_patterns() {
+        for (int kd = 0; kd < 4; kd++)
+            if (B.st_on[kd])
+                stoch_advance(kd, B.st_phi[kd], B.st_sd[kd] * std::sqrt(1.0 - B.st_phi[kd] * B.st_phi[kd]));
+        if (B.st_on[0] || B.st_on[1] || B.st_on[2]) stoch_grids();
+    }
+    // getstochforcing (stoch_data.f90:254-354) on the physics pass's grid
+    // fields: the streamfunction forcing into an_w group 0
+    void stoch_forcing() {
+        const Plan<T> &p = E.plan;
+        const gfsdc_stoch_cfg &sc = B.sc;
+        const int nf = B.nm * B.nl;
+        const long tot = (long)nf * p.nlm, gtot = (long)nf * B.ng;
+        cplx<T> *w = (cplx<T> *)B.an_w;
+        T *g = (T *)B.an_g;
+        const int diss = sc.addnoise_dissfact != 0;
+        hipLaunchKernelGGL(k_an_prep<T>, dim3(1024), dim3(256), 0, s, w, (const cplx<T> *)B.st_pat[3],
+                           (const cplx<T> *)B.spec, (const T *)B.disspec, B.nl, B.nfs, B.nm, p.nlm,
+                           sc.addnoise_vfilt > 0 ? 1 : 0, diss);
+        if (diss) {
+            // the dissipation wind and the noise on the grid (groups 0, 1, 2)
+            launch_synth_gather(p, w, p.nlm, 1, B.sd_an, 3 * nf, W.sx, W.ldc, 1.0, s);
+            launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 6 * nf, s);
+            launch_fft_synth(p, W.fs, W.ldc, g, B.ng, 3 * nf, B.rf_an, s);
+            hipLaunchKernelGGL(k_an_kediss<T>, dim3(2048), dim3(256), 0, s, g, (const T *)B.gsyn, B.nl, B.nsyn, B.nm,
+                               B.ng);
+            // spectral smoothing of sqrt(ke dissipation): group 0 grid -> group 1 spectral -> group 1 grid
+            launch_fft_anal(p, g, B.ng, nf, B.rf0, W.fa, W.ldc, s);
+            launch_leg_anal(p, W.fa, W.ldc, W.sx, W.ldc, 2 * nf, s);
+            launch_anal_scatter(p, W.sx, W.ldc, B.ad_an, nf, w + tot, p.nlm, 1, E.cfg.rerth, s);
+            hipLaunchKernelGGL(k_spec_scale<T>, dim3(1024), dim3(256), 0, s, w + tot, (const T *)B.an_sm, (long)nf,
+                               p.nlm);
+            launch_synth_gather(p, w, p.nlm, 1, B.sd_an + 3 * nf, nf, W.sx, W.ldc, 1.0, s);
+            launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+            launch_fft_synth(p, W.fs, W.ldc, g + gtot, B.ng, nf, B.rf0, s);
+        } else {
+            launch_synth_gather(p, w, p.nlm, 1, B.sd_an + 4 * nf, nf, W.sx, W.ldc, 1.0, s);
+            launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+            launch_fft_synth(p, W.fs, W.ldc, g + 2 * gtot, B.ng, nf, B.rf0, s);
+        }
+        hipLaunchKernelGGL(k_an_modulate<T>, dim3(2048), dim3(256), 0, s, g, (const T *)(g + 2 * gtot),
+                           diss ? (const T *)(g + gtot) : (const T *)nullptr, (const T *)B.st_vf + 3L * B.nl, B.nl,
+                           B.nm, B.ng);
+        launch_fft_anal(p, g, B.ng, nf, B.rf0, W.fa, W.ldc, s);
+        launch_leg_anal(p, W.fa, W.ldc, W.sx, W.ldc, 2 * nf, s);
+        launch_anal_scatter(p, W.sx, W.ldc, B.ad_an, nf, w, p.nlm, 1, E.cfg.rerth, s);
+        mark("stoch_forcing");
+    }
     void step() {
         const bool pat = B.tpat != nullptr;
         if (B.iau_n) iau_forcing();
-        // run.f90:272-292: the random pattern used this step
+        // run.f90:272-292: the random patterns used this step
         if (pat) pattern_advance(B.tpat_phi, E.cfg.tpat_amp * std::sqrt(1.0 - B.tpat_phi * B.tpat_phi));
+        stoch_patterns();
         long n = 2L * E.plan.nlm * B.nfs * B.nm;
         hipLaunchKernelGGL(k_copy<T>, dim3(2048), dim3(256), 0, s, (T *)B.spec0, (const T *)B.spec, n);
         mark("save_state");
@@ -3245,6 +3497,201 @@ template <typename T> int Engine<T>::dfi(int ntdfi, int use_graph) {
     return set_time(t0 + ntdfi * cfg.dt);
 }
 
+// ---- stochastic physics (stoch_data.f90:37-238 init_stochdata) -----------------
+// vertical factors: the sigma 0.10 -> 0.05 taper (svc :63-71, sppt :85-93,
+// addnoise :128-136) and exp((sl - 1) / 0.2) for shum (:103-106)
+static std::vector<double> stoch_vfact(int kd, const std::vector<double> &sl) {
+    std::vector<double> vf(sl.size());
+    const double b = SGLF(0.10), t = SGLF(0.05);
+    for (size_t k = 0; k < sl.size(); k++) {
+        if (kd == 2) {
+            vf[k] = std::exp((sl[k] - 1.) / SGLF(0.2));
+            continue;
+        }
+        vf[k] = 1.0;
+        if (sl[k] < b && sl[k] > t) vf[k] = (sl[k] - t) / (b - t);
+        else if (sl[k] <= t) vf[k] = 0.0;
+    }
+    return vf;
+}
+
+template <typename T> int Engine<T>::stoch_init(const gfsdc_stoch_cfg *sc) {
+    DycoreBuffers *B = dyn;
+    if (!B || !sc) {
+        set_error(!B ? "context has no dycore" : "null argument");
+        return !B ? GFSDC_ESTATE : GFSDC_EINVAL;
+    }
+    const double amp[4] = {sc->svc, sc->sppt, sc->shum, sc->addnoise};
+    const double tau[4] = {sc->svc_tau, sc->sppt_tau, sc->shum_tau, sc->addnoise_tau};
+    const double lsc[4] = {sc->svc_lscale, sc->sppt_lscale, sc->shum_lscale, sc->addnoise_lscale};
+    const int seed[4] = {sc->iseed_svc, sc->iseed_sppt, sc->iseed_shum, sc->iseed_addnoise};
+    for (int kd = 0; kd < 4; kd++)
+        if (amp[kd] > FLT_MIN && (!(tau[kd] > 0) || lsc[kd] < 0)) {
+            set_error("a stochastic scheme with amplitude > 0 needs tau > 0 and lscale >= 0");
+            return GFSDC_EINVAL;
+        }
+    if (sc->addnoise > FLT_MIN && !sc->addnoise_vrtonly) {
+        set_error("addnoise_vrtonly = 0 needs the GSI balance operator file (berror_stats), which is not supported");
+        return GFSDC_EINVAL;
+    }
+    if (sc->addnoise > FLT_MIN && sc->addnoise_dissfact && !(sc->addnoise_lscale > 0)) {
+        set_error("addnoise_dissfact needs addnoise_lscale > 0");
+        return GFSDC_EINVAL;
+    }
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    void *old[] = {B->st_pat2, B->st_pat[3], B->st_var, B->st_vf, B->st_ctr, B->st_grd, B->sd_pat, B->rf0,
+                   B->vcg, B->sd_vc, B->rf1, B->an_w, B->an_g, B->an_sm, B->sd_an, B->ad_an, B->rf_an};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    B->st_pat2 = B->st_pat[0] = B->st_pat[1] = B->st_pat[2] = B->st_pat[3] = nullptr;
+    B->st_var = nullptr;
+    B->st_vf = B->st_grd = B->vcg = B->an_w = B->an_g = B->an_sm = nullptr;
+    B->st_ctr = nullptr;
+    B->sd_pat = B->sd_vc = B->sd_an = nullptr;
+    B->ad_an = nullptr;
+    B->rf0 = B->rf1 = B->rf_an = nullptr;
+    B->vc_on = false;
+    for (int kd = 0; kd < 4; kd++) B->st_on[kd] = 0;
+    drop_graph(B);
+    B->sc = *sc;
+    const int nl = B->nl, nm = B->nm, nlm = plan.nlm, T_ = plan.ntrunc;
+    const long ng = B->ng;
+    bool any = false;
+    for (int kd = 0; kd < 4; kd++) {
+        B->st_on[kd] = amp[kd] > FLT_MIN;
+        any = any || B->st_on[kd];
+    }
+    B->vc_on = sc->vcamp > FLT_MIN || sc->svc > FLT_MIN;
+    const int nf = nm * nl;
+    int rc;
+    if (any) {
+        std::vector<double> var(4 * (size_t)nlm, 0.0), vf(4 * (size_t)nl, 0.0);
+        for (int kd = 0; kd < 4; kd++) {
+            if (!B->st_on[kd]) continue;
+            const std::vector<double> v = pattern_varspec(T_, lsc[kd], cfg.rerth), f = stoch_vfact(kd, B->sl);
+            std::copy(v.begin(), v.end(), var.begin() + (long)kd * nlm);
+            std::copy(f.begin(), f.end(), vf.begin() + (long)kd * nl);
+            B->st_nf[kd] = kd == 3 ? nl : 1;
+            B->st_phi[kd] = std::exp(-cfg.dt / tau[kd]);
+            B->st_sd[kd] = amp[kd];
+            B->st_seed[kd] = (unsigned long long)(long long)seed[kd];
+        }
+        void *pv;
+        if ((rc = dupload<double>(&pv, var, stream))) return rc;
+        B->st_var = (double *)pv;
+        if ((rc = dupload<T>(&B->st_vf, castv<T>(vf), stream))) return rc;
+        B->st_pat2 = dalloc<cplx<T>>(3 * (size_t)nm * nlm);
+        B->st_ctr = dalloc<unsigned long long>(4);
+        B->st_grd = dalloc<T>(3 * (size_t)nm * ng);
+        std::vector<SynthDesc> sd(3 * (size_t)nm);
+        for (int i = 0; i < 3 * nm; i++) sd[i] = SynthDesc{SK_COPY, i, -1, 1.0, 0.0, 0, 0};
+        if ((rc = dupload<SynthDesc>(&pv, sd, stream))) return rc;
+        B->sd_pat = (SynthDesc *)pv;
+        if ((rc = dupload<unsigned char>(&pv, std::vector<unsigned char>(std::max(3 * nm, 3 * nf), 0), stream)))
+            return rc;
+        B->rf0 = (unsigned char *)pv;
+        if (!B->st_pat2 || !B->st_ctr || !B->st_grd) {
+            set_error("device allocation failed for the stochastic patterns");
+            return GFSDC_ENOMEM;
+        }
+        GFSDC_CHECK(hipMemsetAsync(B->st_pat2, 0, sizeof(cplx<T>) * 3 * nm * nlm, stream));
+        GFSDC_CHECK(hipMemsetAsync(B->st_ctr, 0, 4 * sizeof(unsigned long long), stream));
+        for (int kd = 0; kd < 3; kd++) B->st_pat[kd] = (cplx<T> *)B->st_pat2 + (long)kd * nm * nlm;
+    }
+    if (B->st_on[3]) {
+        B->st_pat[3] = dalloc<cplx<T>>((size_t)nf * nlm);
+        B->an_w = dalloc<cplx<T>>(3 * (size_t)nf * nlm);
+        B->an_g = dalloc<T>(3 * (size_t)nf * ng);
+        if (!B->st_pat[3] || !B->an_w || !B->an_g) {
+            set_error("device allocation failed for the additive noise");
+            return GFSDC_ENOMEM;
+        }
+        GFSDC_CHECK(hipMemsetAsync(B->st_pat[3], 0, sizeof(cplx<T>) * nf * nlm, stream));
+        GFSDC_CHECK(hipMemsetAsync(B->an_w, 0, sizeof(cplx<T>) * 3 * nf * nlm, stream));
+        // getstochforcing :269-276: Gaussian smoothing at three times the noise scale
+        std::vector<double> sm(nlm, 1.0);
+        if (sc->addnoise_dissfact) {
+            const double rnn1 = 2. * cfg.rerth / (3. * sc->addnoise_lscale), rnn0 = rnn1 * (rnn1 + 1.);
+            for (int m = 0, k = 0; m <= T_; m++)
+                for (int n = m; n <= T_; n++, k++) sm[k] = std::exp(-(n * (n + 1.) / rnn0));
+        }
+        if ((rc = dupload<T>(&B->an_sm, castv<T>(sm), stream))) return rc;
+        // descriptors: [ud, vd, psi] grids of the wind of (-disspec vrt, -disspec div)
+        // and of the noise, then group-1 and group-0 copies
+        std::vector<SynthDesc> d(5 * (size_t)nf);
+        std::vector<unsigned char> rf(3 * (size_t)nf, 0);
+        std::vector<AnalDesc> ad(nf);
+        const double a = cfg.rerth;
+        for (int i = 0; i < nf; i++) {
+            d[i] = SynthDesc{SK_UHAT, nf + i, 2 * nf + i, a, a, 1, 1};   // getuv (stoch_data.f90:280)
+            d[nf + i] = SynthDesc{SK_VHAT, nf + i, 2 * nf + i, a, a, 1, 1};
+            d[2 * nf + i] = SynthDesc{SK_COPY, i, -1, 1.0, 0.0, 0, 0};
+            d[3 * nf + i] = SynthDesc{SK_COPY, nf + i, -1, 1.0, 0.0, 0, 0};
+            d[4 * nf + i] = SynthDesc{SK_COPY, i, -1, 1.0, 0.0, 0, 0};
+            rf[i] = rf[nf + i] = 1;
+            ad[i] = AnalDesc{AK_COPY, i, -1, 1.0, 0, 0, -1, 0.0};
+        }
+        void *pv;
+        if ((rc = dupload<SynthDesc>(&pv, d, stream))) return rc;
+        B->sd_an = (SynthDesc *)pv;
+        if ((rc = dupload<AnalDesc>(&pv, ad, stream))) return rc;
+        B->ad_an = (AnalDesc *)pv;
+        if ((rc = dupload<unsigned char>(&pv, rf, stream))) return rc;
+        B->rf_an = (unsigned char *)pv;
+    }
+    if (B->vc_on) {
+        B->vcg = dalloc<T>(2 * (size_t)nf * ng);
+        if (!B->vcg) {
+            set_error("device allocation failed for the vorticity gradient");
+            return GFSDC_ENOMEM;
+        }
+        // getgrad(vrtspec) of each level: sphtor_to_spat(0, vrt) / a
+        std::vector<SynthDesc> d(2 * (size_t)nf);
+        for (int mem = 0; mem < nm; mem++)
+            for (int k = 0; k < nl; k++) {
+                d[(size_t)mem * 2 * nl + k] = SynthDesc{SK_UHAT, -1, mem * B->nfs + k, 0.0, 1.0 / cfg.rerth, 0, 0};
+                d[(size_t)mem * 2 * nl + nl + k] = SynthDesc{SK_VHAT, -1, mem * B->nfs + k, 0.0, 1.0 / cfg.rerth, 0, 0};
+            }
+        void *pv;
+        if ((rc = dupload<SynthDesc>(&pv, d, stream))) return rc;
+        B->sd_vc = (SynthDesc *)pv;
+        if ((rc = dupload<unsigned char>(&pv, std::vector<unsigned char>(2 * (size_t)nf, 1), stream))) return rc;
+        B->rf1 = (unsigned char *)pv;
+    }
+    if (any) {
+        // stdev x varspectrum x noise, then 5 e-folding times of spin-up, and the grids
+        Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+        for (int kd = 0; kd < 4; kd++) {
+            if (!B->st_on[kd]) continue;
+            st.stoch_advance(kd, 0.0, B->st_sd[kd]);
+            const int nspinup = (int)(5.0 * tau[kd] / cfg.dt);
+            for (int i = 0; i < nspinup; i++)
+                st.stoch_advance(kd, B->st_phi[kd], B->st_sd[kd] * std::sqrt(1.0 - B->st_phi[kd] * B->st_phi[kd]));
+        }
+        st.stoch_grids();
+    }
+    GFSDC_CHECK(hipGetLastError());
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename T> int Engine<T>::get_stoch(int which, void *out, int dtype, int loc) {
+    DycoreBuffers *B = dyn;
+    if (!B) {
+        set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    const int kd = which < 3 ? which : which < 6 ? which - 3 : 3;
+    if (which < 0 || which > 7 || !B->st_on[kd]) {
+        set_error("no such stochastic field, or its scheme is off");
+        return GFSDC_EINVAL;
+    }
+    const size_t nm = B->nm, nlm = plan.nlm, nl = B->nl;
+    if (which < 3) return output((const T *)B->st_grd + (long)which * nm * B->ng, out, nm * B->ng, dtype, loc, 1);
+    if (which < 6) return output((const T *)B->st_pat[kd], out, 2 * nm * nlm, dtype, loc, 1);
+    return output((const T *)(which == 6 ? B->st_pat[3] : B->an_w), out, 2 * nm * nl * nlm, dtype, loc, 1);
+}
+
 // ---- checkpoint / resume ----------------------------------------------------
 // One contiguous image of everything the next steps read that the caller did
 // not configure: the spectral state, orography, pdryini, the clock, the
@@ -3253,7 +3700,7 @@ template <typename T> int Engine<T>::dfi(int ntdfi, int use_graph) {
 namespace {
 struct CkptHeader {
     char magic[8];
-    int32_t version, esz, nlm, nfs, nm, nl, ntr, has_pattern, has_judge, pad0;
+    int32_t version, esz, nlm, nfs, nm, nl, ntr, has_pattern, has_judge, stoch_mask;
     double time;
     uint64_t tpat_ctr, jctr, bctr;
     char pad[128 - 8 - 11 * 4 - 8 - 3 * 8];
@@ -3262,11 +3709,21 @@ static_assert(sizeof(CkptHeader) == 128, "checkpoint header is 128 bytes");
 const char CKPT_MAGIC[8] = {'G', 'F', 'S', 'D', 'C', 'C', 'K', '1'};
 }  // namespace
 
+static int stoch_mask(const DycoreBuffers *B) {
+    int m = 0;
+    for (int kd = 0; kd < 4; kd++)
+        if (B->st_on[kd]) m |= 1 << kd;
+    return m;
+}
+
 template <typename T> size_t Engine<T>::checkpoint_size() {
     DycoreBuffers *B = dyn;
     if (!B) return 0;
     const size_t c = sizeof(cplx<T>), nlm = plan.nlm, nm = B->nm;
-    return sizeof(CkptHeader) + c * nlm * B->nfs * nm + c * nlm * nm + 8 * nm + (B->tpat ? c * nlm * nm : 0);
+    size_t n = sizeof(CkptHeader) + c * nlm * B->nfs * nm + c * nlm * nm + 8 * nm + (B->tpat ? c * nlm * nm : 0);
+    for (int kd = 0; kd < 4; kd++)
+        if (B->st_on[kd]) n += c * nlm * nm * B->st_nf[kd];
+    return n + (stoch_mask(B) ? 4 * 8 : 0);
 }
 
 template <typename T> int Engine<T>::checkpoint_save(void *buf, size_t nbytes, int loc) {
@@ -3291,6 +3748,7 @@ template <typename T> int Engine<T>::checkpoint_save(void *buf, size_t nbytes, i
     h.ntr = B->ntr;
     h.has_pattern = B->tpat ? 1 : 0;
     h.has_judge = B->jctr ? 1 : 0;
+    h.stoch_mask = stoch_mask(B);
     h.bctr = B->bctr;
     GFSDC_CHECK(hipMemcpyAsync(&h.time, B->clock, sizeof(double), hipMemcpyDeviceToHost, stream));
     if (B->tpat_ctr)
@@ -3307,7 +3765,10 @@ template <typename T> int Engine<T>::checkpoint_save(void *buf, size_t nbytes, i
         const void *p;
         size_t n;
     } secs[] = {{B->spec, c * nlm * B->nfs * nm}, {B->topo, c * nlm * nm}, {B->pdryini, 8 * nm},
-                {B->tpat, B->tpat ? c * nlm * nm : 0}};
+                {B->tpat, B->tpat ? c * nlm * nm : 0},
+                {B->st_pat[0], B->st_on[0] ? c * nlm * nm : 0}, {B->st_pat[1], B->st_on[1] ? c * nlm * nm : 0},
+                {B->st_pat[2], B->st_on[2] ? c * nlm * nm : 0},
+                {B->st_pat[3], B->st_on[3] ? c * nlm * nm * (size_t)B->nl : 0}, {B->st_ctr, h.stoch_mask ? (size_t)32 : (size_t)0}};
     for (auto &sc : secs) {
         if (!sc.n) continue;
         GFSDC_CHECK(hipMemcpyAsync(o, sc.p, sc.n, k, stream));
@@ -3336,7 +3797,7 @@ template <typename T> int Engine<T>::checkpoint_load(const void *buf, size_t nby
         return GFSDC_EINVAL;
     }
     if (h.esz != (int32_t)sizeof(T) || h.nlm != plan.nlm || h.nfs != B->nfs || h.nm != B->nm || h.nl != B->nl ||
-        h.ntr != B->ntr || h.has_pattern != (B->tpat ? 1 : 0)) {
+        h.ntr != B->ntr || h.has_pattern != (B->tpat ? 1 : 0) || h.stoch_mask != stoch_mask(B)) {
         set_error("checkpoint was written by a context of another precision, grid, level count, tracer "
                   "count, member count or pattern setting");
         return GFSDC_EINVAL;
@@ -3354,7 +3815,10 @@ template <typename T> int Engine<T>::checkpoint_load(const void *buf, size_t nby
         void *p;
         size_t n;
     } secs[] = {{B->spec, c * nlm * B->nfs * nm}, {stage[2], c * nlm * nm}, {B->pdryini, 8 * nm},
-                {B->tpat, B->tpat ? c * nlm * nm : 0}};
+                {B->tpat, B->tpat ? c * nlm * nm : 0},
+                {B->st_pat[0], B->st_on[0] ? c * nlm * nm : 0}, {B->st_pat[1], B->st_on[1] ? c * nlm * nm : 0},
+                {B->st_pat[2], B->st_on[2] ? c * nlm * nm : 0},
+                {B->st_pat[3], B->st_on[3] ? c * nlm * nm * (size_t)B->nl : 0}, {B->st_ctr, h.stoch_mask ? (size_t)32 : (size_t)0}};
     for (auto &sc : secs) {
         if (!sc.n) continue;
         GFSDC_CHECK(hipMemcpyAsync(sc.p, o, sc.n, k, stream));

@@ -94,6 +94,8 @@ template <typename T> struct Engine {
     int iau_set(int nfiles, const double *fhrs, double delthrs, const void *vrt, const void *div, const void *tv,
                 const void *trac, const void *lnps, int dtype, int loc);
     int dfi(int ntdfi, int use_graph);
+    int stoch_init(const gfsdc_stoch_cfg *sc);
+    int get_stoch(int which, void *out, int dtype, int loc);
     size_t checkpoint_size();
     int checkpoint_save(void *buf, size_t nbytes, int loc);
     int checkpoint_load(const void *buf, size_t nbytes, int loc);

@@ -44,6 +44,20 @@ class Cfg(C.Structure):
     ]
 
 
+class StochCfg(C.Structure):
+    """gfsdc_stoch_cfg (params.f90:140-160 names)"""
+    _fields_ = [("vcamp", C.c_double),
+                ("svc", C.c_double), ("svc_tau", C.c_double), ("svc_lscale", C.c_double), ("iseed_svc", C.c_int),
+                ("sppt", C.c_double), ("sppt_tau", C.c_double), ("sppt_lscale", C.c_double),
+                ("iseed_sppt", C.c_int),
+                ("shum", C.c_double), ("shum_tau", C.c_double), ("shum_lscale", C.c_double),
+                ("iseed_shum", C.c_int),
+                ("addnoise", C.c_double), ("addnoise_tau", C.c_double), ("addnoise_lscale", C.c_double),
+                ("iseed_addnoise", C.c_int),
+                ("addnoise_vfilt", C.c_int), ("addnoise_kenorm", C.c_int), ("addnoise_dissfact", C.c_int),
+                ("addnoise_vrtonly", C.c_int)]
+
+
 SYMBOLS = [
     "shtns_use_threads_", "shtns_set_size_", "shtns_precompute_", "shtns_calc_nlm_",
     "shtns_cos_array_", "shtns_gauss_wts_", "shtns_spat_to_sh_", "shtns_sh_to_spat_",
@@ -57,7 +71,7 @@ SYMBOLS = [
     "gfsdc_mlp_nparams", "gfsdc_mlp_params", "gfsdc_mlp_forward", "gfsdc_mlp_grad", "gfsdc_mlp_adamw",
     "gfsdc_get_grid", "gfsdc_get_semimp",
     "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_set_time",
-    "gfsdc_get_time", "gfsdc_iau_set", "gfsdc_dfi", "gfsdc_checkpoint_size", "gfsdc_checkpoint_save",
+    "gfsdc_get_time", "gfsdc_iau_set", "gfsdc_dfi", "gfsdc_stoch_defaults", "gfsdc_stoch_init", "gfsdc_get_stoch", "gfsdc_checkpoint_size", "gfsdc_checkpoint_save",
     "gfsdc_checkpoint_load", "gfsdc_sync",
     "gfsdc_stream", "gfsdc_profile_step",
 ]
@@ -122,6 +136,9 @@ def load():
     L.gfsdc_get_time.argtypes = [vp, C.POINTER(d)]
     L.gfsdc_iau_set.argtypes = [vp, i, vp, d, vp, vp, vp, vp, vp, i, i]
     L.gfsdc_dfi.argtypes = [vp, i, i]
+    L.gfsdc_stoch_defaults.argtypes = [C.POINTER(StochCfg)]
+    L.gfsdc_stoch_init.argtypes = [vp, C.POINTER(StochCfg)]
+    L.gfsdc_get_stoch.argtypes = [vp, i, vp, i, i]
     L.gfsdc_checkpoint_size.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.gfsdc_checkpoint_save.argtypes = [vp, vp, C.c_size_t, i]
     L.gfsdc_checkpoint_load.argtypes = [vp, vp, C.c_size_t, i]
@@ -451,6 +468,31 @@ class Context:
 
     def dfi(self, ntdfi, use_graph=False):
         _check(load().gfsdc_dfi(self.h, ntdfi, int(use_graph)))
+
+    STOCH = ["svc", "sppt", "shum", "svc_spec", "sppt_spec", "shum_spec", "addnoise", "psiforcing"]
+
+    def stoch_init(self, **kw):
+        """init_stochdata with the params.f90 stochastic settings given as keywords"""
+        c = StochCfg()
+        load().gfsdc_stoch_defaults(C.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        self._stoch = c
+        _check(load().gfsdc_stoch_init(self.h, C.byref(c)))
+
+    def stoch(self, which):
+        """'svc', 'sppt', 'shum' grids (nmem, nlat, nlon); their spectral patterns
+        'svc_spec', 'sppt_spec', 'shum_spec' (nmem, nlm); 'addnoise' patterns and
+        the last step's 'psiforcing' (nmem, nlev, nlm)"""
+        w = self.STOCH.index(which)
+        if w < 3:
+            out = np.zeros((self.nmem, self.nlat, self.nlon))
+        elif w < 6:
+            out = np.zeros((self.nmem, self.nlm), np.complex128)
+        else:
+            out = np.zeros((self.nmem, self.nlev, self.nlm), np.complex128)
+        _check(load().gfsdc_get_stoch(self.h, w, _p(out), DT_F64, HOST))
+        return out
 
     def checkpoint(self):
         """the device state as one byte image (numpy uint8)"""

@@ -271,6 +271,38 @@ int gfsdc_step_diag(gfsdc_ctx *ctx, double *out /* 4 * nmem */);
  * vorticity of vort_rms (s^-1), member i seeded with seed + i */
 int gfsdc_init_rest_state(gfsdc_ctx *ctx, double t0, double ps0, double vort_rms, int nmin,
                           int nmax, unsigned long long seed);
+/* stochastic physics, the params.f90:140-160 namelist entries (an amplitude of
+ * 0 switches a scheme off): vorticity confinement vcamp (m/s) modulated by
+ * the svc pattern (dyn_run.f90:84-117,230-248); the sppt and shum patterns of
+ * the GFS physics (their grids are made each step, run.f90:277-286, and handed
+ * out by gfsdc_get_stoch; the GFS physics that applies them is not part of
+ * this library); additive streamfunction noise addnoise per level
+ * (stoch_data.f90:254-354, run.f90:288-292,514-518,541-552; the balanced
+ * temperature part needs addnoise_vrtonly = 0 and a GSI balance file, and is
+ * refused). Patterns are drawn from the counter RNG with seed iseed_* + member
+ * (the reference seeds a Mersenne twister, 0 meaning the clock). */
+typedef struct {
+    double vcamp;
+    double svc, svc_tau, svc_lscale;
+    int iseed_svc;
+    double sppt, sppt_tau, sppt_lscale;
+    int iseed_sppt;
+    double shum, shum_tau, shum_lscale;
+    int iseed_shum;
+    double addnoise, addnoise_tau, addnoise_lscale;
+    int iseed_addnoise;
+    int addnoise_vfilt, addnoise_kenorm, addnoise_dissfact, addnoise_vrtonly;
+} gfsdc_stoch_cfg;
+/* params.f90 defaults: everything off, addnoise_vrtonly = 1 */
+void gfsdc_stoch_defaults(gfsdc_stoch_cfg *s);
+/* replaces stoch_data.f90:37 init_stochdata: sets up and spins up the patterns
+ * (5 e-folding times) of every scheme switched on; may be called again */
+int gfsdc_stoch_init(gfsdc_ctx *ctx, const gfsdc_stoch_cfg *s);
+/* which: 0..2 grids of svc, sppt (after its logit transform), shum
+ * (nlon, nlat, nmem); 3..5 their spectral patterns (ndimspec, nmem); 6 the
+ * addnoise patterns (ndimspec, nlev, nmem); 7 the last step's streamfunction
+ * forcing psiforcing (ndimspec, nlev, nmem) */
+int gfsdc_get_stoch(gfsdc_ctx *ctx, int which, void *out, int dtype, int loc);
 /* model clock (s) at the start of the next step: run.f90:53 t = tstart
  * (params.f90: the initial file's fhour * 3600), advanced by dt per step
  * (run.f90:153); it times the IAU forcing */
