@@ -9,6 +9,8 @@ module gfsdc_dycore
   private
   public :: gfsdc_cfg, dyc_defaults, dyc_cfg_size, dyc_check_abi, dyc_create, dyc_destroy, dyc_set_state, dyc_get_state, &
             dyc_getdyntend, dyc_advance, dyc_step_diag, dyc_init_rest_state, dyc_sync, &
+            dyc_set_time, dyc_get_time, dyc_dfi, dyc_iau_set, dyc_checkpoint_size, dyc_checkpoint_save, &
+            dyc_checkpoint_load, gfsdc_stoch_cfg, dyc_stoch_defaults, dyc_stoch_init, dyc_get_stoch, &
             GFSDC_FP64, GFSDC_FP32, GFSDC_HOST, DT_F64, DT_F32, PHYS_NONE, PHYS_HS
 
   integer(c_int), parameter :: GFSDC_FP64 = 0, GFSDC_FP32 = 1, GFSDC_HOST = 0, DT_F64 = 0, DT_F32 = 1
@@ -25,6 +27,20 @@ module gfsdc_dycore
     integer(c_int) :: simple_test, turbflux
     real(c_double) :: tpat_amp, tpat_tau, tpat_lscale
     integer(c_int) :: tpat_seed
+  end type
+
+  ! stochastic physics settings (params.f90:140-160 names, include/gfsdc.h)
+  type, bind(c) :: gfsdc_stoch_cfg
+    real(c_double) :: vcamp
+    real(c_double) :: svc, svc_tau, svc_lscale
+    integer(c_int) :: iseed_svc
+    real(c_double) :: sppt, sppt_tau, sppt_lscale
+    integer(c_int) :: iseed_sppt
+    real(c_double) :: shum, shum_tau, shum_lscale
+    integer(c_int) :: iseed_shum
+    real(c_double) :: addnoise, addnoise_tau, addnoise_lscale
+    integer(c_int) :: iseed_addnoise
+    integer(c_int) :: addnoise_vfilt, addnoise_kenorm, addnoise_dissfact, addnoise_vrtonly
   end type
 
   interface
@@ -81,6 +97,64 @@ module gfsdc_dycore
     integer(c_int) function dyc_sync(h) bind(c, name="gfsdc_sync")
       import :: c_ptr, c_int
       type(c_ptr), value :: h
+    end function
+    ! model clock (s): run.f90:53 t = tstart, t = t + dt per step
+    integer(c_int) function dyc_set_time(h, t) bind(c, name="gfsdc_set_time")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: t
+    end function
+    integer(c_int) function dyc_get_time(h, t) bind(c, name="gfsdc_get_time")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: t
+    end function
+    ! run.f90:58-145 digital filter initialisation (2 ntdfi steps)
+    integer(c_int) function dyc_dfi(h, ntdfi, use_graph) bind(c, name="gfsdc_dfi")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: ntdfi, use_graph
+    end function
+    ! iau_module.f90 init_iau with the increments formed by the caller
+    integer(c_int) function dyc_iau_set(h, nfiles, fhrs, delthrs, vrt, div, tv, trac, lnps, dtype, loc) &
+        bind(c, name="gfsdc_iau_set")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h, fhrs, vrt, div, tv, trac, lnps
+      integer(c_int), value :: nfiles, dtype, loc
+      real(c_double), value :: delthrs
+    end function
+    ! device checkpoint / resume
+    integer(c_int) function dyc_checkpoint_size(h, nbytes) bind(c, name="gfsdc_checkpoint_size")
+      import :: c_ptr, c_int, c_size_t
+      type(c_ptr), value :: h
+      integer(c_size_t), intent(out) :: nbytes
+    end function
+    integer(c_int) function dyc_checkpoint_save(h, buf, nbytes, loc) bind(c, name="gfsdc_checkpoint_save")
+      import :: c_ptr, c_int, c_size_t
+      type(c_ptr), value :: h, buf
+      integer(c_size_t), value :: nbytes
+      integer(c_int), value :: loc
+    end function
+    integer(c_int) function dyc_checkpoint_load(h, buf, nbytes, loc) bind(c, name="gfsdc_checkpoint_load")
+      import :: c_ptr, c_int, c_size_t
+      type(c_ptr), value :: h, buf
+      integer(c_size_t), value :: nbytes
+      integer(c_int), value :: loc
+    end function
+    ! stoch_data.f90 init_stochdata
+    subroutine dyc_stoch_defaults(s) bind(c, name="gfsdc_stoch_defaults")
+      import :: gfsdc_stoch_cfg
+      type(gfsdc_stoch_cfg), intent(inout) :: s
+    end subroutine
+    integer(c_int) function dyc_stoch_init(h, s) bind(c, name="gfsdc_stoch_init")
+      import :: c_ptr, c_int, gfsdc_stoch_cfg
+      type(c_ptr), value :: h
+      type(gfsdc_stoch_cfg), intent(in) :: s
+    end function
+    integer(c_int) function dyc_get_stoch(h, which, out, dtype, loc) bind(c, name="gfsdc_get_stoch")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, out
+      integer(c_int), value :: which, dtype, loc
     end function
     ! AR(1) temperature pattern of every member (ndimspec complex per member)
     integer(c_int) function dyc_get_pattern(h, spec, dtype, loc) bind(c, name="gfsdc_get_pattern")

@@ -3,7 +3,10 @@
 ! libgfsdc and prints run.f90's per-step diagnostic line (run.f90:159-163).
 ! Also exercises module shtns (the transform wrapper) with a round trip and a
 ! solid-body rotation through getuv.
-! Usage: gfsdc_hs_driver [nsteps] [T42L20|T254L64] [fp64|fp32]
+! With a fourth argument ntdfi > 0 the run starts with run.f90's digital
+! filter window of 2 ntdfi steps (run.f90:58-145) and continues from the
+! window's middle.
+! Usage: gfsdc_hs_driver [nsteps] [T42L20|T254L64] [fp64|fp32] [ntdfi]
 !        gfsdc_hs_driver abi   (layout check of gfsdc_cfg only; no GPU call)
 program hs_driver
   use iso_c_binding
@@ -13,7 +16,7 @@ program hs_driver
   implicit none
   type(gfsdc_cfg) :: cfg
   type(c_ptr) :: h
-  integer :: nsteps, nt, k, nlev
+  integer :: nsteps, nt, k, nlev, ntdfi
   character(len=16) :: arg, cname, prec
   real(c_double), allocatable, target :: ak(:), bk(:), diag(:)
   complex(r_kind), allocatable :: spec(:), spec2(:), zero(:)
@@ -34,6 +37,11 @@ program hs_driver
   end if
   if (command_argument_count() >= 2) call get_command_argument(2, cname)
   if (command_argument_count() >= 3) call get_command_argument(3, prec)
+  ntdfi = 0
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, arg)
+    read (arg, *) ntdfi
+  end if
   call dyc_defaults(cfg)
   if (cname == 'T254L64') then
     cfg%nlon = 768; cfg%nlat = 384; cfg%ntrunc = 254; cfg%nlev = 64; cfg%dt = 600.d0
@@ -81,13 +89,19 @@ program hs_driver
   call getuv(spec, zero, ug, vg, real(cfg%rerth, r_kind))
   err = max(maxval(abs(ug - u0 * cos(lats))), maxval(abs(vg))) / u0
   print '(a,es10.3)', 'shtns getuv solid-body max error = ', err
-  t = 0.d0
+  if (ntdfi > 0) then
+    ! run.f90:58-145: the filter window on the device, then the state reset to its
+    ! weighted average and the clock to the window's middle
+    if (dyc_dfi(h, ntdfi, 1) /= 0) stop 'dfi failed'
+    if (dyc_get_time(h, t) /= 0) stop 'clock failed'
+    print '(a,f0.3,a)', 'done with dfi loop, restarting at t = ', t / 3600.d0, ' hrs'
+  end if
   do nt = 1, nsteps
     call cpu_time(tc0)
     if (dyc_advance(h, 1, 1) /= 0) stop 'advance failed'
     if (dyc_step_diag(h, c_loc(diag)) /= 0) stop 'diag failed'
     call cpu_time(tc1)
-    t = t + cfg%dt
+    if (dyc_get_time(h, t) /= 0) stop 'clock failed'
     write (6, 8998) t / 3600.d0, diag(1), diag(2) / 100., diag(3) / 100., diag(4), tc1 - tc0
   end do
 8998 format('t = ',f0.3,' hrs, spdmax = ',f7.3,', min/max ps = ',f7.2,'/',f7.2,', pstend = ',f0.3,', cpu time = ',f0.3)

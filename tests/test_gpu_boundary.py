@@ -218,3 +218,29 @@ def test_fortran_host_layer_hs_driver():
         assert abs(spd - d["spdmax"]) <= 6e-4 + 1e-6 * d["spdmax"], (spd, d)
         assert abs(pmin - d["psmin"] / 100) <= 6e-3 and abs(pmax - d["psmax"] / 100) <= 6e-3, (row, d)
         assert abs(pst - d["pstend"]) <= 6e-4 + 1e-6 * abs(d["pstend"]), (pst, d)
+
+
+def test_fortran_host_layer_dfi():
+    """run.f90's digital filter window through the Fortran host (dyc_dfi), then
+    three steps from the window's middle: diagnostics equal to the oracle's"""
+    drv = os.path.join(ROOT, "gfs-dycore_amd", "gfsdc_hs_driver")
+    print("running gfsdc_hs_driver 3 T42L20 fp64 2", flush=True)
+    r = subprocess.run([drv, "3", "T42L20", "fp64", "2"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"done with dfi loop, restarting at t = (\S+) hrs", r.stdout)
+    assert m and abs(float(m.group(1)) - 2 * 1200.0 / 3600.0) < 1e-3, r.stdout
+    rows = re.findall(r"t = (\S+) hrs, spdmax =\s*(\S+), min/max ps =\s*(\S+)/\s*(\S+), pstend = (\S+),", r.stdout)
+    assert len(rows) == 3, r.stdout
+    ak, bk = levels.sigma_levels(20)
+    o = O.Model(128, 64, 42, 20, ak, bk, 1200.0)
+    vrt, div, tv, lnps = cases.rest_state(o.sht.degree, o.sht.order, 20, seed=0)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    o.dfi(2)
+    for row in rows:
+        o.advance(1)
+        d = o.step_diag()
+        th, spd, pmin, pmax, pst = (float(x) for x in row)
+        assert abs(th - o.time / 3600.0) < 1e-3
+        assert abs(spd - d["spdmax"]) <= 6e-4 + 1e-6 * d["spdmax"], (spd, d)
+        assert abs(pmin - d["psmin"] / 100) <= 6e-3 and abs(pmax - d["psmax"] / 100) <= 6e-3, (row, d)
+        assert abs(pst - d["pstend"]) <= 6e-4 + 1e-6 * abs(d["pstend"]), (pst, d)
