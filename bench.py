@@ -93,9 +93,12 @@ def algorithmic_work(name, ctx, esz):
     nlat, nlon, T, nlm = ctx.nlat, ctx.nlon, ctx.ntrunc, ctx.nlm
     nl, nm = ctx.nlev, ctx.nmem
     nsyn = (7 + 3 * ctx.ntrac) * nl + 3
-    if ctx.cfg.physics != gfsdc.PHYS_NONE:
-        # three dynamics passes of nsyn fields and the physics pass, which skips
-        # the temperature and tracer gradients: the mean fields per launch
+    if ctx.cfg.physics == gfsdc.PHYS_HS:
+        # three dynamics passes of nsyn fields and the lean Held-Suarez pass of the
+        # profiled step (vorticity, divergence, temperature, ln ps): the mean per launch
+        nsyn = (3 * nsyn + 3 * nl + 1) / 4
+    elif ctx.cfg.physics != gfsdc.PHYS_NONE:
+        # the physics pass skips the temperature and tracer gradients
         nsyn = (3 * nsyn + (5 + ctx.ntrac) * nl + 3) / 4
     ng = nlat * nlon
     four = (T + 1) * nlat * 2 * esz  # Fourier coefficients of one field, both hemispheres

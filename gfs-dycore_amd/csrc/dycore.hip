@@ -146,18 +146,39 @@ struct DycoreBuffers {
     SynthDesc *sdesc_ph = nullptr;
     unsigned char *rflag_ph = nullptr;
     int *slot_ph = nullptr;
+    // the lean Held-Suarez physics pass of the steps before an advance's last:
+    // vorticity, divergence, temperature and ln ps only (nhl per member), the
+    // grids the forcing reads (phy_run_hs.f90:66-76); the winds, their
+    // gradients and getomega serve only the post-step grids and diagnostics
+    int nhl = 0;
+    SynthDesc *sdesc_hl = nullptr;
+    unsigned char *rflag_hl = nullptr;
+    int *slot_hl = nullptr;
+    // the grids after the last step are those of a lean pass (profile_step):
+    // get_grid and step_diag refuse them
+    bool grids_partial = false;
     AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr, *adesc_sp = nullptr;
     unsigned char *rflag_syn = nullptr, *rflag_an = nullptr, *rflag_hs = nullptr, *rflag_sp = nullptr;
     // host copies of the FP64 operators
     std::vector<double> amhyb, bmhyb, svhyb, tor_hyb, d_hyb_m, disspec_h, diff_h, dmp_h, sl;
     std::vector<double> ym;                       // tor (x) sv + amhyb bmhyb (semimp_data.f90:225-234)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr, graph_lean = nullptr;
+    hipGraphExec_t gexec = nullptr, gexec_lean = nullptr;
     // profiling
     bool profiling = false;
     std::vector<hipEvent_t> ev;
     std::vector<std::string> evname;
 };
+
+// drop the captured steps: the next graph advance captures them again
+static void drop_graph(DycoreBuffers *B) {
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->gexec_lean) (void)hipGraphExecDestroy(B->gexec_lean);
+    if (B->graph_lean) (void)hipGraphDestroy(B->graph_lean);
+    B->gexec = B->gexec_lean = nullptr;
+    B->graph = B->graph_lean = nullptr;
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -227,6 +248,42 @@ __device__ inline LayerCoef<T> layer_coef(T p1, T p2, T dpk, T bk1, T bk2, T ck,
     return c;
 }
 
+// getomega's layer terms, evaluated identically wherever they are needed: with
+// contraction off, each is one rounded product of the same operands, so the
+// column total of pass 1 and the suffix sums of pass 3 add the same numbers.
+template <typename T> struct OmegaTerms {
+    T cg, tdb, tcb;  // V . grad ln ps, div dp_K, cg dbk_K
+};
+template <typename T>
+__device__ __forceinline__ OmegaTerms<T> omega_terms(T u, T v, T dv, T dpx, T dpy, T dpk, T dbk) {
+#pragma clang fp contract(off)
+    OmegaTerms<T> o;
+    o.cg = u * dpx + v * dpy;
+    o.tdb = dv * dpk;
+    o.tcb = o.cg * dbk;
+    return o;
+}
+// A sum carried as an unevaluated pair hi + lo (Neumaier's compensated
+// summation with the two-sum of Knuth): its error stays at eps^2 of the summed
+// magnitudes, so a prefix sum of the top layers can be formed as the column
+// total minus the sum of the layers below without cancelling its digits.
+template <typename T> struct PairSum {
+    T hi = 0, lo = 0;
+    __device__ __forceinline__ void add(T t) {
+#pragma clang fp contract(off)
+        const T s = hi + t, bp = s - hi;
+        lo += (hi - (s - bp)) + (t - bp);
+        hi = s;
+    }
+    // this - o, rounded once
+    __device__ __forceinline__ T minus(const PairSum &o) const {
+#pragma clang fp contract(off)
+        const T s = hi - o.hi, bp = s - hi;
+        const T e = (hi - (s - bp)) + (-o.hi - bp);
+        return s + (e + (lo - o.lo));
+    }
+};
+
 // Column kernel: calc_pressdata + getomega (+ getpresgrad, getvadv and the
 // tendency assembly unless inverse_only). One thread per grid column and member.
 // Level indices: K counts interfaces/layers top -> bottom (1-based, as pk, dpk,
@@ -242,7 +299,7 @@ __global__ __launch_bounds__(256) void k_columns(
     const T *__restrict__ ck_, const T *__restrict__ dbk_, const T *__restrict__ dak_, const T *__restrict__ sinlat,
     int nl, int ntr, int nsyn, int nan, long ng, int nlon, T rd, T kappa, T cvcp, T twoom, int inverse_only,
     const cplx<T> *__restrict__ spec, int nfs, T *__restrict__ colq, const T *__restrict__ vcg,
-    const T *__restrict__ svcg, const T *__restrict__ vfsvc, T vcamp) {
+    const T *__restrict__ svcg, const T *__restrict__ vfsvc, T vcamp, int store) {
     __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1], ck[MAXLEV], dbk[MAXLEV], dak[MAXLEV];
     for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
         ak[i] = ak_[i];
@@ -270,15 +327,34 @@ __global__ __launch_bounds__(256) void k_columns(
     const T alfa1 = T(SGLF(0.69314718055994530942)), rlnp1 = T(SGLF(99999.99));
 #define LV(P, L) (P)[(long)((L) - 1) * ng]
     auto pk = [&](int K) { return ak[K - 1] + bk[K - 1] * ps; };
-    auto dp = [&](int K) { return dak[K - 1] + dbk[K - 1] * ps; };  // pk(K + 1) - pk(K)
-    // pass 1 (top -> bottom): getomega's running sums db, cb, omega/p (dlnpdtg)
-    // on layers, and etadot at interfaces less its term -ps bk_K dlnpsdt, which
-    // needs the column totals. The etadot buffer keeps that partial value; its
-    // readers add the term (passes 3 and 4 here, and k_etadot_complete for the
-    // etadot grid handed to callers)
+    auto dp = [&](int K) { return fma(dbk[K - 1], ps, dak[K - 1]); };  // pk(K + 1) - pk(K), one rounding
+    // omega / p of layer K (getomega's workb, workc) from the sums db, cb over
+    // the layers above it
+    auto omega_p = [&](int K, T dba, T cba, const OmegaTerms<T> &o, T dv, T dpk, T rdp, T rlnp, T alfa) {
+        T workb, workc;
+        if (K == 1) {
+            workb = alfa * (dv * dpk + ps * (o.cg * dbk[0]) * dbk[0]);
+            workc = ps * o.cg * dbk[0];
+        } else {
+            workb = rlnp * (dba + ps * cba) + alfa * (dv * dpk + ps * o.cg * dbk[K - 1]);
+            workc = ps * o.cg * (dbk[K - 1] + ck[K - 1] * rlnp * rdp);
+        }
+        return (workc - workb) * rdp;
+    };
+    // pass 1 (top -> bottom): getomega's running sums db, cb and their column
+    // totals. Where a caller sees the grids (store) it also writes omega/p
+    // (dlnpdtg) on layers and etadot at interfaces less its term -ps bk_K
+    // dlnpsdt, which needs the column totals; the etadot buffer keeps that
+    // partial value and its readers add the term (pass 4 here, which needs it
+    // with tracers, and k_etadot_complete for the grid handed to callers).
+    // Pass 3 recomputes both from the layer data and the compensated sums.
+    const bool st_et = store || ntr > 0;
     T db = 0, cb = 0;
-    ET[0] = T(0);
-    ET[(long)nl * ng] = T(0);
+    PairSum<T> sdb, scb;
+    if (st_et) {
+        ET[0] = T(0);
+        ET[(long)nl * ng] = T(0);
+    }
     constexpr int P1G = 4;  // levels whose loads are issued together
     for (int K0 = 1; K0 <= nl; K0 += P1G) {
         T gu[P1G], gv[P1G], gd[P1G];
@@ -294,27 +370,23 @@ __global__ __launch_bounds__(256) void k_columns(
             const int K = K0 + i;
             if (K > nl) break;
             int L = nl + 1 - K;
-            T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K), rdp = recip(dpk);
-            T rlnp = rlnp1, alfa = alfa1;
-            if (K > 1) {
-                const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
-                rlnp = lc.rlnp;
-                alfa = lc.alfa;
+            const T dpk = dp(K), dv = gd[i];
+            const OmegaTerms<T> o = omega_terms(gu[i], gv[i], dv, dpx, dpy, dpk, dbk[K - 1]);
+            if (store) {
+                T p1 = pk(K), p2 = pk(K + 1), rdp = recip(dpk);
+                T rlnp = rlnp1, alfa = alfa1;
+                if (K > 1) {
+                    const LayerCoef<T> lc = layer_coef(p1, p2, dpk, bk[K - 1], bk[K], ck[K - 1], dbk[K - 1]);
+                    rlnp = lc.rlnp;
+                    alfa = lc.alfa;
+                }
+                LV(WD, L) = omega_p(K, db, cb, o, dv, dpk, rdp, rlnp, alfa);
             }
-            T cg = gu[i] * dpx + gv[i] * dpy;
-            T dv = gd[i];
-            T workb, workc;
-            if (K == 1) {
-                workb = alfa * (dv * dpk + ps * (cg * dbk[0]) * dbk[0]);
-                workc = ps * cg * dbk[0];
-            } else {
-                workb = rlnp * (db + ps * cb) + alfa * (dv * dpk + ps * cg * dbk[K - 1]);
-                workc = ps * cg * (dbk[K - 1] + ck[K - 1] * rlnp * rdp);
-            }
-            LV(WD, L) = (workc - workb) * rdp;
-            db += dv * dpk;
-            cb += cg * dbk[K - 1];
-            if (K <= nl - 1) ET[(long)K * ng] = -(ps * cb) - db;
+            db += o.tdb;
+            cb += o.tcb;
+            sdb.add(o.tdb);
+            scb.add(o.tcb);
+            if (st_et && K <= nl - 1) ET[(long)K * ng] = -(ps * cb) - db;
         }
     }
     const T dlnpsdt = -db / ps - cb;
@@ -338,15 +410,18 @@ __global__ __launch_bounds__(256) void k_columns(
     const T f = twoom * sinlat[col / nlon];
     const T dphx = dphisdx[(long)mem * ng + col], dphy = dphisdy[(long)mem * ng + col];
     // pass 3: bottom -> top (K = nl..1): getpresgrad accumulations, getvadv, assembly.
-    // u, v, tv of the levels below / at / above L slide through registers; each
-    // interface's etadot is completed (and stored) as the top of the layer below it
+    // u, v, tv of the levels below / at / above L slide through registers. The
+    // getomega sums over the layers above K are the column totals less the sums
+    // over K..nl (rdb, rcb), from which omega / p of the layer and etadot at its
+    // top interface follow; etadot at its bottom interface is the previous top.
     T cofb_acc = 0, px3u = 0, px3v = 0;
     T um = 0, vm = 0, tm = 0;                          // level L - 1
     T u = LV(U, 1), v = LV(V, 1), tv = LV(TV, 1);      // level L
     T eb = T(0);                                       // etadot at the layer's bottom interface
+    PairSum<T> rdb, rcb;
     constexpr int P3G = 2;  // levels whose loads are issued together
     for (int K0 = nl; K0 >= 1; K0 -= P3G) {
-        T gup[P3G], gvp[P3G], gtp[P3G], gtx[P3G], gty[P3G], gwd[P3G], gvr[P3G];
+        T gup[P3G], gvp[P3G], gtp[P3G], gtx[P3G], gty[P3G], gdv[P3G], gvr[P3G];
 #pragma unroll
         for (int i = 0; i < P3G; i++) {
             const int Lc = min(nl + 1 - (K0 - i), nl), Lp = min(Lc + 1, nl);  // clamped to the top level
@@ -355,7 +430,7 @@ __global__ __launch_bounds__(256) void k_columns(
             gtp[i] = LV(TV, Lp);
             gtx[i] = LV(TX, Lc);
             gty[i] = LV(TY, Lc);
-            gwd[i] = LV(WD, Lc);
+            gdv[i] = LV(DIV, Lc);
             gvr[i] = LV(VRT, Lc);
         }
 #pragma unroll
@@ -369,11 +444,14 @@ __global__ __launch_bounds__(256) void k_columns(
                 vp = gvp[i];
                 tp = gtp[i];
             }
-            T ea = T(0);                                   // etadot at the top interface K - 1
-            if (K > 1) {
-                ea = ET[(long)(K - 1) * ng] + et_fix(K - 1);
-            }
             T p1 = pk(K), p2 = pk(K + 1), dpk = dp(K), rdp = recip(dpk);
+            const T dv = gdv[i];
+            const OmegaTerms<T> o = omega_terms(u, v, dv, dpx, dpy, dpk, dbk[K - 1]);
+            rdb.add(o.tdb);
+            rcb.add(o.tcb);
+            const T dba = sdb.minus(rdb), cba = scb.minus(rcb);  // sums over the layers above K
+            T ea = T(0);                                   // etadot at the top interface K - 1
+            if (K > 1) ea = (-(ps * cba) - dba) + et_fix(K - 1);
             T rlnp, alfa, cofa, dbp;
             if (K == 1) {
                 const T rp1 = recip(p1), rp2 = recip(p2);
@@ -410,7 +488,7 @@ __global__ __launch_bounds__(256) void k_columns(
                 vv = h * (eb * (vm - v) + ea * (v - vp));
                 vt = h * (eb * (tm - tv) + ea * (tv - tp));
             }
-            T vq = kappa * gwd[i] * tv;
+            T vq = kappa * omega_p(K, dba, cba, o, dv, dpk, rdp, rlnp, alfa) * tv;
             if (ntr > 0) vq = vq / (T(1) + cvcp * LV(Q, L));
             vv -= prsgy;
             vu -= prsgx;
@@ -529,6 +607,19 @@ template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa, cons
 }
 template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa) {
     return phillips_prs(p1, p2, kappa, prs_series(kappa));
+}
+
+// Surface pressure of the lean Held-Suarez pass: ps = exp(mean + ln ps'), the
+// mean from the spectral state in FP64, as the column kernel forms it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_psg(const T *__restrict__ gsyn, const cplx<T> *__restrict__ spec,
+                                             T *__restrict__ psg, int nl, int ntr, int nsyn, int nfs, long ng) {
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = blockIdx.y;
+    if (col >= ng) return;
+    const T *LNPS = gsyn + (long)mem * nsyn * ng + (7L + 3L * ntr) * nl * ng + col;
+    const double c0 = (double)spec[(long)mem * nfs + 3L * nl + (long)ntr * nl].x * 0.28209479177387814;  // 1/sqrt(4 pi)
+    psg[(long)mem * ng + col] = (T)exp(c0 + (double)LNPS[0]);
 }
 
 // Held-Suarez forcing on the post-dynamics grid fields (phy_run_hs.f90:42-77)
@@ -2394,22 +2485,33 @@ template <typename T> int Engine<T>::dyn_init() {
         for (int f = 7 * nl; f < 7 * nl + ntr * nl; f++) keep.push_back(f);
         for (int f = 7 * nl + 3 * ntr * nl; f < B->nsyn; f++) keep.push_back(f);
         B->nph = (int)keep.size();
-        std::vector<SynthDesc> dph;
-        std::vector<unsigned char> rph;
-        std::vector<int> sph;
-        for (int mem = 0; mem < nm; mem++)
-            for (int f : keep) {
-                dph.push_back(sd[(size_t)mem * B->nsyn + f]);
-                rph.push_back(rs[(size_t)mem * B->nsyn + f]);
-                sph.push_back(mem * B->nsyn + f);
-            }
-        void *q;
-        if ((rc = dupload<SynthDesc>(&q, dph, stream))) return rc;
-        B->sdesc_ph = (SynthDesc *)q;
-        if ((rc = dupload<unsigned char>(&q, rph, stream))) return rc;
-        B->rflag_ph = (unsigned char *)q;
-        if ((rc = dupload<int>(&q, sph, stream))) return rc;
-        B->slot_ph = (int *)q;
+        // the lean Held-Suarez pass: vrt, div, tv (2nl..5nl) and ln ps
+        std::vector<int> lean;
+        for (int f = 2 * nl; f < 5 * nl; f++) lean.push_back(f);
+        lean.push_back(7 * nl + 3 * ntr * nl);
+        B->nhl = (int)lean.size();
+        auto upload_subset = [&](const std::vector<int> &fs, SynthDesc **d, unsigned char **r, int **sl) {
+            std::vector<SynthDesc> dsub;
+            std::vector<unsigned char> rsub;
+            std::vector<int> ssub;
+            for (int mem = 0; mem < nm; mem++)
+                for (int f : fs) {
+                    dsub.push_back(sd[(size_t)mem * B->nsyn + f]);
+                    rsub.push_back(rs[(size_t)mem * B->nsyn + f]);
+                    ssub.push_back(mem * B->nsyn + f);
+                }
+            void *q;
+            int e;
+            if ((e = dupload<SynthDesc>(&q, dsub, stream))) return e;
+            *d = (SynthDesc *)q;
+            if ((e = dupload<unsigned char>(&q, rsub, stream))) return e;
+            *r = (unsigned char *)q;
+            if ((e = dupload<int>(&q, ssub, stream))) return e;
+            *sl = (int *)q;
+            return 0;
+        };
+        if ((rc = upload_subset(keep, &B->sdesc_ph, &B->rflag_ph, &B->slot_ph))) return rc;
+        if ((rc = upload_subset(lean, &B->sdesc_hl, &B->rflag_hl, &B->slot_hl))) return rc;
     }
     void *p;
     if ((rc = dupload<SynthDesc>(&p, sd, stream))) return rc;
@@ -2435,8 +2537,7 @@ template <typename T> int Engine<T>::dyn_init() {
 template <typename T> void Engine<T>::dyn_destroy() {
     DycoreBuffers *B = dyn;
     if (!B) return;
-    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
+    drop_graph(B);
     for (auto e : B->ev) (void)hipEventDestroy(e);
     void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
                     B->tphys, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
@@ -2446,7 +2547,8 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
-                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph,
+                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph, B->sdesc_hl,
+                    B->rflag_hl, B->slot_hl,
                     B->clock, B->iau_hrs, B->iau_inc, B->iau_F, B->st_pat2, B->st_pat[3], B->st_var, B->st_vf,
                     B->st_ctr, B->st_grd, B->sd_pat, B->rf0, B->vcg, B->sd_vc, B->rf1, B->an_w, B->an_g,
                     B->an_sm, B->sd_an, B->ad_an, B->rf_an};
@@ -2468,6 +2570,11 @@ template <typename T> struct Step {
     DycoreBuffers &B;
     Work<T> &W;
     hipStream_t s;
+    bool lean = false;  // run the lean Held-Suarez physics pass (a step before an advance's last)
+    bool grids = false; // the dynamics passes keep the omega / p and etadot grids (a getdyntend call)
+    // the Held-Suarez forcing reads no grid that the lean pass leaves out; the
+    // additive noise reads the winds of the physics pass
+    bool lean_ok() const { return E.cfg.physics == GFSDC_PHYS_HS && !B.st_on[3]; }
     void mark(const char *name) {
         if (!B.profiling) return;
         hipEvent_t e;
@@ -2500,6 +2607,23 @@ template <typename T> struct Step {
         launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_ph, s, (const T *)B.gmean, B.slot_ph);
         mark("fft_synth");
     }
+    // the lean Held-Suarez pass: vorticity, divergence, temperature and ln ps
+    // into their grid slots, then ps as the column kernel forms it
+    void synth_hs_lean() {
+        const Plan<T> &p = E.plan;
+        int nf = B.nm * B.nhl;
+        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc_hl, nf, W.sx, W.ldc, 1.0, s,
+                            (T *)B.gmean);
+        mark("synth_gather");
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+        mark("leg_synth");
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_hl, s, (const T *)B.gmean, B.slot_hl);
+        mark("fft_synth");
+        dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
+        hipLaunchKernelGGL(k_psg<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const cplx<T> *)B.spec, (T *)B.psg,
+                           B.nl, B.ntr, B.nsyn, B.nfs, B.ng);
+        mark("psg");
+    }
     // dyn_run.f90:114-116: the vorticity gradient of vorticity confinement
     void synth_vc() {
         const Plan<T> &p = E.plan;
@@ -2513,6 +2637,10 @@ template <typename T> struct Step {
         const gfsdc_cfg &c = E.cfg;
         dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
         const bool vc = B.vc_on && !inverse_only;
+        // the omega / p and etadot grids are stored where a caller sees them: by
+        // the inverse pass of the physics, a getdyntend call, or with no physics
+        // the dynamics passes themselves
+        const int store = (inverse_only || grids || c.physics == GFSDC_PHYS_NONE) ? 1 : 0;
         if (vc) synth_vc();
         auto kern = vc ? k_columns<T, true> : k_columns<T, false>;
         hipLaunchKernelGGL(kern, g, dim3(256), 0, s, (const T *)B.gsyn, (T *)B.gan, (T *)B.etadot,
@@ -2522,7 +2650,7 @@ template <typename T> struct Step {
                            (T)(c.rd / c.cp), (T)(c.cvap / c.cp - 1.0), (T)(2.0 * c.omega), inverse_only,
                            (const cplx<T> *)B.spec, B.nfs, colq ? (T *)B.colq : (T *)nullptr, (const T *)B.vcg,
                            B.st_on[0] ? (const T *)B.st_grd : (const T *)nullptr, (const T *)B.st_vf,
-                           (T)B.sc.vcamp);
+                           (T)B.sc.vcamp, store);
         mark(inverse_only ? "columns_inverse" : "columns");
     }
     // FP64 area sums: S(psg) -> pmean, S(b) -> bsum; pdryini = S(psg) - S(b) where 0
@@ -2595,8 +2723,12 @@ template <typename T> struct Step {
     }
     void physics() {
         const gfsdc_cfg &c = E.cfg;
-        synth_phys();
-        columns(1);
+        if (lean && lean_ok()) {
+            synth_hs_lean();
+        } else {
+            synth_phys();
+            columns(1);
+        }
         // run.f90:517-518: the additive noise forcing, before the physics
         if (B.st_on[3]) stoch_forcing();
         if (c.physics == GFSDC_PHYS_SIMPLE) {
@@ -2883,10 +3015,7 @@ template <typename T> int Engine<T>::autotune(double *ms, int *best) {
     plan.fft_nf = nf_syn;
     plan.fft_nf_an = nf_an;
     if (best) *best = nf_syn;
-    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    B->gexec = nullptr;
-    B->graph = nullptr;
+    drop_graph(B);
     return 0;
 }
 
@@ -3100,10 +3229,7 @@ template <typename T> int Engine<T>::mlp_enable(int on, int judge_stride, double
     GFSDC_CHECK(hipMemsetAsync(B->jstat, 0, sizeof(double) * 3, stream));
     GFSDC_CHECK(hipStreamSynchronize(stream));
     // the captured step changes: capture again on the next graph advance
-    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    B->gexec = nullptr;
-    B->graph = nullptr;
+    drop_graph(B);
     return 0;
 }
 
@@ -3279,6 +3405,8 @@ int Engine<T>::getdyntend(int rkstage, int inverse_only, void *dvrt, void *ddiv,
         return GFSDC_EINVAL;
     }
     Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    st.grids = true;
+    B->grids_partial = false;
     if (inverse_only) {
         st.synth_all();
         st.columns(1, rkstage == 1 && B->ntr > 0);
@@ -3319,28 +3447,34 @@ template <typename T> int Engine<T>::advance(int nsteps, int use_graph) {
         return GFSDC_ESTATE;
     }
     Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    // every step but the last may run the lean physics pass: only the last
+    // step's grids are seen by callers (get_grid, step_diag)
+    const bool lean = st.lean_ok() && nsteps > 1;
+    if (nsteps > 0) B->grids_partial = false;
     if (!use_graph) {
-        for (int i = 0; i < nsteps; i++) st.step();
+        for (int i = 0; i < nsteps; i++) {
+            st.lean = lean && i < nsteps - 1;
+            st.step();
+        }
         GFSDC_CHECK(hipGetLastError());
         return 0;
     }
-    if (!B->gexec) {
+    auto capture = [&](bool ln, hipGraph_t *g, hipGraphExec_t *x) -> int {
+        st.lean = ln;
         GFSDC_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         st.step();
-        GFSDC_CHECK(hipStreamEndCapture(stream, &B->graph));
-        GFSDC_CHECK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
-    }
-    for (int i = 0; i < nsteps; i++) GFSDC_CHECK(hipGraphLaunch(B->gexec, stream));
+        GFSDC_CHECK(hipStreamEndCapture(stream, g));
+        GFSDC_CHECK(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
+        return 0;
+    };
+    int rc;
+    if (!B->gexec && (rc = capture(false, &B->graph, &B->gexec))) return rc;
+    if (lean && !B->gexec_lean && (rc = capture(true, &B->graph_lean, &B->gexec_lean))) return rc;
+    for (int i = 0; i < nsteps; i++)
+        GFSDC_CHECK(hipGraphLaunch(lean && i < nsteps - 1 ? B->gexec_lean : B->gexec, stream));
     return 0;
 }
 
-// drop the captured step: the next graph advance captures it again
-static void drop_graph(DycoreBuffers *B) {
-    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-    if (B->graph) (void)hipGraphDestroy(B->graph);
-    B->gexec = nullptr;
-    B->graph = nullptr;
-}
 
 // run.f90:53 t = tstart (params.f90: tstart = fhour of the initial file, in s)
 template <typename T> int Engine<T>::set_time(double t) {
@@ -3894,6 +4028,11 @@ template <typename T> int Engine<T>::get_grid(int which, void *out, int dtype, i
         set_error("context has no dycore");
         return GFSDC_ESTATE;
     }
+    if (B->grids_partial) {
+        set_error("the last step ran the lean Held-Suarez physics pass (profile_step), which keeps only the "
+                  "grids its forcing reads: advance or getdyntend first");
+        return GFSDC_ESTATE;
+    }
     const int nl = B->nl, nm = B->nm;
     const long ng = B->ng;
     const T *src = nullptr;
@@ -3966,6 +4105,11 @@ template <typename T> int Engine<T>::step_diag(double *out) {
     DycoreBuffers *B = dyn;
     if (!B) {
         set_error("context has no dycore");
+        return GFSDC_ESTATE;
+    }
+    if (B->grids_partial) {
+        set_error("the last step ran the lean Held-Suarez physics pass (profile_step), which keeps only the "
+                  "grids its forcing reads: advance or getdyntend first");
         return GFSDC_ESTATE;
     }
     hipLaunchKernelGGL(k_diag<T>, dim3(B->nm), dim3(1024), 0, stream, (const T *)B->gsyn, (const T *)B->psg,
@@ -4047,6 +4191,9 @@ template <typename T> int Engine<T>::profile_step(char names[][48], double *ms, 
     B->evname.clear();
     B->profiling = true;
     Step<T> st{*this, *B, *(Work<T> *)B->Work, stream};
+    // the step as advance runs all but its last (lean physics pass where it applies)
+    st.lean = st.lean_ok();
+    B->grids_partial = st.lean;
     st.mark("start");
     st.step();
     B->profiling = false;

@@ -187,7 +187,11 @@ int gfsdc_get_pdryini(gfsdc_ctx *ctx, double *pdryini);
 int gfsdc_getdyntend(gfsdc_ctx *ctx, int rkstage, int inverse_only, void *dvrt, void *ddiv,
                      void *dtv, void *dtrac, void *dlnps, int dtype, int loc);
 /* replaces run.f90:194 advance, nsteps times (device resident; captured in a
- * HIP graph after the first step when use_graph != 0) */
+ * HIP graph after the first step when use_graph != 0). With Held-Suarez
+ * physics every step but the last runs a lean physics pass that synthesises
+ * only the grids the forcing reads (vorticity, divergence, temperature, ln ps);
+ * the state is the same bit for bit, and the last step leaves every grid field
+ * for gfsdc_get_grid and gfsdc_step_diag (run.f90:157-162). */
 int gfsdc_advance(gfsdc_ctx *ctx, int nsteps, int use_graph);
 /* Held-Suarez tendencies on the current grid fields (phy_run_hs.f90:22) */
 int gfsdc_hs_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
@@ -335,7 +339,10 @@ int gfsdc_sync(gfsdc_ctx *ctx);
 void *gfsdc_stream(gfsdc_ctx *ctx);
 
 /* per-kernel timing of one step via HIP events on the context stream:
- * fills up to nmax (name, milliseconds) pairs, returns the count */
+ * fills up to nmax (name, milliseconds) pairs, returns the count. The step is
+ * run as gfsdc_advance runs all but its last, so after a lean Held-Suarez
+ * step gfsdc_get_grid and gfsdc_step_diag return GFSDC_ESTATE until the next
+ * advance or getdyntend. */
 int gfsdc_profile_step(gfsdc_ctx *ctx, char names[][48], double *ms, int nmax);
 
 #ifdef __cplusplus
