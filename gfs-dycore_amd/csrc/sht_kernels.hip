@@ -869,8 +869,31 @@ __device__ __forceinline__ void wave_lds_sync() {
 // st(x, p, v), p being x's permuted LDS index. Only this wavefront touches its
 // field, so the stages are ordered by wave_lds_sync instead of workgroup
 // barriers. The first stage (Ns = 1) has unit twiddles.
-template <typename T, int R, int N, int Ns, typename Ld, typename St>
-__device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int sign, Ld ld, St st) {
+// Twiddles of one Stockham stage in registers: R - 1 per butterfly of this
+// lane (none in the first stage, whose twiddles are 1), loaded one stage ahead
+// of their use so that the loads overlap a whole stage of LDS work.
+template <typename T, int R, int N, int Ns> struct StageTw {
+    static constexpr int nbf = N / R, NB = (nbf + 63) / 64, NT = Ns > 1 ? R - 1 : 0;
+    cplx<T> t[NB][NT > 0 ? NT : 1];
+    __device__ __forceinline__ void load(const cplx<T> *__restrict__ tw) {
+        if constexpr (NT > 0) {
+            const int lane = threadIdx.x & 63;
+#pragma unroll
+            for (int bb = 0; bb < NB; bb++) {
+                const int jm = min(lane + 64 * bb, nbf - 1) % Ns;  // clamped: lanes past nbf load in bounds
+#pragma unroll
+                for (int r = 1; r < R; r++) t[bb][r - 1] = tw[Ns - 1 + (r - 1) * Ns + jm];
+            }
+        }
+    }
+};
+struct NoTw {
+    template <typename P> __device__ __forceinline__ void load(const P *) {}
+};
+
+template <typename T, int R, int N, int Ns, typename Ld, typename St, typename TN>
+__device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int sign, Ld ld, St st,
+                                            const StageTw<T, R, N, Ns> &tc, TN &tn) {
     constexpr int nbf = N / R, NB = (nbf + 63) / 64;
     constexpr bool SR = split_all(N, R, Ns, false), SW = split_all(N, R, Ns, true);
     const int lane = threadIdx.x & 63;
@@ -886,6 +909,7 @@ __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int 
             }
         }
     }
+    tn.load(tw);      // the next stage's twiddles
     wave_lds_sync();  // in place: every read of the stage before any write
 #pragma unroll
     for (int bb = 0; bb < NB; bb++) {
@@ -895,7 +919,7 @@ __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int 
             if constexpr (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    cplx<T> t = tw[Ns - 1 + (r - 1) * Ns + jm];
+                    cplx<T> t = tc.t[bb][r - 1];
                     if (sign > 0) t.y = -t.y;
                     v[bb][r] = cmul(v[bb][r], t);
                 }
@@ -912,28 +936,39 @@ __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int 
     wave_lds_sync();
 }
 
-// Stages S.. of the length-N transform of field bq (one wavefront). FIRST reads
-// the first stage's inputs through ld instead of LDS; LAST hands the last
-// stage's outputs to st instead of LDS (the last stage writes x = j + r Ns with
-// j < Ns, consecutive across lanes, so a global store from it is coalesced).
+// Stages S.. of the length-N transform of field bq (one wavefront), stage S's
+// twiddles tc already loaded. FIRST reads the first stage's inputs through ld
+// instead of LDS; LAST hands the last stage's outputs to st instead of LDS (the
+// last stage writes x = j + r Ns with j < Ns, consecutive across lanes, so a
+// global store from it is coalesced).
 template <typename T, int N, int S, int Ns, bool FIRST, bool LAST, typename Ld, typename St>
-__device__ __forceinline__ void fft_stages_w(cplx<T> *bq, const cplx<T> *tw, int sign, Ld ld, St st) {
+__device__ __forceinline__ void fft_stages_w(cplx<T> *bq, const cplx<T> *tw, int sign, Ld ld, St st,
+                                             const StageTw<T, rad_plan(N).r[S], N, Ns> &tc) {
     constexpr RadPlan P = rad_plan(N);
     if constexpr (S < P.n) {
         constexpr int R = P.r[S];
         auto lds_ld = [&](int, int p) { return bq[p]; };
         auto lds_st = [&](int, int p, const cplx<T> &v) { bq[p] = v; };
-        if constexpr (FIRST && S == 0) {
-            if constexpr (LAST && S == P.n - 1)
-                fft_stage_w<T, R, N, Ns>(tw, sign, ld, st);
-            else
-                fft_stage_w<T, R, N, Ns>(tw, sign, ld, lds_st);
-        } else if constexpr (LAST && S == P.n - 1) {
-            fft_stage_w<T, R, N, Ns>(tw, sign, lds_ld, st);
+        auto run = [&](auto &tn) {
+            if constexpr (FIRST && S == 0) {
+                if constexpr (LAST && S == P.n - 1)
+                    fft_stage_w<T, R, N, Ns>(tw, sign, ld, st, tc, tn);
+                else
+                    fft_stage_w<T, R, N, Ns>(tw, sign, ld, lds_st, tc, tn);
+            } else if constexpr (LAST && S == P.n - 1) {
+                fft_stage_w<T, R, N, Ns>(tw, sign, lds_ld, st, tc, tn);
+            } else {
+                fft_stage_w<T, R, N, Ns>(tw, sign, lds_ld, lds_st, tc, tn);
+            }
+        };
+        if constexpr (S + 1 < P.n) {
+            StageTw<T, P.r[S + 1], N, Ns * R> tn;
+            run(tn);
+            fft_stages_w<T, N, S + 1, Ns * R, FIRST, LAST>(bq, tw, sign, ld, st, tn);
         } else {
-            fft_stage_w<T, R, N, Ns>(tw, sign, lds_ld, lds_st);
+            NoTw tn;
+            run(tn);
         }
-        fft_stages_w<T, N, S + 1, Ns * R, FIRST, LAST>(bq, tw, sign, ld, st);
     }
 }
 
@@ -980,26 +1015,45 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
     const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
+    // wavefront path: this wave's field, its 1/cos scaling and added mean, loaded
+    // with the coefficients (field clamped into range; a wave past nf stores nothing)
+    T w_s = T(1), w_c = T(0);
+    long w_g = 0;
+    if constexpr (WAVE) {
+        const int fw = min(f0 + (tid >> 6), nf - 1);
+        const T rc = rcos[j];  // unconditional, so that no load waits for another
+        w_s = rflag[fw] ? rc : T(1);
+        w_c = mean ? mean[fw] : T(0);
+        w_g = slot ? slot[fw] : fw;
+    }
     constexpr int MAXI = (NFFT * (LN / 2) + NT - 1) / NT;
     const int nitems = (ntrunc + 1) * NFFT;
+    // every load of the batch is issued before the first is used: the indices
+    // are clamped into the buffer (item nitems - 1, field nf - 1) and the values
+    // of the clamped lanes are zeroed afterwards, so no load sits in a branch
+    // that would make it wait for the one before
+    cplx<T> ev[MAXI], ov[MAXI];
+#pragma unroll
+    for (int c = 0; c < MAXI; c++) {
+        ev[c] = ov[c] = cplx<T>{T(0), T(0)};
+        if (c * NT < nitems) {  // uniform
+            const int idx = min(tid + c * NT, nitems - 1);
+            const int m = idx / NFFT, q = idx - m * NFFT, f = min(f0 + q, nf - 1);
+            // rows j / j2 hold the even / odd Legendre sums; field f is Fourier field f + fofs
+            const int cf = 2 * (f + fofs);
+            ev[c] = *(const cplx<T> *)(fs + fs_index(j, m, cf, ldf, ntrunc + 1));
+            ov[c] = *(const cplx<T> *)(fs + fs_index(j2, m, cf, ldf, ntrunc + 1));
+        }
+    }
+    // hemispheres: north E + O, south E - O
     cplx<T> xn[MAXI], xs[MAXI];
 #pragma unroll
     for (int c = 0; c < MAXI; c++) {
-        int idx = tid + c * NT;
-        xn[c] = cplx<T>{T(0), T(0)};
-        xs[c] = xn[c];
-        if (idx < nitems) {
-            int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
-            if (f < nf) {
-                // rows j / j2 hold the even / odd Legendre sums: north E + O, south E - O;
-                // field f is Fourier field f + fofs
-                const int cf = 2 * (f + fofs);
-                const cplx<T> e = *(const cplx<T> *)(fs + fs_index(j, m, cf, ldf, ntrunc + 1));
-                const cplx<T> o = *(const cplx<T> *)(fs + fs_index(j2, m, cf, ldf, ntrunc + 1));
-                xn[c] = cplx<T>{e.x + o.x, e.y + o.y};
-                xs[c] = cplx<T>{e.x - o.x, e.y - o.y};
-            }
-        }
+        const int idx = tid + c * NT, q = idx % NFFT;
+        const bool ok = idx < nitems && f0 + q < nf;
+        const cplx<T> e = ev[c], o = ov[c];
+        xn[c] = ok ? cplx<T>{e.x + o.x, e.y + o.y} : cplx<T>{T(0), T(0)};
+        xs[c] = ok ? cplx<T>{e.x - o.x, e.y - o.y} : cplx<T>{T(0), T(0)};
     }
     if constexpr (!GTW)
         for (int t = tid; t < N; t += NT) tws[t] = tw[t];
@@ -1026,16 +1080,17 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
         // wavefront q transforms field f0 + q and stores the grid rows from its last stage
         const int q = tid >> 6, f = f0 + q;
         if (f >= nf) return;
-        const T s = rflag[f] ? rcos[j] : T(1);
-        const T c = mean ? mean[f] : T(0);
-        const long gf = slot ? slot[f] : f;
+        const T s = w_s, c = w_c;
+        const long gf = w_g;  // grid field the transform is written to
         T *gn = grid + gf * gstride + (long)j * nlon, *gs = grid + gf * gstride + (long)j2 * nlon;
         auto none = [](int, int) { return cplx<T>{T(0), T(0)}; };
         auto out = [&](int x, int, const cplx<T> &z) {
             gn[x] = z.x * s + c;
             gs[x] = z.y * s + c;
         };
-        fft_stages_w<T, NC, 0, 1, false, true>(buf + q * NP, GTW ? tw : tws, +1, none, out);
+        constexpr RadPlan P = rad_plan(NC);
+        fft_stages_w<T, NC, 0, 1, false, true>(buf + q * NP, GTW ? tw : tws, +1, none, out,
+                                               StageTw<T, P.r[0], NC, 1>{});
         return;
     }
     if constexpr (NC > 0)
@@ -1087,9 +1142,17 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
             for (int t = tid; t < N; t += NT) tws[t] = tw[t];
         auto in = [&](int x, int) { return cplx<T>{gn[x], gs[x]}; };
         auto lds_st = [&](int, int p, const cplx<T> &v) { bq[p] = v; };
-        fft_stage_w<T, P.r[0], NC, 1>(GTW ? tw : tws, -1, in, lds_st);
+        // the second stage's twiddles load during the first stage from the global
+        // table; from the LDS table only after the barrier that completes it
+        StageTw<T, P.r[1], NC, P.r[0]> t1;
+        NoTw none;
+        if constexpr (GTW)
+            fft_stage_w<T, P.r[0], NC, 1>(tw, -1, in, lds_st, StageTw<T, P.r[0], NC, 1>{}, t1);
+        else
+            fft_stage_w<T, P.r[0], NC, 1>(tws, -1, in, lds_st, StageTw<T, P.r[0], NC, 1>{}, none);
         __syncthreads();  // the twiddle table
-        fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, GTW ? tw : tws, -1, in, lds_st);
+        if constexpr (!GTW) t1.load(tws);
+        fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, GTW ? tw : tws, -1, in, lds_st, t1);
         __syncthreads();  // the coefficients below are read across fields
     } else {
         constexpr int MAXG = (NFFT * LN + 255) / 256;
