@@ -1202,18 +1202,15 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
 
 // ---------------------------------------------------------------------------
 // Gather into extended spectral columns and scatter out of them.
-template <typename T>
-__device__ inline cplx<T> src_coef(const cplx<T> *__restrict__ src, long sf, long sk, int f, const int *koff,
-                                   int m, int n, int ntrunc, double sc, int inv) {
-    if (f < 0 || n < m || n > ntrunc) return cplx<T>{T(0), T(0)};
-    cplx<T> v = src[f * sf + (long)(koff[m] + n - m) * sk];
-    T s = T(sc);
-    if (inv) s *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
-    return cplx<T>{v.x * s, v.y * s};
-}
-
 constexpr int GATHER_KE = 8;
+constexpr int GATHER_QB = 4;  // rows whose loads are issued together
 
+// Each output element is a combination of at most three source coefficients
+// ("taps") of the row's order m: U-hat takes S at n - 1 and n + 1 and T at n,
+// V-hat takes S at n and T at n - 1 and n + 1, a copy takes its field at n.
+// The taps of GATHER_QB rows are loaded before any is used, with indices
+// clamped into the state and the values of taps outside m <= n <= ntrunc (or
+// of a missing field) zeroed afterwards, so that no load waits for another.
 template <typename T>
 __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk, const SynthDesc *__restrict__ desc,
                                int nfo, T *__restrict__ sx, int ldx, const int2 *__restrict__ kemn,
@@ -1228,34 +1225,71 @@ __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk
         if (d.kind == SK_COPY_MEAN) c = T((double)src[(long)d.a * sf].x * d.sa * 0.28209479177387814);
         mean[fo] = c;
     }
-    // GATHER_KE extended coefficients per workgroup: one descriptor load, GATHER_KE rows
-    for (int ke = blockIdx.y * GATHER_KE; ke < min(nke, (blockIdx.y + 1) * GATHER_KE); ke++) {
-        const int2 mn = kemn[ke];
-        const int m = mn.x, n = mn.y;
-        cplx<T> r{T(0), T(0)};
-        if (d.kind == SK_COPY || ((d.kind == SK_COPY_DEV || d.kind == SK_COPY_MEAN) && n > 0)) {
-            r = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
-        } else if (d.kind == SK_UHAT || d.kind == SK_VHAT) {
-            const T *ep = eps + eoff[m];
-            T en = ep[n - m], en1 = ep[n + 1 - m];
-            T cm1 = T(n - 1) * en, cp1 = T(n + 2) * en1, fm = T(m);
-            if (d.kind == SK_UHAT) {
-                // U = (n-1) eps_n S_{n-1} - (n+2) eps_{n+1} S_{n+1} + i m T_n
-                cplx<T> sm = src_coef(src, sf, sk, d.a, koff, m, n - 1, ntrunc, d.sa, d.inv_a);
-                cplx<T> sp = src_coef(src, sf, sk, d.a, koff, m, n + 1, ntrunc, d.sa, d.inv_a);
-                cplx<T> t0 = src_coef(src, sf, sk, d.b, koff, m, n, ntrunc, d.sb, d.inv_b);
-                r.x = cm1 * sm.x - cp1 * sp.x - fm * t0.y;
-                r.y = cm1 * sm.y - cp1 * sp.y + fm * t0.x;
-            } else {
-                // V = i m S_n - (n-1) eps_n T_{n-1} + (n+2) eps_{n+1} T_{n+1}
-                cplx<T> s0 = src_coef(src, sf, sk, d.a, koff, m, n, ntrunc, d.sa, d.inv_a);
-                cplx<T> tm = src_coef(src, sf, sk, d.b, koff, m, n - 1, ntrunc, d.sb, d.inv_b);
-                cplx<T> tp = src_coef(src, sf, sk, d.b, koff, m, n + 1, ntrunc, d.sb, d.inv_b);
-                r.x = -fm * s0.y - cm1 * tm.x + cp1 * tp.x;
-                r.y = fm * s0.x - cm1 * tm.y + cp1 * tp.y;
+    const bool uh = d.kind == SK_UHAT, vh = d.kind == SK_VHAT, vec = uh || vh;
+    const bool dev = d.kind == SK_COPY_DEV || d.kind == SK_COPY_MEAN;  // without the n = 0 coefficient
+    const bool none = !(vec || dev || d.kind == SK_COPY);
+    // taps: field, degree offset, multiplier, invlap flag
+    const int fa = max(d.a, 0), fb = max(d.b, 0);  // a missing field (-1) reads field 0, zeroed below
+    const int tf0 = fa, td0 = uh ? -1 : 0;
+    const int tf1 = uh ? fa : fb, td1 = uh ? 1 : -1;
+    const int tf2 = fb, td2 = uh ? 0 : 1;
+    const bool okA = d.a >= 0, okB = d.b >= 0;
+    const bool ok0 = okA && !none, ok1 = uh ? okA : okB, ok2 = okB;
+    const T sa = T(d.sa), sb = T(d.sb);
+    const T s0 = sa, s1 = uh ? sa : sb, s2 = sb;
+    const int i0 = d.inv_a, i1 = uh ? d.inv_a : d.inv_b, i2 = d.inv_b;
+    auto tap = [&](int f, int m, int n) {
+        const int nc = min(max(n, m), ntrunc);
+        return src[f * sf + (long)(koff[m] + nc - m) * sk];
+    };
+    auto fac = [&](bool ok, int m, int n, T sc, int inv) {
+        if (!ok || n < m || n > ntrunc) return T(0);
+        if (inv) sc *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
+        return sc;
+    };
+    const int kb0 = blockIdx.y * GATHER_KE, kb1 = min(nke, kb0 + GATHER_KE);
+    for (int kb = kb0; kb < kb1; kb += GATHER_QB) {
+        int2 mn[GATHER_QB];
+        cplx<T> v0[GATHER_QB], v1[GATHER_QB], v2[GATHER_QB];
+#pragma unroll
+        for (int q = 0; q < GATHER_QB; q++) {
+            mn[q] = kemn[min(kb + q, kb1 - 1)];  // clamped: rows past kb1 are not stored
+            const int m = mn[q].x, n = mn[q].y;
+            v0[q] = tap(tf0, m, n + td0);
+            if (vec) {
+                v1[q] = tap(tf1, m, n + td1);
+                v2[q] = tap(tf2, m, n + td2);
             }
         }
-        *(cplx<T> *)(sx + (long)ke * ldx + 2 * fo) = r;
+#pragma unroll
+        for (int q = 0; q < GATHER_QB; q++) {
+            const int ke = kb + q;
+            if (ke >= kb1) break;
+            const int m = mn[q].x, n = mn[q].y;
+            cplx<T> r{T(0), T(0)};
+            if (vec) {
+                const T *ep = eps + eoff[m];
+                const T en = ep[n - m], en1 = ep[n + 1 - m];
+                const T cm1 = T(n - 1) * en, cp1 = T(n + 2) * en1, fm = T(m);
+                const T c0 = fac(ok0, m, n + td0, s0, i0), c1 = fac(ok1, m, n + td1, s1, i1);
+                const T c2 = fac(ok2, m, n + td2, s2, i2);
+                const cplx<T> a0{v0[q].x * c0, v0[q].y * c0}, a1{v1[q].x * c1, v1[q].y * c1};
+                const cplx<T> a2{v2[q].x * c2, v2[q].y * c2};
+                if (uh) {
+                    // U = (n-1) eps_n S_{n-1} - (n+2) eps_{n+1} S_{n+1} + i m T_n
+                    r.x = cm1 * a0.x - cp1 * a1.x - fm * a2.y;
+                    r.y = cm1 * a0.y - cp1 * a1.y + fm * a2.x;
+                } else {
+                    // V = i m S_n - (n-1) eps_n T_{n-1} + (n+2) eps_{n+1} T_{n+1}
+                    r.x = -fm * a0.y - cm1 * a1.x + cp1 * a2.x;
+                    r.y = fm * a0.x - cm1 * a1.y + cp1 * a2.y;
+                }
+            } else if (!none && (!dev || n > 0)) {
+                const T c0 = fac(ok0, m, n, s0, i0);
+                r = cplx<T>{v0[q].x * c0, v0[q].y * c0};
+            }
+            *(cplx<T> *)(sx + (long)ke * ldx + 2 * fo) = r;
+        }
     }
 }
 
