@@ -1131,6 +1131,7 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
     const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
+    const T wf_j = wfac[j], rc_j = rcos[j];  // the output weights, loaded with the grid rows
     if constexpr (WAVE) {
         // wavefront q transforms field f0 + q, its first stage loading the grid rows;
         // a field past nf repeats field nf - 1 and is not stored
@@ -1150,8 +1151,10 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
             fft_stage_w<T, P.r[0], NC, 1>(tw, -1, in, lds_st, StageTw<T, P.r[0], NC, 1>{}, t1);
         else
             fft_stage_w<T, P.r[0], NC, 1>(tws, -1, in, lds_st, StageTw<T, P.r[0], NC, 1>{}, none);
-        __syncthreads();  // the twiddle table
-        if constexpr (!GTW) t1.load(tws);
+        if constexpr (!GTW) {
+            __syncthreads();  // the twiddle table (the global one needs no barrier)
+            t1.load(tws);
+        }
         fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, GTW ? tw : tws, -1, in, lds_st, t1);
         __syncthreads();  // the coefficients below are read across fields
     } else {
@@ -1185,14 +1188,17 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
         else
             fft_run<T, NFFT, MAXN>(buf, N, rd.r, rd.n, tws, -1);
     }
-    const T wj = wfac[j], rc = rcos[j];
+    // a thread's field is the same in every iteration (NT is a multiple of NFFT):
+    // its weight is formed once
+    static_assert(NT % NFFT == 0, "output field per thread");
+    const int qo = tid % NFFT, fo = f0 + qo;
+    const T wf = rflag[min(fo, nf - 1)] ? wf_j * rc_j : wf_j;
     for (int idx = tid; idx < (ntrunc + 1) * NFFT; idx += NT) {
-        int m = idx / NFFT, q = idx - m * NFFT, f = f0 + q;
+        int m = idx / NFFT, q = qo, f = fo;
         if (f >= nf) continue;
         cplx<T> zk = buf[q * NP + fft_pad(m)], zn = buf[q * NP + fft_pad(m == 0 ? 0 : N - m)];
         T xr = T(0.5) * (zk.x + zn.x), xi = T(0.5) * (zk.y - zn.y);
         T yr = T(0.5) * (zk.y + zn.y), yi = T(-0.5) * (zk.x - zn.x);
-        T wf = rflag[f] ? wj * rc : wj;
         T *pe = fa + ((long)(2 * m) * nhp + jp) * ldf + 2 * f;
         T *po = fa + ((long)(2 * m + 1) * nhp + jp) * ldf + 2 * f;
         *(cplx<T> *)pe = cplx<T>{wf * (xr + yr), wf * (xi + yi)};
