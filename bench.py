@@ -115,8 +115,18 @@ def algorithmic_work(name, ctx, esz):
         return "mfma", nm * nan * 2.0 * nlat * nlm
     if name == "fft_synth":
         return "hbm", nm * nsyn * (four + ng * esz)
+    if name == "mlp_forward" and getattr(ctx, "mlp_shape", None):
+        # bf16 GEMMs of the column network over every grid column
+        nin, h, nhid, nout = ctx.mlp_shape
+        return "mfma_bf16", 2.0 * nm * ng * (nin * h + (nhid - 1) * h * h + h * nout)
     if name == "columns":
-        return "hbm", nm * ((7 * nl + 3) + (4 * nl + 1) + (2 * nl + 1) + 2) * ng * esz
+        # reads u, v, vrt, div, tv and its gradient (and each tracer with its
+        # gradient) per level and ln ps with its gradient; writes the tendencies,
+        # ps and d ln ps / dt; the omega/p and etadot grids only without physics
+        # (with physics the bottom-up pass recomputes them and stores nothing)
+        ntr = ctx.ntrac
+        grids = (2 * nl + 1) if ctx.cfg.physics == gfsdc.PHYS_NONE else 0
+        return "hbm", nm * (((7 + 3 * ntr) * nl + 3) + ((4 + ntr) * nl + 1) + grids + 2) * ng * esz
     return "hbm", None
 
 
@@ -130,6 +140,8 @@ def roofline(tab, ctx, esz, workload):
     if work is not None:
         if kind == "mfma":
             roof.update(achieved=work / avg_s / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s")
+        elif kind == "mfma_bf16":
+            roof.update(bound="mfma", achieved=work / avg_s / 1e12, peak=PEAK["bf16_mfma"] / 1e12, unit="TFLOP/s")
         else:
             roof.update(achieved=work / avg_s / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s")
         roof["frac"] = roof["achieved"] / roof["peak"]
@@ -390,6 +402,7 @@ def run_moist(args):
     mlp = None
     if args.moist == "mlp":
         ctx.mlp_init(512, 3, ctx.nlat * ctx.nlon, 1 << 17, seed=1)  # one forward batch = the whole grid
+        ctx.mlp_shape = (4 * ctx.nlev + 3, 512, 3, 4 * ctx.nlev)
         ctx.mlp_collect(1, 8)
         ctx.mlp_normalize()
         w = ctx.mlp_params()
