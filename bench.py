@@ -23,6 +23,8 @@ Other workloads (each prints its own JSON line instead of the headline):
   --micro                  transform microbenchmark: spectogrd + grdtospec +
                            getuv on 64 levels x 4 fields, FP64 / FP32 / BF16X3
   --moist {plain,mlp}      moist run with the simple physics or the column MLP
+  --moist c4train          the C4 network trained on 1e7 T126L64 moist columns,
+                           then run at T254L64 in the simple physics' place
   --copy-calibration       (with any workload) also run a 1 GiB device-to-device
                            copy, the reference kernel of known size for scaling
                            rocprofv3 FETCH_SIZE / WRITE_SIZE into HBM bytes
@@ -383,6 +385,85 @@ def run_micro(args):
         speedup_bf16x3_vs_fp64=res["FP64"]["ms_per_iteration"] / res["BF16X3"]["ms_per_iteration"])
 
 
+def moist_context(config):
+    """FP32 moist context (simple physics, SST mode 2, surface fluxes) from the
+    resting state with the moist humidity grid"""
+    ctx = make(config, gfsdc.FP32, ntrac=1, physics=gfsdc.PHYS_SIMPLE, simple_test=2, turbflux=1)
+    ak, bk = levels.levels_for(config)
+    qg = levels.moist_grid(ctx.lats, ctx.nlon, ak, bk)
+    st = ctx.get_state()
+    ctx.set_state(st["vrt"], st["div"], st["tv"], st["lnps"], trac=ctx.grdtospec(qg)[None, None])
+    return ctx
+
+
+def run_c4train(args):
+    """The column network of config C4 trained as specified and run in place of
+    the physics it learned (phy_run_simple.f90:66-67): 259 -> 3 x 512 -> 256,
+    trained on --c4-columns columns of a T126L64 moist run with the simple
+    physics (every 4th column of each step after a 12-hour spin-up, so the
+    samples cover several model days), AdamW on batches of 4096. The last 10 %
+    of the columns, later in time than any training column, are held out. The
+    network and its normalisation then move to a T254L64 moist context (same
+    64 levels), where it replaces the simple physics inside the timed step,
+    judged on every 50th column against the plain physics with the fallback
+    above a normalised rms error of 1. value = T254L64 simulated days per
+    wall-day with the network; the plain-physics step is timed beside it."""
+    steps, warmup = args.steps, args.warmup
+    src = moist_context("T126L64")
+    src.autotune()
+    src.advance(72, use_graph=True)                     # 12 h at dt = 600 s
+    stride = 4
+    per_step = src.nlat * src.nlon // stride
+    nsteps = -(-args.c4_columns // per_step)
+    src.mlp_init(512, 3, 4096, nsteps * per_step, seed=1)
+    device_sync(src)
+    t0 = time.perf_counter()
+    n = src.mlp_collect(nsteps, stride)
+    device_sync(src)
+    t_collect = time.perf_counter() - t0
+    src.mlp_normalize()
+    ntrain = int(n * 0.9)
+    t0 = time.perf_counter()
+    losses = src.mlp_train(args.c4_train_steps, 4096, 1e-3, 1e-4, ntrain=ntrain)
+    device_sync(src)
+    t_train = time.perf_counter() - t0
+    err_tr, base_tr = src.mlp_eval(0, min(ntrain, 500000))
+    err_ho, base_ho = src.mlp_eval(ntrain, n - ntrain)
+    params, norm = src.mlp_params(), src.mlp_norm()
+    del src
+    dst = moist_context("T254L64")
+    dst.autotune()
+    t_plain = time_steps(dst, steps, warmup)
+    dst.mlp_init(512, 3, dst.nlat * dst.nlon, 1, seed=1)
+    dst.mlp_shape = (4 * dst.nlev + 3, 512, 3, 4 * dst.nlev)
+    dst.mlp_params(set=params)
+    dst.mlp_norm(set=norm)
+    dst.mlp_enable(True, judge_stride=50, threshold=1.0)
+    t = time_steps(dst, steps, warmup)
+    judge = dst.mlp_judge_stats()
+    tab = kernel_table(dst, 1)
+    roof, per_step_k = roofline(tab, dst, 4, "T254L64_moist_c4train")
+    dt = levels.CONFIGS["T254L64"]["dt"]
+    last = losses[-min(200, len(losses)):]
+    return base_line(
+        args, dt / t, "simulated days/wall-day",
+        "moist T254L64 step, trained column MLP physics",
+        {"workload": "T254L64 moist dycore + 259-3x512-256 column MLP trained on T126L64 moist columns "
+                     "(judged on every 50th column, fallback above normalised rms error 1), "
+                     f"SST 302.15 K - 30 K sin^2 lat, dt={dt:g} s", "members": 1, "hip_graph": True},
+        ms_per_step=t * 1e3, plain_physics_ms_per_step=t_plain * 1e3,
+        training={"source": "T126L64 moist run with the simple physics, every 4th column of "
+                            f"{nsteps} steps after a 12-hour spin-up",
+                  "columns": n, "train_columns": ntrain, "held_out_columns": n - ntrain,
+                  "batch": 4096, "adamw_steps": len(losses), "epochs": len(losses) * 4096 / ntrain,
+                  "loss_first20": float(np.mean(losses[:20])), "loss_last200": float(np.mean(last)),
+                  "collect_s": t_collect, "train_s": t_train,
+                  "rms_error_train": err_tr, "mean_predictor_train": base_tr,
+                  "rms_error_held_out": err_ho, "mean_predictor_held_out": base_ho},
+        judge=judge, roofline=roof,
+        kernels_ms_per_step={k: round(v, 4) for k, v in sorted(per_step_k.items(), key=lambda x: -x[1])})
+
+
 def run_moist(args):
     """moist T254L64 (or --config) step with the simple physics, or with the
     column MLP in its place (512-wide, 3 hidden layers, judge on every 50th
@@ -393,11 +474,7 @@ def run_moist(args):
     twenty steps). Its accuracy is covered by the tests."""
     cfg = levels.CONFIGS[args.config]
     dt = cfg["dt"]
-    ctx = make(args.config, gfsdc.FP32, ntrac=1, physics=gfsdc.PHYS_SIMPLE, simple_test=2, turbflux=1)
-    ak, bk = levels.levels_for(args.config)
-    qg = levels.moist_grid(ctx.lats, ctx.nlon, ak, bk)
-    st = ctx.get_state()
-    ctx.set_state(st["vrt"], st["div"], st["tv"], st["lnps"], trac=ctx.grdtospec(qg)[None, None])
+    ctx = moist_context(args.config)
     ctx.autotune()
     mlp = None
     if args.moist == "mlp":
@@ -445,7 +522,9 @@ def main():
     ap.add_argument("--members", type=int, default=0, help="ensemble members (ensemble workload)")
     ap.add_argument("--pattern", action="store_true", help="AR(1) temperature pattern per member")
     ap.add_argument("--micro", action="store_true", help="transform microbenchmark")
-    ap.add_argument("--moist", choices=("plain", "mlp"), default=None)
+    ap.add_argument("--moist", choices=("plain", "mlp", "c4train"), default=None)
+    ap.add_argument("--c4-columns", type=int, default=10_000_000, help="training columns of --moist c4train")
+    ap.add_argument("--c4-train-steps", type=int, default=20000, help="AdamW steps of --moist c4train")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (profiler counter calibration)")
     args = ap.parse_args()
@@ -459,6 +538,8 @@ def main():
     elif args.members:
         args.config = args.config or "T126L64"
         line = run_ensemble(args)
+    elif args.moist == "c4train":
+        line = run_c4train(args)
     elif args.moist:
         args.config = args.config or "T254L64"
         line = run_moist(args)

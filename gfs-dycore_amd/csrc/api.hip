@@ -202,6 +202,7 @@ long gfsdc_mlp_nparams(gfsdc_ctx *c) {
     return c->ef ? c->ef->mlp_nparams() : (c->ed ? c->ed->mlp_nparams() : 0);
 }
 int gfsdc_mlp_params(gfsdc_ctx *c, float *get, const float *set) { DISPATCH(c, mlp_params(get, set)); }
+int gfsdc_mlp_norm(gfsdc_ctx *c, float *get, const float *set) { DISPATCH(c, mlp_norm(get, set)); }
 int gfsdc_mlp_forward(gfsdc_ctx *c, const float *x, long n, float *y) { DISPATCH(c, mlp_forward_host(x, n, y)); }
 int gfsdc_mlp_grad(gfsdc_ctx *c, const float *x, const float *t, int n, float *grad, double *loss) {
     DISPATCH(c, mlp_grad_host(x, t, n, grad, loss));

@@ -3256,6 +3256,36 @@ template <typename T> int Engine<T>::mlp_params(float *get, const float *set) {
     return 0;
 }
 
+template <typename T> int Engine<T>::mlp_norm(float *get, const float *set) {
+    DycoreBuffers *B = dyn;
+    int rc = mlp_need<T>(B, false);
+    if (rc) return rc;
+    const int nin = B->mlp->nin, nout = B->mlp->nout, n = 2 * (nin + nout);
+    if (set) {
+        // means finite, spreads positive and finite (the features divide by them)
+        for (int i = 0; i < n; i++) {
+            const bool spread = (i >= nin && i < 2 * nin) || i >= 2 * nin + nout;
+            if (!std::isfinite(set[i]) || (spread && !(set[i] > 0.f))) {
+                set_error("gfsdc_mlp_norm: entry " + std::to_string(i) + " is not a finite mean or a positive spread");
+                return GFSDC_EINVAL;
+            }
+        }
+        GFSDC_CHECK(hipMemcpyAsync(B->nrm, set, sizeof(float) * n, hipMemcpyHostToDevice, stream));
+        GFSDC_CHECK(hipStreamSynchronize(stream));
+        B->nrm_ready = true;
+        drop_graph(B);  // a captured step holds no copy, but keep captures in step with the network state
+    }
+    if (get) {
+        if (!B->nrm_ready) {
+            set_error("column MLP has no normalisation: collect data and call gfsdc_mlp_normalize first");
+            return GFSDC_ESTATE;
+        }
+        GFSDC_CHECK(hipMemcpyAsync(get, B->nrm, sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+        GFSDC_CHECK(hipStreamSynchronize(stream));
+    }
+    return 0;
+}
+
 template <typename T> int Engine<T>::mlp_forward_host(const float *x, long n, float *y) {
     DycoreBuffers *B = dyn;
     int rc = mlp_need<T>(B, false);

@@ -79,6 +79,7 @@ template <typename T> struct Engine {
     int mlp_judge_stats(double *out);
     long mlp_nparams();
     int mlp_params(float *get, const float *set);
+    int mlp_norm(float *get, const float *set);
     int mlp_forward_host(const float *x, long n, float *y);
     int mlp_grad_host(const float *x, const float *t, int n, float *grad, double *loss);
     int mlp_adamw_host(const float *grad, double lr, double wd);

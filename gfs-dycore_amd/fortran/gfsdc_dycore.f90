@@ -11,10 +11,12 @@ module gfsdc_dycore
             dyc_getdyntend, dyc_advance, dyc_step_diag, dyc_init_rest_state, dyc_sync, &
             dyc_set_time, dyc_get_time, dyc_dfi, dyc_iau_set, dyc_checkpoint_size, dyc_checkpoint_save, &
             dyc_checkpoint_load, gfsdc_stoch_cfg, dyc_stoch_defaults, dyc_stoch_init, dyc_get_stoch, &
-            GFSDC_FP64, GFSDC_FP32, GFSDC_HOST, DT_F64, DT_F32, PHYS_NONE, PHYS_HS
+            dyc_mlp_init, dyc_mlp_collect, dyc_mlp_normalize, dyc_mlp_train, dyc_mlp_enable, &
+            dyc_mlp_judge_stats, dyc_mlp_params, dyc_mlp_norm, &
+            GFSDC_FP64, GFSDC_FP32, GFSDC_HOST, DT_F64, DT_F32, PHYS_NONE, PHYS_HS, PHYS_SIMPLE
 
   integer(c_int), parameter :: GFSDC_FP64 = 0, GFSDC_FP32 = 1, GFSDC_HOST = 0, DT_F64 = 0, DT_F32 = 1
-  integer(c_int), parameter :: PHYS_NONE = 0, PHYS_HS = 1
+  integer(c_int), parameter :: PHYS_NONE = 0, PHYS_HS = 1, PHYS_SIMPLE = 2
 
   type, bind(c) :: gfsdc_cfg
     integer(c_int) :: nlon, nlat, ntrunc, nlev, ntrac, nmem, precision
@@ -197,6 +199,16 @@ module gfsdc_dycore
     integer(c_int) function dyc_mlp_judge_stats(h, out3) bind(c, name="gfsdc_mlp_judge_stats")
       import :: c_ptr, c_int
       type(c_ptr), value :: h, out3
+    end function
+    ! parameters and normalisation of the network (get and / or set; c_null_ptr
+    ! skips a direction): a network trained in one context run in another
+    integer(c_int) function dyc_mlp_params(h, get, set) bind(c, name="gfsdc_mlp_params")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, get, set
+    end function
+    integer(c_int) function dyc_mlp_norm(h, get, set) bind(c, name="gfsdc_mlp_norm")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, get, set
     end function
   end interface
 

@@ -68,7 +68,7 @@ SYMBOLS = [
     "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
     "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_autotune", "gfsdc_bench_transforms", "gfsdc_bench_copy", "gfsdc_mlp_init", "gfsdc_mlp_collect",
     "gfsdc_mlp_normalize", "gfsdc_mlp_train", "gfsdc_mlp_eval", "gfsdc_mlp_enable", "gfsdc_mlp_judge_stats",
-    "gfsdc_mlp_nparams", "gfsdc_mlp_params", "gfsdc_mlp_forward", "gfsdc_mlp_grad", "gfsdc_mlp_adamw",
+    "gfsdc_mlp_nparams", "gfsdc_mlp_params", "gfsdc_mlp_norm", "gfsdc_mlp_forward", "gfsdc_mlp_grad", "gfsdc_mlp_adamw",
     "gfsdc_get_grid", "gfsdc_get_semimp",
     "gfsdc_get_damp", "gfsdc_step_diag", "gfsdc_init_rest_state", "gfsdc_set_time",
     "gfsdc_get_time", "gfsdc_iau_set", "gfsdc_dfi", "gfsdc_stoch_defaults", "gfsdc_stoch_init", "gfsdc_get_stoch", "gfsdc_checkpoint_size", "gfsdc_checkpoint_save",
@@ -124,6 +124,7 @@ def load():
     L.gfsdc_mlp_nparams.argtypes = [vp]
     L.gfsdc_mlp_nparams.restype = lg
     L.gfsdc_mlp_params.argtypes = [vp, vp, vp]
+    L.gfsdc_mlp_norm.argtypes = [vp, vp, vp]
     L.gfsdc_mlp_forward.argtypes = [vp, vp, lg, vp]
     L.gfsdc_mlp_grad.argtypes = [vp, vp, vp, i, vp, C.POINTER(d)]
     L.gfsdc_mlp_adamw.argtypes = [vp, vp, d, d]
@@ -391,6 +392,16 @@ class Context:
         src = None if set is None else np.ascontiguousarray(set, np.float32)
         _check(load().gfsdc_mlp_params(self.h, _p(out), _p(src) if src is not None else None))
         return out
+
+    def mlp_norm(self, set=None):
+        """the normalisation [mean_x | std_x | mean_y | std_y]; set= installs one
+        (with mlp_params, a network trained in another context of the same levels)"""
+        out = np.zeros(2 * (self.mlp_nin + self.mlp_nout), np.float32)
+        src = None if set is None else np.ascontiguousarray(set, np.float32)
+        if src is not None and src.size != out.size:
+            raise GfsdcError(f"mlp_norm: {src.size} values, expected {out.size}")
+        _check(load().gfsdc_mlp_norm(self.h, _p(out) if set is None else None, _p(src) if src is not None else None))
+        return out if set is None else src
 
     def mlp_forward(self, x):
         x = np.ascontiguousarray(x, np.float32)

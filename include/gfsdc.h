@@ -249,6 +249,12 @@ int gfsdc_mlp_judge_stats(gfsdc_ctx *ctx, double *out3);
 /* parameters in the layout W_1[nin][hidden], b_1[hidden], ..., W_L[hidden][nout], b_L[nout] */
 long gfsdc_mlp_nparams(gfsdc_ctx *ctx);
 int gfsdc_mlp_params(gfsdc_ctx *ctx, float *get, const float *set);
+/* the input and output normalisation [mean_x | std_x | mean_y | std_y]
+ * (2 (nin + nout) floats; gfsdc_mlp_normalize makes it from the collected
+ * columns): with gfsdc_mlp_params it moves a trained network to another
+ * context of the same levels (a network trained on T126L64 columns run at
+ * T254L64). Setting it refuses non-finite means and non-positive spreads. */
+int gfsdc_mlp_norm(gfsdc_ctx *ctx, float *get, const float *set);
 /* network on n normalised host rows x[n][nin] -> y[n][nout] */
 int gfsdc_mlp_forward(gfsdc_ctx *ctx, const float *x, long n, float *y);
 /* mean-squared-error loss and its parameter gradient on n <= maxbatch normalised host rows */

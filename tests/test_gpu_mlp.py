@@ -153,3 +153,45 @@ def test_fallback_reproduces_the_plain_physics():
     assert a.mlp_judge_stats()["fallbacks"] == 2
     for k in ("vrt", "div", "tv", "lnps", "trac"):
         assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_network_moves_to_another_context():
+    """a network trained in one context, installed in another through
+    gfsdc_mlp_params and gfsdc_mlp_norm, computes the same outputs and the same
+    judged steps; the normalisation refuses non-positive spreads"""
+    import gfsdc
+    print("mlp transfer", flush=True)
+    a = moist_ctx(seed=6)
+    a.mlp_init(HID, NLAY, 1024, 20000, seed=5)
+    a.mlp_collect(6, 8)
+    a.mlp_normalize()
+    a.mlp_train(200, 256, 1e-3, 1e-4)
+    p, nrm = a.mlp_params(), a.mlp_norm()
+    assert nrm.size == 2 * (a.mlp_nin + a.mlp_nout)
+    b = moist_ctx(seed=6)
+    b.mlp_init(HID, NLAY, 1024, 1, seed=99)       # other initial weights, no data
+    with pytest.raises(gfsdc.GfsdcError, match="normalisation"):
+        b.mlp_enable(True)                         # nothing installed yet
+    b.mlp_params(set=p)
+    bad = nrm.copy()
+    bad[a.mlp_nin] = 0.0                           # a zero input spread
+    with pytest.raises(gfsdc.GfsdcError, match="positive spread"):
+        b.mlp_norm(set=bad)
+    b.mlp_norm(set=nrm)
+    assert np.array_equal(b.mlp_norm(), nrm)
+    assert np.array_equal(b.mlp_params(), p)
+    x = np.random.default_rng(2).standard_normal((300, a.mlp_nin)).astype(np.float32)
+    assert np.array_equal(a.mlp_forward(x), b.mlp_forward(x))
+    # the same judged steps from the same state
+    c = moist_ctx(seed=7)
+    st = c.get_state()
+    for g in (a, b):
+        g.set_state(st["vrt"], st["div"], st["tv"], st["lnps"], trac=st["trac"])
+        g.pdryini = np.zeros(1)                    # both fix the dry mass of this state
+        g.mlp_enable(True, judge_stride=16, threshold=10.0)
+        g.advance(2, use_graph=True)
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("vrt", "div", "tv", "lnps", "trac"):
+        assert np.array_equal(sa[k], sb[k]), k
+    # (the judged columns differ: each context keeps its own sampling counter)
+    assert a.mlp_judge_stats()["judged"] == b.mlp_judge_stats()["judged"] == 2
