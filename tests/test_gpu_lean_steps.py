@@ -12,6 +12,7 @@ with two members, the dry-mass fixer and the stochastic temperature pattern
 import numpy as np
 import pytest
 
+import cases
 import levels
 
 pytestmark = pytest.mark.gpu
@@ -70,3 +71,27 @@ def test_profiled_lean_step_refuses_the_grids():
         a.step_diag()
     a.getdyntend(1, inverse_only=True)
     a.grid("ug")
+
+
+@pytest.mark.parametrize("physics", [0, 1])
+def test_grids_after_advance_match_oracle(physics):
+    """the grid fields an advance leaves (grid_data / pressure_data after the
+    last getdyntend of its last step: the physics pass's with Held-Suarez, the
+    third dynamics pass's without physics, where the column kernel stores the
+    omega/p and etadot grids itself) against the FP64 oracle after the same
+    steps, to 1e-9"""
+    import gfsdc
+    import oracle as O
+    print(f"grids after advance physics={physics}", flush=True)
+    ak, bk = levels.hybrid_levels(NL)
+    o = O.Model(NLON, NLAT, 42, NL, ak, bk, 1200.0, physics=physics)
+    g = gfsdc.Context(NLON, NLAT, 42, gfsdc.FP64, nlev=NL, ak=ak, bk=bk, dt=1200.0, physics=physics)
+    vrt, div, tv, lnps = cases.rest_state(o.sht.degree, o.sht.order, NL, seed=4, vort_rms=2e-5)
+    o.set_state(vrt[0], div[0], tv[0], lnps[0])
+    g.set_state(vrt, div, tv, lnps)
+    o.advance(3)
+    g.advance(3, use_graph=True)
+    for name in ("ug", "vg", "vrtg", "divg", "virtempg", "dlnpdtg", "etadot", "psg", "dlnpsdt"):
+        a, b = g.grid(name)[0], o.grid(name)
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        assert err < 1e-9, (name, err)
