@@ -592,14 +592,21 @@ template <typename T> __device__ inline PrsSeries<T> prs_series(T kappa) {
                         (a6 - T(1)) / (T(181440) * kappa), -(a8 - T(1)) / (T(9676800) * kappa),
                         (a10 - T(1)) / (T(479001600) * kappa)};
 }
+// FP32 natural logarithm and exponential on the hardware base-2 instructions
+// (v_log_f32 / v_exp_f32, about 1 ulp for the normal positive arguments here)
+__device__ __forceinline__ float ln_hw(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+__device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
 template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa, const PrsSeries<T> &c) {
     if constexpr (sizeof(T) == 4) {
         if (p1 > T(0)) {
-            const T y = log1p((p2 - p1) * recip(p1));
+            // y only enters through S = O(y^2 / 12): the rounding of 1 + x in the
+            // hardware logarithm leaves S, and the pressure, exact to far below an ulp
+            const T y = ln_hw(T(1) + (p2 - p1) * recip(p1));
             if (y <= T(1)) {
                 const T y2 = y * y;
                 const T S = y2 * (c.d1 + y2 * (c.d2 + y2 * (c.d3 + y2 * (c.d4 + y2 * c.d5))));
-                return sqrt(p1 * p2) * exp(S);
+                return sqrt(p1 * p2) * exp_hw(S);
             }
         }
     }
@@ -663,9 +670,17 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
         T prs = phillips_prs(p1, p2, kappa, prc);
         T blprof = prs * rps;
         // (prs / p0)^(2/7) as exp(2/7 ln(prs / p0)), sharing the logarithm of the
-        // temperature profile
-        const T lr = log(prs / p0);
-        T teq = exp(expo * lr) * (T(315) - deltmp * sl * sl - delthz * lr * cl2);
+        // temperature profile (in FP32 the base-2 hardware instructions)
+        T lr, pw;
+        if constexpr (sizeof(T) == 4) {
+            const float l2 = __builtin_amdgcn_logf(prs / p0);  // log2, on the hardware
+            lr = l2 * 0.693147180559945309f;
+            pw = __builtin_amdgcn_exp2f(expo * l2);
+        } else {
+            lr = log(prs / p0);
+            pw = exp(expo * lr);
+        }
+        T teq = pw * (T(315) - deltmp * sl * sl - delthz * lr * cl2);
         blprof = (blprof - sigbot) * rsb;
         if (blprof < T(0)) blprof = T(0);
         if (teq < tempstrat) teq = tempstrat;
