@@ -23,8 +23,9 @@ Other workloads (each prints its own JSON line instead of the headline):
   --micro                  transform microbenchmark: spectogrd + grdtospec +
                            getuv on 64 levels x 4 fields, FP64 / FP32 / BF16X3
   --moist {plain,mlp}      moist run with the simple physics or the column MLP
-  --moist c4train          the C4 network trained on 1e7 T126L64 moist columns,
-                           then run at T254L64 in the simple physics' place
+  --moist c4train          the 259-3x512-256 column network trained on 1e7
+                           T126L64 moist columns, then run at T254L64 in the
+                           simple physics' place
   --copy-calibration       (with any workload) also run a 1 GiB device-to-device
                            copy, the reference kernel of known size for scaling
                            rocprofv3 FETCH_SIZE / WRITE_SIZE into HBM bytes
@@ -397,9 +398,8 @@ def moist_context(config):
 
 
 def run_c4train(args):
-    """The column network of config C4 trained as specified and run in place of
-    the physics it learned (phy_run_simple.f90:66-67): 259 -> 3 x 512 -> 256,
-    trained on --c4-columns columns of a T126L64 moist run with the simple
+    """The 259 -> 3 x 512 -> 256 column network trained and run in place of the
+    physics it learned (phy_run_simple.f90:66-67): trained on --c4-columns columns of a T126L64 moist run with the simple
     physics (every 4th column of each step after a 12-hour spin-up, so the
     samples cover several model days), AdamW on batches of 4096. The last 10 %
     of the columns, later in time than any training column, are held out. The

@@ -2,10 +2,10 @@
 
 * T126L64 and T254L64 (hybrid levels, Held-Suarez): FP64 getdyntend and one
   advance step equal the oracle to 1e-9; the FP32 tendencies to 2e-4.
-* C3: 32 members of T126L64 with the AR(1) temperature pattern folded into
+* 32 members of T126L64 with the AR(1) temperature pattern folded into
   every launch. Each member equals its own single-member run (1e-12, FP64),
   and each member's pattern equals the oracle's for its seed.
-* C4: T254L64 moist run with the 259 -> 3 x 512 -> 256 column network. Its
+* the T254L64 moist run with the 259 -> 3 x 512 -> 256 column network. Its
   forward and gradient equal the oracle network evaluated with the same bf16
   storage rounding (a row sample); with a 2 % judge (every 50th column) and a
   zero threshold every step falls back, and the step is then bit for bit the
@@ -84,7 +84,7 @@ def test_c3_ensemble_32_members_with_pattern():
                                        dt=c["dt"], nmem=nmem, tpat_seed=s, **PAT)
     ens = mk(nm, seed)
     ens.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=seed)
-    print("C3: 32-member T126L64 ensemble, 2 steps", flush=True)
+    print("32-member T126L64 ensemble, 2 steps", flush=True)
     ens.advance(2, use_graph=True)
     se, pe = ens.get_state(), ens.pattern()
     del ens
@@ -137,7 +137,7 @@ def test_c4_network_forward_and_gradient_full_size():
     rows = rng.choice(1500, 192, replace=False)
     _, _, yo = net.loss_grad(p, x[rows], np.zeros((rows.size, nout)), grad=False, bf16=True)
     e = rel2(y[rows], yo)
-    print(f"C4 forward rel L2 vs bf16 oracle: {e:.2e}")
+    print(f"259-3x512-256 forward rel L2 vs bf16 oracle: {e:.2e}")
     assert e < 4e-3, e
     xb, tb = x[:256], rng.normal(0, 1, (256, nout)).astype(np.float32)
     grad, loss = g.mlp_grad(xb, tb)
@@ -147,7 +147,7 @@ def test_c4_network_forward_and_gradient_full_size():
     for K, N in [(nin, 512), (512, 512), (512, 512), (512, nout)]:
         for sl in (slice(o, o + K * N), slice(o + K * N, o + K * N + N)):
             e = rel2(grad[sl], go[sl])
-            print(f"C4 gradient block {K}x{N} rel L2 vs bf16 oracle: {e:.2e}")
+            print(f"259-3x512-256 gradient block {K}x{N} rel L2 vs bf16 oracle: {e:.2e}")
             assert e < 6e-3, (K, N, e)
         o += K * N + N
 
