@@ -399,9 +399,10 @@ def moist_context(config):
 
 def run_c4train(args):
     """The 259 -> 3 x 512 -> 256 column network trained and run in place of the
-    physics it learned (phy_run_simple.f90:66-67): trained on --c4-columns columns of a T126L64 moist run with the simple
-    physics (every 4th column of each step after a 12-hour spin-up, so the
-    samples cover several model days), AdamW on batches of 4096. The last 10 %
+    physics it learned (phy_run_simple.f90:66-67): trained on --c4-columns
+    columns of a T126L64 moist run with the simple physics (every 4th column of
+    each step after a 12-hour spin-up, so the samples cover several model
+    days), AdamW on batches of 4096. The last 10 %
     of the columns, later in time than any training column, are held out. The
     network and its normalisation then move to a T254L64 moist context (same
     64 levels), where it replaces the simple physics inside the timed step,
