@@ -890,6 +890,18 @@ template <typename T, int R, int N, int Ns> struct StageTw {
 struct NoTw {
     template <typename P> __device__ __forceinline__ void load(const P *) {}
 };
+// Twiddles of a last stage of 256 butterflies run four consecutive butterflies
+// per lane (j = 4 lane + bb), loaded during the stage before it
+template <typename T, int R, int NS> struct QuadTw {
+    cplx<T> t[4][R - 1];
+    __device__ __forceinline__ void load(const cplx<T> *__restrict__ tw) {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++)
+#pragma unroll
+            for (int r = 1; r < R; r++) t[bb][r - 1] = tw[NS - 1 + (r - 1) * NS + 4 * lane + bb];
+    }
+};
 
 template <typename T, int R, int N, int Ns, typename Ld, typename St, typename TN>
 __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int sign, Ld ld, St st,
@@ -941,11 +953,13 @@ __device__ __forceinline__ void fft_stage_w(const cplx<T> *__restrict__ tw, int 
 // instead of LDS; LAST hands the last stage's outputs to st instead of LDS (the
 // last stage writes x = j + r Ns with j < Ns, consecutive across lanes, so a
 // global store from it is coalesced).
-template <typename T, int N, int S, int Ns, bool FIRST, bool LAST, typename Ld, typename St>
+template <typename T, int N, int S, int Ns, bool FIRST, bool LAST, int SEND, typename Ld, typename St,
+          typename TT = NoTw>
 __device__ __forceinline__ void fft_stages_w(cplx<T> *bq, const cplx<T> *tw, int sign, Ld ld, St st,
-                                             const StageTw<T, rad_plan(N).r[S], N, Ns> &tc) {
+                                             const StageTw<T, rad_plan(N).r[S], N, Ns> &tc, TT *tail = nullptr) {
     constexpr RadPlan P = rad_plan(N);
-    if constexpr (S < P.n) {
+    constexpr int SE = SEND < P.n ? SEND : P.n;  // stages S .. SE - 1 run here
+    if constexpr (S < SE) {
         constexpr int R = P.r[S];
         auto lds_ld = [&](int, int p) { return bq[p]; };
         auto lds_st = [&](int, int p, const cplx<T> &v) { bq[p] = v; };
@@ -961,10 +975,12 @@ __device__ __forceinline__ void fft_stages_w(cplx<T> *bq, const cplx<T> *tw, int
                 fft_stage_w<T, R, N, Ns>(tw, sign, lds_ld, lds_st, tc, tn);
             }
         };
-        if constexpr (S + 1 < P.n) {
+        if constexpr (S + 1 < SE) {
             StageTw<T, P.r[S + 1], N, Ns * R> tn;
             run(tn);
-            fft_stages_w<T, N, S + 1, Ns * R, FIRST, LAST>(bq, tw, sign, ld, st, tn);
+            fft_stages_w<T, N, S + 1, Ns * R, FIRST, LAST, SEND>(bq, tw, sign, ld, st, tn, tail);
+        } else if (tail) {
+            run(*tail);  // the twiddles of the stage that follows this call's last
         } else {
             NoTw tn;
             run(tn);
@@ -1019,6 +1035,13 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     // with the coefficients (field clamped into range; a wave past nf stores nothing)
     T w_s = T(1), w_c = T(0);
     long w_g = 0;
+    // a last stage of 256 butterflies (the 768-point transform: radix 3, Ns = 256)
+    // runs with four consecutive butterflies per lane, so that each lane's outputs
+    // of one row are four consecutive grid points (one 16-byte store); its
+    // twiddles come from the global table during the stage before it
+    constexpr RadPlan PS = rad_plan(NC > 0 ? NC : 2);
+    constexpr int RL = PS.r[PS.n > 0 ? PS.n - 1 : 0], NSL = (NC > 0 ? NC : 2) / RL;
+    constexpr bool QUADL = WAVE && NSL == 256 && RL <= 4;
     if constexpr (WAVE) {
         const int fw = min(f0 + (tid >> 6), nf - 1);
         const T rc = rcos[j];  // unconditional, so that no load waits for another
@@ -1084,13 +1107,53 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
         const long gf = w_g;  // grid field the transform is written to
         T *gn = grid + gf * gstride + (long)j * nlon, *gs = grid + gf * gstride + (long)j2 * nlon;
         auto none = [](int, int) { return cplx<T>{T(0), T(0)}; };
+        constexpr RadPlan P = rad_plan(NC);
+        if constexpr (QUADL) {
+            cplx<T> *bq = buf + q * NP;
+            auto lds_st = [&](int, int p, const cplx<T> &v) { bq[p] = v; };
+            QuadTw<T, RL, NSL> tl;
+            fft_stages_w<T, NC, 0, 1, false, false, P.n - 1>(bq, GTW ? tw : tws, +1, none, lds_st,
+                                                             StageTw<T, P.r[0], NC, 1>{}, &tl);
+            // the last stage: butterflies j = 4 lane + bb, outputs x = j + r NSL
+            const int lane = tid & 63;
+            cplx<T> v[4][RL];
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++)
+#pragma unroll
+                for (int r = 0; r < RL; r++) v[bb][r] = bq[fft_pad(4 * lane + bb + r * NSL)];
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) {
+#pragma unroll
+                for (int r = 1; r < RL; r++) {
+                    cplx<T> t = tl.t[bb][r - 1];
+                    t.y = -t.y;  // sign +1
+                    v[bb][r] = cmul(v[bb][r], t);
+                }
+                small_dft<T, RL>(v[bb], +1);
+            }
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                const int x = 4 * lane + r * NSL;
+                if constexpr (sizeof(T) == 4) {
+                    *(float4 *)(gn + x) = float4{v[0][r].x * s + c, v[1][r].x * s + c, v[2][r].x * s + c,
+                                                 v[3][r].x * s + c};
+                    *(float4 *)(gs + x) = float4{v[0][r].y * s + c, v[1][r].y * s + c, v[2][r].y * s + c,
+                                                 v[3][r].y * s + c};
+                } else {
+                    *(double2 *)(gn + x) = double2{v[0][r].x * s + c, v[1][r].x * s + c};
+                    *(double2 *)(gn + x + 2) = double2{v[2][r].x * s + c, v[3][r].x * s + c};
+                    *(double2 *)(gs + x) = double2{v[0][r].y * s + c, v[1][r].y * s + c};
+                    *(double2 *)(gs + x + 2) = double2{v[2][r].y * s + c, v[3][r].y * s + c};
+                }
+            }
+            return;
+        }
         auto out = [&](int x, int, const cplx<T> &z) {
             gn[x] = z.x * s + c;
             gs[x] = z.y * s + c;
         };
-        constexpr RadPlan P = rad_plan(NC);
-        fft_stages_w<T, NC, 0, 1, false, true>(buf + q * NP, GTW ? tw : tws, +1, none, out,
-                                               StageTw<T, P.r[0], NC, 1>{});
+        fft_stages_w<T, NC, 0, 1, false, true, P.n>(buf + q * NP, GTW ? tw : tws, +1, none, out,
+                                                    StageTw<T, P.r[0], NC, 1>{});
         return;
     }
     if constexpr (NC > 0)
@@ -1155,7 +1218,7 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
             __syncthreads();  // the twiddle table (the global one needs no barrier)
             t1.load(tws);
         }
-        fft_stages_w<T, NC, 1, P.r[0], false, false>(bq, GTW ? tw : tws, -1, in, lds_st, t1);
+        fft_stages_w<T, NC, 1, P.r[0], false, false, P.n>(bq, GTW ? tw : tws, -1, in, lds_st, t1);
         __syncthreads();  // the coefficients below are read across fields
     } else {
         constexpr int MAXG = (NFFT * LN + 255) / 256;
