@@ -89,33 +89,37 @@ def kernel_table(ctx, nprof):
     return {k: (v[0] / v[1], v[1] / nprof) for k, v in acc.items()}
 
 
-def algorithmic_work(name, ctx, esz):
-    """(kind, work per launch) of one launch of a step kernel: flops for the
-    Legendre GEMMs (2 nlat ndimspec per scalar field), bytes otherwise
-    (each input / output array read or written once)."""
+def algorithmic_work(name, ctx, esz, launches):
+    """(kind, work per launch) of a step kernel, the mean over the step's
+    `launches` launches of it: flops for the Legendre GEMMs (2 nlat ndimspec
+    per scalar field), bytes otherwise (each input / output array read or
+    written once)."""
     nlat, nlon, T, nlm = ctx.nlat, ctx.nlon, ctx.ntrunc, ctx.nlm
     nl, nm = ctx.nlev, ctx.nmem
+    # fields of a dynamics pass: synthesised, and analysed (T, the two flux
+    # components and KE per level, ln ps and the tracers)
     nsyn = (7 + 3 * ctx.ntrac) * nl + 3
+    nan = (4 + ctx.ntrac) * nl + 1
     if ctx.cfg.physics == gfsdc.PHYS_HS:
-        # three dynamics passes of nsyn fields and the lean Held-Suarez pass of the
-        # profiled step (vorticity, divergence, temperature, ln ps): the mean per launch
-        nsyn = (3 * nsyn + 3 * nl + 1) / 4
+        # three dynamics passes and the lean Held-Suarez pass of the profiled
+        # step: temperature and ln ps, then vorticity and divergence on the
+        # levels where the drag acts; the forcing analysed on the same levels
+        lean = (nl + 1) + 2 * ctx.hs_drag_levels()
+        syn_step, an_step = 3 * nsyn + lean, 3 * nan + lean
     elif ctx.cfg.physics != gfsdc.PHYS_NONE:
         # the physics pass skips the temperature and tracer gradients
-        nsyn = (3 * nsyn + (5 + ctx.ntrac) * nl + 3) / 4
+        syn_step, an_step = 3 * nsyn + (5 + ctx.ntrac) * nl + 3, None
+    else:
+        syn_step, an_step = 3 * nsyn, 3 * nan
+    nsyn = syn_step / launches  # mean fields per launch
     ng = nlat * nlon
     four = (T + 1) * nlat * 2 * esz  # Fourier coefficients of one field, both hemispheres
     if name == "leg_synth":
         return "mfma", nm * nsyn * 2.0 * nlat * nlm
     if name == "leg_anal":
-        # dynamics passes: T, the two flux components and KE per level, ln ps and
-        # the tracers; the Held-Suarez forcing: T, vorticity, divergence, ln ps
-        nan = (4 + ctx.ntrac) * nl + 1
-        if ctx.cfg.physics == gfsdc.PHYS_HS:
-            nan = (3 * nan + 3 * nl + 1) / 4
-        elif ctx.cfg.physics != gfsdc.PHYS_NONE:
+        if an_step is None:
             return "mfma", None
-        return "mfma", nm * nan * 2.0 * nlat * nlm
+        return "mfma", nm * an_step / launches * 2.0 * nlat * nlm
     if name == "fft_synth":
         return "hbm", nm * nsyn * (four + ng * esz)
     if name == "mlp_forward" and getattr(ctx, "mlp_shape", None):
@@ -136,7 +140,7 @@ def algorithmic_work(name, ctx, esz):
 def roofline(tab, ctx, esz, workload):
     per_step = {k: v[0] * v[1] for k, v in tab.items()}
     dom = max(per_step, key=per_step.get)
-    kind, work = algorithmic_work(dom, ctx, esz)
+    kind, work = algorithmic_work(dom, ctx, esz, tab[dom][1])
     avg_s = tab[dom][0] * 1e-3
     roof = {"kernel": dom, "bound": kind, "avg_launch_ms": tab[dom][0],
             "traffic": measured_traffic(dom, workload)}
@@ -288,8 +292,10 @@ def run_headline(args):
     roof, per_step = roofline(tab, c32, 4, args.config)
     def mfma_frac(name):
         leg = tab.get(name)
-        _, f = algorithmic_work(name, c32, 4)
-        if not leg or f is None:
+        if not leg:
+            return None
+        _, f = algorithmic_work(name, c32, 4, leg[1])
+        if f is None:
             return None
         return {"kernel": name, "avg_launch_ms": leg[0], "achieved_tflops": f / (leg[0] * 1e-3) / 1e12,
                 "peak_tflops": PEAK["fp32_mfma"] / 1e12, "frac": f / (leg[0] * 1e-3) / PEAK["fp32_mfma"]}

@@ -171,6 +171,7 @@ int gfsdc_advance(gfsdc_ctx *c, int nsteps, int use_graph) { DISPATCH(c, advance
 int gfsdc_hs_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, int dtype, int loc) {
     DISPATCH(c, hs_phytend(dvrt, ddiv, dtv, dtype, loc));
 }
+int gfsdc_hs_drag_levels(gfsdc_ctx *c, int *n) { DISPATCH(c, hs_drag_levels(n)); }
 int gfsdc_simple_phytend(gfsdc_ctx *c, void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc) {
     DISPATCH(c, simple_phytend(dvrt, ddiv, dtv, dtrac, dtype, loc));
 }

@@ -146,14 +146,22 @@ struct DycoreBuffers {
     SynthDesc *sdesc_ph = nullptr;
     unsigned char *rflag_ph = nullptr;
     int *slot_ph = nullptr;
-    // the lean Held-Suarez physics pass of the steps before an advance's last:
-    // vorticity, divergence, temperature and ln ps only (nhl per member), the
-    // grids the forcing reads (phy_run_hs.f90:66-76); the winds, their
-    // gradients and getomega serve only the post-step grids and diagnostics
-    int nhl = 0;
-    SynthDesc *sdesc_hl = nullptr;
-    unsigned char *rflag_hl = nullptr;
-    int *slot_hl = nullptr;
+    // the lean Held-Suarez physics pass of the steps before an advance's last
+    // synthesises only the grids the forcing reads (phy_run_hs.f90:66-76); the
+    // winds, their gradients and getomega serve only the post-step grids and
+    // diagnostics. Its first synthesis holds temperature and ln ps (nhla =
+    // nm (nl + 1) fields, member-major); its second vorticity and divergence,
+    // level-major from the lowest level (nhlb = 2 nm nl fields: level k, member,
+    // vrt / div), of which it transforms only the levels where the drag can act
+    int nhla = 0, nhlb = 0;
+    SynthDesc *sdesc_hla = nullptr, *sdesc_hlb = nullptr;
+    unsigned char *rflag_hla = nullptr, *rflag_hlb = nullptr;
+    int *slot_hla = nullptr, *slot_hlb = nullptr;
+    // device counts of the lean pass, written by k_area_final: [0] fields of
+    // the second synthesis (2 nm nact), [1] Held-Suarez forcing fields to
+    // analyse (nm (nl + 1) + 2 nm nact), [2] nact, the levels counted from the
+    // lowest whose drag is evaluated
+    int *hsdyn = nullptr;
     // the grids after the last step are those of a lean pass (profile_step):
     // get_grid and step_diag refuse them
     bool grids_partial = false;
@@ -616,6 +624,16 @@ template <typename T> __device__ inline T phillips_prs(T p1, T p2, T kappa) {
     return phillips_prs(p1, p2, kappa, prs_series(kappa));
 }
 
+// Held-Suarez forcing fields in the analysis grid buffer: every member's
+// temperature forcing of levels 0..nl-1 (level 0 the lowest) with its
+// mass-fixer field as level nl, then the vorticity and divergence forcing
+// level-major from the lowest level (level k, member, vrt / div), so that the
+// levels where the drag acts form a prefix of the buffer.
+enum { HS_T = 0, HS_VRT = 1, HS_DIV = 2 };
+__host__ __device__ inline int hs_field(int nl, int nm, int mem, int kind, int k) {
+    return kind == HS_T ? mem * (nl + 1) + k : nm * (nl + 1) + 2 * (k * nm + mem) + (kind - HS_VRT);
+}
+
 // Surface pressure of the lean Held-Suarez pass: ps = exp(mean + ln ps'), the
 // mean from the spectral state in FP64, as the column kernel forms it.
 template <typename T>
@@ -629,8 +647,32 @@ __global__ __launch_bounds__(256) void k_psg(const T *__restrict__ gsyn, const c
     psg[(long)mem * ng + col] = (T)exp(c0 + (double)LNPS[0]);
 }
 
+// The levels on which the Held-Suarez drag can act, for the lean pass.
+// phy_run_hs.f90:52-58 sets the drag profile max(0, (p_k / ps - sigbot) /
+// (1 - sigbot)): where p_k / ps stays below sigbot in every column, a level's
+// vorticity and divergence forcing -0 * vrtg is zero on the whole grid, and so
+// is its transform. The layer pressure p_k lies below the layer's lower
+// interface ak + bk ps, and (ak + bk ps) / ps = ak / ps + bk is monotone in ps,
+// so over all columns it is largest at the least or the greatest ps (1 / ps =
+// rmin, rmax). A level where it stays below 0.999 sigbot at both (a margin far
+// above the rounding of p_k) has no drag. The lean pass counts the levels up
+// to the highest other one: nact, with the fields of its vorticity and
+// divergence synthesis (2 nm nact) and the Held-Suarez forcing fields it
+// analyses (nm (nl + 1) + 2 nm nact), in cnt[2], cnt[0] and cnt[1].
+template <typename T>
+__device__ inline bool hs_level_drags(int k, double rmin, double rmax, const T *__restrict__ ak,
+                                      const T *__restrict__ bk, int nl) {
+    const double thr = 0.999 * SGLF(0.7);
+    const double a = (double)ak[nl - k], b = (double)bk[nl - k];  // grid level k: interfaces nl - k - 1, nl - k
+    return a * rmin + b > thr || a * rmax + b > thr;
+}
+
 // Held-Suarez forcing on the post-dynamics grid fields (phy_run_hs.f90:42-77)
-// and the dry-mass-fixer field log(ps * pcorr) (dyn_run.f90:626-631).
+// and the dry-mass-fixer field log(ps * pcorr) (dyn_run.f90:626-631), into the
+// forcing fields of hs_field. With cnt (the lean pass) the vorticity and
+// divergence forcing is formed on the levels below cnt[2] only: hs_level_drags
+// found the drag zero on every level above them, and their vorticity and
+// divergence were not synthesised.
 constexpr int HS_LCH = 8;  // level chunks of the Held-Suarez kernel (levels are independent)
 
 template <typename T>
@@ -639,7 +681,8 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
                                                     const T *__restrict__ bk_, const T *__restrict__ sinlat,
                                                     const double *__restrict__ pdryini,
                                                     const double *__restrict__ pmean, int nl, int ntr, int nsyn,
-                                                    int nhs, long ng, int nlon, T kappa, int massfix) {
+                                                    int nm, long ng, int nlon, T kappa, int massfix,
+                                                    const int *__restrict__ cnt) {
     __shared__ T ak[MAXLEV + 1], bk[MAXLEV + 1];
     for (int i = threadIdx.x; i <= nl; i += blockDim.x) {
         ak[i] = ak_[i];
@@ -651,9 +694,10 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
     const int mem = blockIdx.y / HS_LCH, ch = blockIdx.y - mem * HS_LCH;
     const int CS = (nl + HS_LCH - 1) / HS_LCH, L0 = ch * CS + 1, L1 = min(nl, (ch + 1) * CS);
     if (col >= ng) return;
+    const int nact = cnt ? cnt[2] : nl;
     const T *G = gsyn + (long)mem * nsyn * ng + col;
     const T *VRT = G + 2L * nl * ng, *DIV = G + 3L * nl * ng, *TV = G + 4L * nl * ng;
-    T *A = gan + (long)mem * nhs * ng + col;
+    T *AT = gan + (long)hs_field(nl, nm, mem, HS_T, 0) * ng + col;
     const T ps = psg[(long)mem * ng + col];
     const T sl = sinlat[col / nlon];
     const T cl2 = T(1) - sl * sl;
@@ -685,14 +729,17 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
         if (blprof < T(0)) blprof = T(0);
         if (teq < tempstrat) teq = tempstrat;
         long o = (long)(L - 1) * ng;
-        A[o] = (krada + (kradb - krada) * blprof * cl2 * cl2) * (teq - TV[o]);
-        A[(long)nl * ng + o] = -(blprof * kdrag) * VRT[o];
-        A[2L * nl * ng + o] = -(blprof * kdrag) * DIV[o];
+        AT[o] = (krada + (kradb - krada) * blprof * cl2 * cl2) * (teq - TV[o]);
+        if (L - 1 < nact) {
+            T *AV = gan + (long)hs_field(nl, nm, mem, HS_VRT, L - 1) * ng + col;  // divergence follows it
+            AV[0] = -(blprof * kdrag) * VRT[o];
+            AV[ng] = -(blprof * kdrag) * DIV[o];
+        }
     }
     if (massfix && ch == 0) {
         // log(ps * pcorr) - c0 = ln ps' + log(pcorr)
         const T *LNPS = G + (7L + 3L * ntr) * nl * ng;
-        A[3L * nl * ng] = LNPS[0] + (T)log(pdryini[mem] / pmean[mem]);
+        AT[(long)nl * ng] = LNPS[0] + (T)log(pdryini[mem] / pmean[mem]);
     }
 }
 
@@ -702,7 +749,10 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
 // sums per member, then a fixed-order final sum. The final stage optionally
 // sets pdryini = S_a - S_b where it is still 0 (dyn_run.f90:135-149).
 constexpr int PS_BLOCKS = 128;
+static_assert(MAXLEV <= PS_BLOCKS, "k_area_final tests one level per thread");
 
+// part holds four values per (member, block): S_a, S_b and the least and the
+// greatest a of the block's columns
 template <typename T>
 __global__ __launch_bounds__(256) void k_area_partial(const T *__restrict__ a, const T *__restrict__ b,
                                                       const double *__restrict__ areaw, double *__restrict__ part,
@@ -710,35 +760,50 @@ __global__ __launch_bounds__(256) void k_area_partial(const T *__restrict__ a, c
     const int mem = blockIdx.y;
     const T *pa = a + (long)mem * ng;
     const T *pb = b ? b + (long)mem * ng : nullptr;
-    double sa = 0.0, sb = 0.0;
+    double sa = 0.0, sb = 0.0, lo = INFINITY, hi = -INFINITY;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < ng; i += (long)PS_BLOCKS * 256) {
-        const double w = areaw[i / nlon];
-        sa += w * (double)pa[i];
+        const double w = areaw[i / nlon], v = (double)pa[i];
+        sa += w * v;
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
         if (pb) sb += w * (double)pb[i];
     }
-    __shared__ double ra[256], rb[256];
+    __shared__ double ra[256], rb[256], rl[256], rh[256];
     ra[threadIdx.x] = sa;
     rb[threadIdx.x] = sb;
+    rl[threadIdx.x] = lo;
+    rh[threadIdx.x] = hi;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             ra[threadIdx.x] += ra[threadIdx.x + w];
             rb[threadIdx.x] += rb[threadIdx.x + w];
+            rl[threadIdx.x] = fmin(rl[threadIdx.x], rl[threadIdx.x + w]);
+            rh[threadIdx.x] = fmax(rh[threadIdx.x], rh[threadIdx.x + w]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[2 * (mem * PS_BLOCKS + blockIdx.x)] = ra[0];
-        part[2 * (mem * PS_BLOCKS + blockIdx.x) + 1] = rb[0];
+        double *q = part + 4 * (mem * PS_BLOCKS + blockIdx.x);
+        q[0] = ra[0];
+        q[1] = rb[0];
+        q[2] = rl[0];
+        q[3] = rh[0];
     }
 }
 
+// One workgroup per member sums its partials; with cnt, workgroup 0 also
+// takes the least and greatest a (ps) over every member and block and writes
+// the lean Held-Suarez pass's counts from them (hs_level_drags).
+template <typename T>
 __global__ __launch_bounds__(PS_BLOCKS) void k_area_final(const double *__restrict__ part, double *__restrict__ out_a,
-                                                           double *__restrict__ out_b, double *__restrict__ pdryini) {
-    const int mem = blockIdx.x;
+                                                           double *__restrict__ out_b, double *__restrict__ pdryini,
+                                                           const T *__restrict__ ak, const T *__restrict__ bk, int nl,
+                                                           int *__restrict__ cnt) {
+    const int mem = blockIdx.x, nm = gridDim.x;
     __shared__ double ra[PS_BLOCKS], rb[PS_BLOCKS];
-    ra[threadIdx.x] = part[2 * (mem * PS_BLOCKS + threadIdx.x)];
-    rb[threadIdx.x] = part[2 * (mem * PS_BLOCKS + threadIdx.x) + 1];
+    ra[threadIdx.x] = part[4 * (mem * PS_BLOCKS + threadIdx.x)];
+    rb[threadIdx.x] = part[4 * (mem * PS_BLOCKS + threadIdx.x) + 1];
     __syncthreads();
     for (int w = PS_BLOCKS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
@@ -751,6 +816,41 @@ __global__ __launch_bounds__(PS_BLOCKS) void k_area_final(const double *__restri
         if (out_a) out_a[mem] = ra[0];
         if (out_b) out_b[mem] = rb[0];
         if (pdryini && pdryini[mem] == 0.0) pdryini[mem] = ra[0] - rb[0];
+    }
+    if (cnt && mem == 0) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int i = threadIdx.x; i < nm * PS_BLOCKS; i += PS_BLOCKS) {
+            lo = fmin(lo, part[4 * i + 2]);
+            hi = fmax(hi, part[4 * i + 3]);
+        }
+        __syncthreads();  // ra, rb reused
+        ra[threadIdx.x] = lo;
+        rb[threadIdx.x] = hi;
+        __syncthreads();
+        for (int w = PS_BLOCKS / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                ra[threadIdx.x] = fmin(ra[threadIdx.x], ra[threadIdx.x + w]);
+                rb[threadIdx.x] = fmax(rb[threadIdx.x], rb[threadIdx.x + w]);
+            }
+            __syncthreads();
+        }
+        // one level per thread (nl <= MAXLEV <= PS_BLOCKS); every level for a
+        // state without a finite positive ps
+        const double pmin = ra[0], pmax = rb[0];
+        const bool ok = pmin > 0.0 && pmax < INFINITY;
+        const int k = threadIdx.x;
+        __shared__ int top[PS_BLOCKS];
+        top[k] = k < nl && (!ok || hs_level_drags(k, 1.0 / pmin, 1.0 / pmax, ak, bk, nl)) ? k + 1 : 0;
+        __syncthreads();
+        for (int w = PS_BLOCKS / 2; w > 0; w >>= 1) {
+            if (k < w) top[k] = max(top[k], top[k + w]);
+            __syncthreads();
+        }
+        if (k == 0) {
+            cnt[0] = 2 * nm * top[0];
+            cnt[1] = nm * (nl + 1) + 2 * nm * top[0];
+            cnt[2] = top[0];
+        }
     }
 }
 
@@ -2364,7 +2464,7 @@ template <typename T> int Engine<T>::dyn_init() {
     B->dphisdy = dalloc<T>((size_t)nm * ng);
     B->pdryini = dalloc<double>(nm);
     B->pmean = dalloc<double>(nm);
-    B->pspart = dalloc<double>((size_t)nm * PS_BLOCKS * 2);
+    B->pspart = dalloc<double>((size_t)nm * PS_BLOCKS * 4);
     B->pwsum = dalloc<double>(nm);
     B->diag = dalloc<double>(4 * nm);
     B->clock = dalloc<double>(1);
@@ -2372,10 +2472,11 @@ template <typename T> int Engine<T>::dyn_init() {
     B->pwat = dalloc<T>((size_t)nm * ng);
     B->precip = dalloc<T>((size_t)nm * ng);
     B->pblscr = dalloc<T>(cfg.physics == GFSDC_PHYS_SIMPLE ? (size_t)nm * 2 * nl * ng : 1);
+    B->hsdyn = dalloc<int>(8);
     void *ptrs[] = {B->spec, B->spec0, B->ten[0], B->ten[1], B->ten[2], B->lin[0], B->lin[1], B->lin[2],
                     B->tphys, B->topo, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
                     B->dphisdy, B->pdryini, B->pmean, B->diag, B->pspart, B->pwsum, B->colq, B->pwat,
-                    B->precip, B->pblscr, B->gmean, B->clock};
+                    B->precip, B->pblscr, B->gmean, B->clock, B->hsdyn};
     for (void *p : ptrs)
         if (!p) {
             set_error("device allocation failed for the dycore state");
@@ -2391,6 +2492,7 @@ template <typename T> int Engine<T>::dyn_init() {
     GFSDC_CHECK(hipMemsetAsync(B->precip, 0, sizeof(T) * nm * ng, stream));
     GFSDC_CHECK(hipMemsetAsync(B->pwsum, 0, sizeof(double) * nm, stream));
     GFSDC_CHECK(hipMemsetAsync(B->gmean, 0, sizeof(T) * nm * B->nsyn, stream));
+    GFSDC_CHECK(hipMemsetAsync(B->hsdyn, 0, sizeof(int) * 8, stream));
     Work<T> *W = new Work<T>();
     B->Work = W;
     if ((rc = work_alloc(*W, plan, nm * std::max(B->nsyn, B->nan)))) return rc;
@@ -2456,7 +2558,7 @@ template <typename T> int Engine<T>::dyn_init() {
         d[o + 2] = SynthDesc{SK_VHAT, -1, lp, 0.0, ra_, 0, 0};  // dlnpsdy
         r[o + 1] = r[o + 2] = 1;
         // analysis: grid fields of the member (Tten, Fx, Fy, KE, dlnpsdt, dq) -> tendencies
-        int g0 = mem * B->nan, h0 = mem * B->nhs;
+        int g0 = mem * B->nan;
         AnalDesc *e = &ad[(size_t)mem * B->nfs];
         AnalDesc *hh = &ah[(size_t)mem * B->nfs];
         for (int k = 0; k < nl; k++) {
@@ -2466,17 +2568,18 @@ template <typename T> int Engine<T>::dyn_init() {
                                  1.0 / (rerth * rerth)};
             e[2 * nl + k] = AnalDesc{AK_COPY, g0 + k, -1, 1.0, 0, 0, -1, 0.0};
             ra[g0 + nl + k] = ra[g0 + 2 * nl + k] = 1;
-            // Held-Suarez: (Tforce, vrt force, div force, log(ps*pcorr))
-            hh[k] = AnalDesc{AK_COPY, h0 + nl + k, -1, 1.0, 0, 0, -1, 0.0};
-            hh[nl + k] = AnalDesc{AK_COPY, h0 + 2 * nl + k, -1, 1.0, 0, 0, -1, 0.0};
-            hh[2 * nl + k] = AnalDesc{AK_COPY, h0 + k, -1, 1.0, 0, 0, -1, 0.0};
+            // Held-Suarez forcing fields (hs_field): T forcing, vrt force, div force
+            hh[k] = AnalDesc{AK_COPY, hs_field(nl, nm, mem, HS_VRT, k), -1, 1.0, 0, 0, -1, 0.0};
+            hh[nl + k] = AnalDesc{AK_COPY, hs_field(nl, nm, mem, HS_DIV, k), -1, 1.0, 0, 0, -1, 0.0};
+            hh[2 * nl + k] = AnalDesc{AK_COPY, hs_field(nl, nm, mem, HS_T, k), -1, 1.0, 0, 0, -1, 0.0};
         }
         for (int k = 0; k < ntr * nl; k++) {
             e[3 * nl + k] = AnalDesc{AK_COPY, g0 + 4 * nl + 1 + k, -1, 1.0, 0, 0, -1, 0.0};
             hh[3 * nl + k] = zero;  // Held-Suarez leaves tracers unchanged
         }
         e[3 * nl + ntr * nl] = AnalDesc{AK_COPY, g0 + 4 * nl, -1, 1.0, 0, 0, -1, 0.0};
-        hh[3 * nl + ntr * nl] = cfg.massfix ? AnalDesc{AK_COPY, h0 + 3 * nl, -1, 1.0, 0, 0, -1, 0.0} : zero;
+        hh[3 * nl + ntr * nl] =
+            cfg.massfix ? AnalDesc{AK_COPY, hs_field(nl, nm, mem, HS_T, nl), -1, 1.0, 0, 0, -1, 0.0} : zero;
         // simple physics (phy_run_simple.f90:160-171): grid changes / dt ->
         // tv, (vrt, div) = getvrtdivspec(du, dv), tracer 1; mass-fixer field
         int p0 = mem * B->nsp;
@@ -2495,26 +2598,35 @@ template <typename T> int Engine<T>::dyn_init() {
     {
         // grid fields the physics pass needs: u, v, vrt, div, tv (0..5nl), the
         // tracers (7nl..7nl+ntr nl) and ln ps with its gradient (last 3)
-        std::vector<int> keep;
-        for (int f = 0; f < 5 * nl; f++) keep.push_back(f);
-        for (int f = 7 * nl; f < 7 * nl + ntr * nl; f++) keep.push_back(f);
-        for (int f = 7 * nl + 3 * ntr * nl; f < B->nsyn; f++) keep.push_back(f);
-        B->nph = (int)keep.size();
-        // the lean Held-Suarez pass: vrt, div, tv (2nl..5nl) and ln ps
-        std::vector<int> lean;
-        for (int f = 2 * nl; f < 5 * nl; f++) lean.push_back(f);
-        lean.push_back(7 * nl + 3 * ntr * nl);
-        B->nhl = (int)lean.size();
-        auto upload_subset = [&](const std::vector<int> &fs, SynthDesc **d, unsigned char **r, int **sl) {
+        // (member, grid field) pairs in transform order
+        std::vector<std::pair<int, int>> keep, lean_a, lean_b;
+        for (int mem = 0; mem < nm; mem++) {
+            for (int f = 0; f < 5 * nl; f++) keep.push_back({mem, f});
+            for (int f = 7 * nl; f < 7 * nl + ntr * nl; f++) keep.push_back({mem, f});
+            for (int f = 7 * nl + 3 * ntr * nl; f < B->nsyn; f++) keep.push_back({mem, f});
+            // the lean Held-Suarez pass: temperature (4nl..5nl) and ln ps, then
+            for (int k = 0; k < nl; k++) lean_a.push_back({mem, 4 * nl + k});
+            lean_a.push_back({mem, 7 * nl + 3 * ntr * nl});
+        }
+        // vorticity (2nl..3nl) and divergence (3nl..4nl) level-major from the lowest level
+        for (int k = 0; k < nl; k++)
+            for (int mem = 0; mem < nm; mem++) {
+                lean_b.push_back({mem, 2 * nl + k});
+                lean_b.push_back({mem, 3 * nl + k});
+            }
+        B->nph = (int)keep.size() / nm;
+        B->nhla = (int)lean_a.size();
+        B->nhlb = (int)lean_b.size();
+        auto upload_subset = [&](const std::vector<std::pair<int, int>> &fs, SynthDesc **d, unsigned char **r,
+                                 int **sl) {
             std::vector<SynthDesc> dsub;
             std::vector<unsigned char> rsub;
             std::vector<int> ssub;
-            for (int mem = 0; mem < nm; mem++)
-                for (int f : fs) {
-                    dsub.push_back(sd[(size_t)mem * B->nsyn + f]);
-                    rsub.push_back(rs[(size_t)mem * B->nsyn + f]);
-                    ssub.push_back(mem * B->nsyn + f);
-                }
+            for (auto [mem, f] : fs) {
+                dsub.push_back(sd[(size_t)mem * B->nsyn + f]);
+                rsub.push_back(rs[(size_t)mem * B->nsyn + f]);
+                ssub.push_back(mem * B->nsyn + f);
+            }
             void *q;
             int e;
             if ((e = dupload<SynthDesc>(&q, dsub, stream))) return e;
@@ -2526,7 +2638,8 @@ template <typename T> int Engine<T>::dyn_init() {
             return 0;
         };
         if ((rc = upload_subset(keep, &B->sdesc_ph, &B->rflag_ph, &B->slot_ph))) return rc;
-        if ((rc = upload_subset(lean, &B->sdesc_hl, &B->rflag_hl, &B->slot_hl))) return rc;
+        if ((rc = upload_subset(lean_a, &B->sdesc_hla, &B->rflag_hla, &B->slot_hla))) return rc;
+        if ((rc = upload_subset(lean_b, &B->sdesc_hlb, &B->rflag_hlb, &B->slot_hlb))) return rc;
     }
     void *p;
     if ((rc = dupload<SynthDesc>(&p, sd, stream))) return rc;
@@ -2562,8 +2675,8 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
-                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph, B->sdesc_hl,
-                    B->rflag_hl, B->slot_hl,
+                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph, B->sdesc_hla,
+                    B->rflag_hla, B->slot_hla, B->sdesc_hlb, B->rflag_hlb, B->slot_hlb, B->hsdyn,
                     B->clock, B->iau_hrs, B->iau_inc, B->iau_F, B->st_pat2, B->st_pat[3], B->st_var, B->st_vf,
                     B->st_ctr, B->st_grd, B->sd_pat, B->rf0, B->vcg, B->sd_vc, B->rf1, B->an_w, B->an_g,
                     B->an_sm, B->sd_an, B->ad_an, B->rf_an};
@@ -2622,22 +2735,32 @@ template <typename T> struct Step {
         launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_ph, s, (const T *)B.gmean, B.slot_ph);
         mark("fft_synth");
     }
-    // the lean Held-Suarez pass: vorticity, divergence, temperature and ln ps
-    // into their grid slots, then ps as the column kernel forms it
+    // the lean Held-Suarez pass: temperature and ln ps into their grid slots,
+    // ps as the column kernel forms it (k_psg), the levels where the drag can
+    // act (with the area sums), then the vorticity and divergence of those levels
     void synth_hs_lean() {
         const Plan<T> &p = E.plan;
-        int nf = B.nm * B.nhl;
-        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc_hl, nf, W.sx, W.ldc, 1.0, s,
-                            (T *)B.gmean);
+        const cplx<T> *spec = (const cplx<T> *)B.spec;
+        const long sk = (long)B.nfs * B.nm;
+        launch_synth_gather(p, spec, 1, sk, B.sdesc_hla, B.nhla, W.sx, W.ldc, 1.0, s, (T *)B.gmean);
         mark("synth_gather");
-        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * B.nhla, s);
         mark("leg_synth");
-        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_hl, s, (const T *)B.gmean, B.slot_hl);
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, B.nhla, B.rflag_hla, s, (const T *)B.gmean, B.slot_hla);
         mark("fft_synth");
         dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
-        hipLaunchKernelGGL(k_psg<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const cplx<T> *)B.spec, (T *)B.psg,
-                           B.nl, B.ntr, B.nsyn, B.nfs, B.ng);
+        hipLaunchKernelGGL(k_psg<T>, g, dim3(256), 0, s, (const T *)B.gsyn, spec, (T *)B.psg, B.nl, B.ntr, B.nsyn,
+                           B.nfs, B.ng);
         mark("psg");
+        // the global means of the forcing's mass fixer, and the drag's levels
+        area_sums(nullptr, E.cfg.massfix ? B.pmean : (double *)nullptr, nullptr, false, true);
+        launch_synth_gather(p, spec, 1, sk, B.sdesc_hlb, B.nhlb, W.sx, W.ldc, 1.0, s, (T *)nullptr, B.hsdyn);
+        mark("synth_gather");
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * B.nhlb, s, B.hsdyn);
+        mark("leg_synth");
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, B.nhlb, B.rflag_hlb, s, (const T *)nullptr, B.slot_hlb,
+                         0, B.hsdyn);
+        mark("fft_synth");
     }
     // dyn_run.f90:114-116: the vorticity gradient of vorticity confinement
     void synth_vc() {
@@ -2668,21 +2791,27 @@ template <typename T> struct Step {
                            (T)B.sc.vcamp, store);
         mark(inverse_only ? "columns_inverse" : "columns");
     }
-    // FP64 area sums: S(psg) -> pmean, S(b) -> bsum; pdryini = S(psg) - S(b) where 0
-    void area_sums(const void *b, double *pmean, double *bsum, bool set_pdry) {
+    // FP64 area sums: S(psg) -> pmean, S(b) -> bsum; pdryini = S(psg) - S(b) where 0;
+    // with drag_count also the lean Held-Suarez pass's counts (hsdyn) from psg's range
+    void area_sums(const void *b, double *pmean, double *bsum, bool set_pdry, bool drag_count = false) {
         hipLaunchKernelGGL(k_area_partial<T>, dim3(PS_BLOCKS, B.nm), dim3(256), 0, s, (const T *)B.psg,
                            (const T *)b, (const double *)B.areaw, B.pspart, B.ng, E.plan.nlon);
-        hipLaunchKernelGGL(k_area_final, dim3(B.nm), dim3(PS_BLOCKS), 0, s, (const double *)B.pspart, pmean, bsum,
-                           set_pdry ? B.pdryini : (double *)nullptr);
+        hipLaunchKernelGGL(k_area_final<T>, dim3(B.nm), dim3(PS_BLOCKS), 0, s, (const double *)B.pspart, pmean,
+                           bsum, set_pdry ? B.pdryini : (double *)nullptr, (const T *)B.ak, (const T *)B.bk, B.nl,
+                           drag_count ? B.hsdyn : (int *)nullptr);
         mark("ps_mean");
     }
-    void analyse(const AnalDesc *desc, const unsigned char *rflag, int ngf, cplx<T> *dst) {
+    // ngf grid fields of gan -> spectral fields of dst; with nfdyn only the
+    // first *nfdyn grid fields (the others are zero on the grid)
+    void analyse(const AnalDesc *desc, const unsigned char *rflag, int ngf, cplx<T> *dst,
+                 const int *nfdyn = nullptr) {
         const Plan<T> &p = E.plan;
-        launch_fft_anal(p, (const T *)B.gan, B.ng, ngf, rflag, W.fa, W.ldc, s);
+        launch_fft_anal(p, (const T *)B.gan, B.ng, ngf, rflag, W.fa, W.ldc, s, nfdyn);
         mark("fft_anal");
-        launch_leg_anal(p, W.fa, W.ldc, W.sx, W.ldc, 2 * ngf, s);
+        launch_leg_anal(p, W.fa, W.ldc, W.sx, W.ldc, 2 * ngf, s, nfdyn);
         mark("leg_anal");
-        launch_anal_scatter(p, W.sx, W.ldc, desc, B.nm * B.nfs, dst, 1, (long)B.nfs * B.nm, E.cfg.rerth, s);
+        launch_anal_scatter(p, W.sx, W.ldc, desc, B.nm * B.nfs, dst, 1, (long)B.nfs * B.nm, E.cfg.rerth, s,
+                            nfdyn);
         mark("anal_scatter");
     }
     void getdyntend(int stage) {
@@ -2738,7 +2867,8 @@ template <typename T> struct Step {
     }
     void physics() {
         const gfsdc_cfg &c = E.cfg;
-        if (lean && lean_ok()) {
+        const bool lean_pass = lean && lean_ok();
+        if (lean_pass) {
             synth_hs_lean();
         } else {
             synth_phys();
@@ -2763,14 +2893,16 @@ template <typename T> struct Step {
             phys_update();
         }
         if (c.physics == GFSDC_PHYS_HS) {
-            area_sums(nullptr, c.massfix ? B.pmean : (double *)nullptr, nullptr, false);
+            if (!lean_pass) area_sums(nullptr, c.massfix ? B.pmean : (double *)nullptr, nullptr, false);
             dim3 g((unsigned)((B.ng + 255) / 256), B.nm * HS_LCH);
+            // the lean pass forms and transforms the drag only where it can act (hs_level_drags)
+            const int *cnt = lean_pass ? B.hsdyn : nullptr;
             hipLaunchKernelGGL(k_hs_columns<T>, g, dim3(256), 0, s, (const T *)B.gsyn, (const T *)B.psg, (T *)B.gan,
                                (const T *)B.ak, (const T *)B.bk, (const T *)B.sinlat, B.pdryini, B.pmean, B.nl,
-                               B.ntr, B.nsyn, B.nhs, B.ng, E.plan.nlon, (T)(c.rd / c.cp), c.massfix);
+                               B.ntr, B.nsyn, B.nm, B.ng, E.plan.nlon, (T)(c.rd / c.cp), c.massfix, cnt);
             mark("hs_columns");
             // HS fields per member: T, vrt, div forcing and the mass-fixer field
-            analyse(B.adesc_hs, B.rflag_hs, B.nm * B.nhs, (cplx<T> *)B.tphys);
+            analyse(B.adesc_hs, B.rflag_hs, B.nm * B.nhs, (cplx<T> *)B.tphys, cnt ? cnt + 1 : nullptr);
             phys_update();
         }
         // run.f90:541-552: the streamfunction forcing into vorticity, after the physics
@@ -3544,6 +3676,17 @@ template <typename T> int Engine<T>::get_time(double *t) {
     return 0;
 }
 
+template <typename T> int Engine<T>::hs_drag_levels(int *n) {
+    DycoreBuffers *B = dyn;
+    if (!B || !n) {
+        set_error(!B ? "context has no dycore" : "null argument");
+        return !B ? GFSDC_ESTATE : GFSDC_EINVAL;
+    }
+    GFSDC_CHECK(hipMemcpyAsync(n, B->hsdyn + 2, sizeof(int), hipMemcpyDeviceToHost, stream));
+    GFSDC_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
 // init_iau (iau_module.f90:20-81) with the increments already formed by the
 // caller (analysis - first guess); each field has a trailing nfiles dimension
 // after the set_state layout. nfiles = 0 switches the forcing off.
@@ -4045,7 +4188,8 @@ template <typename T> int Engine<T>::hs_phytend(void *dvrt, void *ddiv, void *dt
     dim3 g((unsigned)((B->ng + 255) / 256), B->nm * HS_LCH);
     hipLaunchKernelGGL(k_hs_columns<T>, g, dim3(256), 0, stream, (const T *)B->gsyn, (const T *)B->psg,
                        (T *)B->gan, (const T *)B->ak, (const T *)B->bk, (const T *)B->sinlat, B->pdryini, B->pmean,
-                       B->nl, B->ntr, B->nsyn, B->nhs, B->ng, plan.nlon, (T)(cfg.rd / cfg.cp), 0);
+                       B->nl, B->ntr, B->nsyn, B->nm, B->ng, plan.nlon, (T)(cfg.rd / cfg.cp), 0,
+                       (const int *)nullptr);
     st.analyse(B->adesc_hs, B->rflag_hs, B->nm * B->nhs, (cplx<T> *)B->tphys);
     return pack_tphys(dvrt, ddiv, dtv, nullptr, dtype, loc);
 }

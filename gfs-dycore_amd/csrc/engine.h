@@ -64,6 +64,7 @@ template <typename T> struct Engine {
                    void *dlnps, int dtype, int loc);
     int advance(int nsteps, int use_graph);
     int hs_phytend(void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
+    int hs_drag_levels(int *n);
     int simple_phytend(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
     int pack_tphys(void *dvrt, void *ddiv, void *dtv, void *dtrac, int dtype, int loc);
     int pattern_spinup();

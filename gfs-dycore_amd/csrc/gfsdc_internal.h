@@ -112,23 +112,31 @@ struct AnalDesc {
 
 // Kernel launchers (sht_kernels.hip), all asynchronous on stream s.
 // Spectral source element (field f, coefficient k) sits at src[f*sf + k*sk] (complex).
+// nfdyn, where given, is a device-resident field count that an earlier kernel of
+// the stream wrote: the launch is sized for the static count (its maximum) and
+// the workgroups past *nfdyn return at once; the scatter writes zero for an
+// output whose source field lies past it.
 template <typename T>
 void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc,
-                         int nfo, T *sx, int ldx, double rsphere, hipStream_t s, T *mean = nullptr);
+                         int nfo, T *sx, int ldx, double rsphere, hipStream_t s, T *mean = nullptr,
+                         const int *nfdyn = nullptr);
 template <typename T>
-void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s);
+void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s,
+                      const int *nfdyn = nullptr);
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
                       const unsigned char *rcos_flag, hipStream_t s, const T *mean = nullptr,
-                      const int *slot = nullptr, int fofs = 0);
+                      const int *slot = nullptr, int fofs = 0, const int *nfdyn = nullptr);
 template <typename T>
 void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rcos_flag,
-                     T *fa, int ldf, hipStream_t s);
+                     T *fa, int ldf, hipStream_t s, const int *nfdyn = nullptr);
 template <typename T>
-void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s);
+void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s,
+                     const int *nfdyn = nullptr);
 template <typename T>
 void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc *desc, int nfo,
-                         cplx<T> *dst, long df, long dk, double rsphere, hipStream_t s);
+                         cplx<T> *dst, long df, long dk, double rsphere, hipStream_t s,
+                         const int *nfdyn = nullptr);
 
 // Workspace for one batched transform pipeline.
 template <typename T> struct Work {

@@ -11,6 +11,7 @@
 // and analysis is the mirror image (FFT -> Legendre GEMM -> scatter).
 // The arithmetic follows the transforms src/shtns.f90 requests from SHTns:
 // orthonormal harmonics without Condon-Shortley phase on a Gauss grid.
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 
@@ -252,7 +253,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, const long *__restrict__ poff,
                                                    const int *__restrict__ keoff, const T *__restrict__ sx,
                                                    int ldx, T *__restrict__ fs, int ldf, int nhalf, int nhp,
-                                                   int nlat, int ntrunc, int ntj, int ncol) {
+                                                   int nlat, int ntrunc, int ntj, int ncol,
+                                                   const int *__restrict__ nfdyn) {
     typedef typename MF<T>::acc acc_t;
     typedef typename Vec<T>::t V;
     constexpr int VW = Vec<T>::n;
@@ -271,6 +273,8 @@ __global__ __launch_bounds__(256) void k_leg_synth(const T *__restrict__ plm, co
     const T *P = plm + poff[m] + (par ? (long)Ke * nhp : 0L);
     const T *S = sx + (long)keoff[m] * ldx + (long)par * ldx;
     const int j0 = tj * GEMM_BM, c0 = tc * BN;
+    if (nfdyn) ncol = min(ncol, 2 * *nfdyn);
+    if (c0 >= ncol) return;  // a column tile past the device field count
     // operand chunks; after the k-loop (STG) each wave's 16 x 64 output slab
     constexpr int STG_P = BN / 2 + 4;
     constexpr int LDS_OPS = GEMM_BK * (GEMM_BM + LDS_PAD) + GEMM_BK * (BN + LDS_PAD);
@@ -421,7 +425,7 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plmt, co
                                                   const int *__restrict__ keoff, const int *__restrict__ mtile_m,
                                                   const int *__restrict__ mtile_i, const T *__restrict__ fa,
                                                   int ldf, T *__restrict__ sx, int ldx, int nhp, int ntrunc, int ncol,
-                                                  int nmt) {
+                                                  int nmt, const int *__restrict__ nfdyn) {
     typedef typename MF<T>::acc acc_t;
     typedef typename Vec<T>::t V;
     constexpr int VW = Vec<T>::n;
@@ -436,6 +440,8 @@ __global__ __launch_bounds__(256) void k_leg_anal(const T *__restrict__ plmt, co
     if (i0 >= Kp) return;  // an odd-parity tile past the last odd degree (the tiles follow the even block)
     const T *F = fa + (long)(2 * m + par) * nhp * ldf;
     const int c0 = tc * BN;
+    if (nfdyn) ncol = min(ncol, 2 * *nfdyn);
+    if (c0 >= ncol) return;  // a column tile past the device field count
     // [parity][j][i'] rows of Kw = roundup(Ke, GEMM_BM) degrees, zero past the
     // parity's last degree
     const int Kw = round_up_d(Ke, GEMM_BM);
@@ -1020,7 +1026,8 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
                                                    long gstride, int nf, const unsigned char *__restrict__ rflag,
                                                    const T *__restrict__ mean, const int *__restrict__ slot,
                                                    const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
-                                                   int nlon, int nlat, int ntrunc, Radices rd, int fofs) {
+                                                   int nlon, int nlat, int ntrunc, Radices rd, int fofs,
+                                                   const int *__restrict__ nfdyn) {
     constexpr int LN = NC ? NC : MAXN;
     // wavefront path; with 8 fields the twiddles are read from the (cached)
     // global table so that three workgroups fit a CU's LDS
@@ -1030,6 +1037,10 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     __shared__ cplx<T> tws[GTW ? 1 : LN];
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
+    if (nfdyn) {
+        nf = min(nf, *nfdyn);
+        if (f0 >= nf) return;  // a field group past the device field count
+    }
     const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
     // wavefront path: this wave's field, its 1/cos scaling and added mean, loaded
     // with the coefficients (field clamped into range; a wave past nf stores nothing)
@@ -1185,7 +1196,8 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
                                                   const unsigned char *__restrict__ rflag,
                                                   const T *__restrict__ rcos, const T *__restrict__ wfac,
                                                   const cplx<T> *__restrict__ tw, T *__restrict__ fa, int ldf,
-                                                  int nlon, int nlat, int nhp, int ntrunc, Radices rd) {
+                                                  int nlon, int nlat, int nhp, int ntrunc, Radices rd,
+                                                  const int *__restrict__ nfdyn) {
     constexpr int LN = NC ? NC : MAXN;
     constexpr bool WAVE = fft_wave(NFFT, NC), GTW = WAVE && NFFT == 8;
     constexpr int NT = fft_threads(NFFT, NC);
@@ -1193,6 +1205,10 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_anal(const T *__r
     __shared__ cplx<T> tws[GTW ? 1 : LN];
     const int nfg = (nf + NFFT - 1) / NFFT, tile = xcd_tile(blockIdx.x, gridDim.x);
     const int jp = tile / nfg, f0 = (tile - jp * nfg) * NFFT;
+    if (nfdyn) {
+        nf = min(nf, *nfdyn);
+        if (f0 >= nf) return;  // a field group past the device field count
+    }
     const int j = jp, j2 = nlat - 1 - jp, N = NC ? NC : nlon, NP = fft_pitch(N), tid = threadIdx.x;
     const T wf_j = wfac[j], rc_j = rcos[j];  // the output weights, loaded with the grid rows
     if constexpr (WAVE) {
@@ -1284,8 +1300,9 @@ template <typename T>
 __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk, const SynthDesc *__restrict__ desc,
                                int nfo, T *__restrict__ sx, int ldx, const int2 *__restrict__ kemn,
                                const int *__restrict__ koff, const T *__restrict__ eps, const int *__restrict__ eoff,
-                               int ntrunc, int nke, T *__restrict__ mean) {
+                               int ntrunc, int nke, T *__restrict__ mean, const int *__restrict__ nfdyn) {
     const int fo = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nfdyn) nfo = min(nfo, *nfdyn);
     if (fo >= nfo) return;
     const SynthDesc d = desc[fo];
     if (mean && blockIdx.y == 0) {
@@ -1370,12 +1387,15 @@ template <typename T>
 __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc *__restrict__ desc, int nfo,
                                cplx<T> *__restrict__ dst, long df, long dk, const int2 *__restrict__ kmn,
                                const int *__restrict__ keoff, const T *__restrict__ eps,
-                               const int *__restrict__ eoff, T rrs, int nlm) {
+                               const int *__restrict__ eoff, T rrs, int nlm, const int *__restrict__ nfdyn) {
     const int fo = blockIdx.x * blockDim.x + threadIdx.x;
     if (fo >= nfo) return;
     const AnalDesc d = desc[fo];
     const int k0 = blockIdx.y * SCATTER_KE, kend = min(nlm, k0 + SCATTER_KE);
-    if (d.kind == AK_ZERO) {
+    // a source field past the device field count was not transformed (the
+    // caller has established that it is zero on the grid): its output is zero
+    const int lim = nfdyn ? *nfdyn : INT_MAX;
+    if (d.kind == AK_ZERO || d.a >= lim || d.b >= lim || d.c >= lim) {
         if (!d.accum)
             for (int k = k0; k < kend; k++) dst[fo * df + (long)k * dk] = cplx<T>{T(0), T(0)};
         return;
@@ -1641,15 +1661,18 @@ template <typename T> void work_free(Work<T> &w) {
 
 template <typename T>
 void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc, int nfo,
-                         T *sx, int ldx, double rsphere, hipStream_t s, T *mean) {
+                         T *sx, int ldx, double rsphere, hipStream_t s, T *mean, const int *nfdyn) {
     (void)rsphere;
     dim3 grid((nfo + 255) / 256, (p.nlme + GATHER_KE - 1) / GATHER_KE);
     hipLaunchKernelGGL(k_synth_gather<T>, grid, dim3(256), 0, s, src, sf, sk, desc, nfo, sx, ldx, p.kemn, p.koff,
-                       p.eps, p.eoff, p.ntrunc, p.nlme, mean);
+                       p.eps, p.eoff, p.ntrunc, p.nlme, mean, nfdyn);
 }
 
+// The BF16x3 Legendre kernels take no device field count: they transform every
+// column of the launch (the columns past it hold earlier data and are not read).
 template <typename T>
-void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s) {
+void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s,
+                      const int *nfdyn) {
     int ntj = p.nhp / GEMM_BM, ntc = (ncol + GT<T>::BN - 1) / GT<T>::BN;
     if constexpr (sizeof(T) == 4) {
         if (p.leg_bf16x3) {
@@ -1659,11 +1682,12 @@ void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, in
         }
     }
     hipLaunchKernelGGL(k_leg_synth<T>, dim3(ntc * 2 * (p.ntrunc + 1) * ntj), dim3(256), 0, s, p.plm, p.poff,
-                       p.keoff, sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj, ncol);
+                       p.keoff, sx, ldx, fs, ldf, p.nhalf, p.nhp, p.nlat, p.ntrunc, ntj, ncol, nfdyn);
 }
 
 template <typename T>
-void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s) {
+void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int ncol, hipStream_t s,
+                     const int *nfdyn) {
     int ntc = (ncol + GT<T>::BN - 1) / GT<T>::BN;
     if constexpr (sizeof(T) == 4) {
         if (p.leg_bf16x3) {
@@ -1673,7 +1697,7 @@ void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int
         }
     }
     hipLaunchKernelGGL(k_leg_anal<T>, dim3(ntc * p.n_mtiles * 2), dim3(256), 0, s, p.plmt, p.tpoff, p.keoff,
-                       p.mtile_m, p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol, p.n_mtiles);
+                       p.mtile_m, p.mtile_i, fa, ldf, sx, ldx, p.nhp, p.ntrunc, ncol, p.n_mtiles, nfdyn);
 }
 
 static Radices radices_of(int n, const int *r) {
@@ -1685,13 +1709,15 @@ static Radices radices_of(int n, const int *r) {
 
 template <typename T, int NF>
 static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                         const unsigned char *rflag, const T *mean, const int *slot, int fofs, hipStream_t s) {
+                         const unsigned char *rflag, const T *mean, const int *slot, int fofs, hipStream_t s,
+                         const int *nfdyn) {
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_SYN(NC)                                                                                             \
     hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(fft_threads(NF, NC)), 0, s, fs, ldf, grid, gstride, nf,   \
                        rflag, mean, slot,                                                                     \
-                       p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad), fofs)
+                       p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad), fofs, \
+                       nfdyn)
     switch (p.nlon) {
     case 768: FFT_SYN(768); break;
     case 384: FFT_SYN(384); break;
@@ -1703,13 +1729,14 @@ static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long g
 
 template <typename T, int NF>
 static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
-                        int ldf, hipStream_t s) {
+                        int ldf, hipStream_t s, const int *nfdyn) {
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_ANA(NC)                                                                                             \
     hipLaunchKernelGGL((k_fft_anal<T, NF, MN, NC>), g, dim3(fft_threads(NF, NC)), 0, s, grid, gstride, nf, rflag,    \
                        p.rcos, p.wfac,                                                                        \
-                       (const cplx<T> *)p.tw, fa, ldf, p.nlon, p.nlat, p.nhp, p.ntrunc, radices_of(p.nrad, p.rad))
+                       (const cplx<T> *)p.tw, fa, ldf, p.nlon, p.nlat, p.nhp, p.ntrunc, radices_of(p.nrad, p.rad), \
+                       nfdyn)
     switch (p.nlon) {
     case 768: FFT_ANA(768); break;
     case 384: FFT_ANA(384); break;
@@ -1725,31 +1752,32 @@ static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, c
 // workgroups per CU (LDS = fields * nlon * sizeof(complex))
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
-                      const unsigned char *rflag, hipStream_t s, const T *mean, const int *slot, int fofs) {
+                      const unsigned char *rflag, hipStream_t s, const T *mean, const int *slot, int fofs,
+                      const int *nfdyn) {
     switch (p.fft_nf) {
-    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s); break;
-    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s); break;
-    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s); break;
+    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn); break;
+    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn); break;
+    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn); break;
     }
 }
 
 template <typename T>
 void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rflag, T *fa,
-                     int ldf, hipStream_t s) {
+                     int ldf, hipStream_t s, const int *nfdyn) {
     switch (p.fft_nf_an) {
-    case 2: fft_anal_nf<T, 2>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
-    case 4: fft_anal_nf<T, 4>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
-    default: fft_anal_nf<T, 8>(p, grid, gstride, nf, rflag, fa, ldf, s); break;
+    case 2: fft_anal_nf<T, 2>(p, grid, gstride, nf, rflag, fa, ldf, s, nfdyn); break;
+    case 4: fft_anal_nf<T, 4>(p, grid, gstride, nf, rflag, fa, ldf, s, nfdyn); break;
+    default: fft_anal_nf<T, 8>(p, grid, gstride, nf, rflag, fa, ldf, s, nfdyn); break;
     }
 }
 
 template <typename T>
 void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc *desc, int nfo, cplx<T> *dst,
-                         long df, long dk, double rsphere, hipStream_t s) {
+                         long df, long dk, double rsphere, hipStream_t s, const int *nfdyn) {
     const int bs = nfo >= 256 ? 256 : 64 * ((nfo + 63) / 64);
     dim3 grid((nfo + bs - 1) / bs, (p.nlm + SCATTER_KE - 1) / SCATTER_KE);
     hipLaunchKernelGGL(k_anal_scatter<T>, grid, dim3(bs), 0, s, sx, ldx, desc, nfo, dst, df, dk, p.kmn, p.keoff,
-                       p.eps, p.eoff, (T)(1.0 / rsphere), p.nlm);
+                       p.eps, p.eoff, (T)(1.0 / rsphere), p.nlm, nfdyn);
 }
 
 #define INST(T)                                                                                                \
@@ -1758,15 +1786,16 @@ void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc 
     template int work_alloc<T>(Work<T> &, const Plan<T> &, int);                                               \
     template void work_free<T>(Work<T> &);                                                                     \
     template void launch_synth_gather<T>(const Plan<T> &, const cplx<T> *, long, long, const SynthDesc *, int,  \
-                                         T *, int, double, hipStream_t, T *);                                  \
-    template void launch_leg_synth<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);            \
-    template void launch_leg_anal<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t);             \
+                                         T *, int, double, hipStream_t, T *, const int *);                     \
+    template void launch_leg_synth<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t,             \
+                                      const int *);                                                            \
+    template void launch_leg_anal<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t, const int *); \
     template void launch_fft_synth<T>(const Plan<T> &, const T *, int, T *, long, int, const unsigned char *,  \
-                                      hipStream_t, const T *, const int *, int);                               \
+                                      hipStream_t, const T *, const int *, int, const int *);                  \
     template void launch_fft_anal<T>(const Plan<T> &, const T *, long, int, const unsigned char *, T *, int,   \
-                                     hipStream_t);                                                             \
+                                     hipStream_t, const int *);                                                \
     template void launch_anal_scatter<T>(const Plan<T> &, const T *, int, const AnalDesc *, int, cplx<T> *,    \
-                                         long, long, double, hipStream_t);
+                                         long, long, double, hipStream_t, const int *);
 INST(float)
 INST(double)
 

@@ -66,7 +66,7 @@ SYMBOLS = [
     "gfsdc_griddata", "gfsdc_spectogrd", "gfsdc_spectogrd_noalloc", "gfsdc_grdtospec", "gfsdc_getuv",
     "gfsdc_getvrtdivspec", "gfsdc_getgrad", "gfsdc_set_state", "gfsdc_get_state",
     "gfsdc_set_topo", "gfsdc_set_pdryini", "gfsdc_get_pdryini", "gfsdc_getdyntend",
-    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_autotune", "gfsdc_bench_transforms", "gfsdc_bench_copy", "gfsdc_mlp_init", "gfsdc_mlp_collect",
+    "gfsdc_advance", "gfsdc_hs_phytend", "gfsdc_hs_drag_levels", "gfsdc_simple_phytend", "gfsdc_get_pattern", "gfsdc_autotune", "gfsdc_bench_transforms", "gfsdc_bench_copy", "gfsdc_mlp_init", "gfsdc_mlp_collect",
     "gfsdc_mlp_normalize", "gfsdc_mlp_train", "gfsdc_mlp_eval", "gfsdc_mlp_enable", "gfsdc_mlp_judge_stats",
     "gfsdc_mlp_nparams", "gfsdc_mlp_params", "gfsdc_mlp_norm", "gfsdc_mlp_forward", "gfsdc_mlp_grad", "gfsdc_mlp_adamw",
     "gfsdc_get_grid", "gfsdc_get_semimp",
@@ -108,6 +108,7 @@ def load():
     L.gfsdc_getdyntend.argtypes = [vp, i, i, vp, vp, vp, vp, vp, i, i]
     L.gfsdc_advance.argtypes = [vp, i, i]
     L.gfsdc_hs_phytend.argtypes = [vp, vp, vp, vp, i, i]
+    L.gfsdc_hs_drag_levels.argtypes = [vp, C.POINTER(i)]
     L.gfsdc_simple_phytend.argtypes = [vp, vp, vp, vp, vp, i, i]
     L.gfsdc_get_pattern.argtypes = [vp, vp, i, i]
     L.gfsdc_autotune.argtypes = [vp, vp, C.POINTER(i)]
@@ -322,6 +323,14 @@ class Context:
         a, b, c = self._s3(), self._s3(), self._s3()
         _check(load().gfsdc_hs_phytend(self.h, _p(a), _p(b), _p(c), DT_F64, HOST))
         return dict(dvrt=a, ddiv=b, dtv=c)
+
+    def hs_drag_levels(self):
+        """levels, counted from the lowest, on which the last lean Held-Suarez
+        physics pass of advance formed the drag (0 before the first); above
+        them p / ps < sigbot in every column, so the drag there is zero"""
+        n = C.c_int(0)
+        _check(load().gfsdc_hs_drag_levels(self.h, C.byref(n)))
+        return n.value
 
     def simple_phytend(self):
         a, b, c = self._s3(), self._s3(), self._s3()

@@ -195,6 +195,12 @@ int gfsdc_getdyntend(gfsdc_ctx *ctx, int rkstage, int inverse_only, void *dvrt, 
 int gfsdc_advance(gfsdc_ctx *ctx, int nsteps, int use_graph);
 /* Held-Suarez tendencies on the current grid fields (phy_run_hs.f90:22) */
 int gfsdc_hs_phytend(gfsdc_ctx *ctx, void *dvrt, void *ddiv, void *dtv, int dtype, int loc);
+/* levels, counted from the lowest, on which the last lean Held-Suarez physics
+ * pass of gfsdc_advance formed and transformed the drag (0 before the first):
+ * on every level above them p / ps stayed below phy_run_hs.f90's sigbot in
+ * every column, so their drag and its vorticity and divergence tendencies are
+ * zero. A diagnostic with no counterpart in the reference. */
+int gfsdc_hs_drag_levels(gfsdc_ctx *ctx, int *nlev);
 /* replaces phy_run_simple.f90:28 getphytend (simple moist physics on the current
  * grid fields, i.e. after gfsdc_getdyntend(ctx, 3, 1, ...)): tendencies of
  * vrt, div, tv (ndimspec, nlev, nmem) and trac (ndimspec, nlev, ntrac, nmem);

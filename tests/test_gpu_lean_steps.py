@@ -41,6 +41,10 @@ def test_lean_steps_are_bit_identical(prec, graph):
     a.advance(5, use_graph=graph)          # 4 lean steps, then a full one
     for _ in range(5):
         b.advance(1, use_graph=graph)      # full steps only
+    # the lean passes formed the drag on the lowest levels only (6 or 7 of these
+    # 20 near ps = 1e5), the full passes on all of them
+    assert 0 < a.hs_drag_levels() < NL
+    assert b.hs_drag_levels() == 0
     sa, sb = a.get_state(), b.get_state()
     for k in ("vrt", "div", "tv", "lnps"):
         assert np.array_equal(sa[k], sb[k]), k
@@ -51,6 +55,46 @@ def test_lean_steps_are_bit_identical(prec, graph):
     for k in da:
         assert np.array_equal(da[k], db[k]), k
     assert a.time == b.time == 5 * 1200.0
+
+
+def phillips_sigma(ak, bk, ps, kappa=2.0 / 7.0):
+    """p_k / ps of every level (level 0 the lowest) from the Phillips layer
+    pressure of pressure_data.f90:87-88, in FP64"""
+    pi = ak + bk * ps
+    p1, p2 = pi[:-1], pi[1:]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        p = ((p2 ** (kappa + 1) - p1 ** (kappa + 1)) / ((kappa + 1) * (p2 - p1))) ** (1 / kappa)
+    return p[::-1] / ps
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_drag_levels_cover_the_boundary_layer(prec):
+    """On sigma levels p / ps does not depend on ps, so the levels with drag
+    (p / ps > sigbot = 0.7, phy_run_hs.f90:52-58) are fixed: the lowest 6 of
+    the 20 (sigma 0.975 .. 0.725). The lean pass may take a level more (its
+    test uses the layer's lower interface), never fewer; the state still
+    matches full steps bit for bit."""
+    import gfsdc
+    p = gfsdc.FP64 if prec == "fp64" else gfsdc.FP32
+    print(f"drag levels {prec}", flush=True)
+    ak, bk = levels.levels_for("T42L20")
+    assert np.all(ak == 0)
+    sig = phillips_sigma(ak, bk, 1.0e5)
+    nbl = int(np.sum(sig > 0.7))
+    assert nbl == 6 and np.all(sig[:nbl] > 0.7)
+
+    def mk():
+        c = gfsdc.Context(NLON, NLAT, T, p, nlev=NL, ak=ak, bk=bk, dt=1200.0)
+        c.init_rest_state(300.0, 1.0e5, 2e-5, 1, 21, seed=5)
+        return c
+    a, b = mk(), mk()
+    a.advance(3, use_graph=True)
+    for _ in range(3):
+        b.advance(1, use_graph=True)
+    assert nbl <= a.hs_drag_levels() <= nbl + 1
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("vrt", "div", "tv", "lnps"):
+        assert np.array_equal(sa[k], sb[k]), k
 
 
 def test_profiled_lean_step_refuses_the_grids():
