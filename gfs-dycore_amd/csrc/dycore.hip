@@ -165,7 +165,9 @@ struct DycoreBuffers {
     // the grids after the last step are those of a lean pass (profile_step):
     // get_grid and step_diag refuse them
     bool grids_partial = false;
-    AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr, *adesc_sp = nullptr;
+    // adesc_hsu: the Held-Suarez forcing's analysis applied to the state in
+    // place (state += dt tendency, the mass fixer's ln ps update) for advance
+    AnalDesc *adesc_dyn = nullptr, *adesc_hs = nullptr, *adesc_hsu = nullptr, *adesc_sp = nullptr;
     unsigned char *rflag_syn = nullptr, *rflag_an = nullptr, *rflag_hs = nullptr, *rflag_sp = nullptr;
     // host copies of the FP64 operators
     std::vector<double> amhyb, bmhyb, svhyb, tor_hyb, d_hyb_m, disspec_h, diff_h, dmp_h, sl;
@@ -2528,6 +2530,7 @@ template <typename T> int Engine<T>::dyn_init() {
     std::vector<SynthDesc> sd((size_t)nm * B->nsyn);
     std::vector<unsigned char> rs((size_t)nm * B->nsyn, 0), ra((size_t)nm * B->nan, 0), rh((size_t)nm * B->nan, 0);
     std::vector<AnalDesc> ad((size_t)nm * B->nfs), ah((size_t)nm * B->nfs), ap((size_t)nm * B->nfs);
+    std::vector<AnalDesc> au((size_t)nm * B->nfs);
     std::vector<unsigned char> rp((size_t)nm * B->nsp, 0);
     const AnalDesc zero{AK_ZERO, -1, -1, 0.0, 0, 0, -1, 0.0};
     const double a = rerth, ra_ = 1.0 / rerth;
@@ -2580,6 +2583,16 @@ template <typename T> int Engine<T>::dyn_init() {
         e[3 * nl + ntr * nl] = AnalDesc{AK_COPY, g0 + 4 * nl, -1, 1.0, 0, 0, -1, 0.0};
         hh[3 * nl + ntr * nl] =
             cfg.massfix ? AnalDesc{AK_COPY, hs_field(nl, nm, mem, HS_T, nl), -1, 1.0, 0, 0, -1, 0.0} : zero;
+        // the same outputs applied in place (run.f90:527-538): vrt, div, tv += dt
+        // tendency, tracers unchanged, ln ps through the mass fixer
+        AnalDesc *uu = &au[(size_t)mem * B->nfs];
+        const AnalDesc keep{AK_ZERO, -1, -1, 0.0, 0, 1, -1, 0.0};
+        for (int f = 0; f < B->nfs; f++) {
+            uu[f] = hh[f].kind == AK_COPY ? hh[f] : keep;
+            uu[f].scale = cfg.dt;
+            uu[f].accum = 1;
+        }
+        if (cfg.massfix) uu[3 * nl + ntr * nl].kind = AK_MASSFIX;
         // simple physics (phy_run_simple.f90:160-171): grid changes / dt ->
         // tv, (vrt, div) = getvrtdivspec(du, dv), tracer 1; mass-fixer field
         int p0 = mem * B->nsp;
@@ -2648,6 +2661,8 @@ template <typename T> int Engine<T>::dyn_init() {
     B->adesc_dyn = (AnalDesc *)p;
     if ((rc = dupload<AnalDesc>(&p, ah, stream))) return rc;
     B->adesc_hs = (AnalDesc *)p;
+    if ((rc = dupload<AnalDesc>(&p, au, stream))) return rc;
+    B->adesc_hsu = (AnalDesc *)p;
     if ((rc = dupload<AnalDesc>(&p, ap, stream))) return rc;
     B->adesc_sp = (AnalDesc *)p;
     if ((rc = dupload<unsigned char>(&p, rp, stream))) return rc;
@@ -2671,7 +2686,8 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->tphys, B->gsyn, B->gan, B->etadot, B->dlnpdt, B->psg, B->dlnpsdt, B->dphisdx,
                     B->dphisdy, B->topo, B->at, B->bt, B->dt3, B->sv, B->tor, B->disspec, B->diffp, B->dmpp,
                     B->ak, B->bk, B->ck, B->dbk, B->sinlat, B->coslat, B->areaw, B->kbydeg, B->degree, B->si_tiles,
-                    B->pdryini, B->pmean, B->pspart, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->rflag_syn,
+                    B->pdryini, B->pmean, B->pspart, B->diag, B->sdesc, B->adesc_dyn, B->adesc_hs, B->adesc_hsu,
+                    B->rflag_syn,
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
@@ -2804,14 +2820,14 @@ template <typename T> struct Step {
     // ngf grid fields of gan -> spectral fields of dst; with nfdyn only the
     // first *nfdyn grid fields (the others are zero on the grid)
     void analyse(const AnalDesc *desc, const unsigned char *rflag, int ngf, cplx<T> *dst,
-                 const int *nfdyn = nullptr) {
+                 const int *nfdyn = nullptr, bool massfix = false) {
         const Plan<T> &p = E.plan;
         launch_fft_anal(p, (const T *)B.gan, B.ng, ngf, rflag, W.fa, W.ldc, s, nfdyn);
         mark("fft_anal");
         launch_leg_anal(p, W.fa, W.ldc, W.sx, W.ldc, 2 * ngf, s, nfdyn);
         mark("leg_anal");
         launch_anal_scatter(p, W.sx, W.ldc, desc, B.nm * B.nfs, dst, 1, (long)B.nfs * B.nm, E.cfg.rerth, s,
-                            nfdyn);
+                            nfdyn, massfix);
         mark("anal_scatter");
     }
     void getdyntend(int stage) {
@@ -2901,9 +2917,10 @@ template <typename T> struct Step {
                                (const T *)B.ak, (const T *)B.bk, (const T *)B.sinlat, B.pdryini, B.pmean, B.nl,
                                B.ntr, B.nsyn, B.nm, B.ng, E.plan.nlon, (T)(c.rd / c.cp), c.massfix, cnt);
             mark("hs_columns");
-            // HS fields per member: T, vrt, div forcing and the mass-fixer field
-            analyse(B.adesc_hs, B.rflag_hs, B.nm * B.nhs, (cplx<T> *)B.tphys, cnt ? cnt + 1 : nullptr);
-            phys_update();
+            // HS fields per member: T, vrt, div forcing and the mass-fixer field,
+            // analysed straight into the state (the scatter applies the update)
+            analyse(B.adesc_hsu, B.rflag_hs, B.nm * B.nhs, (cplx<T> *)B.spec, cnt ? cnt + 1 : nullptr,
+                    c.massfix != 0);
         }
         // run.f90:541-552: the streamfunction forcing into vorticity, after the physics
         if (B.st_on[3]) {

@@ -98,8 +98,12 @@ struct SynthDesc {
     int inv_a, inv_b;  // multiply S / T by invlap(n)
 };
 // AK_ZERO writes a zero coefficient (an output with no source, e.g. a tracer a
-// physics package leaves unchanged).
-enum AnalKind : int { AK_COPY = 0, AK_VRT = 1, AK_DIV = 2, AK_ZERO = 3 };
+// physics package leaves unchanged; with accum it leaves the output alone).
+// AK_MASSFIX applies the dry-mass fixer to ln ps in place (dyn_run.f90:626-633
+// with the update of run.f90:535-538): column a holds the analysed deviation
+// field of log(ps pcorr), scale is dt, and the output s becomes s + dt (t - s) / dt
+// with t the analysis plus, at n = 0, s.
+enum AnalKind : int { AK_COPY = 0, AK_VRT = 1, AK_DIV = 2, AK_ZERO = 3, AK_MASSFIX = 4 };
 struct AnalDesc {
     int kind;
     int a, b;      // analysed columns (field indices): COPY uses a; VRT/DIV use (U~=a, V~=b)
@@ -115,7 +119,8 @@ struct AnalDesc {
 // nfdyn, where given, is a device-resident field count that an earlier kernel of
 // the stream wrote: the launch is sized for the static count (its maximum) and
 // the workgroups past *nfdyn return at once; the scatter writes zero for an
-// output whose source field lies past it.
+// output whose source field lies past it (leaves it alone with accum). The
+// scatter takes AK_MASSFIX descriptors only with massfix set.
 template <typename T>
 void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk, const SynthDesc *desc,
                          int nfo, T *sx, int ldx, double rsphere, hipStream_t s, T *mean = nullptr,
@@ -136,7 +141,7 @@ void launch_leg_anal(const Plan<T> &p, const T *fa, int ldf, T *sx, int ldx, int
 template <typename T>
 void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc *desc, int nfo,
                          cplx<T> *dst, long df, long dk, double rsphere, hipStream_t s,
-                         const int *nfdyn = nullptr);
+                         const int *nfdyn = nullptr, bool massfix = false);
 
 // Workspace for one batched transform pipeline.
 template <typename T> struct Work {

@@ -1383,7 +1383,9 @@ __global__ void k_synth_gather(const cplx<T> *__restrict__ src, long sf, long sk
 // threads over the output fields.
 constexpr int SCATTER_KE = 8;
 
-template <typename T>
+// MFX: the descriptors may hold AK_MASSFIX (a separate instantiation keeps its
+// division out of the plain scatter)
+template <typename T, bool MFX>
 __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc *__restrict__ desc, int nfo,
                                cplx<T> *__restrict__ dst, long df, long dk, const int2 *__restrict__ kmn,
                                const int *__restrict__ keoff, const T *__restrict__ eps,
@@ -1403,6 +1405,7 @@ __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc
     // the inputs of QB rows first (loads all in flight together), then those rows;
     // FP64 takes half the rows per batch to stay within the registers
     constexpr int QB = sizeof(T) == 4 ? SCATTER_KE : SCATTER_KE / 2;
+    const bool mfx = MFX && d.kind == AK_MASSFIX, copy = d.kind == AK_COPY || mfx, rd = d.accum || mfx;
     const int ca = (d.kind == AK_VRT) ? d.b : d.a;  // i m (V for vrt, U for div; the field for a copy)
     const int cb = (d.kind == AK_VRT) ? d.a : d.b;  // recurrence partner (U for vrt, V for div)
     for (int kb = k0; kb < kend; kb += QB) {
@@ -1415,15 +1418,15 @@ __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc
             const int m = mn.x, n = mn.y;
             mnq[q] = mn;
             const T *X = sx + (long)(keoff[m] + n - m) * ldx;
-            const T *C = (d.kind == AK_COPY) ? X + 2 * d.a : X + 2 * ca;
+            const T *C = copy ? X + 2 * d.a : X + 2 * ca;
             z0[q] = cplx<T>{C[0], C[1]};
             zp[q] = zm[q] = zc[q] = za[q] = cplx<T>{T(0), T(0)};
-            if (d.kind != AK_COPY) {
+            if (!copy) {
                 zp[q] = cplx<T>{X[ldx + 2 * cb], X[ldx + 2 * cb + 1]};
                 if (n > m) zm[q] = cplx<T>{X[-ldx + 2 * cb], X[-ldx + 2 * cb + 1]};
             }
             if (d.c >= 0) zc[q] = cplx<T>{X[2 * d.c], X[2 * d.c + 1]};
-            if (d.accum) za[q] = dst[fo * df + (long)k * dk];
+            if (rd) za[q] = dst[fo * df + (long)k * dk];
         }
 #pragma unroll
         for (int q = 0; q < QB; q++) {
@@ -1431,7 +1434,7 @@ __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc
             if (k >= kend) break;
             const int m = mnq[q].x, n = mnq[q].y;
             cplx<T> r = z0[q];
-            if (d.kind != AK_COPY) {
+            if (!copy) {
                 const T *ep = eps + eoff[m];
                 T en = ep[n - m], en1 = ep[n + 1 - m];
                 T cp = T(n) * en1, cm = T(n + 1) * en, fm = T(m);
@@ -1447,8 +1450,18 @@ __global__ void k_anal_scatter(const T *__restrict__ sx, int ldx, const AnalDesc
                 r.y += qq * zc[q].y;
             }
             T sc = T(d.scale);
-            if (d.inv) sc *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
             cplx<T> *o = dst + fo * df + (long)k * dk;
+            if (MFX && mfx) {
+                // ln ps + dt (t - ln ps) / dt, t the analysed deviation plus the mean at n = 0
+                const T dt = sc;
+                const cplx<T> s = za[q];
+                cplx<T> t = r;
+                if (k == 0) t.x += s.x;
+                const cplx<T> dd{(t.x - s.x) / dt, (t.y - s.y) / dt};
+                *o = cplx<T>{s.x + dt * dd.x, s.y + dt * dd.y};
+                continue;
+            }
+            if (d.inv) sc *= (n == 0) ? T(0) : T(-1.0 / ((double)n * (n + 1.0)));
             if (d.accum)
                 *o = cplx<T>{za[q].x + sc * r.x, za[q].y + sc * r.y};
             else
@@ -1773,11 +1786,12 @@ void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, cons
 
 template <typename T>
 void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc *desc, int nfo, cplx<T> *dst,
-                         long df, long dk, double rsphere, hipStream_t s, const int *nfdyn) {
+                         long df, long dk, double rsphere, hipStream_t s, const int *nfdyn, bool massfix) {
     const int bs = nfo >= 256 ? 256 : 64 * ((nfo + 63) / 64);
     dim3 grid((nfo + bs - 1) / bs, (p.nlm + SCATTER_KE - 1) / SCATTER_KE);
-    hipLaunchKernelGGL(k_anal_scatter<T>, grid, dim3(bs), 0, s, sx, ldx, desc, nfo, dst, df, dk, p.kmn, p.keoff,
-                       p.eps, p.eoff, (T)(1.0 / rsphere), p.nlm, nfdyn);
+    auto kern = massfix ? k_anal_scatter<T, true> : k_anal_scatter<T, false>;
+    hipLaunchKernelGGL(kern, grid, dim3(bs), 0, s, sx, ldx, desc, nfo, dst, df, dk, p.kmn, p.keoff, p.eps, p.eoff,
+                       (T)(1.0 / rsphere), p.nlm, nfdyn);
 }
 
 #define INST(T)                                                                                                \
@@ -1795,7 +1809,7 @@ void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc 
     template void launch_fft_anal<T>(const Plan<T> &, const T *, long, int, const unsigned char *, T *, int,   \
                                      hipStream_t, const int *);                                                \
     template void launch_anal_scatter<T>(const Plan<T> &, const T *, int, const AnalDesc *, int, cplx<T> *,    \
-                                         long, long, double, hipStream_t, const int *);
+                                         long, long, double, hipStream_t, const int *, bool);
 INST(float)
 INST(double)
 
