@@ -149,18 +149,20 @@ struct DycoreBuffers {
     // the lean Held-Suarez physics pass of the steps before an advance's last
     // synthesises only the grids the forcing reads (phy_run_hs.f90:66-76); the
     // winds, their gradients and getomega serve only the post-step grids and
-    // diagnostics. Its first synthesis holds temperature and ln ps (nhla =
-    // nm (nl + 1) fields, member-major); its second vorticity and divergence,
-    // level-major from the lowest level (nhlb = 2 nm nl fields: level k, member,
-    // vrt / div), of which it transforms only the levels where the drag can act
+    // diagnostics. Its fields: temperature and ln ps (nhla = nm (nl + 1),
+    // member-major), then vorticity and divergence level-major from the lowest
+    // level (nhlb = 2 nm nl: level k, member, vrt / div), of which it
+    // transforms only the levels where the drag can act
     int nhla = 0, nhlb = 0;
-    SynthDesc *sdesc_hla = nullptr, *sdesc_hlb = nullptr;
-    unsigned char *rflag_hla = nullptr, *rflag_hlb = nullptr;
-    int *slot_hla = nullptr, *slot_hlb = nullptr;
+    SynthDesc *sdesc_hl = nullptr;
+    unsigned char *rflag_hl = nullptr;
+    int *slot_hl = nullptr;
     // device counts of the lean pass, written by k_area_final: [0] fields of
-    // the second synthesis (2 nm nact), [1] Held-Suarez forcing fields to
-    // analyse (nm (nl + 1) + 2 nm nact), [2] nact, the levels counted from the
-    // lowest whose drag is evaluated
+    // the vorticity / divergence synthesis that follows the drag's count (0
+    // where the first synthesis already held every level with drag), [1]
+    // Held-Suarez forcing fields to analyse, [2] nact, the levels counted from
+    // the lowest whose drag is evaluated, [3] fields of the next lean pass's
+    // first synthesis, [4] the levels it takes (the last nact plus one)
     int *hsdyn = nullptr;
     // the grids after the last step are those of a lean pass (profile_step):
     // get_grid and step_diag refuse them
@@ -657,10 +659,9 @@ __global__ __launch_bounds__(256) void k_psg(const T *__restrict__ gsyn, const c
 // interface ak + bk ps, and (ak + bk ps) / ps = ak / ps + bk is monotone in ps,
 // so over all columns it is largest at the least or the greatest ps (1 / ps =
 // rmin, rmax). A level where it stays below 0.999 sigbot at both (a margin far
-// above the rounding of p_k) has no drag. The lean pass counts the levels up
-// to the highest other one: nact, with the fields of its vorticity and
-// divergence synthesis (2 nm nact) and the Held-Suarez forcing fields it
-// analyses (nm (nl + 1) + 2 nm nact), in cnt[2], cnt[0] and cnt[1].
+// above the rounding of p_k) has no drag. The lean pass takes the levels up
+// to the highest other one (nact, which k_area_final turns into the counts the
+// pass's kernels read).
 template <typename T>
 __device__ inline bool hs_level_drags(int k, double rmin, double rmax, const T *__restrict__ ak,
                                       const T *__restrict__ bk, int nl) {
@@ -710,7 +711,26 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
     // per column: the layer-pressure series, 1 / ps, 1 / (1 - sigbot)
     const PrsSeries<T> prc = prs_series(kappa);
     const T rps = recip(ps), rsb = T(1) / (T(1) - sigbot);
-    for (int L = L0; L <= L1; L++) {
+    // the chunk's grid values first, all loads in flight together (levels
+    // clamped into the chunk; the vorticity and divergence where the chunk has
+    // a level with drag)
+    constexpr int CSM = (MAXLEV + HS_LCH - 1) / HS_LCH;
+    const bool drag = L0 - 1 < nact;
+    T tv[CSM], vr[CSM], dv[CSM];
+#pragma unroll
+    for (int i = 0; i < CSM; i++) {
+        const long o = (long)(min(L0 + i, L1) - 1) * ng;
+        tv[i] = TV[o];
+        vr[i] = dv[i] = T(0);
+        if (drag) {
+            vr[i] = VRT[o];
+            dv[i] = DIV[o];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CSM; i++) {
+        const int L = L0 + i;
+        if (L > L1) break;
         int K = nl + 1 - L;
         T p1 = ak[K - 1] + bk[K - 1] * ps, p2 = ak[K] + bk[K] * ps;
         T prs = phillips_prs(p1, p2, kappa, prc);
@@ -731,11 +751,11 @@ __global__ __launch_bounds__(256) void k_hs_columns(const T *__restrict__ gsyn, 
         if (blprof < T(0)) blprof = T(0);
         if (teq < tempstrat) teq = tempstrat;
         long o = (long)(L - 1) * ng;
-        AT[o] = (krada + (kradb - krada) * blprof * cl2 * cl2) * (teq - TV[o]);
+        AT[o] = (krada + (kradb - krada) * blprof * cl2 * cl2) * (teq - tv[i]);
         if (L - 1 < nact) {
             T *AV = gan + (long)hs_field(nl, nm, mem, HS_VRT, L - 1) * ng + col;  // divergence follows it
-            AV[0] = -(blprof * kdrag) * VRT[o];
-            AV[ng] = -(blprof * kdrag) * DIV[o];
+            AV[0] = -(blprof * kdrag) * vr[i];
+            AV[ng] = -(blprof * kdrag) * dv[i];
         }
     }
     if (massfix && ch == 0) {
@@ -795,8 +815,9 @@ __global__ __launch_bounds__(256) void k_area_partial(const T *__restrict__ a, c
 }
 
 // One workgroup per member sums its partials; with cnt, workgroup 0 also
-// takes the least and greatest a (ps) over every member and block and writes
-// the lean Held-Suarez pass's counts from them (hs_level_drags).
+// takes the least and greatest a (ps) over every member and block, finds the
+// levels with drag from them (hs_level_drags) and writes the lean Held-Suarez
+// pass's counts (cnt[0..4], as DycoreBuffers::hsdyn describes them).
 template <typename T>
 __global__ __launch_bounds__(PS_BLOCKS) void k_area_final(const double *__restrict__ part, double *__restrict__ out_a,
                                                            double *__restrict__ out_b, double *__restrict__ pdryini,
@@ -849,9 +870,12 @@ __global__ __launch_bounds__(PS_BLOCKS) void k_area_final(const double *__restri
             __syncthreads();
         }
         if (k == 0) {
-            cnt[0] = 2 * nm * top[0];
-            cnt[1] = nm * (nl + 1) + 2 * nm * top[0];
-            cnt[2] = top[0];
+            const int nact = top[0], took = cnt[4], next = min(nl, nact + 1);
+            cnt[0] = nact > took ? 2 * nm * nact : 0;
+            cnt[1] = nm * (nl + 1) + 2 * nm * nact;
+            cnt[2] = nact;
+            cnt[3] = nm * (nl + 1) + 2 * nm * next;
+            cnt[4] = next;
         }
     }
 }
@@ -2651,8 +2675,13 @@ template <typename T> int Engine<T>::dyn_init() {
             return 0;
         };
         if ((rc = upload_subset(keep, &B->sdesc_ph, &B->rflag_ph, &B->slot_ph))) return rc;
-        if ((rc = upload_subset(lean_a, &B->sdesc_hla, &B->rflag_hla, &B->slot_hla))) return rc;
-        if ((rc = upload_subset(lean_b, &B->sdesc_hlb, &B->rflag_hlb, &B->slot_hlb))) return rc;
+        std::vector<std::pair<int, int>> lean = lean_a;
+        lean.insert(lean.end(), lean_b.begin(), lean_b.end());
+        if ((rc = upload_subset(lean, &B->sdesc_hl, &B->rflag_hl, &B->slot_hl))) return rc;
+        // the first lean pass takes every level
+        const int c0[8] = {0, 0, 0, B->nhla + B->nhlb, nl, 0, 0, 0};
+        GFSDC_CHECK(hipMemcpyAsync(B->hsdyn, c0, sizeof(c0), hipMemcpyHostToDevice, stream));
+        GFSDC_CHECK(hipStreamSynchronize(stream));
     }
     void *p;
     if ((rc = dupload<SynthDesc>(&p, sd, stream))) return rc;
@@ -2691,8 +2720,8 @@ template <typename T> void Engine<T>::dyn_destroy() {
                     B->rflag_an, B->rflag_hs, B->pwsum, B->colq, B->pwat, B->precip, B->pblscr, B->latd,
                     B->adesc_sp, B->rflag_sp, B->tpat, B->tpat_var, B->tpat_vf, B->tpat_ctr, B->ds_x, B->ds_y,
                     B->nrm, B->xin, B->yout, B->tbx, B->tby, B->ganj, B->pwatj, B->precipj, B->jpart, B->jstat,
-                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph, B->sdesc_hla,
-                    B->rflag_hla, B->slot_hla, B->sdesc_hlb, B->rflag_hlb, B->slot_hlb, B->hsdyn,
+                    B->jflag, B->jctr, B->gmean, B->dak, B->sdesc_ph, B->rflag_ph, B->slot_ph, B->sdesc_hl,
+                    B->rflag_hl, B->slot_hl, B->hsdyn,
                     B->clock, B->iau_hrs, B->iau_inc, B->iau_F, B->st_pat2, B->st_pat[3], B->st_var, B->st_vf,
                     B->st_ctr, B->st_grd, B->sd_pat, B->rf0, B->vcg, B->sd_vc, B->rf1, B->an_w, B->an_g,
                     B->an_sm, B->sd_an, B->ad_an, B->rf_an};
@@ -2751,18 +2780,23 @@ template <typename T> struct Step {
         launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_ph, s, (const T *)B.gmean, B.slot_ph);
         mark("fft_synth");
     }
-    // the lean Held-Suarez pass: temperature and ln ps into their grid slots,
-    // ps as the column kernel forms it (k_psg), the levels where the drag can
-    // act (with the area sums), then the vorticity and divergence of those levels
+    // the lean Held-Suarez pass: temperature, ln ps and the vorticity and
+    // divergence of the levels where the last lean pass found drag (and one
+    // more) into their grid slots; ps as the column kernel forms it (k_psg);
+    // the levels where the drag can act now (with the area sums); and, only
+    // where those reach past the first synthesis, the vorticity and divergence
+    // of all of them
     void synth_hs_lean() {
         const Plan<T> &p = E.plan;
         const cplx<T> *spec = (const cplx<T> *)B.spec;
         const long sk = (long)B.nfs * B.nm;
-        launch_synth_gather(p, spec, 1, sk, B.sdesc_hla, B.nhla, W.sx, W.ldc, 1.0, s, (T *)B.gmean);
+        const int nf = B.nhla + B.nhlb, na = B.nhla;
+        launch_synth_gather(p, spec, 1, sk, B.sdesc_hl, nf, W.sx, W.ldc, 1.0, s, (T *)B.gmean, B.hsdyn + 3);
         mark("synth_gather");
-        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * B.nhla, s);
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s, B.hsdyn + 3);
         mark("leg_synth");
-        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, B.nhla, B.rflag_hla, s, (const T *)B.gmean, B.slot_hla);
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_hl, s, (const T *)B.gmean, B.slot_hl, 0,
+                         B.hsdyn + 3);
         mark("fft_synth");
         dim3 g((unsigned)((B.ng + 255) / 256), B.nm);
         hipLaunchKernelGGL(k_psg<T>, g, dim3(256), 0, s, (const T *)B.gsyn, spec, (T *)B.psg, B.nl, B.ntr, B.nsyn,
@@ -2770,13 +2804,13 @@ template <typename T> struct Step {
         mark("psg");
         // the global means of the forcing's mass fixer, and the drag's levels
         area_sums(nullptr, E.cfg.massfix ? B.pmean : (double *)nullptr, nullptr, false, true);
-        launch_synth_gather(p, spec, 1, sk, B.sdesc_hlb, B.nhlb, W.sx, W.ldc, 1.0, s, (T *)nullptr, B.hsdyn);
-        mark("synth_gather");
+        launch_synth_gather(p, spec, 1, sk, B.sdesc_hl + na, B.nhlb, W.sx, W.ldc, 1.0, s, (T *)nullptr, B.hsdyn);
+        mark("synth_gather_more");
         launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * B.nhlb, s, B.hsdyn);
-        mark("leg_synth");
-        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, B.nhlb, B.rflag_hlb, s, (const T *)nullptr, B.slot_hlb,
-                         0, B.hsdyn);
-        mark("fft_synth");
+        mark("leg_synth_more");
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, B.nhlb, B.rflag_hl + na, s, (const T *)nullptr,
+                         B.slot_hl + na, 0, B.hsdyn);
+        mark("fft_synth_more");
     }
     // dyn_run.f90:114-116: the vorticity gradient of vorticity confinement
     void synth_vc() {

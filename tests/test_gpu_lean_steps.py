@@ -97,6 +97,43 @@ def test_drag_levels_cover_the_boundary_layer(prec):
         assert np.array_equal(sa[k], sb[k]), k
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_drag_reaching_more_levels_than_the_last_pass(prec):
+    """The lean pass first synthesises the vorticity and divergence of the
+    levels where the previous lean pass found drag, plus one; where the drag
+    now reaches higher it synthesises the rest before the forcing. Halving ps
+    between two advances (7 levels with drag near 1000 hPa on these hybrid
+    levels, 12 near 500 hPa) takes that path; the state still matches full
+    steps bit for bit, through the mass fixer's ln ps update as well."""
+    import gfsdc
+    p = gfsdc.FP64 if prec == "fp64" else gfsdc.FP32
+    print(f"drag levels rising {prec}", flush=True)
+    ak, bk = levels.hybrid_levels(NL)
+
+    def mk():
+        c = gfsdc.Context(NLON, NLAT, T, p, nlev=NL, ak=ak, bk=bk, dt=1200.0)
+        c.init_rest_state(300.0, 1.0e5, 2e-5, 1, 21, seed=6)
+        return c
+    a, b = mk(), mk()
+    a.advance(3, use_graph=True)
+    for _ in range(3):
+        b.advance(1, use_graph=True)
+    n0 = a.hs_drag_levels()
+    assert n0 in (6, 7, 8)
+    for c in (a, b):
+        st = c.get_state()
+        st["lnps"][:, 0] -= np.log(2.0) * np.sqrt(4 * np.pi)  # the (0, 0) coefficient: ps / 2
+        c.set_state(st["vrt"], st["div"], st["tv"], st["lnps"])
+    # one lean step (the dry-mass fixer then restores ps) and a full one
+    a.advance(2, use_graph=True)
+    for _ in range(2):
+        b.advance(1, use_graph=True)
+    assert a.hs_drag_levels() >= n0 + 2
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("vrt", "div", "tv", "lnps"):
+        assert np.array_equal(sa[k], sb[k]), k
+
+
 def test_profiled_lean_step_refuses_the_grids():
     import gfsdc
     print("lean profile_step", flush=True)
