@@ -12,7 +12,8 @@ value = mixed-precision (FP32 Legendre/FFT/columns, FP64 tables and global
 sums) simulated days per wall-day; the FP64 GPU path is timed beside it and
 the spectral error of the mixed path after --error-steps steps (default one
 model day) against the FP64 path is reported. roofline = the dominant kernel
-of the step, timed with HIP events on the context stream; cpu_baseline = the
+of the step, timed with HIP events on the context stream (fft_synth_dynamics_pass
+the Fourier synthesis on the dynamics passes alone); cpu_baseline = the
 FP64 CPU oracle timed on the same workload on the host cores (median of 5
 steps after a warm-up step on every physical core of the process's CPU set,
 plus one single-thread step).
@@ -78,15 +79,33 @@ def time_steps(ctx, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def kernel_table(ctx, nprof):
-    """mean per-launch time (ms) and launches per step, by kernel name"""
+def kernel_table(ctx, nprof, raw=None):
+    """mean per-launch time (ms) and launches per step, by kernel name; each
+    profiled step's (name, ms) list is appended to raw where given"""
     acc = {}
     for _ in range(nprof):
-        for name, ms in ctx.profile_step():
+        prof = ctx.profile_step()
+        if raw is not None:
+            raw.append(prof)
+        for name, ms in prof:
             a = acc.setdefault(name, [0.0, 0])
             a[0] += ms
             a[1] += 1
     return {k: (v[0] / v[1], v[1] / nprof) for k, v in acc.items()}
+
+
+def dynamics_fft_synth(raw, ctx, esz):
+    """k_fft_synth on the dynamics passes alone (the first three launches of
+    each profiled step, (7 + 3 ntrac) nl + 3 fields each) against HBM peak"""
+    ms = [t for prof in raw for t in [t for n, t in prof if n == "fft_synth"][:3]]
+    if not ms:
+        return None
+    nf = ctx.nmem * ((7 + 3 * ctx.ntrac) * ctx.nlev + 3)
+    work = nf * ((ctx.ntrunc + 1) * ctx.nlat * 2 * esz + ctx.nlat * ctx.nlon * esz)
+    avg = sum(ms) / len(ms)
+    ach = work / (avg * 1e-3) / 1e9
+    return {"kernel": "fft_synth", "pass": "dynamics", "fields": nf, "avg_launch_ms": avg, "achieved": ach,
+            "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": ach / (PEAK["hbm"] / 1e9)}
 
 
 def algorithmic_work(name, ctx, esz, launches):
@@ -288,8 +307,10 @@ def run_headline(args):
         out["fp64_days_per_day"] = dt / t64
         out["speedup_mixed_vs_fp64"] = t64 / t32
         del c64
-    tab = kernel_table(c32, args.prof_steps)
+    raw = []
+    tab = kernel_table(c32, args.prof_steps, raw)
     roof, per_step = roofline(tab, c32, 4, args.config)
+    fsd = dynamics_fft_synth(raw, c32, 4)
     def mfma_frac(name):
         leg = tab.get(name)
         if not leg:
@@ -317,7 +338,7 @@ def run_headline(args):
          "members": 1, "hip_graph": True},
         ms_per_step=t32 * 1e3, **out, error_vs_fp64_after_steps=args.error_steps if err else 0,
         spectral_rel_error_vs_fp64=err, zonal_mean_drift_vs_fp64=drift, roofline=roof, legendre_gemm=legendre,
-        legendre_gemm_analysis=legendre_anal,
+        legendre_gemm_analysis=legendre_anal, fft_synth_dynamics_pass=fsd,
         kernels_ms_per_step={k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda x: -x[1])},
         cpu_baseline=cpu)
     return line
