@@ -115,26 +115,30 @@ def algorithmic_work(name, ctx, esz, launches):
     written once)."""
     nlat, nlon, T, nlm = ctx.nlat, ctx.nlon, ctx.ntrunc, ctx.nlm
     nl, nm = ctx.nlev, ctx.nmem
-    # fields of a dynamics pass: synthesised, and analysed (T, the two flux
-    # components and KE per level, ln ps and the tracers)
+    # fields of a dynamics pass: synthesised on the grid, synthesised by the
+    # Legendre GEMM (all but the x-gradients of temperature, the tracers and ln ps,
+    # which the Fourier synthesis forms from their scalars), and analysed (T, the
+    # two flux components and KE per level, ln ps and the tracers)
     nsyn = (7 + 3 * ctx.ntrac) * nl + 3
+    nleg = nsyn - (1 + ctx.ntrac) * nl - 1
     nan = (4 + ctx.ntrac) * nl + 1
     if ctx.cfg.physics == gfsdc.PHYS_HS:
         # three dynamics passes and the lean Held-Suarez pass of the profiled
         # step: temperature and ln ps, then vorticity and divergence on the
         # levels where the drag acts; the forcing analysed on the same levels
         lean = (nl + 1) + 2 * ctx.hs_drag_levels()
-        syn_step, an_step = 3 * nsyn + lean, 3 * nan + lean
+        syn_step, leg_step, an_step = 3 * nsyn + lean, 3 * nleg + lean, 3 * nan + lean
     elif ctx.cfg.physics != gfsdc.PHYS_NONE:
         # the physics pass skips the temperature and tracer gradients
-        syn_step, an_step = 3 * nsyn + (5 + ctx.ntrac) * nl + 3, None
+        phys = (5 + ctx.ntrac) * nl + 3
+        syn_step, leg_step, an_step = 3 * nsyn + phys, 3 * nleg + phys, None
     else:
-        syn_step, an_step = 3 * nsyn, 3 * nan
+        syn_step, leg_step, an_step = 3 * nsyn, 3 * nleg, 3 * nan
     nsyn = syn_step / launches  # mean fields per launch
     ng = nlat * nlon
     four = (T + 1) * nlat * 2 * esz  # Fourier coefficients of one field, both hemispheres
     if name == "leg_synth":
-        return "mfma", nm * nsyn * 2.0 * nlat * nlm
+        return "mfma", nm * leg_step / launches * 2.0 * nlat * nlm
     if name == "leg_anal":
         if an_step is None:
             return "mfma", None

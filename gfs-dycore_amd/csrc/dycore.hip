@@ -139,7 +139,13 @@ struct DycoreBuffers {
     int iau_n = 0;
     double *iau_hrs = nullptr, iau_dt = 0.0;
     void *iau_inc = nullptr, *iau_F = nullptr;
-    SynthDesc *sdesc = nullptr;                  // nm*nsyn
+    // the dynamics synthesis: the gather and Legendre GEMM take nleg fields per
+    // member (every grid field but the x-gradients dtv/dx, dq/dx, dlnps/dx,
+    // which the Fourier synthesis forms from its scalar's Legendre output through
+    // fmap_dyn); the Fourier synthesis writes all nsyn grid fields
+    SynthDesc *sdesc = nullptr;                  // nm*nleg
+    int nleg = 0;
+    FieldMap fmap_dyn;
     // the physics pass synthesises every field but the temperature and tracer
     // gradients (nph per member): descriptors, 1/cos flags and grid slots
     int nph = 0;
@@ -2683,8 +2689,51 @@ template <typename T> int Engine<T>::dyn_init() {
         GFSDC_CHECK(hipMemcpyAsync(B->hsdyn, c0, sizeof(c0), hipMemcpyHostToDevice, stream));
         GFSDC_CHECK(hipStreamSynchronize(stream));
     }
+    // the dynamics synthesis' Legendre fields and the map of its grid fields onto
+    // them: runs of grid slots (u..tv, dtv/dx, dtv/dy, q, dq/dx, dq/dy, ln ps,
+    // dlnps/dx, dlnps/dy) and the Legendre field each run starts at
+    std::vector<SynthDesc> sdl;
+    {
+        const int T3 = ntr * nl, o = 7 * nl + 3 * T3, lo = 6 * nl + 2 * T3;
+        const int st[9] = {0, 5 * nl, 6 * nl, 7 * nl, 7 * nl + T3, 7 * nl + 2 * T3, o, o + 1, o + 2};
+        const int sr[9] = {0, 4 * nl, 5 * nl, 6 * nl, 6 * nl, 6 * nl + T3, lo, lo, lo + 1};
+        const unsigned dxm = (1u << 1) | (1u << 4) | (1u << 7);
+        FieldMap &M = B->fmap_dyn;
+        M.nseg = 9;
+        M.per_dst = B->nsyn;
+        M.per_src = B->nleg = B->nsyn - nl - T3 - 1;
+        M.dx = dxm;
+        for (int i = 0; i < 9; i++) {
+            M.start[i] = st[i];
+            M.src[i] = sr[i];
+        }
+        for (int mem = 0; mem < nm; mem++)
+            for (int f = 0; f < B->nsyn; f++) {
+                bool dx;
+                const int l = fmap_field(M, mem * B->nsyn + f, dx) - mem * B->nleg;
+                if (!dx) {
+                    if (l != (int)sdl.size() - mem * B->nleg) {
+                        set_error("internal: dynamics synthesis map");
+                        return GFSDC_EINVAL;
+                    }
+                    sdl.push_back(sd[(size_t)mem * B->nsyn + f]);
+                }
+            }
+        // every x-gradient is i m / a times the Legendre field of its own scalar
+        for (int mem = 0; mem < nm; mem++)
+            for (int f = 0; f < B->nsyn; f++) {
+                bool dx;
+                const int l = fmap_field(M, mem * B->nsyn + f, dx);
+                const SynthDesc &g = sd[(size_t)mem * B->nsyn + f];
+                if (dx && (g.kind != SK_UHAT || g.a >= 0 || g.sb != ra_ || g.inv_b || sdl[l].a != g.b ||
+                           (sdl[l].kind != SK_COPY && sdl[l].kind != SK_COPY_DEV && sdl[l].kind != SK_COPY_MEAN))) {
+                    set_error("internal: dynamics synthesis x-gradients");
+                    return GFSDC_EINVAL;
+                }
+            }
+    }
     void *p;
-    if ((rc = dupload<SynthDesc>(&p, sd, stream))) return rc;
+    if ((rc = dupload<SynthDesc>(&p, sdl, stream))) return rc;
     B->sdesc = (SynthDesc *)p;
     if ((rc = dupload<AnalDesc>(&p, ad, stream))) return rc;
     B->adesc_dyn = (AnalDesc *)p;
@@ -2758,13 +2807,14 @@ template <typename T> struct Step {
     }
     void synth_all() {
         const Plan<T> &p = E.plan;
-        int nf = B.nm * B.nsyn;
-        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc, nf, W.sx, W.ldc, 1.0, s,
+        const int nlg = B.nm * B.nleg, nf = B.nm * B.nsyn;
+        launch_synth_gather(p, (const cplx<T> *)B.spec, 1, (long)B.nfs * B.nm, B.sdesc, nlg, W.sx, W.ldc, 1.0, s,
                             (T *)B.gmean);
         mark("synth_gather");
-        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nf, s);
+        launch_leg_synth(p, W.sx, W.ldc, W.fs, W.ldc, 2 * nlg, s);
         mark("leg_synth");
-        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_syn, s, (const T *)B.gmean);
+        launch_fft_synth(p, W.fs, W.ldc, (T *)B.gsyn, B.ng, nf, B.rflag_syn, s, (const T *)B.gmean, nullptr, 0,
+                         nullptr, &B.fmap_dyn, 1.0 / E.cfg.rerth);
         mark("fft_synth");
     }
     // the same chain for the physics pass, without the temperature and tracer
@@ -3189,7 +3239,7 @@ template <typename T> int Engine<T>::autotune(double *ms, int *best) {
         for (int rep = 0; rep < 2; rep++) {  // first pass warms up
             GFSDC_CHECK(hipEventRecord(e0, stream));
             launch_fft_synth(plan, W.fs, W.ldc, (T *)B->gsyn, B->ng, B->nm * B->nsyn, B->rflag_syn, stream,
-                             (const T *)B->gmean);
+                             (const T *)B->gmean, nullptr, 0, nullptr, &B->fmap_dyn, 1.0 / cfg.rerth);
             GFSDC_CHECK(hipEventRecord(e1, stream));
             launch_fft_anal(plan, (const T *)B->gan, B->ng, B->nm * B->nan, B->rflag_an, W.fa, W.ldc, stream);
             GFSDC_CHECK(hipEventRecord(e2, stream));

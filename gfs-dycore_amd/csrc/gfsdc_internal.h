@@ -97,6 +97,32 @@ struct SynthDesc {
     double sa, sb;     // multipliers of S and T
     int inv_a, inv_b;  // multiply S / T by invlap(n)
 };
+// Map from the grid fields of a Fourier synthesis to the Legendre-synthesised
+// fields it reads: per member, per_dst grid fields in nseg runs; run s starts
+// at grid field start[s] (ascending; an empty run shares its start with the
+// next) and reads Legendre fields from src[s] on. A run whose bit is set in dx
+// is the x-gradient (1/(a cos phi)) d/dlambda of its source: the Fourier
+// synthesis multiplies the source's coefficients by i m / a instead of taking
+// a Legendre synthesis of its own (the 1/cos phi comes with the field's flag).
+// nseg = 0 is the identity.
+constexpr int FMAP_SEGS = 10;
+struct FieldMap {
+    int nseg = 0, per_dst = 0, per_src = 0;
+    int start[FMAP_SEGS] = {0}, src[FMAP_SEGS] = {0};
+    unsigned dx = 0;
+};
+__host__ __device__ inline int fmap_field(const FieldMap &M, int f, bool &dx) {
+    if (M.nseg == 0) {
+        dx = false;
+        return f;
+    }
+    const int mem = f / M.per_dst, fl = f - mem * M.per_dst;
+    int s = 0;
+    for (int i = 1; i < M.nseg; i++) s = fl >= M.start[i] ? i : s;
+    dx = (M.dx >> s) & 1u;
+    return mem * M.per_src + M.src[s] + (fl - M.start[s]);
+}
+
 // AK_ZERO writes a zero coefficient (an output with no source, e.g. a tracer a
 // physics package leaves unchanged; with accum it leaves the output alone).
 // AK_MASSFIX applies the dry-mass fixer to ln ps in place (dyn_run.f90:626-633
@@ -128,10 +154,13 @@ void launch_synth_gather(const Plan<T> &p, const cplx<T> *src, long sf, long sk,
 template <typename T>
 void launch_leg_synth(const Plan<T> &p, const T *sx, int ldx, T *fs, int ldf, int ncol, hipStream_t s,
                       const int *nfdyn = nullptr);
+// fmap (with rdx = 1 / a for its x-gradients) maps the grid fields to the
+// Legendre fields they read; the mean is indexed by the Legendre field
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
                       const unsigned char *rcos_flag, hipStream_t s, const T *mean = nullptr,
-                      const int *slot = nullptr, int fofs = 0, const int *nfdyn = nullptr);
+                      const int *slot = nullptr, int fofs = 0, const int *nfdyn = nullptr,
+                      const FieldMap *fmap = nullptr, double rdx = 0.0);
 template <typename T>
 void launch_fft_anal(const Plan<T> &p, const T *grid, long gstride, int nf, const unsigned char *rcos_flag,
                      T *fa, int ldf, hipStream_t s, const int *nfdyn = nullptr);

@@ -1027,7 +1027,7 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
                                                    const T *__restrict__ mean, const int *__restrict__ slot,
                                                    const T *__restrict__ rcos, const cplx<T> *__restrict__ tw,
                                                    int nlon, int nlat, int ntrunc, Radices rd, int fofs,
-                                                   const int *__restrict__ nfdyn) {
+                                                   const int *__restrict__ nfdyn, FieldMap fmap, T rdx) {
     constexpr int LN = NC ? NC : MAXN;
     // wavefront path; with 8 fields the twiddles are read from the (cached)
     // global table so that three workgroups fit a CU's LDS
@@ -1055,9 +1055,12 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     constexpr bool QUADL = WAVE && NSL == 256 && RL <= 4;
     if constexpr (WAVE) {
         const int fw = min(f0 + (tid >> 6), nf - 1);
+        bool dxw;
+        const int lw = fmap_field(fmap, fw, dxw);
         const T rc = rcos[j];  // unconditional, so that no load waits for another
         w_s = rflag[fw] ? rc : T(1);
-        w_c = mean ? mean[fw] : T(0);
+        w_c = mean ? mean[lw] : T(0);
+        if (dxw) w_c = T(0);
         w_g = slot ? slot[fw] : fw;
     }
     constexpr int MAXI = (NFFT * (LN / 2) + NT - 1) / NT;
@@ -1067,25 +1070,33 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     // of the clamped lanes are zeroed afterwards, so no load sits in a branch
     // that would make it wait for the one before
     cplx<T> ev[MAXI], ov[MAXI];
+    // a thread's items all belong to field tid % NFFT (NT is a multiple of NFFT);
+    // field f reads Legendre field fmap(f) + fofs
+    bool dxl;
+    const int cf = 2 * (fmap_field(fmap, min(f0 + (int)(tid % NFFT), nf - 1), dxl) + fofs);
 #pragma unroll
     for (int c = 0; c < MAXI; c++) {
         ev[c] = ov[c] = cplx<T>{T(0), T(0)};
         if (c * NT < nitems) {  // uniform
             const int idx = min(tid + c * NT, nitems - 1);
-            const int m = idx / NFFT, q = idx - m * NFFT, f = min(f0 + q, nf - 1);
-            // rows j / j2 hold the even / odd Legendre sums; field f is Fourier field f + fofs
-            const int cf = 2 * (f + fofs);
+            const int m = idx / NFFT;
+            // rows j / j2 hold the even / odd Legendre sums
             ev[c] = *(const cplx<T> *)(fs + fs_index(j, m, cf, ldf, ntrunc + 1));
             ov[c] = *(const cplx<T> *)(fs + fs_index(j2, m, cf, ldf, ntrunc + 1));
         }
     }
-    // hemispheres: north E + O, south E - O
+    // hemispheres: north E + O, south E - O; an x-gradient times i m / a
     cplx<T> xn[MAXI], xs[MAXI];
 #pragma unroll
     for (int c = 0; c < MAXI; c++) {
         const int idx = tid + c * NT, q = idx % NFFT;
         const bool ok = idx < nitems && f0 + q < nf;
-        const cplx<T> e = ev[c], o = ov[c];
+        cplx<T> e = ev[c], o = ov[c];
+        if (dxl) {
+            const T k = T(idx / NFFT) * rdx;
+            e = cplx<T>{-k * e.y, k * e.x};
+            o = cplx<T>{-k * o.y, k * o.x};
+        }
         xn[c] = ok ? cplx<T>{e.x + o.x, e.y + o.y} : cplx<T>{T(0), T(0)};
         xs[c] = ok ? cplx<T>{e.x - o.x, e.y - o.y} : cplx<T>{T(0), T(0)};
     }
@@ -1723,14 +1734,14 @@ static Radices radices_of(int n, const int *r) {
 template <typename T, int NF>
 static void fft_synth_nf(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
                          const unsigned char *rflag, const T *mean, const int *slot, int fofs, hipStream_t s,
-                         const int *nfdyn) {
+                         const int *nfdyn, const FieldMap &fmap, T rdx) {
     constexpr int MN = FftCfg<T>::MAXN;
     dim3 g(p.nhalf * ((nf + NF - 1) / NF));
 #define FFT_SYN(NC)                                                                                             \
     hipLaunchKernelGGL((k_fft_synth<T, NF, MN, NC>), g, dim3(fft_threads(NF, NC)), 0, s, fs, ldf, grid, gstride, nf,   \
                        rflag, mean, slot,                                                                     \
                        p.rcos, (const cplx<T> *)p.tw, p.nlon, p.nlat, p.ntrunc, radices_of(p.nrad, p.rad), fofs, \
-                       nfdyn)
+                       nfdyn, fmap, rdx)
     switch (p.nlon) {
     case 768: FFT_SYN(768); break;
     case 384: FFT_SYN(384); break;
@@ -1766,11 +1777,12 @@ static void fft_anal_nf(const Plan<T> &p, const T *grid, long gstride, int nf, c
 template <typename T>
 void launch_fft_synth(const Plan<T> &p, const T *fs, int ldf, T *grid, long gstride, int nf,
                       const unsigned char *rflag, hipStream_t s, const T *mean, const int *slot, int fofs,
-                      const int *nfdyn) {
+                      const int *nfdyn, const FieldMap *fmap, double rdx) {
+    const FieldMap id{}, &fm = fmap ? *fmap : id;
     switch (p.fft_nf) {
-    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn); break;
-    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn); break;
-    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn); break;
+    case 2: fft_synth_nf<T, 2>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn, fm, (T)rdx); break;
+    case 4: fft_synth_nf<T, 4>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn, fm, (T)rdx); break;
+    default: fft_synth_nf<T, 8>(p, fs, ldf, grid, gstride, nf, rflag, mean, slot, fofs, s, nfdyn, fm, (T)rdx); break;
     }
 }
 
@@ -1805,7 +1817,8 @@ void launch_anal_scatter(const Plan<T> &p, const T *sx, int ldx, const AnalDesc 
                                       const int *);                                                            \
     template void launch_leg_anal<T>(const Plan<T> &, const T *, int, T *, int, int, hipStream_t, const int *); \
     template void launch_fft_synth<T>(const Plan<T> &, const T *, int, T *, long, int, const unsigned char *,  \
-                                      hipStream_t, const T *, const int *, int, const int *);                  \
+                                      hipStream_t, const T *, const int *, int, const int *, const FieldMap *, \
+                                      double);                                                                 \
     template void launch_fft_anal<T>(const Plan<T> &, const T *, long, int, const unsigned char *, T *, int,   \
                                      hipStream_t, const int *);                                                \
     template void launch_anal_scatter<T>(const Plan<T> &, const T *, int, const AnalDesc *, int, cplx<T> *,    \
