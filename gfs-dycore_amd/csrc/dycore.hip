@@ -2707,6 +2707,11 @@ template <typename T> int Engine<T>::dyn_init() {
             M.start[i] = st[i];
             M.src[i] = sr[i];
         }
+        fmap_set_divisor(M);
+        if ((unsigned long long)nm * B->nsyn * B->nsyn >= ((unsigned long long)1 << 32)) {
+            set_error("internal: dynamics synthesis map size");
+            return GFSDC_EINVAL;
+        }
         for (int mem = 0; mem < nm; mem++)
             for (int f = 0; f < B->nsyn; f++) {
                 bool dx;

@@ -104,23 +104,50 @@ struct SynthDesc {
 // is the x-gradient (1/(a cos phi)) d/dlambda of its source: the Fourier
 // synthesis multiplies the source's coefficients by i m / a instead of taking
 // a Legendre synthesis of its own (the 1/cos phi comes with the field's flag).
-// nseg = 0 is the identity.
+// nseg = 0 is the identity. The member index is f * rdiv >> 32 with rdiv =
+// ceil(2^32 / per_dst) (exact for f per_dst < 2^32), and the runs are tested
+// with compile-time indices only, so that the kernel keeps the map in scalar
+// registers and loads nothing for it.
 constexpr int FMAP_SEGS = 10;
 struct FieldMap {
     int nseg = 0, per_dst = 0, per_src = 0;
+    unsigned rdiv = 0;
     int start[FMAP_SEGS] = {0}, src[FMAP_SEGS] = {0};
     unsigned dx = 0;
 };
+inline void fmap_set_divisor(FieldMap &M) {
+    M.rdiv = (unsigned)((((unsigned long long)1 << 32) + (unsigned)M.per_dst - 1) / (unsigned)M.per_dst);
+}
 __host__ __device__ inline int fmap_field(const FieldMap &M, int f, bool &dx) {
     if (M.nseg == 0) {
         dx = false;
         return f;
     }
-    const int mem = f / M.per_dst, fl = f - mem * M.per_dst;
-    int s = 0;
-    for (int i = 1; i < M.nseg; i++) s = fl >= M.start[i] ? i : s;
-    dx = (M.dx >> s) & 1u;
-    return mem * M.per_src + M.src[s] + (fl - M.start[s]);
+    const int mem = (int)(((unsigned long long)(unsigned)f * M.rdiv) >> 32), fl = f - mem * M.per_dst;
+    int off = M.src[0] - M.start[0];
+    bool d = M.dx & 1u;
+#pragma unroll
+    for (int i = 1; i < FMAP_SEGS; i++)
+        if (i < M.nseg && fl >= M.start[i]) {
+            off = M.src[i] - M.start[i];
+            d = (M.dx >> i) & 1u;
+        }
+    dx = d;
+    return mem * M.per_src + fl + off;
+}
+// grid fields fa..fb (fa <= fb) read consecutive Legendre fields of one kind:
+// one member, no run starting inside
+__host__ __device__ inline bool fmap_linear(const FieldMap &M, int fa, int fb) {
+    if (M.nseg == 0) return true;
+    const int ma = (int)(((unsigned long long)(unsigned)fa * M.rdiv) >> 32);
+    const int mb = (int)(((unsigned long long)(unsigned)fb * M.rdiv) >> 32);
+    if (ma != mb) return false;
+    const int la = fa - ma * M.per_dst, lb = fb - ma * M.per_dst;
+    bool ok = true;
+#pragma unroll
+    for (int i = 1; i < FMAP_SEGS; i++)
+        if (i < M.nseg && M.start[i] > la && M.start[i] <= lb) ok = false;
+    return ok;
 }
 
 // AK_ZERO writes a zero coefficient (an output with no source, e.g. a tracer a

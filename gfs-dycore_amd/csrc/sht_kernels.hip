@@ -1053,10 +1053,24 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     constexpr RadPlan PS = rad_plan(NC > 0 ? NC : 2);
     constexpr int RL = PS.r[PS.n > 0 ? PS.n - 1 : 0], NSL = (NC > 0 ? NC : 2) / RL;
     constexpr bool QUADL = WAVE && NSL == 256 && RL <= 4;
+    // the Legendre fields of the workgroup's grid fields: where they are
+    // consecutive (every group but those at a run's or a member's end) the map
+    // is taken once for the group, with workgroup-uniform values
+    const int fa = min(f0, nf - 1), fb = min(f0 + NFFT - 1, nf - 1);
+    bool dxa;
+    const int la = fmap_field(fmap, fa, dxa);
+    const bool lin = fmap_linear(fmap, fa, fb);
+    auto lfield = [&](int q, bool &dx) {
+        if (lin) {
+            dx = dxa;
+            return la + min(q, fb - fa);
+        }
+        return fmap_field(fmap, min(f0 + q, nf - 1), dx);
+    };
     if constexpr (WAVE) {
         const int fw = min(f0 + (tid >> 6), nf - 1);
         bool dxw;
-        const int lw = fmap_field(fmap, fw, dxw);
+        const int lw = lfield(tid >> 6, dxw);
         const T rc = rcos[j];  // unconditional, so that no load waits for another
         w_s = rflag[fw] ? rc : T(1);
         w_c = mean ? mean[lw] : T(0);
@@ -1073,7 +1087,7 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     // a thread's items all belong to field tid % NFFT (NT is a multiple of NFFT);
     // field f reads Legendre field fmap(f) + fofs
     bool dxl;
-    const int cf = 2 * (fmap_field(fmap, min(f0 + (int)(tid % NFFT), nf - 1), dxl) + fofs);
+    const int cf = 2 * (lfield((int)(tid % NFFT), dxl) + fofs);
 #pragma unroll
     for (int c = 0; c < MAXI; c++) {
         ev[c] = ov[c] = cplx<T>{T(0), T(0)};
