@@ -172,7 +172,10 @@ def test_grids_after_advance_match_oracle(physics):
     g.set_state(vrt, div, tv, lnps)
     o.advance(3)
     g.advance(3, use_graph=True)
-    for name in ("ug", "vg", "vrtg", "divg", "virtempg", "dlnpdtg", "etadot", "psg", "dlnpsdt"):
+    # dlnpsdx is formed in the Fourier synthesis from ln ps's Legendre sums (i m / a),
+    # dlnpsdy by a Legendre synthesis of its own
+    for name in ("ug", "vg", "vrtg", "divg", "virtempg", "dlnpdtg", "etadot", "psg", "dlnpsdt", "dlnpsdx",
+                 "dlnpsdy"):
         a, b = g.grid(name)[0], o.grid(name)
         err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
         assert err < 1e-9, (name, err)
