@@ -96,12 +96,14 @@ def kernel_table(ctx, nprof, raw=None):
 
 def dynamics_fft_synth(raw, ctx, esz):
     """k_fft_synth on the dynamics passes alone (the first three launches of
-    each profiled step, (7 + 3 ntrac) nl + 3 fields each) against HBM peak"""
+    each profiled step, (7 + 3 ntrac) nl + 3 grid fields each, the Fourier
+    coefficients of all but the x-gradients) against HBM peak"""
     ms = [t for prof in raw for t in [t for n, t in prof if n == "fft_synth"][:3]]
     if not ms:
         return None
     nf = ctx.nmem * ((7 + 3 * ctx.ntrac) * ctx.nlev + 3)
-    work = nf * ((ctx.ntrunc + 1) * ctx.nlat * 2 * esz + ctx.nlat * ctx.nlon * esz)
+    nleg = nf - ctx.nmem * ((1 + ctx.ntrac) * ctx.nlev + 1)  # all but the x-gradients
+    work = nleg * (ctx.ntrunc + 1) * ctx.nlat * 2 * esz + nf * ctx.nlat * ctx.nlon * esz
     avg = sum(ms) / len(ms)
     ach = work / (avg * 1e-3) / 1e9
     return {"kernel": "fft_synth", "pass": "dynamics", "fields": nf, "avg_launch_ms": avg, "achieved": ach,
@@ -134,7 +136,6 @@ def algorithmic_work(name, ctx, esz, launches):
         syn_step, leg_step, an_step = 3 * nsyn + phys, 3 * nleg + phys, None
     else:
         syn_step, leg_step, an_step = 3 * nsyn, 3 * nleg, 3 * nan
-    nsyn = syn_step / launches  # mean fields per launch
     ng = nlat * nlon
     four = (T + 1) * nlat * 2 * esz  # Fourier coefficients of one field, both hemispheres
     if name == "leg_synth":
@@ -144,7 +145,9 @@ def algorithmic_work(name, ctx, esz, launches):
             return "mfma", None
         return "mfma", nm * an_step / launches * 2.0 * nlat * nlm
     if name == "fft_synth":
-        return "hbm", nm * nsyn * (four + ng * esz)
+        # the Fourier coefficients of each Legendre-synthesised field once (an
+        # x-gradient reads those of its scalar) and the grid of every field
+        return "hbm", nm * (leg_step * four + syn_step * ng * esz) / launches
     if name == "mlp_forward" and getattr(ctx, "mlp_shape", None):
         # bf16 GEMMs of the column network over every grid column
         nin, h, nhid, nout = ctx.mlp_shape
