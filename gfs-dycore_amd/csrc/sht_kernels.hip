@@ -1201,8 +1201,10 @@ __global__ __launch_bounds__(fft_threads(NFFT, NC)) void k_fft_synth(const T *__
     for (int q = 0; q < NFFT; q++) {
         const int f = f0 + q;
         if (f >= nf) break;
+        bool dxf;
+        const int lf = lfield(q, dxf);  // the mean belongs to the Legendre field; an x-gradient has none
         const T s = rflag[f] ? rc : T(1);
-        const T c = mean ? mean[f] : T(0);
+        const T c = (mean && !dxf) ? mean[lf] : T(0);
         const long gf = slot ? slot[f] : f;  // grid field the transform is written to
         T *gn = grid + gf * gstride + (long)j * nlon, *gs = grid + gf * gstride + (long)j2 * nlon;
         const cplx<T> *bq = buf + q * NP;

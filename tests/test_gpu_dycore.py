@@ -223,8 +223,14 @@ def test_topography_enters_as_the_geopotential_gradient(prec):
         assert np.abs(t1["dlnps"][mem] - t0["dlnps"][mem]).max() == 0, mem
 
 
-def test_ensemble_members_independent():
+@pytest.mark.parametrize("nfw", [2, 8])
+def test_ensemble_members_independent(nfw, monkeypatch):
+    """three members folded into every launch, each equal to its own
+    single-member run; with 2 fields per Fourier workgroup (the general path)
+    and 8 (the wavefront path), both of which map each member's x-gradients and
+    temperature mean onto its own Legendre fields"""
     import gfsdc
+    monkeypatch.setenv("GFSDC_FFT_NF", str(nfw))
     ak, bk = coords("sigma", 20)
     ens = gfsdc.Context(NLON, NLAT, T, gfsdc.FP64, nlev=20, ak=ak, bk=bk, dt=1200.0, nmem=3)
     ens.init_rest_state(300.0, 1.0e5, 1e-6, 1, 21, seed=100)
