@@ -189,7 +189,8 @@ int gfsdc_getdyntend(gfsdc_ctx *ctx, int rkstage, int inverse_only, void *dvrt, 
 /* replaces run.f90:194 advance, nsteps times (device resident; captured in a
  * HIP graph after the first step when use_graph != 0). With Held-Suarez
  * physics every step but the last runs a lean physics pass that synthesises
- * only the grids the forcing reads (vorticity, divergence, temperature, ln ps);
+ * only the grids the forcing reads (temperature, ln ps, and vorticity and
+ * divergence on the levels where the drag can act: gfsdc_hs_drag_levels);
  * the state is the same bit for bit, and the last step leaves every grid field
  * for gfsdc_get_grid and gfsdc_step_diag (run.f90:157-162). */
 int gfsdc_advance(gfsdc_ctx *ctx, int nsteps, int use_graph);
