@@ -1907,7 +1907,9 @@ __global__ __launch_bounds__(256) void k_si_mfma(
     __shared__ long sbase[C], slbase[C];
     __shared__ int sk[C];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int2 tl = tiles[blockIdx.x];
+    // tiles run in degree order, each XCD a contiguous range of them, so that
+    // the tiles of one degree share its operators in one L2
+    const int2 tl = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const int n = tl.x, i0 = tl.y, per = (n + 1) * nm;
     const int cnt = min(C, per - i0);
     const long NFS = (long)nfs * nm, nlin = 2L * nl + 1;

@@ -23,6 +23,17 @@ void set_error(const std::string &msg);
 
 template <typename T> struct cplx { T x, y; };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8).
+// Give each XCD a contiguous range of logical tiles, so that tiles sharing
+// cache lines (the field groups of one latitude pair; the Legendre block and
+// the spectral columns of one order m; the level operators of one degree) run
+// on the same XCD and meet in its L2 instead of fetching the same lines
+// through eight L2s.
+__device__ inline int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
 // Legendre GEMM tile shape (rows x columns of one workgroup, 4 waves).
 constexpr int GEMM_BM = 64;
 constexpr int GEMM_BN = 128;

@@ -147,16 +147,6 @@ __global__ void k_legendre_table(T *plm, const long *poff, const double *mu, int
 // consecutive columns.
 constexpr int LDS_PAD = 16;
 
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8).
-// Give each XCD a contiguous range of logical tiles, so that tiles sharing
-// cache lines (the field groups of one latitude pair; the Legendre block and
-// the spectral columns of one order m) run on the same XCD and meet in its L2
-// instead of fetching the same lines through eight L2s.
-__device__ inline int xcd_tile(int b, int nb) {
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + min(x, r) + k;
-}
-
 template <typename T> struct Vec;
 template <> struct Vec<float> {
     typedef float4 t;
